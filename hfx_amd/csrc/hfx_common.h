@@ -1,0 +1,288 @@
+/*
+ * hfx_common.h -- shared device/host definitions of the MI355X SEQUEST core.
+ *
+ * Canonical arithmetic (identical on the CPU oracle and on gfx950):
+ *   FP32 with one IEEE rounding per operation (built with -ffp-contract=off),
+ *   correctly rounded division (HIP's default for FP32 '/'), rint (half to even)
+ *   for every m/z -> bin conversion, b ladder = sequential FP32 prefix.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/hfx_hip.h"
+
+/* ------------------------------------------------------------------ */
+/* Constants                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Device ion constants, ion_series.h:16-26. */
+#define HFX_BIN_WIDTH_MONO 1.0005079f
+#define HFX_BIN_WIDTH_AVG 1.0011413f
+#define HFX_DEV_H2O 18.01056f
+#define HFX_DEV_NH3 17.02655f
+#define HFX_DEV_CO 27.9949f
+#define HFX_DEV_H 1.0078246f
+
+/* Largest spectrum length (bins) the LDS-resident paths accept. */
+#define HFX_MAX_LEN 4096
+/* Words of a per-wave peak map: one nibble per bin. */
+#define HFX_MAP_WORDS (HFX_MAX_LEN / 8)
+/* Highest fragment charge handled (ion_series.cu:334-342 supports 1..4). */
+#define HFX_MAX_CHARGE 4
+/* Fixed-point scale of the exact score: round(w*x*2^36) summed in int64. */
+#define HFX_FX_SCALE 68719476736.0
+
+#define HFX_WAVE 64
+
+/* ion byte packing, utils.h:92-101 */
+#define HFX_ACID_IDX(i) ((i)&0x1Fu)
+#define HFX_ACID_MOD(i) (((i)&0xE0u) >> 5)
+
+/* ------------------------------------------------------------------ */
+/* Error handling (host)                                                */
+/* ------------------------------------------------------------------ */
+
+void hfx_set_error(const char *fmt, ...);
+
+#define HFX_CHECK_HIP(expr)                                                          \
+    do {                                                                             \
+        hipError_t e__ = (expr);                                                     \
+        if (e__ != hipSuccess) {                                                     \
+            hfx_set_error("%s:%d: %s: %s", __FILE__, __LINE__, #expr,                \
+                          hipGetErrorString(e__));                                   \
+            return HFX_ERR_HIP;                                                      \
+        }                                                                            \
+    } while (0)
+
+#define HFX_CHECK_LAUNCH() HFX_CHECK_HIP(hipGetLastError())
+
+#define HFX_REQUIRE(cond, code, ...)                                                 \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            hfx_set_error(__VA_ARGS__);                                              \
+            return (code);                                                           \
+        }                                                                            \
+    } while (0)
+
+static inline hipStream_t hfx_stream(void *s) { return (hipStream_t)s; }
+
+/* Scratch slots of a context (one buffer per slot, grown on demand, never
+ * shrunk); entry points that may run back to back use distinct slots. */
+enum {
+    HFX_SLOT_SORT = 0,    /* radix sort ping-pong buffers        */
+    HFX_SLOT_SUM = 1,     /* device-side reductions to the host  */
+    HFX_SLOT_SCAN = 2,    /* prepare_scoring offsets             */
+    HFX_SLOT_TILES = 3,   /* xcorr tile offsets                  */
+    HFX_SLOT_TOPN = 4,    /* top-N bookkeeping                   */
+    HFX_SLOT_MVM = 5,     /* addIons / dense helpers             */
+    HFX_SLOT_MODS = 6,    /* modified-peptide enumeration        */
+    HFX_SLOT_MODS2 = 7,
+    HFX_SLOT_BATCH = 8,   /* batched path                        */
+    HFX_SLOT_BATCH2 = 9,
+    HFX_SLOT_DENSE = 10,  /* BF16 dense path                     */
+};
+/* Scratch owned by a context: grown on demand, never shrunk. */
+int hfx_ctx_scratch(hfx_ctx *ctx, int slot, size_t bytes, void **out);
+int hfx_ctx_pinned(hfx_ctx *ctx, size_t bytes, void **out);
+int hfx_ctx_num_cus(hfx_ctx *ctx);
+hfx_ctx *hfx_default_ctx(void);
+
+/* ------------------------------------------------------------------ */
+/* Device helpers                                                        */
+/* ------------------------------------------------------------------ */
+
+#ifdef __HIP_DEVICE_COMPILE__
+#define HFX_DEV __device__ __forceinline__
+#else
+#define HFX_DEV __device__ __host__ inline
+#endif
+
+/* binMZ with IEEE division (ion_series.cu:98-102): rint(mz / 1.0005079). */
+__device__ __forceinline__ int32_t hfx_bin(float mz)
+{
+    return (int32_t)__builtin_rintf(mz / HFX_BIN_WIDTH_MONO);
+}
+
+/* Unary weight codes: OR of codes == code of the maximum weight
+ * (10 -> 0b001, 25 -> 0b011, 50 -> 0b111); weight = {0,10,25,50}[popc]. */
+#define HFX_CODE10 1u
+#define HFX_CODE25 3u
+#define HFX_CODE50 7u
+
+__device__ __forceinline__ float hfx_code_weight(uint32_t nib)
+{
+    uint32_t c = __builtin_popcount(nib);
+    return c == 3 ? 50.0f : (c == 2 ? 25.0f : (c == 1 ? 10.0f : 0.0f));
+}
+
+/* Order-preserving map of an FP32 key to uint32 (larger float -> larger). */
+__device__ __forceinline__ uint32_t hfx_f2key(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float hfx_key2f(uint32_t k)
+{
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+__device__ __forceinline__ int64_t hfx_fx_term(float w, float x)
+{
+    double p = (double)w * (double)x * HFX_FX_SCALE; /* exact: 6 x 24 bits, 2^36 */
+    return (int64_t)__builtin_rint(p);
+}
+
+__device__ __forceinline__ void hfx_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float hfx_wave_sum_f32(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int64_t hfx_wave_sum_i64(int64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t hfx_lane() { return __lane_id(); }
+
+/*
+ * Theoretical peaks of one candidate into a per-wave LDS nibble map.
+ *
+ * masses for positions [0, n) are produced by `mass_of(j)` (called by lane j
+ * for j within the current 64-residue chunk); the b ladder is the sequential
+ * FP32 prefix, reproduced exactly by broadcasting each residue mass to every
+ * lane and letting lane k add masses 0..k in order.  For each ladder position
+ * and each charge 1..zmax the peaks of addIonsAB/addIonsY
+ * (ion_series.cu:111-156) are OR-ed into `map` (bins >= len dropped, max per
+ * bin through the unary weight code).
+ */
+template <class MassFn>
+__device__ __forceinline__ void hfx_wave_peaks(uint32_t *map, uint32_t n, float r,
+                                               uint32_t zmax, uint32_t len, MassFn mass_of)
+{
+    const uint32_t lane = hfx_lane();
+    float carry = 0.0f;
+    for (uint32_t c0 = 0; c0 < n; c0 += HFX_WAVE) {
+        const uint32_t cn = min((uint32_t)HFX_WAVE, n - c0);
+        float m = (lane < cn) ? mass_of(c0 + lane) : 0.0f;
+        float b = carry;
+        for (uint32_t j = 0; j < cn; ++j) {
+            const float mj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), j));
+            if (j <= lane) b = b + mj;
+        }
+        carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), cn - 1));
+        if (lane < cn) {
+            const float y = r - b;
+            for (uint32_t z = 1; z <= zmax; ++z) {
+                const float fz = (float)z;
+                const float zh = HFX_DEV_H * fz;
+                int32_t x;
+#define HFX_PUT(xx, code)                                                            \
+    do {                                                                             \
+        x = (xx);                                                                    \
+        if (x >= 0 && (uint32_t)x < len)                                             \
+            atomicOr(&map[(uint32_t)x >> 3], (code) << (((uint32_t)x & 7u) * 4u));     \
+    } while (0)
+                HFX_PUT(hfx_bin(((b - HFX_DEV_CO) + zh) / fz), HFX_CODE10);
+                const float mb = (b + zh) / fz;
+                const int32_t xb = hfx_bin(mb);
+                HFX_PUT(xb, HFX_CODE50);
+                HFX_PUT(xb + 1, HFX_CODE25);
+                HFX_PUT(xb - 1, HFX_CODE25);
+                HFX_PUT(hfx_bin(mb - HFX_DEV_H2O / fz), HFX_CODE10);
+                HFX_PUT(hfx_bin(mb - HFX_DEV_NH3 / fz), HFX_CODE10);
+                const float my = ((y + HFX_DEV_H2O) + zh) / fz;
+                const int32_t xy = hfx_bin(my);
+                HFX_PUT(xy, HFX_CODE50);
+                HFX_PUT(xy + 1, HFX_CODE25);
+                HFX_PUT(xy - 1, HFX_CODE25);
+                HFX_PUT(hfx_bin(my - HFX_DEV_NH3 / fz), HFX_CODE10);
+#undef HFX_PUT
+            }
+        }
+    }
+}
+
+/* Dot of the map with the LDS spectrum, FP32, clearing the map as it goes.
+ * Lane l scans words l, l+64, ... (conflict-free), bins ascending. */
+__device__ __forceinline__ float hfx_wave_dot_f32(uint32_t *map, const float *xs, uint32_t len)
+{
+    const uint32_t lane = hfx_lane();
+    const uint32_t nw = (len + 7) >> 3;
+    float acc = 0.0f;
+    for (uint32_t w = lane; w < nw; w += HFX_WAVE) {
+        uint32_t word = map[w];
+        if (word) {
+            map[w] = 0;
+            while (word) {
+                uint32_t sh = __builtin_ctz(word) & ~3u;
+                uint32_t nib = (word >> sh) & 0xFu;
+                word &= ~(0xFu << sh);
+                acc = acc + hfx_code_weight(nib) * xs[(w << 3) + (sh >> 2)];
+            }
+        }
+    }
+    return hfx_wave_sum_f32(acc);
+}
+
+/* FP32 and exact dots of the map in one pass, clearing it. */
+__device__ __forceinline__ void hfx_wave_dot_both(uint32_t *map, const float *xs, uint32_t len,
+                                                  float *f32_out, int64_t *fx_out)
+{
+    const uint32_t lane = hfx_lane();
+    const uint32_t nw = (len + 7) >> 3;
+    float acc = 0.0f;
+    int64_t fx = 0;
+    for (uint32_t w = lane; w < nw; w += HFX_WAVE) {
+        uint32_t word = map[w];
+        if (word) {
+            map[w] = 0;
+            while (word) {
+                uint32_t sh = __builtin_ctz(word) & ~3u;
+                uint32_t nib = (word >> sh) & 0xFu;
+                word &= ~(0xFu << sh);
+                const float wt = hfx_code_weight(nib);
+                const float xv = xs[(w << 3) + (sh >> 2)];
+                acc = acc + wt * xv;
+                fx += hfx_fx_term(wt, xv);
+            }
+        }
+    }
+    *f32_out = hfx_wave_sum_f32(acc);
+    *fx_out = hfx_wave_sum_i64(fx);
+}
+
+/* Exact (fixed-point) dot of the map with the LDS spectrum, clearing it. */
+__device__ __forceinline__ int64_t hfx_wave_dot_fx(uint32_t *map, const float *xs, uint32_t len)
+{
+    const uint32_t lane = hfx_lane();
+    const uint32_t nw = (len + 7) >> 3;
+    int64_t acc = 0;
+    for (uint32_t w = lane; w < nw; w += HFX_WAVE) {
+        uint32_t word = map[w];
+        if (word) {
+            map[w] = 0;
+            while (word) {
+                uint32_t sh = __builtin_ctz(word) & ~3u;
+                uint32_t nib = (word >> sh) & 0xFu;
+                word &= ~(0xFu << sh);
+                acc += hfx_fx_term(hfx_code_weight(nib), xs[(w << 3) + (sh >> 2)]);
+            }
+        }
+    }
+    return hfx_wave_sum_i64(acc);
+}

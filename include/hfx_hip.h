@@ -1,0 +1,269 @@
+/*
+ * hfx_hip.h -- C ABI of libhfx_hip.so, the MI355X-native SEQUEST search core.
+ *
+ * Drop-in for HFX's GPU search path.  Two groups of entry points:
+ *
+ *  1. The reference's own C ABI (src/common/include/algorithms.h), same names,
+ *     same argument meaning, same return convention, so that
+ *     src/haskell/Foreign/CUDA/Algorithms.hs binds them unchanged.  These run
+ *     on the null stream (except addModIons, which takes a stream like the
+ *     reference) and the count-returning ones synchronise, as the reference's
+ *     Thrust-based versions do.  Errors: the reference has none; here a failure
+ *     returns 0 / does nothing and records a message read by hfx_last_error().
+ *
+ *  2. New hfx_* entry points with an int status (HFX_OK == 0), an explicit
+ *     stream (a hipStream_t passed as void*), and caller-owned buffers; scratch
+ *     comes from an hfx_ctx created once.  They replace the dense
+ *     addIons + cuBLAS sgemv + b40c sort sequence of Search.hs:177-277 with a
+ *     fused sparse XCorr and a top-N select, move fast-XCorr preprocessing
+ *     (Correlation.hs:37-109) onto the GPU, and replace the CUDA binding
+ *     package (packages/cuda-0.4.1.0) with thin memory/stream helpers.
+ *
+ * All pointers named d_* are device pointers; h_* are host pointers.
+ */
+#ifndef HFX_HIP_H
+#define HFX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                          */
+/* ------------------------------------------------------------------ */
+#define HFX_OK 0
+#define HFX_ERR_ARG 1     /* invalid argument / size beyond a stated limit */
+#define HFX_ERR_HIP 2     /* a HIP runtime call failed */
+#define HFX_ERR_NOMEM 3   /* device allocation failed */
+#define HFX_ERR_DATA 4    /* input data the reference cannot process */
+
+typedef struct hfx_ctx hfx_ctx;
+
+/* Device-resident peptide database (Fragment.hs:178-188 DeviceSeqDB plus the
+ * sorted residual copy that sort_idx_f leaves behind, Fragment.hs:250-257). */
+typedef struct hfx_db {
+    const uint8_t *d_ions;        /* packed residues, prepare_ions layout        */
+    const float *d_r;             /* residual mass per peptide (no water)        */
+    const uint32_t *d_tc;         /* first residue index                         */
+    const uint32_t *d_tn;         /* ladder end (Fragment.hs:386-390)            */
+    const float *d_mass;          /* [26] residue mass table incl. static mods   */
+    const uint32_t *d_idx_sorted; /* peptide indices sorted by residual          */
+    const float *d_r_sorted;      /* residuals in that order                     */
+    uint32_t num_pep;
+    uint64_t num_ions;
+} hfx_db;
+
+/* Preprocessed spectra resident on the device. */
+typedef struct hfx_spectra {
+    const float *d_x;        /* [num_spec * x_stride] fast-XCorr arrays           */
+    uint32_t x_stride;       /* floats between consecutive spectra (>= max len)   */
+    const uint32_t *d_len;   /* [num_spec] array length (Correlation.hs:62-67)    */
+    const uint32_t *d_zmax;  /* [num_spec] max fragment charge max(1, z-1)        */
+    const float *d_mass;     /* [num_spec] window centre ms2Mass (Data.hs:58-59)  */
+    uint32_t num_spec;
+} hfx_spectra;
+
+/* ------------------------------------------------------------------ */
+/* Runtime: replaces packages/cuda-0.4.1.0 (Runtime/Marshal.chs:126-376, */
+/* Runtime/Stream.chs:54-89, Runtime/Device.chs:267-273,                 */
+/* Driver/Marshal.chs:548) and the cuBLAS handle of Execution.hs:49-53.  */
+/* ------------------------------------------------------------------ */
+const char *hfx_last_error(void);
+int hfx_ctx_create(int device, hfx_ctx **out);
+int hfx_ctx_destroy(hfx_ctx *ctx);
+int hfx_malloc(void **d_ptr, size_t bytes);
+int hfx_free(void *d_ptr);
+int hfx_host_alloc(void **h_ptr, size_t bytes); /* pinned, Marshal.chs mallocHostArray */
+int hfx_host_free(void *h_ptr);
+int hfx_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int hfx_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes, void *stream);
+int hfx_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int hfx_memset(void *d_dst, int value, size_t bytes, void *stream);
+int hfx_stream_create(void **stream);
+int hfx_stream_destroy(void *stream);
+int hfx_stream_sync(void *stream);
+int hfx_device_sync(void);
+int hfx_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ------------------------------------------------------------------ */
+/* Reference ABI (algorithms.h), same names and meaning.                 */
+/* ------------------------------------------------------------------ */
+
+/* algorithms.h:48, sort.cu:28-35: d_idx = init..init+N-1 stably sorted by the
+ * keys (ascending); the keys are sorted in place. */
+void sort_idx_f(float *d_keys_raw, uint32_t *d_idx_raw, uint32_t N, uint32_t init);
+
+/* algorithms.h:42, sort.cu:20-26: stable ascending key-value sort in place. */
+void sort_val_f(float *d_keys, uint32_t *d_vals, uint32_t N);
+
+/* algorithms.h:54, sort_b40c.cu:34-41: stable ascending key-value sort in
+ * place (b40c radix sort; the alternate-buffer selector quirk is not kept). */
+void sort_b40c_f(float *d_keys, uint32_t *d_vals, uint32_t N);
+
+/* algorithms.h:80-87, prepare_ions.cu:105-132: pack each residue byte as
+ * (c - 'A') | mod << 5 (all N bytes, also N < 512). */
+void prepare_ions(uint8_t *d_ions, uint32_t N, uint8_t *d_ma, uint32_t num_ma);
+
+/* algorithms.h:99-113, filter.cu:75-127: per spectrum the residual-sorted
+ * window [lower_bound(m-eps), upper_bound(m+eps)); returns the total. */
+uint32_t findSpecCandsByMass(uint32_t *d_spec_begin_raw, uint32_t *d_spec_end_raw,
+                             uint32_t *d_spec_num_pep_raw, const float *d_r_raw,
+                             const uint32_t *d_pep_idx_r_sorted_raw, const uint32_t num_pep,
+                             const float *d_spec_masses_raw, const uint32_t num_spec,
+                             const float eps);
+
+/* algorithms.h:115-131, filter_by_mod.cu:35-143: per modification the range
+ * [(m-eps)-delta, (m+eps)-delta], exclusive scan of counts; returns total. */
+uint32_t findBeginEnd_f(uint32_t *d_begin_raw, uint32_t *d_end_raw, uint32_t *d_num_pep_scan_raw,
+                        const float *d_r_raw, const uint32_t *d_pep_idx_r_sorted_raw,
+                        const uint32_t num_pep, const float *d_mod_delta_raw,
+                        const uint32_t num_mod, const float mass_, const float eps);
+
+/* algorithms.h:133-159, filter_by_mod.cu:265-378. */
+uint32_t findModablePeptides(uint32_t *d_out_pep_idx_raw, uint32_t *d_out_pep_mod_idx_raw,
+                             uint32_t *d_out_pep_ma_count_raw, uint32_t *d_out_spec_num_valid_scan,
+                             const uint32_t *d_spec_num_cand_scan, const uint32_t num_spec,
+                             uint32_t num_cand_total, const uint8_t *d_ions,
+                             const uint32_t *d_tc, const uint32_t *d_tn,
+                             const uint32_t *d_pep_idx_r_sorted_raw, const uint32_t *d_begin_raw,
+                             const uint32_t *d_end_raw, const uint32_t *d_num_pep_scan_raw,
+                             const uint32_t *d_mod_ma_count, const uint32_t num_mod,
+                             const uint8_t *d_ma, const uint32_t num_ma);
+
+/* algorithms.h:161-173, calc_mod_cands.cu:76-119. */
+uint32_t calcTotalModCands(uint32_t *d_out_pep_num_mpep_raw, uint32_t *d_out_pep_ma_num_comb_scan_raw,
+                           const uint32_t *d_mod_ma_count_raw, const uint32_t *d_pep_idx_raw,
+                           const uint32_t *d_pep_mod_idx_raw, const uint32_t *d_pep_ma_count_raw,
+                           const uint32_t nPep, const uint32_t num_mod, const uint32_t num_ma);
+
+/* algorithms.h:175-192, prepare_gen_mod.cu:79-129. */
+uint32_t prepareGenMod(uint32_t *d_out_mpep_pep_idx, uint32_t *d_out_mpep_pep_mod_idx,
+                       uint32_t *d_out_mpep_rank, uint32_t *d_out_mpep_ith_valid,
+                       uint32_t *d_out_mpep_mod_ma_count_sum_scan,
+                       const uint32_t *d_mod_ma_count_sum, const uint32_t *d_pep_idx,
+                       const uint32_t *d_pep_mod_idx, const uint32_t *d_pep_num_mpep,
+                       const uint32_t num_pep, const uint32_t num_mpep);
+
+/* algorithms.h:194-212, gen_mod_pep.cu:229-312. */
+void genModCands(uint32_t *d_out_mpep_unrank, const uint32_t *d_mod_ma_count,
+                 const uint32_t *d_mpep_ith_valid, const uint32_t *d_mpep_rank,
+                 const uint32_t *d_mpep_mod_ma_count_sum_scan, const uint32_t *d_pep_mod_idx,
+                 const uint32_t *d_pep_ma_count, const uint32_t *d_pep_ma_num_comb_scan,
+                 const uint32_t total, const uint32_t mod_num_ma);
+
+/* algorithms.h:214-241, mion_series.cu:443-556: dense theoretical rows of
+ * modified peptides (max per bin, written as floats 0/10/25/50). */
+void addModIons(float *d_out_mspec, const float *d_residual, const float *d_mass,
+                const uint8_t *d_ions, const uint32_t *d_tc, const uint32_t *d_tn,
+                const uint32_t *d_mpep_pep_idx, const uint32_t *d_mpep_pep_mod_idx,
+                const uint32_t *d_mpep_unrank, const uint32_t *d_mpep_mod_ma_count_sum_scan,
+                const uint32_t len_unrank, const uint32_t num_mpep,
+                const uint32_t *d_mod_ma_count, const float *d_mod_delta, const uint8_t *d_ma,
+                const float *d_ma_mass, const uint32_t num_ma, const uint32_t len_spec,
+                const uint32_t max_charge, void *stream);
+
+/* algorithms.h:247-260, ion_series.cu:319-343: dense theoretical rows. */
+void addIons(float *d_spec, const float *d_residual, const float *d_mass, const uint8_t *d_ions,
+             const uint32_t *d_tc, const uint32_t *d_tn, const uint32_t *d_idx,
+             const uint32_t num_idx, const uint32_t max_charge, const uint32_t len_spec);
+
+/* algorithms.h:279-287, prepare_scoring.cu:47-67. */
+void prepare_scoring(uint32_t *d_out_spec_pep_idx, const uint32_t *d_pep_idx_r_sorted_raw,
+                     const uint32_t *d_spec_begin, const uint32_t *d_spec_num_pep,
+                     const uint32_t num_spec);
+
+/* ------------------------------------------------------------------ */
+/* Host helpers: the Haskell host functions whose outputs the kernels    */
+/* consume, for callers without the Haskell driver.                      */
+/* ------------------------------------------------------------------ */
+
+/* Config.hs:266-272 initializeAAMasses: out26[c] = static_add26[c] + mass(c). */
+int hfx_mass_table(const float *static_add26, int mono, float *out26);
+
+/* Config.hs:415-431 getDigestionRule: mask26[c] = 1 for cleavage residues. */
+int hfx_digestion_rule(int rule, uint8_t *mask26);
+
+/* Fragment.hs:363-405 digest (fragment + splice + mass filter) of the
+ * proteins seq[seg[p] .. seg[p+1]); p_rule / min_len / max_len are harness
+ * extensions (0 = off).  Writes (r, tc, tn) per peptide. */
+int hfx_digest(const uint8_t *h_seq, const uint32_t *h_seg, uint32_t nprot, const float *h_mass26,
+               const uint8_t *h_rule26, uint32_t missed, float min_mass, float max_mass, int p_rule,
+               uint32_t min_len, uint32_t max_len, float *h_out_r, uint32_t *h_out_tc,
+               uint32_t *h_out_tn, uint64_t cap, uint64_t *out_n);
+
+/* Fragment.hs:260-289 makeModInfo (modable acids, modification vectors,
+ * their count sums and mass deltas). */
+int hfx_mod_info(const float *h_var26, uint32_t max_ma, uint8_t *h_out_ma, float *h_out_ma_mass,
+                 uint32_t *out_num_ma, uint32_t *h_out_mod_ma_count, uint32_t *h_out_mod_sum,
+                 float *h_out_mod_delta, uint32_t cap_mod, uint32_t *out_num_mod);
+
+/* Spectrum/Data.hs:58-59 ms2Mass (window centre) and DTA.hs:51 precursor. */
+float hfx_ms2_mass(float pcr, float charge);
+float hfx_dta_precursor(float mh, float charge);
+
+/* ------------------------------------------------------------------ */
+/* New entry points                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Replaces CUBLAS.sgemv(T) of Search.hs:242-245 / :527-529
+ * (packages/cublas/.../Level2.chs:65-77): d_y[i] = sum_j d_A[i*lda + j] * d_x[j]
+ * for i < m, j < n, row-major A (the dense rows addIons writes). */
+int hfx_mvm_rows(hfx_ctx *ctx, float *d_y, const float *d_A, uint32_t lda, const float *d_x,
+                 uint32_t m, uint32_t n, void *stream);
+
+/* Fast-XCorr preprocessing on the GPU, Correlation.hs:37-109, for num_spec
+ * spectra whose peaks (sorted by m/z) are d_mz/d_int[d_peak_off[s] ..
+ * d_peak_off[s+1]).  Writes d_x[s*x_stride .. +len) (zero padded to
+ * x_stride), d_len[s] and d_status[s] (0 ok, 2 = the reference's sel == 0
+ * division by zero, 3 = len > x_stride, 1 = no peaks).  mean_mode 0 sums the
+ * 151-bin window in FP32 like the reference; 1 sums it in FP64. */
+int hfx_preprocess(hfx_ctx *ctx, float *d_x, uint32_t x_stride, uint32_t *d_len,
+                   uint32_t *d_status, const float *d_mz, const float *d_int,
+                   const uint32_t *d_peak_off, const float *d_pcr, const float *d_charge,
+                   uint32_t num_spec, int remove_precursor, int mono, int mean_mode,
+                   void *stream);
+
+/* Window filter for all spectra of a batch in one launch (findSpecCandsByMass
+ * without the synchronous return): d_begin/d_num per spectrum. */
+int hfx_window(hfx_ctx *ctx, uint32_t *d_begin, uint32_t *d_num, const hfx_db *db,
+               const float *d_spec_mass, uint32_t num_spec, float eps, void *stream);
+
+/* Exclusive scan of n uint32 counts into n+1 uint64 offsets. */
+int hfx_scan_offsets(hfx_ctx *ctx, uint64_t *d_off, const uint32_t *d_count, uint32_t n,
+                     void *stream);
+
+/* Plain FP32 sparse XCorr (replaces addIons + sgemv, Search.hs:177-245):
+ * for spectrum s and its i-th window candidate (residual-sorted position
+ * d_begin[s] + i), d_scores[d_off[s] + i] = sum over the candidate's
+ * theoretical bins b < len_s of maxweight(b) * x_s[b], FP32.  `total` is
+ * sum_s d_num[s] (findSpecCandsByMass's return value). */
+int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                     const uint32_t *d_begin, const uint32_t *d_num, uint64_t total,
+                     const hfx_db *db, const hfx_spectra *sp, void *stream);
+
+/* Per-spectrum top-N (replaces sort_b40c_f + full D2H, Search.hs:250-276):
+ * selects the top N + margin candidates by FP32 score, re-scores them exactly
+ * (int64 fixed point, 2^-36 resolution, reported as double), and writes the
+ * best N ordered by (exact score desc, candidate position asc):
+ * d_top_pos[s*N + k] (position within the spectrum's window), d_top_exact,
+ * d_top_f32, d_top_cnt[s].  d_flag[s] = 1 when a candidate outside the
+ * re-scored set could still rank in the top N (FP32 error bound); flagged
+ * spectra are completed by hfx_topn_exact. */
+int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+             uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
+             const float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
+             const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp, void *stream);
+
+/* Exact top-N of the listed spectra (all candidates scored in fixed point). */
+int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                   uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_spec_list,
+                   uint32_t num_list, const uint32_t *d_begin, const uint32_t *d_num,
+                   const hfx_db *db, const hfx_spectra *sp, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,270 @@
+"""GPU parity of the plain (unmodified) search path against the CPU oracle.
+
+Integer and index work (packing, sorts, windows, bins, rankings) must be
+bit-exact; FP32 scores must be within 1e-4 of the exact score relative to
+max(|exact|, 100) (100 raw units = 0.01 displayed XCorr); exact re-scores
+and preprocessing (same mean mode) are bit-exact.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STRIDE = 2112
+REL = 1e-4
+FLOOR = 100.0
+
+
+@pytest.fixture(scope="module")
+def env(small_workload, orc):
+    import hfx_amd
+    from hfx_amd import pipeline
+
+    w = small_workload
+    ctx = pipeline.Ctx(0)
+    hdb = pipeline.HostDB(ions=w["prot"].seq, r=w["r"], tc=w["tc"], tn=w["tn"], mass=w["mass"])
+    db = pipeline.DeviceDB(hdb)
+    sp = w["sp"]
+    dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE,
+                                 mean_mode=1)
+    pipeline.sync()
+    return dict(ctx=ctx, db=db, dsp=dsp, w=w, L=hfx_amd.lib(), pl=pipeline)
+
+
+def test_prepare_ions(env, orc):
+    pl, L = env["pl"], env["L"]
+    rng = np.random.default_rng(1)
+    letters = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", np.uint8)
+    for n in (1, 100, 511, 513, 100003):
+        seq = letters[rng.integers(0, 20, n)]
+        for ma in (b"", b"MSTY", b"Q"):
+            t = pl.dev(seq)
+            tma = pl.dev(np.frombuffer(ma, np.uint8)) if ma else None
+            L.prepare_ions(pl.ptr(t), n, pl.ptr(tma) if tma is not None else None, len(ma))
+            pl.sync()
+            assert (pl.host(t, np.uint8, n) == orc.prepare_ions(seq, ma)).all(), (n, ma)
+
+
+def test_db_packing_and_sort(env, orc):
+    db, w = env["db"], env["w"]
+    assert (db.packed_ions() == orc.prepare_ions(w["prot"].seq)).all()
+    idx, rs = orc.sort_idx(w["r"])
+    assert (db.idx_sorted() == idx).all()
+    assert (env["pl"].host(db.t_rsorted, np.float32, db.num_pep) == rs).all()
+
+
+def test_sort_b40c_f_stable(env):
+    pl, L = env["pl"], env["L"]
+    rng = np.random.default_rng(2)
+    for n in (1, 7, 2048, 2049, 300001):
+        keys = rng.integers(-50, 50, n).astype(np.float32) / np.float32(4)
+        keys[::11] = -0.0
+        vals = np.arange(n, dtype=np.uint32)
+        tk, tv = pl.dev(keys), pl.dev(vals)
+        L.sort_b40c_f(pl.ptr(tk), pl.ptr(tv), n)
+        pl.sync()
+        # radix order treats -0.0 < +0.0; ties keep input order
+        key2 = keys.view(np.uint32).astype(np.int64)
+        order_key = np.where(key2 & 0x80000000, -(key2 & 0x7FFFFFFF) - 1, key2)
+        o = np.argsort(order_key, kind="stable")
+        assert (pl.host(tv, np.uint32, n) == vals[o]).all()
+        assert (pl.host(tk, np.float32, n).view(np.uint32) == keys[o].view(np.uint32)).all()
+
+
+def test_windows(env, orc):
+    pl, L, db, dsp, w = env["pl"], env["L"], env["db"], env["dsp"], env["w"]
+    S = dsp.num_spec
+    idx, _ = orc.sort_idx(w["r"])
+    rng = np.random.default_rng(3)
+    masses = np.concatenate([dsp.mass, rng.uniform(300, 8000, 200).astype(np.float32)])
+    for eps in (3.0, 0.01, 50.0):
+        b, e, n, tot = orc.find_spec_cands_by_mass(w["r"], idx, masses, eps)
+        tb, te, tnn = pl.empty(4 * len(masses)), pl.empty(4 * len(masses)), pl.empty(4 * len(masses))
+        tm = pl.dev(masses)
+        got = L.findSpecCandsByMass(pl.ptr(tb), pl.ptr(te), pl.ptr(tnn), pl.ptr(db.t_r),
+                                    pl.ptr(db.t_idx), db.num_pep, pl.ptr(tm), len(masses), eps)
+        assert got == tot
+        assert (pl.host(tb, np.uint32, len(masses)) == b).all()
+        assert (pl.host(te, np.uint32, len(masses)) == e).all()
+        assert (pl.host(tnn, np.uint32, len(masses)) == n).all()
+        # batched form over the sorted copy
+        env["L"].hfx_window(env["ctx"].h, pl.ptr(tb), pl.ptr(tnn), C.byref(db.desc), pl.ptr(tm),
+                            len(masses), eps, pl.stream_handle())
+        pl.sync()
+        assert (pl.host(tb, np.uint32, len(masses)) == b).all()
+        assert (pl.host(tnn, np.uint32, len(masses)) == n).all()
+        # prepare_scoring, prepare_scoring.cu:47-67
+        out = pl.empty(4 * max(tot, 1))
+        tb2, tn2 = pl.dev(b), pl.dev(n)
+        L.prepare_scoring(pl.ptr(out), pl.ptr(db.t_idx), pl.ptr(tb2), pl.ptr(tn2), len(masses))
+        pl.sync()
+        assert (pl.host(out, np.uint32, tot) == orc.prepare_scoring(idx, b, n)).all()
+
+
+def test_find_begin_end(env, orc):
+    pl, L, db, w = env["pl"], env["L"], env["db"], env["w"]
+    idx, _ = orc.sort_idx(w["r"])
+    delta = np.array([15.9949, 79.9663, 95.9612, -17.0, 0.0, 159.9326], np.float32)
+    td = pl.dev(delta)
+    for mass in (np.float32(1234.5), np.float32(2500.25), np.float32(100.0)):
+        b, e, sc, tot = orc.find_begin_end(w["r"], idx, delta, mass, 3.0)
+        tb, te, ts = pl.empty(4 * 6), pl.empty(4 * 6), pl.empty(4 * 6)
+        got = L.findBeginEnd_f(pl.ptr(tb), pl.ptr(te), pl.ptr(ts), pl.ptr(db.t_r),
+                               pl.ptr(db.t_idx), db.num_pep, pl.ptr(td), 6, mass, 3.0)
+        assert got == tot
+        assert (pl.host(tb, np.uint32, 6) == b).all() and (pl.host(te, np.uint32, 6) == e).all()
+        assert (pl.host(ts, np.uint32, 6) == sc).all()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_preprocess_bit_exact(env, orc, mode):
+    pl, w = env["pl"], env["w"]
+    sp = w["sp"]
+    dsp = pl.DeviceSpectra(env["ctx"], sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE,
+                           mean_mode=mode)
+    X, lens, st = orc.preprocess_batch(sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE, mode)
+    assert (st == 0).all()
+    assert (dsp.status() == 0).all()
+    assert (dsp.lengths() == lens).all()
+    assert (dsp.x().view(np.uint32) == X.view(np.uint32)).all()
+
+
+def test_preprocess_edge_cases(env, orc):
+    pl = env["pl"]
+    # no peaks; sel == 0; precursor removal off; a peak exactly at the cutoff
+    pcr = np.array([500.0, 500.0, 700.0, 650.0], np.float32)
+    ch = np.array([2, 2, 3, 2], np.float32)
+    peaks = [[], [(5.0, 1.0), (9.0, 2.0)], [(100.0, 1.0), (690.0, 3.0), (710.0, 2.0), (1500.0, 1.0)],
+             [(200.0, 1.0), (1350.0, 2.0), (1360.0, 0.5)]]
+    mz = np.array([p[0] for pk in peaks for p in pk], np.float32)
+    it = np.array([p[1] for pk in peaks for p in pk], np.float32)
+    off = np.cumsum([0] + [len(p) for p in peaks]).astype(np.uint32)
+    for remove in (True, False):
+        dsp = pl.DeviceSpectra(env["ctx"], pcr, ch, mz, it, off, STRIDE, remove_precursor=remove)
+        stat = dsp.status()
+        assert stat[0] == 1 and stat[1] == 2
+        for s in (2, 3):
+            rc, x = orc.preprocess(pcr[s], ch[s], mz[off[s]:off[s + 1]], it[off[s]:off[s + 1]],
+                                   remove, True, 1, STRIDE)
+            assert rc == 0 and stat[s] == 0
+            got = dsp.x()[s]
+            assert dsp.lengths()[s] == len(x)
+            assert (got[: len(x)].view(np.uint32) == x.view(np.uint32)).all()
+
+
+def test_add_ions_dense_rows(env, orc):
+    pl, L, db, w = env["pl"], env["L"], env["db"], env["w"]
+    rng = np.random.default_rng(4)
+    ions = db.packed_ions()
+    sel = rng.integers(0, db.num_pep, 300).astype(np.uint32)
+    tsel = pl.dev(sel)
+    for zmax in (1, 2, 3, 4):
+        for length in (2075, 1000, 333):
+            out = pl.empty(4 * len(sel) * length)
+            L.addIons(pl.ptr(out), pl.ptr(db.t_r), pl.ptr(db.t_mass), pl.ptr(db.t_ions),
+                      pl.ptr(db.t_tc), pl.ptr(db.t_tn), pl.ptr(tsel), len(sel), zmax, length)
+            pl.sync()
+            got = pl.host(out, np.float32).reshape(len(sel), length)
+            for i, p in enumerate(sel):
+                ref = orc.peaks_dense(ions, int(w["tc"][p]), int(w["tn"][p]), w["r"][p], w["mass"],
+                                      zmax, length)
+                assert (got[i] == ref.astype(np.float32)).all(), (zmax, length, i)
+
+
+def test_mvm_rows(env):
+    pl, L = env["pl"], env["L"]
+    rng = np.random.default_rng(5)
+    m, n = 777, 2075
+    A = rng.choice(np.array([0, 10, 25, 50], np.float32), (m, n))
+    x = rng.normal(0, 3, n).astype(np.float32)
+    ta, tx, ty = pl.dev(A), pl.dev(x), pl.empty(4 * m)
+    hfx = __import__("hfx_amd")
+    hfx.check(L.hfx_mvm_rows(env["ctx"].h, pl.ptr(ty), pl.ptr(ta), n, pl.ptr(tx), m, n,
+                             pl.stream_handle()), "mvm")
+    pl.sync()
+    ref = A.astype(np.float64) @ x.astype(np.float64)
+    got = pl.host(ty, np.float32, m)
+    assert np.all(np.abs(got - ref) <= REL * np.maximum(np.abs(ref), FLOOR))
+
+
+def _oracle_plain(orc, env, topn, want_all=True, spectra=None):
+    w, db, dsp = env["w"], env["db"], env["dsp"]
+    idx, _ = orc.sort_idx(w["r"])
+    X = dsp.x()
+    lens = dsp.lengths()
+    return orc.search_plain(db.packed_ions(), w["r"], w["tc"], w["tn"], w["mass"], idx, X, lens,
+                            dsp.zmax, dsp.mass, 3.0, topn, want_all)
+
+
+def test_sparse_scores_and_topn(env, orc):
+    pl = env["pl"]
+    ref = _oracle_plain(orc, env, 5)
+    srch = pl.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=5, margin=16)
+    res = srch.run(keep_scores=True)
+    assert (res.begin == ref["begin"]).all() and (res.num == ref["num"]).all()
+    assert int(res.off[-1]) == len(res.scores) == int(ref["off"][-1])
+    exact = orc.fx_to_double(ref["all_fx"][: len(res.scores)])
+    err = np.abs(res.scores.astype(np.float64) - exact)
+    assert np.all(err <= REL * np.maximum(np.abs(exact), FLOOR)), err.max()
+    # rankings and exact scores identical to the oracle
+    assert (res.top_cnt == ref["top_cnt"]).all()
+    for s in range(len(res.top_cnt)):
+        c = int(res.top_cnt[s])
+        assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), s
+        assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
+    # the spectrum's source peptide is found at rank 1 for most spectra
+    w = env["w"]
+    idx = env["db"].idx_sorted()
+    src = w["sp"].source
+    hits = sum(int(idx[res.begin[s] + res.top_pos[s, 0]]) == int(src[s]) or
+               (w["r"][idx[res.begin[s] + res.top_pos[s, 0]]] == w["r"][src[s]])
+               for s in range(len(src)) if res.top_cnt[s])
+    assert hits >= 0.9 * len(src)
+
+
+def test_exact_fallback_matches(env, orc):
+    pl, L = env["pl"], env["L"]
+    import hfx_amd
+
+    ref = _oracle_plain(orc, env, 5, want_all=False)
+    srch = pl.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=5)
+    srch.window()
+    S = env["dsp"].num_spec
+    lst = pl.dev(np.arange(S, dtype=np.uint32))
+    hfx_amd.check(L.hfx_topn_exact(env["ctx"].h, pl.ptr(srch.t_pos), pl.ptr(srch.t_exact),
+                                   pl.ptr(srch.t_f32), pl.ptr(srch.t_cnt), 5, pl.ptr(lst), S,
+                                   pl.ptr(srch.t_begin), pl.ptr(srch.t_num),
+                                   C.byref(env["db"].desc), C.byref(env["dsp"].desc),
+                                   pl.stream_handle()), "topn_exact")
+    pl.sync()
+    pos = pl.host(srch.t_pos, np.uint32, S * 5).reshape(S, 5)
+    ex = pl.host(srch.t_exact, np.float64, S * 5).reshape(S, 5)
+    cnt = pl.host(srch.t_cnt, np.uint32, S)
+    assert (cnt == ref["top_cnt"]).all()
+    for s in range(S):
+        c = int(cnt[s])
+        assert pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist()
+        assert (ex[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
+
+
+def test_empty_windows_and_high_charge(env, orc):
+    pl, w = env["pl"], env["w"]
+    sp = w["sp"]
+    # precursors far outside the database mass range give empty windows;
+    # charge 7 (max fragment charge 6 > 4) scores 0 like the reference's no-launch
+    pcr = np.array([90.0, sp.pcr[0], sp.pcr[1]], np.float32)
+    ch = np.array([2.0, 7.0, 2.0], np.float32)
+    s0, s1 = int(sp.off[0]), int(sp.off[2])
+    mz = np.concatenate([sp.mz[s0:s1], sp.mz[s0:int(sp.off[1])]])
+    it = np.concatenate([sp.inten[s0:s1], sp.inten[s0:int(sp.off[1])]])
+    n0, n1 = int(sp.off[1] - sp.off[0]), int(sp.off[2] - sp.off[1])
+    off = np.array([0, n0, n0 + n1, n0 + n1 + n0], np.uint32)
+    ch = ch * 1
+    dsp = pl.DeviceSpectra(env["ctx"], pcr, ch, mz, it, off, STRIDE)
+    srch = pl.PlainSearch(env["ctx"], env["db"], dsp, eps=3.0, topn=5)
+    res = srch.run(keep_scores=True)
+    assert res.num[0] == 0 and res.top_cnt[0] == 0
+    a, b = int(res.off[1]), int(res.off[2])
+    assert b > a and np.all(res.scores[a:b] == 0)

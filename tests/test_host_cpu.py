@@ -1,0 +1,97 @@
+"""CPU tests of the C ABI library: it loads, exports every declared symbol,
+and its host helpers (digestion, mass table, modification table, precursor
+mass) agree bit for bit with the oracle.  No kernel is launched here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hfx_amd
+from hfx_amd import pipeline, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "hfx_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
+    skip = {"if", "for", "while", "sizeof", "defined", "return"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def test_library_exports_every_declared_symbol():
+    L = hfx_amd.lib()
+    decl = declared_functions()
+    assert len(decl) > 30
+    missing = [n for n in decl if not hasattr(L, n) and n not in hfx_amd._OPTIONAL]
+    assert not missing, missing
+    # the ctypes table covers the header too
+    assert set(decl) <= set(hfx_amd.exported_symbols())
+
+
+def test_mass_table_matches_oracle(orc):
+    for static, mono in [({}, True), ({"C": 57.02146}, True), ({"C": 57.0, "M": 1.5}, False)]:
+        assert (pipeline.mass_table(static, mono) == orc.mass_table(static, mono)).all()
+
+
+@pytest.mark.parametrize("rule,missed,p_rule,lmin,lmax", [
+    (1, 0, 0, 0, 0), (1, 2, 0, 0, 0), (1, 1, 1, 7, 30), (2, 1, 0, 0, 0), (13, 2, 1, 5, 0),
+    (0, 0, 0, 0, 0), (6, 3, 0, 0, 25),
+])
+def test_digest_matches_oracle(orc, rule, missed, p_rule, lmin, lmax):
+    prot = synth.make_proteins(300, seed=rule * 10 + missed)
+    mass = orc.mass_table({"C": 57.02146})
+    lo, hi = (400.0, 7200.0) if rule else (0.0, 1e9)
+    a = pipeline.digest(prot.seq, prot.seg, mass, rule, missed, lo, hi, p_rule, lmin, lmax)
+    b = orc.digest(prot.seq, prot.seg, mass, rule, missed, lo, hi, p_rule, lmin, lmax)
+    assert len(a[0]) == len(b[0]) and len(a[0]) > 0
+    for x, y in zip(a, b):
+        assert (x == y).all()
+
+
+def test_mod_info_matches_oracle(orc):
+    L = hfx_amd.lib()
+    for var, mx in [({"Q": -17.0}, 2), ({"M": 15.9949, "S": 79.9663, "T": 79.9663, "Y": 79.9663}, 4),
+                    ({"M": 15.9949, "N": 0.984}, 20), ({}, 3)]:
+        v = np.zeros(26, np.float32)
+        for k, x in var.items():
+            v[ord(k) - 65] = x
+        ma = np.zeros(26, np.uint8)
+        mam = np.zeros(26, np.float32)
+        na = C.c_uint32()
+        nm = C.c_uint32()
+        cap = 1 << 14
+        cnt = np.zeros(cap * 26, np.uint32)
+        sm = np.zeros(cap, np.uint32)
+        dl = np.zeros(cap, np.float32)
+        hfx_amd.check(L.hfx_mod_info(v.ctypes.data, mx, ma.ctypes.data, mam.ctypes.data, C.byref(na),
+                                     cnt.ctypes.data, sm.ctypes.data, dl.ctypes.data, cap,
+                                     C.byref(nm)), "hfx_mod_info")
+        o = orc.mod_info(var, mx)
+        a, m = na.value, nm.value
+        assert bytes(ma[:a]) == o[0]
+        assert (mam[:a] == o[1]).all()
+        assert (cnt[: m * a].reshape(m, a) == o[2]).all()
+        assert (sm[:m] == o[3]).all() and (dl[:m] == o[4]).all()
+
+
+def test_precursor_helpers_match_oracle(orc):
+    L = hfx_amd.lib()
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        p = np.float32(rng.uniform(300, 2000))
+        z = np.float32(rng.integers(1, 5))
+        assert np.float32(L.hfx_ms2_mass(p, z)) == orc.ms2_mass(p, z)
+        assert np.float32(L.hfx_dta_precursor(p, z)) == orc.dta_precursor(p, z)
+
+
+def test_pipeline_refuses_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError):
+        pipeline.empty(16)
