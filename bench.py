@@ -118,8 +118,6 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         raise SystemExit(2)
 
-    import torch
-
     from hfx_amd import pipeline
 
     env = setup(args.workload)
@@ -129,26 +127,26 @@ def main():
     def step(ev=None):
         srch.window()
         if ev is not None:
-            ev[0].record()
+            ev[0].record(pipeline.stream_handle())
         srch.score()
         if ev is not None:
-            ev[1].record()
+            ev[1].record(pipeline.stream_handle())
         srch.select()
         return srch.fallback()
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
+    pipeline.sync()
+    # HIP events on the stream the kernels are launched on
+    evs = [(pipeline.Event(), pipeline.Event()) for _ in range(args.steps)]
     flagged = 0
-    torch.cuda.synchronize()
+    pipeline.sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
         flagged += len(step(evs[k]))
-    torch.cuda.synchronize()
+    pipeline.sync()
     elapsed = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    kern_ms = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
     res = srch.run()  # results of one more pass, for the checks below
     scores_per_step = int(res.off[-1])
     value = scores_per_step * args.steps / elapsed

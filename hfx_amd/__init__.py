@@ -76,6 +76,11 @@ _SIGS = {
     "hfx_stream_sync": (_I, [_P]),
     "hfx_device_sync": (_I, []),
     "hfx_mem_info": (_I, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "hfx_device_count": (_I, [C.POINTER(_I)]),
+    "hfx_event_create": (_I, [C.POINTER(_P)]),
+    "hfx_event_destroy": (_I, [_P]),
+    "hfx_event_record": (_I, [_P, _P]),
+    "hfx_event_elapsed_ms": (_I, [C.POINTER(_F), _P, _P]),
     # reference ABI (algorithms.h)
     "sort_idx_f": (None, [_P, _P, _U32, _U32]),
     "sort_val_f": (None, [_P, _P, _U32]),

@@ -203,3 +203,44 @@ extern "C" int hfx_mem_info(size_t *free_bytes, size_t *total_bytes)
     HFX_CHECK_HIP(hipMemGetInfo(free_bytes, total_bytes));
     return HFX_OK;
 }
+
+extern "C" int hfx_device_count(int *count)
+{
+    HFX_REQUIRE(count != nullptr, HFX_ERR_ARG, "hfx_device_count: null out");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        hfx_set_error("hipGetDeviceCount: %s", hipGetErrorString(e));
+        return HFX_ERR_HIP;
+    }
+    return HFX_OK;
+}
+
+/* Events (Runtime/Event.chs of the CUDA binding): kernel timing on a stream. */
+extern "C" int hfx_event_create(void **ev)
+{
+    hipEvent_t e;
+    HFX_CHECK_HIP(hipEventCreate(&e));
+    *ev = (void *)e;
+    return HFX_OK;
+}
+
+extern "C" int hfx_event_destroy(void *ev)
+{
+    if (ev) HFX_CHECK_HIP(hipEventDestroy((hipEvent_t)ev));
+    return HFX_OK;
+}
+
+extern "C" int hfx_event_record(void *ev, void *stream)
+{
+    HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev, hfx_stream(stream)));
+    return HFX_OK;
+}
+
+extern "C" int hfx_event_elapsed_ms(float *ms, void *start, void *stop)
+{
+    HFX_CHECK_HIP(hipEventSynchronize((hipEvent_t)stop));
+    HFX_CHECK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return HFX_OK;
+}

@@ -6,8 +6,9 @@ DeviceSpectra  the preprocessed spectra of one batch (Correlation.hs on GPU).
 search_plain   Search.hs:129-277 searchWithoutMods for a batch: window ->
                sparse XCorr -> top-N (+ exact re-score, + fallback).
 
-Device memory and streams come from PyTorch (plumbing only); every
-computation is a call into libhfx_hip.so.
+Device memory, streams and events come from the library's own runtime
+helpers (hfx_malloc, hfx_memcpy_*, hfx_event_*), so that one HIP runtime is
+loaded in the process; every computation is a call into libhfx_hip.so.
 """
 from __future__ import annotations
 
@@ -18,43 +19,109 @@ import numpy as np
 
 from . import HfxDb, HfxSpectra, check, lib
 
-
-def _torch():
-    import torch
-
-    if not torch.cuda.is_available():
-        raise RuntimeError("hfx_amd.pipeline needs a GPU (torch.cuda is not available)")
-    return torch
+_gpu_checked = False
 
 
-def dev(arr: np.ndarray):
-    """Upload a numpy array to a device tensor (byte-exact)."""
-    torch = _torch()
+def require_gpu():
+    global _gpu_checked
+    if _gpu_checked:
+        return
+    n = C.c_int(0)
+    rc = lib().hfx_device_count(C.byref(n))
+    if rc != 0 or n.value < 1:
+        raise RuntimeError("hfx_amd.pipeline needs a GPU (no HIP device visible)")
+    _gpu_checked = True
+
+
+class DevBuf:
+    """A device allocation (hfx_malloc) freed with the object."""
+
+    __slots__ = ("p", "nbytes")
+
+    def __init__(self, nbytes: int):
+        require_gpu()
+        self.nbytes = max(int(nbytes), 1)
+        p = C.c_void_p()
+        check(lib().hfx_malloc(C.byref(p), self.nbytes), "hfx_malloc")
+        self.p = p.value
+
+    def __del__(self):
+        try:
+            if self.p:
+                lib().hfx_free(self.p)
+        except Exception:
+            pass
+
+    def upload(self, arr: np.ndarray, stream=None):
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        check(lib().hfx_memcpy_h2d(self.p, a.ctypes.data, a.nbytes, stream), "hfx_memcpy_h2d")
+        check(lib().hfx_stream_sync(stream), "hfx_stream_sync")
+
+    def download(self, dtype, count=None, stream=None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dt)
+        if n:
+            check(lib().hfx_memcpy_d2h(out.ctypes.data, self.p, out.nbytes, stream),
+                  "hfx_memcpy_d2h")
+            check(lib().hfx_stream_sync(stream), "hfx_stream_sync")
+        return out
+
+    def zero(self, stream=None):
+        check(lib().hfx_memset(self.p, 0, self.nbytes, stream), "hfx_memset")
+
+
+def dev(arr: np.ndarray) -> DevBuf:
+    """Upload a numpy array to a new device buffer (byte-exact)."""
     a = np.ascontiguousarray(arr)
-    t = torch.from_numpy(a.view(np.uint8).reshape(-1).copy()).to("cuda")
-    return t
+    b = DevBuf(a.nbytes)
+    if a.nbytes:
+        b.upload(a)
+    return b
 
 
-def empty(nbytes: int):
-    torch = _torch()
-    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device="cuda")
+def empty(nbytes: int) -> DevBuf:
+    return DevBuf(nbytes)
 
 
-def host(t, dtype, count=None) -> np.ndarray:
-    a = t.cpu().numpy().view(dtype)
-    return a if count is None else a[:count]
+def host(t: DevBuf, dtype, count=None) -> np.ndarray:
+    return t.download(dtype, count)
 
 
-def ptr(t) -> int:
-    return t.data_ptr()
+def ptr(t: DevBuf) -> int:
+    return t.p
 
 
-def stream_handle() -> int:
-    return _torch().cuda.current_stream().cuda_stream
+def stream_handle():
+    """The stream every pipeline call is issued on (the null stream)."""
+    return None
 
 
 def sync():
-    _torch().cuda.synchronize()
+    check(lib().hfx_device_sync(), "hfx_device_sync")
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().hfx_event_create(C.byref(p)), "hfx_event_create")
+        self.p = p.value
+
+    def record(self, stream=None):
+        check(lib().hfx_event_record(self.p, stream), "hfx_event_record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().hfx_event_elapsed_ms(C.byref(ms), self.p, end.p), "hfx_event_elapsed_ms")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib().hfx_event_destroy(self.p)
+        except Exception:
+            pass
 
 
 class Ctx:
@@ -132,7 +199,6 @@ class DeviceDB:
                        ptr(self.t_ma) if self.t_ma is not None else None, len(ma))
         L.sort_idx_f(ptr(self.t_rsorted), ptr(self.t_idx), self.num_pep, 0)
         sync()
-        err = L.hfx_last_error()
         self.desc = HfxDb(ptr(self.t_ions), ptr(self.t_r), ptr(self.t_tc), ptr(self.t_tn),
                           ptr(self.t_mass), ptr(self.t_idx), ptr(self.t_rsorted),
                           self.num_pep, self.num_ions)
@@ -229,7 +295,7 @@ class PlainSearch:
               "hfx_scan_offsets")
         off = host(self.t_off, np.uint64, S + 1)
         self.total = int(off[-1])
-        if self.t_scores is None or self.t_scores.numel() < 4 * max(self.total, 1):
+        if self.t_scores is None or self.t_scores.nbytes < 4 * max(self.total, 1):
             self.t_scores = empty(4 * max(self.total, 1))
         return self.total
 
@@ -271,7 +337,7 @@ class PlainSearch:
             flagged = self.fallback()
         else:
             flagged = np.zeros(0, np.uint32)
-            self.t_cnt.zero_()
+            self.t_cnt.zero()
         sync()
         return PlainResult(
             begin=host(self.t_begin, np.uint32, S).copy(),

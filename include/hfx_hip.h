@@ -87,6 +87,12 @@ int hfx_stream_destroy(void *stream);
 int hfx_stream_sync(void *stream);
 int hfx_device_sync(void);
 int hfx_mem_info(size_t *free_bytes, size_t *total_bytes);
+int hfx_device_count(int *count);
+/* Runtime/Event.chs: events for timing work on a stream. */
+int hfx_event_create(void **ev);
+int hfx_event_destroy(void *ev);
+int hfx_event_record(void *ev, void *stream);
+int hfx_event_elapsed_ms(float *ms, void *start, void *stop);
 
 /* ------------------------------------------------------------------ */
 /* Reference ABI (algorithms.h), same names and meaning.                 */
