@@ -106,6 +106,26 @@ def cpu_baseline(env, res, topn, target_s=12.0):
                        f"scoring with exact top-{topn}, {dt:.1f}s", topn_identical_to_gpu=same)
 
 
+def calib_read(mib: int):
+    """Streams mib MiB of floats through k_mvm_rows (reads A once, x from
+    cache, writes 4 bytes per row): a kernel of known HBM traffic whose byte
+    counters scale the counters of the search kernels."""
+    from hfx_amd import check, lib, pipeline
+
+    n = 4096
+    m = mib * 1024 * 1024 // (4 * n)
+    A = pipeline.empty(4 * m * n)
+    A.zero()
+    x = pipeline.dev(np.ones(n, np.float32))
+    y = pipeline.empty(4 * m)
+    ctx = pipeline.Ctx(0)
+    for _ in range(3):
+        check(lib().hfx_mvm_rows(ctx.h, y.p, A.p, n, x.p, m, n, None), "hfx_mvm_rows")
+    pipeline.sync()
+    print(json.dumps({"calib_read_bytes": 4 * m * n, "calib_write_bytes": 4 * m,
+                      "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,10 +133,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tryptic-20k-10k")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",
+                    help="only run a streaming read of MIB MiB (k_mvm_rows over a "
+                         "row matrix) to calibrate HBM byte counters, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         raise SystemExit(2)
+    if args.calib_read:
+        calib_read(args.calib_read)
+        return
 
     from hfx_amd import pipeline
 

@@ -254,7 +254,9 @@ def test_empty_windows_and_high_charge(env, orc):
     sp = w["sp"]
     # precursors far outside the database mass range give empty windows;
     # charge 7 (max fragment charge 6 > 4) scores 0 like the reference's no-launch
-    pcr = np.array([90.0, sp.pcr[0], sp.pcr[1]], np.float32)
+    # the 7+ precursor m/z of spectrum 0's peptide keeps the same window
+    pcr7 = (2.0 * float(sp.pcr[0]) + 5.0 * 1.007276) / 7.0
+    pcr = np.array([90.0, pcr7, sp.pcr[1]], np.float32)
     ch = np.array([2.0, 7.0, 2.0], np.float32)
     s0, s1 = int(sp.off[0]), int(sp.off[2])
     mz = np.concatenate([sp.mz[s0:s1], sp.mz[s0:int(sp.off[1])]])
