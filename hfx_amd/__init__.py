@@ -52,6 +52,22 @@ class HfxSpectra(C.Structure):
     ]
 
 
+class HfxModset(C.Structure):
+    """struct hfx_modset (include/hfx_hip.h)."""
+
+    _fields_ = [
+        ("d_mpep_pep_idx", C.c_void_p),
+        ("d_mpep_mod_idx", C.c_void_p),
+        ("d_mpep_unrank_off", C.c_void_p),
+        ("d_unrank", C.c_void_p),
+        ("d_mod_ma_count", C.c_void_p),
+        ("d_mod_delta", C.c_void_p),
+        ("d_ma_mass", C.c_void_p),
+        ("num_ma", C.c_uint32),
+        ("num_mpep", C.c_uint64),
+    ]
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _U64 = C.c_uint64
@@ -133,16 +149,25 @@ _SIGS = {
         _I,
         [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_mpep_offsets": (_I, [_P, _P, _P, _U32, _P, _U32, _P]),
+    "hfx_xcorr_sparse_mod": (
+        _I,
+        [_P, _P, _P, _U64, C.POINTER(HfxModset), C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
+    ),
+    "hfx_topn_mod": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, _U32, _U32, _P, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
+         C.POINTER(HfxSpectra), _P],
+    ),
+    "hfx_topn_exact_mod": (
+        _I,
+        [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
+         C.POINTER(HfxSpectra), _P],
+    ),
 }
 
-# symbols present when the corresponding component is built
-_OPTIONAL = {
-    "findModablePeptides",
-    "calcTotalModCands",
-    "prepareGenMod",
-    "genModCands",
-    "addModIons",
-}
+# symbols that may be absent from a partial build (none at present)
+_OPTIONAL: set[str] = set()
 
 
 def lib() -> C.CDLL:

@@ -1,0 +1,82 @@
+/*
+ * hfx_cand.h -- candidate sets scored by the sparse XCorr and top-N kernels.
+ *
+ * A candidate set maps (spectrum s, k-th candidate of s) to the residues,
+ * residual and residue masses of one theoretical spectrum:
+ *   HfxPlainCands  unmodified peptides of the precursor window, in residual-
+ *                  sorted order (Search.hs:151-165, prepare_scoring.cu:47-67);
+ *   HfxModCands    modified peptides enumerated for the spectrum
+ *                  (Search.hs:358-394: rows of the spectrum expanded to their
+ *                  modified peptides, in enumeration order).
+ */
+#pragma once
+
+#include "hfx_common.h"
+
+/* Stage spectrum s of sp in LDS (xs) and the mass table (mtab); returns the
+ * effective length, never above HFX_MAX_LEN or the stride. */
+__device__ __forceinline__ uint32_t hfx_stage_spectrum(float *xs, float *mtab, const hfx_db &db,
+                                                       const hfx_spectra &sp, uint32_t s)
+{
+    const uint32_t len = min(sp.d_len[s], min(sp.x_stride, (uint32_t)HFX_MAX_LEN));
+    const float *xg = sp.d_x + (size_t)s * sp.x_stride;
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) xs[i] = xg[i];
+    if (threadIdx.x < 32) mtab[threadIdx.x] = threadIdx.x < 26 ? db.d_mass[threadIdx.x] : 0.0f;
+    return len;
+}
+
+/* Max fragment charge actually generated: the reference launches nothing for
+ * max_charge > 4 (ion_series.cu:334-342), leaving zero rows. */
+__device__ __forceinline__ uint32_t hfx_zeff(uint32_t zmax)
+{
+    return zmax <= HFX_MAX_CHARGE ? zmax : 0u;
+}
+
+struct HfxPlainCands {
+    hfx_db db;
+    const uint32_t *begin; /* [S] first residual-sorted position of the window */
+    const uint32_t *num;   /* [S] window size */
+    const uint64_t *off;   /* [S+1] score offsets */
+
+    __device__ __forceinline__ uint32_t count(uint32_t s) const { return num[s]; }
+    __device__ __forceinline__ uint64_t base(uint32_t s) const { return off[s]; }
+    __device__ __forceinline__ uint32_t zeff(uint32_t zmax) const { return hfx_zeff(zmax); }
+
+    /* peaks of candidate k of spectrum s into the wave's map */
+    __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
+                                          uint32_t len, uint32_t zeff) const
+    {
+        const uint32_t pep = db.d_idx_sorted[begin[s] + k];
+        const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
+        const uint32_t n = tn > tc ? tn - tc : 0;
+        hfx_wave_peaks(map, n, db.d_r[pep], zeff, len, HfxPlainMass{db.d_ions + tc, mtab});
+    }
+};
+
+struct HfxModCands {
+    hfx_db db;
+    hfx_modset ms;
+    const uint64_t *off; /* [S+1] first modified peptide of each spectrum */
+
+    __device__ __forceinline__ uint32_t count(uint32_t s) const
+    {
+        return (uint32_t)(off[s + 1] - off[s]);
+    }
+    __device__ __forceinline__ uint64_t base(uint32_t s) const { return off[s]; }
+    /* addModIons dispatches max charges 1..10 (mion_series.cu:335-556) */
+    __device__ __forceinline__ uint32_t zeff(uint32_t zmax) const { return zmax <= 10 ? zmax : 0u; }
+
+    __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
+                                          uint32_t len, uint32_t zeff) const
+    {
+        const uint64_t i = off[s] + k;
+        const uint32_t pep = ms.d_mpep_pep_idx[i], mod = ms.d_mpep_mod_idx[i];
+        const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
+        const uint32_t n = tn > tc ? tn - tc : 0;
+        const uint32_t na = min(ms.num_ma, (uint32_t)HFX_MAX_MA);
+        HfxModMass mm{db.d_ions + tc, mtab, ms.d_mod_ma_count + (size_t)mod * ms.num_ma,
+                      ms.d_unrank + ms.d_mpep_unrank_off[i], ms.d_ma_mass, na, {0, 0, 0, 0, 0, 0, 0}};
+        const float r = db.d_r[pep] + ms.d_mod_delta[mod];
+        hfx_wave_peaks(map, n, r, zeff, len, mm);
+    }
+};

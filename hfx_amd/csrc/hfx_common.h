@@ -89,6 +89,10 @@ int hfx_ctx_scratch(hfx_ctx *ctx, int slot, size_t bytes, void **out);
 int hfx_ctx_pinned(hfx_ctx *ctx, size_t bytes, void **out);
 int hfx_ctx_num_cus(hfx_ctx *ctx);
 hfx_ctx *hfx_default_ctx(void);
+/* sum of n uint32 (wrapping) read back synchronously (null stream) */
+uint32_t hfx_sum_u32_to_host(hfx_ctx *ctx, const uint32_t *d_a, uint32_t n);
+/* exclusive scan in place, one workgroup */
+int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device helpers                                                        */
@@ -162,23 +166,24 @@ __device__ __forceinline__ uint32_t hfx_lane() { return __lane_id(); }
 /*
  * Theoretical peaks of one candidate into a per-wave LDS nibble map.
  *
- * masses for positions [0, n) are produced by `mass_of(j)` (called by lane j
- * for j within the current 64-residue chunk); the b ladder is the sequential
- * FP32 prefix, reproduced exactly by broadcasting each residue mass to every
- * lane and letting lane k add masses 0..k in order.  For each ladder position
- * and each charge 1..zmax the peaks of addIonsAB/addIonsY
+ * Residue masses come in chunks of 64 from `mass_of(c0, cn)`, called by every
+ * lane of the wave (so it may use wave-wide ballots); lane j returns the mass
+ * of residue c0 + j for j < cn and 0 otherwise.  The b ladder is the
+ * sequential FP32 prefix, reproduced exactly by broadcasting each residue mass
+ * to every lane and letting lane k add masses 0..k in order.  For each ladder
+ * position and each charge 1..zmax the peaks of addIonsAB/addIonsY
  * (ion_series.cu:111-156) are OR-ed into `map` (bins >= len dropped, max per
  * bin through the unary weight code).
  */
 template <class MassFn>
 __device__ __forceinline__ void hfx_wave_peaks(uint32_t *map, uint32_t n, float r,
-                                               uint32_t zmax, uint32_t len, MassFn mass_of)
+                                               uint32_t zmax, uint32_t len, MassFn &&mass_of)
 {
     const uint32_t lane = hfx_lane();
     float carry = 0.0f;
     for (uint32_t c0 = 0; c0 < n; c0 += HFX_WAVE) {
         const uint32_t cn = min((uint32_t)HFX_WAVE, n - c0);
-        float m = (lane < cn) ? mass_of(c0 + lane) : 0.0f;
+        const float m = mass_of(c0, cn);
         float b = carry;
         for (uint32_t j = 0; j < cn; ++j) {
             const float mj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), j));
@@ -286,3 +291,54 @@ __device__ __forceinline__ int64_t hfx_wave_dot_fx(uint32_t *map, const float *x
     }
     return hfx_wave_sum_i64(acc);
 }
+
+/* Residue masses of an unmodified candidate (ion_series.cu:163-168). */
+struct HfxPlainMass {
+    const uint8_t *ions; /* first residue of the candidate */
+    const float *mtab;   /* LDS copy of the 26-entry table */
+    __device__ __forceinline__ float operator()(uint32_t c0, uint32_t cn) const
+    {
+        const uint32_t lane = hfx_lane();
+        return lane < cn ? mtab[HFX_ACID_IDX(ions[c0 + lane])] : 0.0f;
+    }
+};
+
+/* Residue masses of a modified candidate: a residue packed with modable-acid
+ * index a+1 (prepare_ions) that is the i-th occurrence of acid a in the
+ * ladder, with i listed in this peptide's unrank slice for a, weighs
+ * table + ma_mass[a] (mion_series.cu:243-286).  The occurrence index counts
+ * acid a only (check_series.cu:61-99), carried across 64-residue chunks. */
+#define HFX_MAX_MA 7
+struct HfxModMass {
+    const uint8_t *ions;
+    const float *mtab;
+    const uint32_t *mod_row; /* counts of each modable acid in this modification */
+    const uint32_t *unrank;  /* this modified peptide's unrank slice */
+    const float *ma_mass;
+    uint32_t num_ma;
+    uint32_t occ[HFX_MAX_MA];
+    __device__ __forceinline__ float operator()(uint32_t c0, uint32_t cn)
+    {
+        const uint32_t lane = hfx_lane();
+        const bool act = lane < cn;
+        const uint32_t ion = act ? ions[c0 + lane] : 0u;
+        const uint32_t mod = HFX_ACID_MOD(ion);
+        float m = act ? mtab[HFX_ACID_IDX(ion)] : 0.0f;
+        const uint64_t lt = (1ull << lane) - 1ull;
+        uint32_t off = 0;
+        for (uint32_t a = 0; a < num_ma; ++a) {
+            const bool is_a = act && mod == a + 1;
+            const uint64_t bal = __ballot(is_a);
+            const uint32_t k = mod_row[a];
+            if (is_a) {
+                const uint32_t ith = occ[a] + (uint32_t)__builtin_popcountll(bal & lt);
+                bool hit = false;
+                for (uint32_t q = 0; q < k; ++q) hit |= unrank[off + q] == ith;
+                if (hit) m = m + ma_mass[a];
+            }
+            occ[a] += (uint32_t)__builtin_popcountll(bal);
+            off += k;
+        }
+        return m;
+    }
+};

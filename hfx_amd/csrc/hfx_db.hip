@@ -167,6 +167,14 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_downsweep(const uint32_t *kin
     }
 }
 
+int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream)
+{
+    if (n == 0) return HFX_OK;
+    hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(1024), 0, hfx_stream(stream), d_a, n);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
 __global__ void k_float_to_key(const float *f, uint32_t *k, uint32_t n)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -292,7 +300,7 @@ __global__ void __launch_bounds__(1024) k_sum_u32(const uint32_t *a, uint32_t n,
     }
 }
 
-static uint32_t sum_to_host(hfx_ctx *ctx, const uint32_t *d_a, uint32_t n)
+uint32_t hfx_sum_u32_to_host(hfx_ctx *ctx, const uint32_t *d_a, uint32_t n)
 {
     void *p, *h;
     if (hfx_ctx_scratch(ctx, HFX_SLOT_SUM, 16, &p) != HFX_OK) return 0;
@@ -318,7 +326,7 @@ extern "C" uint32_t findSpecCandsByMass(uint32_t *d_spec_begin_raw, uint32_t *d_
     hipLaunchKernelGGL(k_window, dim3((num_spec + 255) / 256), dim3(256), 0, 0, d_spec_begin_raw,
                        d_spec_end_raw, d_spec_num_pep_raw, d_r_raw, d_pep_idx_r_sorted_raw, num_pep,
                        d_spec_masses_raw, num_spec, eps);
-    return sum_to_host(ctx, d_spec_num_pep_raw, num_spec);
+    return hfx_sum_u32_to_host(ctx, d_spec_num_pep_raw, num_spec);
 }
 
 extern "C" int hfx_window(hfx_ctx *ctx, uint32_t *d_begin, uint32_t *d_num, const hfx_db *db,
@@ -361,7 +369,7 @@ extern "C" uint32_t findBeginEnd_f(uint32_t *d_begin_raw, uint32_t *d_end_raw,
     hipLaunchKernelGGL(k_begin_end_mod, dim3((num_mod + 255) / 256), dim3(256), 0, 0, d_begin_raw,
                        d_end_raw, d_num_pep_scan_raw, d_r_raw, d_pep_idx_r_sorted_raw, num_pep,
                        d_mod_delta_raw, num_mod, lo0, hi0);
-    uint32_t total = sum_to_host(ctx, d_num_pep_scan_raw, num_mod);
+    uint32_t total = hfx_sum_u32_to_host(ctx, d_num_pep_scan_raw, num_mod);
     hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(1024), 0, 0, d_num_pep_scan_raw, num_mod);
     if (hipStreamSynchronize(0) != hipSuccess) hfx_set_error("findBeginEnd_f: sync failed");
     return total;

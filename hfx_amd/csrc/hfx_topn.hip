@@ -7,107 +7,66 @@
  * (stable ascending sort, then reverse twice: Search.hs:119-123, :276).
  *
  * Here one workgroup per spectrum:
- *   1. radix-selects the T+1 best FP32 scores (key desc, position asc), T =
- *      min(N + margin, n);
- *   2. re-scores the best T exactly: int64 fixed point, sum of
- *      round(w * x * 2^36), independent of summation order, so the GPU and
- *      the CPU oracle agree bit for bit;
- *   3. orders them by (exact desc, position asc) and writes the best N;
- *   4. flags the spectrum when the (T+1)-th FP32 score plus the FP32 error
- *      bound could still reach the N-th exact score (then hfx_topn_exact
- *      scores every candidate exactly).
- * FP32 error bound of a score: |f32 - exact| <= 80 * 2^-24 * sum|w*x| and
- * sum|w*x| <= 50 * sum_b |x_b| (each bin counts once, weights <= 50).
+ *   1. radix-selects the T = min(N + margin, n) best FP32 scores (key desc,
+ *      position asc);
+ *   2. re-scores them exactly: int64 fixed point, sum of round(w*x*2^36),
+ *      independent of summation order, so the GPU and the CPU oracle agree
+ *      bit for bit;
+ *   3. takes the N-th best exact score exN and re-scores every other candidate
+ *      whose FP32 score is >= exN - B (B bounds |FP32 - exact|), so that no
+ *      candidate left out can reach the top N;
+ *   4. orders the re-scored set by (exact desc, position asc), writes the best
+ *      N.  If more than TN_EXT candidates fall within B, the spectrum is
+ *      flagged and hfx_topn_exact scores all of its candidates exactly.
+ * Bound: |f32 - exact| <= 80 * 2^-24 * sum|w*x| (<= 71 roundings per score)
+ * and sum|w*x| <= 50 * sum_b |x_b| (each bin counts once, weights <= 50).
  */
-#include "hfx_common.h"
+#include "hfx_cand.h"
 
 #define TN_THREADS 256
 #define TN_WAVES (TN_THREADS / HFX_WAVE)
 #define TN_MAXT 128
+#define TN_EXT 256
+#define TN_MAXC (TN_MAXT + TN_EXT)
 
-struct TopnArgs {
-    uint32_t *top_pos;
-    double *top_exact;
-    float *top_f32;
-    uint32_t *top_cnt;
-    uint32_t *flag;
-    uint32_t N, margin;
-    const float *scores;
-    const uint64_t *off;
-    const uint32_t *begin;
-    const uint32_t *num;
-    hfx_db db;
-    hfx_spectra sp;
-};
-
-__device__ __forceinline__ bool before_key(uint32_t ka, uint32_t pa, uint32_t kb, uint32_t pb)
-{
-    return ka > kb || (ka == kb && pa < pb);
-}
 __device__ __forceinline__ bool before_fx(int64_t fa, uint32_t pa, int64_t fb, uint32_t pb)
 {
     return fa > fb || (fa == fb && pa < pb);
 }
 
-/* Stage the spectrum and mass table; return the effective length. */
-__device__ __forceinline__ uint32_t stage_spectrum(float *xs, float *mtab, const hfx_db &db,
-                                                   const hfx_spectra &sp, uint32_t s)
-{
-    const uint32_t len = min(sp.d_len[s], min(sp.x_stride, (uint32_t)HFX_MAX_LEN));
-    const float *xg = sp.d_x + (size_t)s * sp.x_stride;
-    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) xs[i] = xg[i];
-    if (threadIdx.x < 32) mtab[threadIdx.x] = threadIdx.x < 26 ? db.d_mass[threadIdx.x] : 0.0f;
-    return len;
-}
-
-/* Exact and FP32 score of window position `pos` of spectrum s by one wave. */
-__device__ __forceinline__ void wave_score(uint32_t *map, const float *xs, const float *mtab,
-                                           const hfx_db &db, uint32_t p, uint32_t len,
-                                           uint32_t zmax, float *f32, int64_t *fx)
-{
-    const uint32_t pep = db.d_idx_sorted[p];
-    const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
-    const uint32_t n = tn > tc ? tn - tc : 0;
-    const uint8_t *ions = db.d_ions + tc;
-    const uint32_t zeff = zmax <= HFX_MAX_CHARGE ? zmax : 0;
-    hfx_wave_peaks(map, n, db.d_r[pep], zeff, len,
-                   [&](uint32_t j) { return mtab[HFX_ACID_IDX(ions[j])]; });
-    hfx_wave_sync();
-    hfx_wave_dot_both(map, xs, len, f32, fx);
-    hfx_wave_sync();
-}
-
-__global__ void __launch_bounds__(TN_THREADS) k_topn(TopnArgs a)
+template <class Cands>
+__global__ void __launch_bounds__(TN_THREADS)
+    k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
+           uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
+           hfx_spectra sp)
 {
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
     __shared__ float mtab[32];
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t sel_k[TN_MAXT + 1], sel_p[TN_MAXT + 1];
-    __shared__ uint32_t srt_k[TN_MAXT + 1], srt_p[TN_MAXT + 1];
-    __shared__ int64_t ex_fx[TN_MAXT];
-    __shared__ float ex_f32[TN_MAXT];
+    __shared__ uint32_t cp[TN_MAXC];
+    __shared__ int64_t cfx[TN_MAXC];
+    __shared__ float cf32[TN_MAXC];
     __shared__ uint32_t wcnt[TN_WAVES];
-    __shared__ uint32_t sh_digit, sh_rem, sh_gt;
     __shared__ float sh_x1[TN_WAVES];
+    __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_last, sh_ext;
+    __shared__ int64_t sh_exN;
 
     const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
-    const uint32_t n = a.num[s];
-    const uint32_t N = a.N;
-    const float *sc = a.scores + a.off[s];
+    const uint32_t n = c.count(s);
+    const float *sc = scores + c.base(s);
     if (n == 0) {
         if (tid == 0) {
-            a.top_cnt[s] = 0;
-            a.flag[s] = 0;
+            top_cnt[s] = 0;
+            flag[s] = 0;
         }
         return;
     }
-    const uint32_t T = min(N + a.margin, n);
-    const uint32_t want = min(T + 1, n);
+    const uint32_t T = min(N + margin, n);
 
-    /* 1. radix select of the `want` largest keys (MSB first, 8-bit digits) */
+    /* 1. radix select of the T largest keys (MSB first, 8-bit digits) */
     uint32_t prefix = 0, pmask = 0;
-    if (tid == 0) sh_rem = want;
+    if (tid == 0) sh_rem = T;
     for (int shift = 24; shift >= 0; shift -= 8) {
         hist[tid] = 0;
         __syncthreads();
@@ -130,24 +89,23 @@ __global__ void __launch_bounds__(TN_THREADS) k_topn(TopnArgs a)
         pmask |= 255u << shift;
     }
     const uint32_t K = prefix;
-    const uint32_t need_eq = sh_rem;
-    const uint32_t n_gt = want - need_eq;
-    if (tid == 0) sh_gt = 0;
+    const uint32_t need_eq = sh_rem; /* >= 1 */
+    const uint32_t n_gt = T - need_eq;
+    if (tid == 0) { sh_gt = 0; sh_ext = 0; sh_last = 0; }
     __syncthreads();
     for (uint32_t i = tid; i < n; i += TN_THREADS) {
-        const uint32_t k = hfx_f2key(sc[i]);
-        if (k > K) {
+        if (hfx_f2key(sc[i]) > K) {
             const uint32_t slot = atomicAdd(&sh_gt, 1u);
-            if (slot < n_gt) { sel_k[slot] = k; sel_p[slot] = i; }
+            if (slot < n_gt) cp[slot] = i;
         }
     }
-    /* keys equal to the threshold: the need_eq smallest positions, in order */
+    /* keys equal to the threshold: the need_eq smallest positions */
     uint32_t eq_base = 0;
     for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
         const uint32_t i = base + tid;
         const bool f = i < n && hfx_f2key(sc[i]) == K;
         const uint64_t bal = __ballot(f);
-        const uint32_t wr = __builtin_popcountll(bal & ((1ull << lane) - 1ull));
+        const uint32_t wr = (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull));
         if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
         __syncthreads();
         uint32_t wpre = 0, tot = 0;
@@ -156,88 +114,97 @@ __global__ void __launch_bounds__(TN_THREADS) k_topn(TopnArgs a)
             tot += wcnt[w];
         }
         const uint32_t r = eq_base + wpre + wr;
-        if (f && r < need_eq) { sel_k[n_gt + r] = K; sel_p[n_gt + r] = i; }
+        if (f && r < need_eq) {
+            cp[n_gt + r] = i;
+            if (r == need_eq - 1) sh_last = i;
+        }
         eq_base += tot;
         __syncthreads();
     }
-    __syncthreads();
-    /* order the selection by (key desc, position asc) */
-    for (uint32_t i = tid; i < want; i += TN_THREADS) {
-        const uint32_t ki = sel_k[i], pi = sel_p[i];
-        uint32_t rk = 0;
-        for (uint32_t j = 0; j < want; ++j) rk += before_key(sel_k[j], sel_p[j], ki, pi) ? 1u : 0u;
-        srt_k[rk] = ki;
-        srt_p[rk] = pi;
-    }
-    /* 2. exact re-score of the best T */
-    const uint32_t len = stage_spectrum(xs, mtab, a.db, a.sp, s);
+
+    /* 2. exact re-score of the selection */
+    const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
     uint32_t *map = maps[wave];
     for (uint32_t w = lane; w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
     __syncthreads();
-    const uint32_t zmax = a.sp.d_zmax[s];
-    const uint32_t b0 = a.begin[s];
+    const uint32_t zeff = c.zeff(sp.d_zmax[s]);
     for (uint32_t i = wave; i < T; i += TN_WAVES) {
-        float f32;
-        int64_t fx;
-        wave_score(map, xs, mtab, a.db, b0 + srt_p[i], len, zmax, &f32, &fx);
+        c.peaks(map, mtab, s, cp[i], len, zeff);
+        hfx_wave_sync();
+        const int64_t fx = hfx_wave_dot_fx(map, xs, len);
+        hfx_wave_sync();
         if (lane == 0) {
-            ex_fx[i] = fx;
-            ex_f32[i] = sc[srt_p[i]];
+            cfx[i] = fx;
+            cf32[i] = sc[cp[i]];
         }
     }
-    /* sum |x| for the FP32 error bound */
     float x1 = 0.0f;
     for (uint32_t i = tid; i < len; i += TN_THREADS) x1 += fabsf(xs[i]);
     x1 = hfx_wave_sum_f32(x1);
     if (lane == 0) sh_x1[wave] = x1;
     __syncthreads();
-    /* 3. order the re-scored set by (exact desc, position asc), emit N */
-    const uint32_t outn = min(N, T);
-    for (uint32_t i = tid; i < T; i += TN_THREADS) {
-        const int64_t fi = ex_fx[i];
-        const uint32_t pi = srt_p[i];
-        uint32_t rk = 0;
-        for (uint32_t j = 0; j < T; ++j) rk += before_fx(ex_fx[j], srt_p[j], fi, pi) ? 1u : 0u;
-        if (rk < outn) {
-            a.top_pos[(size_t)s * N + rk] = pi;
-            a.top_exact[(size_t)s * N + rk] = (double)fi * (1.0 / HFX_FX_SCALE);
-            a.top_f32[(size_t)s * N + rk] = ex_f32[i];
-            if (rk == outn - 1) hist[0] = i; /* index of the N-th, for the flag */
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        a.top_cnt[s] = outn;
-        uint32_t fl = 0;
-        if (n > T) {
-            float xa = 0.0f;
-            for (uint32_t w = 0; w < TN_WAVES; ++w) xa += sh_x1[w];
-            const double bound = 80.0 * 5.9604644775390625e-08 * 50.0 * (double)xa * 1.001 + 1e-6;
-            const double exN = (double)ex_fx[hist[0]] * (1.0 / HFX_FX_SCALE);
-            const double f32ex = (double)hfx_key2f(srt_k[T]);
-            fl = (exN <= f32ex + bound) ? 1u : 0u;
-        }
-        a.flag[s] = fl;
-    }
-}
 
-extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
-                        uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
-                        const float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-                        const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp, void *stream)
-{
-    (void)ctx;
-    HFX_REQUIRE(d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_flag && d_scores && d_off &&
-                    d_begin && d_num && db && sp,
-                HFX_ERR_ARG, "hfx_topn: null argument");
-    HFX_REQUIRE(N >= 1 && N + margin <= TN_MAXT, HFX_ERR_ARG,
-                "hfx_topn: N + margin must be in 1..%d (N=%u margin=%u)", TN_MAXT, N, margin);
-    if (sp->num_spec == 0) return HFX_OK;
-    TopnArgs a{d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-               d_scores,  d_off,       d_begin,   d_num,     *db,    *sp};
-    hipLaunchKernelGGL(k_topn, dim3(sp->num_spec), dim3(TN_THREADS), 0, hfx_stream(stream), a);
-    HFX_CHECK_LAUNCH();
-    return HFX_OK;
+    /* 3. every other candidate that could still reach the N-th exact score */
+    uint32_t M = T;
+    uint32_t fl = 0;
+    if (n > T) {
+        for (uint32_t i = tid; i < T; i += TN_THREADS) {
+            uint32_t rk = 0;
+            for (uint32_t j = 0; j < T; ++j) rk += before_fx(cfx[j], cp[j], cfx[i], cp[i]) ? 1u : 0u;
+            if (rk == N - 1) sh_exN = cfx[i];
+        }
+        __syncthreads();
+        float xa = 0.0f;
+        for (uint32_t w = 0; w < TN_WAVES; ++w) xa += sh_x1[w];
+        const double bound = 80.0 * 5.9604644775390625e-08 * 50.0 * (double)xa * 1.001 + 1e-6;
+        const double tau = (double)sh_exN * (1.0 / HFX_FX_SCALE) - bound;
+        const uint32_t last = sh_last;
+        for (uint32_t i = tid; i < n; i += TN_THREADS) {
+            const float f = sc[i];
+            const uint32_t k = hfx_f2key(f);
+            const bool selected = k > K || (k == K && i <= last);
+            if (!selected && (double)f >= tau) {
+                const uint32_t slot = atomicAdd(&sh_ext, 1u);
+                if (slot < TN_EXT) cp[T + slot] = i;
+            }
+        }
+        __syncthreads();
+        const uint32_t ne = sh_ext;
+        if (ne > TN_EXT) {
+            fl = 1;
+        } else {
+            for (uint32_t i = T + wave; i < T + ne; i += TN_WAVES) {
+                c.peaks(map, mtab, s, cp[i], len, zeff);
+                hfx_wave_sync();
+                const int64_t fx = hfx_wave_dot_fx(map, xs, len);
+                hfx_wave_sync();
+                if (lane == 0) {
+                    cfx[i] = fx;
+                    cf32[i] = sc[cp[i]];
+                }
+            }
+            M = T + ne;
+        }
+        __syncthreads();
+    }
+
+    /* 4. order by (exact desc, position asc), emit N */
+    const uint32_t outn = min(N, n);
+    for (uint32_t i = tid; i < M; i += TN_THREADS) {
+        const int64_t fi = cfx[i];
+        const uint32_t pi = cp[i];
+        uint32_t rk = 0;
+        for (uint32_t j = 0; j < M; ++j) rk += before_fx(cfx[j], cp[j], fi, pi) ? 1u : 0u;
+        if (rk < outn) {
+            top_pos[(size_t)s * N + rk] = pi;
+            top_exact[(size_t)s * N + rk] = (double)fi * (1.0 / HFX_FX_SCALE);
+            top_f32[(size_t)s * N + rk] = cf32[i];
+        }
+    }
+    if (tid == 0) {
+        top_cnt[s] = outn;
+        flag[s] = fl;
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -246,20 +213,10 @@ extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, 
 
 #define TX_MAXN 64
 
-struct TopnExactArgs {
-    uint32_t *top_pos;
-    double *top_exact;
-    float *top_f32;
-    uint32_t *top_cnt;
-    uint32_t N;
-    const uint32_t *list;
-    const uint32_t *begin;
-    const uint32_t *num;
-    hfx_db db;
-    hfx_spectra sp;
-};
-
-__global__ void __launch_bounds__(TN_THREADS) k_topn_exact(TopnExactArgs a)
+template <class Cands>
+__global__ void __launch_bounds__(TN_THREADS)
+    k_topn_exact(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
+                 uint32_t N, const uint32_t *list, Cands c, hfx_spectra sp)
 {
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
@@ -268,27 +225,30 @@ __global__ void __launch_bounds__(TN_THREADS) k_topn_exact(TopnExactArgs a)
     __shared__ uint32_t lpos[TN_WAVES * TX_MAXN];
     __shared__ float lf32[TN_WAVES * TX_MAXN];
     __shared__ uint32_t lcnt[TN_WAVES];
-    const uint32_t s = a.list[blockIdx.x];
+    const uint32_t s = list[blockIdx.x];
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
-    const uint32_t n = a.num[s], N = a.N;
-    const uint32_t len = stage_spectrum(xs, mtab, a.db, a.sp, s);
+    const uint32_t n = c.count(s);
+    const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
     uint32_t *map = maps[wave];
     for (uint32_t w = lane; w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
     if (lane == 0) lcnt[wave] = 0;
     __syncthreads();
-    const uint32_t zmax = a.sp.d_zmax[s], b0 = a.begin[s];
+    const uint32_t zeff = c.zeff(sp.d_zmax[s]);
     int64_t *wf = lfx + wave * TX_MAXN;
     uint32_t *wp = lpos + wave * TX_MAXN;
     float *w32 = lf32 + wave * TX_MAXN;
     for (uint32_t i = wave; i < n; i += TN_WAVES) {
         float f32;
         int64_t fx;
-        wave_score(map, xs, mtab, a.db, b0 + i, len, zmax, &f32, &fx);
+        c.peaks(map, mtab, s, i, len, zeff);
+        hfx_wave_sync();
+        hfx_wave_dot_both(map, xs, len, &f32, &fx);
+        hfx_wave_sync();
         if (lane == 0) {
             /* positions arrive ascending per wave: equal scores keep order */
-            uint32_t c = lcnt[wave];
-            if (c < N || fx > wf[c - 1]) {
-                uint32_t j = c < N ? c++ : c - 1;
+            uint32_t cnum = lcnt[wave];
+            if (cnum < N || fx > wf[cnum - 1]) {
+                uint32_t j = cnum < N ? cnum++ : cnum - 1;
                 while (j > 0 && fx > wf[j - 1]) {
                     wf[j] = wf[j - 1];
                     wp[j] = wp[j - 1];
@@ -298,14 +258,12 @@ __global__ void __launch_bounds__(TN_THREADS) k_topn_exact(TopnExactArgs a)
                 wf[j] = fx;
                 wp[j] = i;
                 w32[j] = f32;
-                lcnt[wave] = c;
+                lcnt[wave] = cnum;
             }
         }
         hfx_wave_sync();
     }
     __syncthreads();
-    uint32_t tot = 0;
-    for (uint32_t w = 0; w < TN_WAVES; ++w) tot += lcnt[w];
     const uint32_t outn = min(N, n);
     for (uint32_t e = tid; e < TN_WAVES * TX_MAXN; e += TN_THREADS) {
         const uint32_t w = e / TX_MAXN, k = e % TX_MAXN;
@@ -319,13 +277,62 @@ __global__ void __launch_bounds__(TN_THREADS) k_topn_exact(TopnExactArgs a)
                 rk += before_fx(lfx[e2], lpos[e2], fi, pi) ? 1u : 0u;
             }
         if (rk < outn) {
-            a.top_pos[(size_t)s * N + rk] = pi;
-            a.top_exact[(size_t)s * N + rk] = (double)fi * (1.0 / HFX_FX_SCALE);
-            a.top_f32[(size_t)s * N + rk] = lf32[e];
+            top_pos[(size_t)s * N + rk] = pi;
+            top_exact[(size_t)s * N + rk] = (double)fi * (1.0 / HFX_FX_SCALE);
+            top_f32[(size_t)s * N + rk] = lf32[e];
         }
     }
-    (void)tot;
-    if (tid == 0) a.top_cnt[s] = outn;
+    if (tid == 0) top_cnt[s] = outn;
+}
+
+/* ------------------------------------------------------------------ */
+/* Entry points                                                          */
+/* ------------------------------------------------------------------ */
+
+template <class Cands>
+static int launch_topn(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                       uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
+                       const float *d_scores, const Cands &c, const hfx_spectra *sp, void *stream)
+{
+    HFX_REQUIRE(d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_flag && d_scores && sp,
+                HFX_ERR_ARG, "hfx_topn: null argument");
+    HFX_REQUIRE(N >= 1 && N + margin <= TN_MAXT, HFX_ERR_ARG,
+                "hfx_topn: N + margin must be in 1..%d (N=%u margin=%u)", TN_MAXT, N, margin);
+    if (sp->num_spec == 0) return HFX_OK;
+    hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS), 0, hfx_stream(stream),
+                       d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c,
+                       *sp);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+template <class Cands>
+static int launch_topn_exact(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                             uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_list,
+                             uint32_t num_list, const Cands &c, const hfx_spectra *sp,
+                             void *stream)
+{
+    HFX_REQUIRE(d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_list && sp, HFX_ERR_ARG,
+                "hfx_topn_exact: null argument");
+    HFX_REQUIRE(N >= 1 && N <= TX_MAXN, HFX_ERR_ARG, "hfx_topn_exact: N must be in 1..%d",
+                TX_MAXN);
+    if (num_list == 0) return HFX_OK;
+    hipLaunchKernelGGL(k_topn_exact<Cands>, dim3(num_list), dim3(TN_THREADS), 0, hfx_stream(stream),
+                       d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_list, c, *sp);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                        uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
+                        const float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
+                        const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp, void *stream)
+{
+    (void)ctx;
+    HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn: null argument");
+    HfxPlainCands c{*db, d_begin, d_num, d_off};
+    return launch_topn(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c,
+                       sp, stream);
 }
 
 extern "C" int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
@@ -335,14 +342,36 @@ extern "C" int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_e
                               void *stream)
 {
     (void)ctx;
-    HFX_REQUIRE(d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_spec_list && d_begin &&
-                    d_num && db && sp,
-                HFX_ERR_ARG, "hfx_topn_exact: null argument");
-    HFX_REQUIRE(N >= 1 && N <= TX_MAXN, HFX_ERR_ARG, "hfx_topn_exact: N must be in 1..%d", TX_MAXN);
-    if (num_list == 0) return HFX_OK;
-    TopnExactArgs a{d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list, d_begin, d_num,
-                    *db,       *sp};
-    hipLaunchKernelGGL(k_topn_exact, dim3(num_list), dim3(TN_THREADS), 0, hfx_stream(stream), a);
-    HFX_CHECK_LAUNCH();
-    return HFX_OK;
+    HFX_REQUIRE(d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn_exact: null argument");
+    HfxPlainCands c{*db, d_begin, d_num, nullptr};
+    return launch_topn_exact(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list, num_list,
+                             c, sp, stream);
+}
+
+extern "C" int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
+                            float *d_top_f32, uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N,
+                            uint32_t margin, const float *d_scores, const uint64_t *d_mp_off,
+                            const hfx_modset *ms, const hfx_db *db, const hfx_spectra *sp,
+                            void *stream)
+{
+    (void)ctx;
+    HFX_REQUIRE(d_mp_off && ms && db, HFX_ERR_ARG, "hfx_topn_mod: null argument");
+    HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_topn_mod: num_ma %u", ms->num_ma);
+    HfxModCands c{*db, *ms, d_mp_off};
+    return launch_topn(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c,
+                       sp, stream);
+}
+
+extern "C" int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
+                                  float *d_top_f32, uint32_t *d_top_cnt, uint32_t N,
+                                  const uint32_t *d_spec_list, uint32_t num_list,
+                                  const uint64_t *d_mp_off, const hfx_modset *ms, const hfx_db *db,
+                                  const hfx_spectra *sp, void *stream)
+{
+    (void)ctx;
+    HFX_REQUIRE(d_mp_off && ms && db, HFX_ERR_ARG, "hfx_topn_exact_mod: null argument");
+    HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_topn_exact_mod: num_ma %u", ms->num_ma);
+    HfxModCands c{*db, *ms, d_mp_off};
+    return launch_topn_exact(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list, num_list,
+                             c, sp, stream);
 }

@@ -13,18 +13,11 @@
  * XC_CHUNK candidates) tile; the spectrum is staged in LDS once per tile; each
  * wave scores one candidate at a time (ladder -> map -> dot).
  */
-#include "hfx_common.h"
+#include "hfx_cand.h"
 
 #define XC_THREADS 256
 #define XC_WAVES (XC_THREADS / HFX_WAVE)
 #define XC_CHUNK 64
-
-/* tiles per spectrum */
-__global__ void k_tile_count(uint32_t *tiles, const uint32_t *num, uint32_t num_spec, uint32_t chunk)
-{
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < num_spec) tiles[s] = (num[s] + chunk - 1) / chunk;
-}
 
 /* largest s with off[s] <= t, off has n+1 entries, off[0] = 0 */
 __device__ __forceinline__ uint32_t find_segment(const uint64_t *off, uint32_t n, uint64_t t)
@@ -37,66 +30,46 @@ __device__ __forceinline__ uint32_t find_segment(const uint64_t *off, uint32_t n
     return lo;
 }
 
-struct XcArgs {
-    float *scores;
-    const uint64_t *score_off;
-    const uint64_t *tile_off;
-    const uint32_t *begin;
-    const uint32_t *num;
-    hfx_db db;
-    hfx_spectra sp;
-};
+template <class Cands>
+__global__ void k_tile_count(uint32_t *tiles, Cands c, uint32_t num_spec, uint32_t chunk)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < num_spec) tiles[s] = (c.count(s) + chunk - 1) / chunk;
+}
 
-__global__ void __launch_bounds__(XC_THREADS) k_xcorr_plain(XcArgs a)
+template <class Cands>
+__global__ void __launch_bounds__(XC_THREADS)
+    k_xcorr(float *scores, const uint64_t *tile_off, Cands c, hfx_spectra sp)
 {
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[XC_WAVES][HFX_MAP_WORDS];
     __shared__ float mtab[32];
-    const uint32_t S = a.sp.num_spec;
+    const uint32_t S = sp.num_spec;
     const uint64_t t = blockIdx.x;
-    if (t >= a.tile_off[S]) return;
-    const uint32_t s = find_segment(a.tile_off, S, t);
-    const uint32_t chunk = (uint32_t)(t - a.tile_off[s]);
-    const uint32_t first = chunk * XC_CHUNK;
-    const uint32_t cnt = min((uint32_t)XC_CHUNK, a.num[s] - first);
-    /* every LDS index below is < len <= HFX_MAX_LEN, whatever the data say */
-    const uint32_t len = min(a.sp.d_len[s], min(a.sp.x_stride, (uint32_t)HFX_MAX_LEN));
-    const uint32_t zmax = a.sp.d_zmax[s];
+    if (t >= tile_off[S]) return;
+    const uint32_t s = find_segment(tile_off, S, t);
+    const uint32_t first = (uint32_t)(t - tile_off[s]) * XC_CHUNK;
+    const uint32_t cnt = min((uint32_t)XC_CHUNK, c.count(s) - first);
     const uint32_t wave = threadIdx.x / HFX_WAVE;
     uint32_t *map = maps[wave];
-
-    const float *xg = a.sp.d_x + (size_t)s * a.sp.x_stride;
-    for (uint32_t i = threadIdx.x; i < len; i += XC_THREADS) xs[i] = xg[i];
-    if (threadIdx.x < 32) mtab[threadIdx.x] = threadIdx.x < 26 ? a.db.d_mass[threadIdx.x] : 0.0f;
+    const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
     for (uint32_t w = hfx_lane(); w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
     __syncthreads();
-
-    const uint32_t base = a.begin[s] + first;
-    const uint64_t out = a.score_off[s] + first;
-    /* zmax > 4: the reference launches nothing (ion_series.cu:334-342) and the
-     * zeroed rows score 0 */
-    const uint32_t zeff = zmax <= HFX_MAX_CHARGE ? zmax : 0;
+    const uint32_t zeff = c.zeff(sp.d_zmax[s]);
+    float *out = scores + c.base(s) + first;
     for (uint32_t i = wave; i < cnt; i += XC_WAVES) {
-        const uint32_t pep = a.db.d_idx_sorted[base + i];
-        const uint32_t tc = a.db.d_tc[pep], tn = a.db.d_tn[pep];
-        const float r = a.db.d_r[pep];
-        const uint32_t n = tn > tc ? tn - tc : 0;
-        const uint8_t *ions = a.db.d_ions + tc;
-        hfx_wave_peaks(map, n, r, zeff, len,
-                       [&](uint32_t j) { return mtab[HFX_ACID_IDX(ions[j])]; });
+        c.peaks(map, mtab, s, first + i, len, zeff);
         hfx_wave_sync();
         const float sc = hfx_wave_dot_f32(map, xs, len);
         hfx_wave_sync();
-        if (hfx_lane() == 0) a.scores[out + i] = sc;
+        if (hfx_lane() == 0) out[i] = sc;
     }
 }
 
-extern "C" int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
-                                const uint32_t *d_begin, const uint32_t *d_num, uint64_t total,
-                                const hfx_db *db, const hfx_spectra *sp, void *stream)
+template <class Cands>
+static int launch_xcorr(hfx_ctx *ctx, float *d_scores, const Cands &c, uint64_t total,
+                        const hfx_spectra *sp, void *stream)
 {
-    HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && db && sp, HFX_ERR_ARG,
-                "hfx_xcorr_sparse: null argument");
     const uint32_t S = sp->num_spec;
     if (S == 0 || total == 0) return HFX_OK;
     hipStream_t st = hfx_stream(stream);
@@ -107,16 +80,38 @@ extern "C" int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d
         return rc;
     uint32_t *tiles = (uint32_t *)p;
     uint64_t *tile_off = (uint64_t *)(((uintptr_t)(tiles + S) + 15) & ~(uintptr_t)15);
-    hipLaunchKernelGGL(k_tile_count, dim3((S + 255) / 256), dim3(256), 0, st, tiles, d_num, S,
+    hipLaunchKernelGGL(k_tile_count<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, tiles, c, S,
                        (uint32_t)XC_CHUNK);
     if ((rc = hfx_scan_offsets(ctx, tile_off, tiles, S, stream)) != HFX_OK) return rc;
-    uint64_t grid = (total + XC_CHUNK - 1) / XC_CHUNK + S;
-    HFX_REQUIRE(grid < (1ull << 31), HFX_ERR_ARG, "hfx_xcorr_sparse: %llu tiles",
-                (unsigned long long)grid);
-    XcArgs a{d_scores, d_off, tile_off, d_begin, d_num, *db, *sp};
-    hipLaunchKernelGGL(k_xcorr_plain, dim3((uint32_t)grid), dim3(XC_THREADS), 0, st, a);
+    /* sum_s ceil(n_s / chunk) <= total / chunk + S */
+    const uint64_t grid = (total + XC_CHUNK - 1) / XC_CHUNK + S;
+    HFX_REQUIRE(grid < (1ull << 31), HFX_ERR_ARG, "xcorr: %llu tiles", (unsigned long long)grid);
+    hipLaunchKernelGGL(k_xcorr<Cands>, dim3((uint32_t)grid), dim3(XC_THREADS), 0, st, d_scores,
+                       tile_off, c, *sp);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
+}
+
+extern "C" int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                                const uint32_t *d_begin, const uint32_t *d_num, uint64_t total,
+                                const hfx_db *db, const hfx_spectra *sp, void *stream)
+{
+    HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && db && sp, HFX_ERR_ARG,
+                "hfx_xcorr_sparse: null argument");
+    HfxPlainCands c{*db, d_begin, d_num, d_off};
+    return launch_xcorr(ctx, d_scores, c, total, sp, stream);
+}
+
+extern "C" int hfx_xcorr_sparse_mod(hfx_ctx *ctx, float *d_scores, const uint64_t *d_mp_off,
+                                    uint64_t total, const hfx_modset *ms, const hfx_db *db,
+                                    const hfx_spectra *sp, void *stream)
+{
+    HFX_REQUIRE(ctx && d_scores && d_mp_off && ms && db && sp, HFX_ERR_ARG,
+                "hfx_xcorr_sparse_mod: null argument");
+    HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_xcorr_sparse_mod: num_ma %u > %d",
+                ms->num_ma, HFX_MAX_MA);
+    HfxModCands c{*db, *ms, d_mp_off};
+    return launch_xcorr(ctx, d_scores, c, total, sp, stream);
 }
 
 /* ------------------------------------------------------------------ */
@@ -124,7 +119,7 @@ extern "C" int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d
 /* ------------------------------------------------------------------ */
 
 /* One wave per row: map -> dense row of floats 0/10/25/50, every element
- * written (no memset needed, no cross-block race, Q5 of the reference). */
+ * written (no memset needed, no cross-block race). */
 __device__ __forceinline__ void wave_map_to_row(uint32_t *map, float *row, uint32_t len)
 {
     const uint32_t nw = (len + 7) >> 3;
@@ -155,9 +150,7 @@ __global__ void __launch_bounds__(XC_THREADS)
         const uint32_t pep = idx[row];
         const uint32_t tc = tc_a[pep], tn = tn_a[pep];
         const uint32_t n = tn > tc ? tn - tc : 0;
-        const uint8_t *ions = ions_all + tc;
-        hfx_wave_peaks(map, n, residual[pep], zmax, len,
-                       [&](uint32_t j) { return mtab[HFX_ACID_IDX(ions[j])]; });
+        hfx_wave_peaks(map, n, residual[pep], zmax, len, HfxPlainMass{ions_all + tc, mtab});
         hfx_wave_sync();
         wave_map_to_row(map, spec + (size_t)row * len, len);
         hfx_wave_sync();
@@ -206,6 +199,7 @@ extern "C" int hfx_mvm_rows(hfx_ctx *ctx, float *d_y, const float *d_A, uint32_t
 {
     (void)ctx;
     HFX_REQUIRE(d_y && d_A && d_x, HFX_ERR_ARG, "hfx_mvm_rows: null argument");
+    HFX_REQUIRE(n <= lda, HFX_ERR_ARG, "hfx_mvm_rows: n %u > lda %u", n, lda);
     if (m == 0) return HFX_OK;
     hipLaunchKernelGGL(k_mvm_rows, dim3((m + 3) / 4), dim3(256), 0, hfx_stream(stream), d_y, d_A,
                        lda, d_x, m, n);

@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import HfxDb, HfxSpectra, check, lib
+from . import HfxDb, HfxModset, HfxSpectra, check, lib
 
 _gpu_checked = False
 
@@ -350,3 +350,169 @@ class PlainSearch:
             flagged=flagged,
             scores=host(self.t_scores, np.float32, self.total).copy() if keep_scores else None,
         )
+
+
+@dataclass
+class ModInfo:
+    """Fragment.hs:260-289 HostModInfo (from hfx_mod_info)."""
+
+    ma: bytes
+    ma_mass: np.ndarray
+    mod_ma_count: np.ndarray  # [num_mod, num_ma]
+    mod_sum: np.ndarray
+    mod_delta: np.ndarray
+
+    @property
+    def num_ma(self) -> int:
+        return len(self.ma)
+
+    @property
+    def num_mod(self) -> int:
+        return len(self.mod_delta)
+
+
+def mod_info(var: dict, max_ma: int) -> ModInfo:
+    L = lib()
+    v = np.zeros(26, np.float32)
+    for k, x in var.items():
+        v[ord(k) - 65] = np.float32(x)
+    ma = np.zeros(26, np.uint8)
+    mam = np.zeros(26, np.float32)
+    na, nm = C.c_uint32(), C.c_uint32()
+    L.hfx_mod_info(v.ctypes.data, max_ma, ma.ctypes.data, mam.ctypes.data, C.byref(na), None,
+                   None, None, 0, C.byref(nm))
+    a, m = na.value, nm.value
+    cnt = np.zeros(max(m * a, 1), np.uint32)
+    sm = np.zeros(max(m, 1), np.uint32)
+    dl = np.zeros(max(m, 1), np.float32)
+    check(L.hfx_mod_info(v.ctypes.data, max_ma, ma.ctypes.data, mam.ctypes.data, C.byref(na),
+                         cnt.ctypes.data, sm.ctypes.data, dl.ctypes.data, m, C.byref(nm)),
+          "hfx_mod_info")
+    return ModInfo(bytes(ma[:a]), mam[:a].copy(), cnt[: m * a].reshape(m, a).copy(), sm[:m].copy(),
+                   dl[:m].copy())
+
+
+@dataclass
+class ModResult:
+    spec_cand: np.ndarray     # candidates by mass per spectrum (rows before modability)
+    num_valid: int
+    num_mpep: int
+    len_unrank: int
+    mp_off: np.ndarray        # [S+1] first modified peptide of each spectrum
+    top_pos: np.ndarray       # [S, N] index within the spectrum's modified peptides
+    top_exact: np.ndarray
+    top_f32: np.ndarray
+    top_cnt: np.ndarray
+    flagged: np.ndarray
+    stages: dict              # every intermediate array, for parity checks
+    scores: np.ndarray | None = None
+
+
+class ModSearch:
+    """Search.hs:282-394 searchWithMods for one batch, through the reference
+    ABI (findBeginEnd_f, findModablePeptides, calcTotalModCands,
+    prepareGenMod, genModCands) and the sparse modified-peptide scorer."""
+
+    def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, mi: ModInfo, eps: float,
+                 topn: int, margin: int = 16):
+        self.ctx, self.db, self.sp, self.mi = ctx, db, sp, mi
+        self.eps, self.topn, self.margin = float(eps), int(topn), int(margin)
+        self.t_ma = dev(np.frombuffer(mi.ma, np.uint8))
+        self.t_ma_mass = dev(mi.ma_mass)
+        self.t_mod_cnt = dev(mi.mod_ma_count.reshape(-1))
+        self.t_mod_sum = dev(mi.mod_sum)
+        self.t_mod_delta = dev(mi.mod_delta)
+
+    def run(self, keep_scores: bool = False) -> ModResult:
+        L = lib()
+        S, M, na = self.sp.num_spec, self.mi.num_mod, self.mi.num_ma
+        db = self.db
+        t_b, t_e, t_sc = empty(4 * S * M), empty(4 * S * M), empty(4 * S * M)
+        spec_cand = np.zeros(S, np.uint32)
+        # Search.hs:317-322: one findBeginEnd per spectrum
+        for s in range(S):
+            o = 4 * s * M
+            spec_cand[s] = L.findBeginEnd_f(t_b.p + o, t_e.p + o, t_sc.p + o, db.t_r.p, db.t_idx.p,
+                                            db.num_pep, self.t_mod_delta.p, M,
+                                            float(self.sp.mass[s]), self.eps)
+        scan = np.zeros(S + 1, np.uint32)
+        np.cumsum(spec_cand, out=scan[1:])
+        total = int(scan[-1])
+        t_scan = dev(scan)
+        st = {"spec_cand": spec_cand, "begin": host(t_b, np.uint32, S * M),
+              "end": host(t_e, np.uint32, S * M), "num_scan": host(t_sc, np.uint32, S * M)}
+        t_pep, t_pmod = empty(4 * total), empty(4 * total)
+        t_pcnt, t_vscan = empty(4 * total * na), empty(4 * S)
+        nv = L.findModablePeptides(t_pep.p, t_pmod.p, t_pcnt.p, t_vscan.p, t_scan.p, S, total,
+                                   db.t_ions.p, db.t_tc.p, db.t_tn.p, db.t_idx.p, t_b.p, t_e.p,
+                                   t_sc.p, self.t_mod_cnt.p, M, self.t_ma.p, na)
+        st.update(valid_pep=host(t_pep, np.uint32, nv), valid_mod=host(t_pmod, np.uint32, nv),
+                  valid_cnt=host(t_pcnt, np.uint32, nv * na), valid_scan=host(t_vscan, np.uint32, S))
+        t_nmp, t_comb = empty(4 * nv), empty(4 * nv * na)
+        nmp = L.calcTotalModCands(t_nmp.p, t_comb.p, self.t_mod_cnt.p, t_pep.p, t_pmod.p, t_pcnt.p,
+                                  nv, M, na) if nv else 0
+        st.update(pep_num_mpep=host(t_nmp, np.uint32, nv), comb=host(t_comb, np.uint32, nv * na))
+        t_mp = [empty(4 * nmp) for _ in range(5)]
+        lu = L.prepareGenMod(*[t.p for t in t_mp], self.t_mod_sum.p, t_pep.p, t_pmod.p, t_nmp.p,
+                             nv, nmp) if nmp else 0
+        for name, t in zip(("mp_pep", "mp_mod", "mp_rank", "mp_ith", "mp_scan"), t_mp):
+            st[name] = host(t, np.uint32, nmp)
+        t_unrank = empty(4 * max(lu, 1))
+        if nmp:
+            L.genModCands(t_unrank.p, self.t_mod_cnt.p, t_mp[3].p, t_mp[2].p, t_mp[4].p, t_pmod.p,
+                          t_pcnt.p, t_comb.p, nmp, na)
+        sync()
+        st["unrank"] = host(t_unrank, np.uint32, lu)
+        t_mpoff = empty(8 * (S + 1))
+        check(L.hfx_mpep_offsets(self.ctx.h, t_mpoff.p, t_nmp.p, nv, t_vscan.p, S, None),
+              "hfx_mpep_offsets")
+        mp_off = host(t_mpoff, np.uint64, S + 1)
+        self.modset = HfxModset(t_mp[0].p, t_mp[1].p, t_mp[4].p, t_unrank.p, self.t_mod_cnt.p,
+                                self.t_mod_delta.p, self.t_ma_mass.p, na, nmp)
+        N = self.topn
+        t_scores = empty(4 * max(nmp, 1))
+        t_pos, t_ex = empty(4 * S * N), empty(8 * S * N)
+        t_f32, t_cnt, t_flag = empty(4 * S * N), empty(4 * S), empty(4 * S)
+        flagged = np.zeros(0, np.uint32)
+        if nmp:
+            check(L.hfx_xcorr_sparse_mod(self.ctx.h, t_scores.p, t_mpoff.p, nmp,
+                                         C.byref(self.modset), C.byref(db.desc),
+                                         C.byref(self.sp.desc), None), "hfx_xcorr_sparse_mod")
+            check(L.hfx_topn_mod(self.ctx.h, t_pos.p, t_ex.p, t_f32.p, t_cnt.p, t_flag.p, N,
+                                 self.margin, t_scores.p, t_mpoff.p, C.byref(self.modset),
+                                 C.byref(db.desc), C.byref(self.sp.desc), None), "hfx_topn_mod")
+            flags = host(t_flag, np.uint32, S)
+            flagged = np.nonzero(flags)[0].astype(np.uint32)
+            if len(flagged):
+                t_l = dev(flagged)
+                check(L.hfx_topn_exact_mod(self.ctx.h, t_pos.p, t_ex.p, t_f32.p, t_cnt.p, N, t_l.p,
+                                           len(flagged), t_mpoff.p, C.byref(self.modset),
+                                           C.byref(db.desc), C.byref(self.sp.desc), None),
+                      "hfx_topn_exact_mod")
+        else:
+            t_cnt.zero()
+        sync()
+        # keep the device arrays alive as long as the result references them
+        self._keep = (t_mp, t_unrank, t_pep, t_pmod, t_pcnt, t_nmp, t_comb, t_mpoff)
+        return ModResult(
+            spec_cand=spec_cand, num_valid=int(nv), num_mpep=int(nmp), len_unrank=int(lu),
+            mp_off=mp_off, top_pos=host(t_pos, np.uint32, S * N).reshape(S, N),
+            top_exact=host(t_ex, np.float64, S * N).reshape(S, N),
+            top_f32=host(t_f32, np.float32, S * N).reshape(S, N),
+            top_cnt=host(t_cnt, np.uint32, S), flagged=flagged, stages=st,
+            scores=host(t_scores, np.float32, nmp) if keep_scores else None)
+
+
+def merge_rankings(plain: PlainResult, mods: ModResult, n: int):
+    """Search.hs:118-123: per spectrum the unmodified and modified matches
+    merged by score descending; on equal scores unmodified matches first, then
+    lower index.  Returns per spectrum a list of (exact score, is_mod, index)."""
+    out = []
+    for s in range(len(plain.top_cnt)):
+        items = [(float(plain.top_exact[s, k]), 0, int(plain.top_pos[s, k]))
+                 for k in range(int(plain.top_cnt[s]))]
+        items += [(float(mods.top_exact[s, k]), 1, int(mods.top_pos[s, k]))
+                  for k in range(int(mods.top_cnt[s]))]
+        items.sort(key=lambda t: (-t[0], t[1], t[2]))
+        out.append(items[:n])
+    return out

@@ -66,6 +66,21 @@ typedef struct hfx_spectra {
     uint32_t num_spec;
 } hfx_spectra;
 
+/* Modified peptides of a batch, as genModCands and its predecessors leave
+ * them (Search.hs:358-394): per modified peptide its peptide, modification
+ * and the offset of its unrank slice. */
+typedef struct hfx_modset {
+    const uint32_t *d_mpep_pep_idx;    /* [num_mpep]                            */
+    const uint32_t *d_mpep_mod_idx;    /* [num_mpep]                            */
+    const uint32_t *d_mpep_unrank_off; /* [num_mpep] mpep_mod_ma_count_sum_scan */
+    const uint32_t *d_unrank;          /* [len_unrank]                          */
+    const uint32_t *d_mod_ma_count;    /* [num_mod * num_ma]                    */
+    const float *d_mod_delta;          /* [num_mod]                             */
+    const float *d_ma_mass;            /* [num_ma]                              */
+    uint32_t num_ma;
+    uint64_t num_mpep;
+} hfx_modset;
+
 /* ------------------------------------------------------------------ */
 /* Runtime: replaces packages/cuda-0.4.1.0 (Runtime/Marshal.chs:126-376, */
 /* Runtime/Stream.chs:54-89, Runtime/Device.chs:267-273,                 */
@@ -268,6 +283,32 @@ int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float
                    uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_spec_list,
                    uint32_t num_list, const uint32_t *d_begin, const uint32_t *d_num,
                    const hfx_db *db, const hfx_spectra *sp, void *stream);
+
+/* Search.hs:381-386 on the device: spectrum s owns the modified peptides of
+ * rows [d_spec_num_valid_scan[s], d_spec_num_valid_scan[s+1]) (the last bound
+ * is num_rows), i.e. d_mp_off[s] = sum of d_pep_num_mpep over the rows before
+ * them; d_mp_off has num_spec + 1 entries. */
+int hfx_mpep_offsets(hfx_ctx *ctx, uint64_t *d_mp_off, const uint32_t *d_pep_num_mpep,
+                     uint32_t num_rows, const uint32_t *d_spec_num_valid_scan, uint32_t num_spec,
+                     void *stream);
+
+/* Sparse FP32 XCorr of modified peptides (replaces addModIons + sgemv,
+ * Search.hs:477-529): d_scores[d_mp_off[s] + k] for the k-th modified
+ * peptide of spectrum s. */
+int hfx_xcorr_sparse_mod(hfx_ctx *ctx, float *d_scores, const uint64_t *d_mp_off, uint64_t total,
+                         const hfx_modset *ms, const hfx_db *db, const hfx_spectra *sp,
+                         void *stream);
+
+/* hfx_topn / hfx_topn_exact over modified peptides; positions are indices
+ * within the spectrum's block of modified peptides (Search.hs:534-622). */
+int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                 uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
+                 const float *d_scores, const uint64_t *d_mp_off, const hfx_modset *ms,
+                 const hfx_db *db, const hfx_spectra *sp, void *stream);
+int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                       uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_spec_list,
+                       uint32_t num_list, const uint64_t *d_mp_off, const hfx_modset *ms,
+                       const hfx_db *db, const hfx_spectra *sp, void *stream);
 
 #ifdef __cplusplus
 }

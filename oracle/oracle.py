@@ -255,6 +255,88 @@ def preprocess_batch(pcr, charge, mz, inten, off, stride, mean_mode=1, remove_pr
     return X, lens, status
 
 
+def find_modable_peptides(spec_num_cand_scan, num_cand_total, ions_packed, tc, tn, idx_sorted,
+                          begin, end, num_pep_scan, mod_ma_count, ma: bytes):
+    S = len(spec_num_cand_scan) - 1
+    na = len(ma)
+    M = len(mod_ma_count)
+    n = int(num_cand_total)
+    pep = np.zeros(max(n, 1), np.uint32)
+    mod = np.zeros(max(n, 1), np.uint32)
+    cnt = np.zeros(max(n * na, 1), np.uint32)
+    vs = np.zeros(max(S, 1), np.uint32)
+    mab = np.frombuffer(ma, np.uint8).copy()
+    nv = lib().orc_find_modable_peptides(
+        _p(pep), _p(mod), _p(cnt), _p(vs), _p(u32(spec_num_cand_scan)), C.c_uint32(S),
+        C.c_uint32(n), _p(np.ascontiguousarray(ions_packed, np.uint8)), _p(u32(tc)), _p(u32(tn)),
+        _p(u32(idx_sorted)), _p(u32(begin)), _p(u32(end)), _p(u32(num_pep_scan)),
+        _p(u32(np.asarray(mod_ma_count).reshape(-1))), C.c_uint32(M), _p(mab), C.c_uint32(na))
+    return int(nv), pep[:nv].copy(), mod[:nv].copy(), cnt[: nv * na].copy(), vs[:S].copy()
+
+
+def calc_total_mod_cands(mod_ma_count, pep_idx, pep_mod_idx, pep_ma_count, num_ma):
+    n = len(pep_idx)
+    num = np.zeros(max(n, 1), np.uint32)
+    comb = np.zeros(max(n * num_ma, 1), np.uint32)
+    tot = lib().orc_calc_total_mod_cands(
+        _p(num), _p(comb), _p(u32(np.asarray(mod_ma_count).reshape(-1))), _p(u32(pep_idx)),
+        _p(u32(pep_mod_idx)), _p(u32(pep_ma_count)), C.c_uint32(n),
+        C.c_uint32(len(mod_ma_count)), C.c_uint32(num_ma))
+    return int(tot), num[:n].copy(), comb[: n * num_ma].copy()
+
+
+def prepare_gen_mod(mod_sum, pep_idx, pep_mod_idx, pep_num_mpep, num_mpep):
+    m = int(num_mpep)
+    outs = [np.zeros(max(m, 1), np.uint32) for _ in range(5)]
+    lu = lib().orc_prepare_gen_mod(*[_p(o) for o in outs], _p(u32(mod_sum)), _p(u32(pep_idx)),
+                                   _p(u32(pep_mod_idx)), _p(u32(pep_num_mpep)),
+                                   C.c_uint32(len(pep_idx)), C.c_uint32(m))
+    return (int(lu),) + tuple(o[:m].copy() for o in outs)
+
+
+def gen_mod_cands(len_unrank, mod_ma_count, mpep_ith, mpep_rank, mpep_scan, pep_mod_idx,
+                  pep_ma_count, comb_scan, num_ma):
+    out = np.zeros(max(int(len_unrank), 1), np.uint32)
+    lib().orc_gen_mod_cands(_p(out), _p(u32(np.asarray(mod_ma_count).reshape(-1))),
+                            _p(u32(mpep_ith)), _p(u32(mpep_rank)), _p(u32(mpep_scan)),
+                            _p(u32(pep_mod_idx)), _p(u32(pep_ma_count)), _p(u32(comb_scan)),
+                            C.c_uint32(len(mpep_ith)), C.c_uint32(num_ma))
+    return out[: int(len_unrank)].copy()
+
+
+def search_mods(ions_packed, r, tc, tn, mass26, idx_sorted, ma: bytes, ma_mass, mod_ma_count,
+                mod_sum, mod_delta, X, lens, zmax, masses, eps, topn, nthreads=0,
+                cap_mpep=1 << 24, cap_unrank=1 << 26):
+    S, stride = X.shape
+    X = np.ascontiguousarray(X, np.float32)
+    na = len(ma)
+    snm = np.zeros(S, np.uint32)
+    top = np.zeros(S * topn, np.uint32)
+    fx = np.zeros(S * topn, np.int64)
+    cnt = np.zeros(S, np.uint32)
+    mp_pep = np.zeros(cap_mpep, np.uint32)
+    mp_mod = np.zeros(cap_mpep, np.uint32)
+    mp_uoff = np.zeros(cap_mpep, np.uint32)
+    unrank = np.zeros(cap_unrank, np.uint32)
+    nmp = C.c_uint64()
+    lu = C.c_uint64()
+    mab = np.frombuffer(ma, np.uint8).copy()
+    rc = lib().orc_search_mods(
+        _p(np.ascontiguousarray(ions_packed, np.uint8)), _p(f32(r)), _p(u32(tc)), _p(u32(tn)),
+        _p(f32(mass26)), _p(u32(idx_sorted)), C.c_uint32(len(r)), _p(mab), _p(f32(ma_mass)),
+        C.c_uint32(na), _p(u32(np.asarray(mod_ma_count).reshape(-1))), _p(u32(mod_sum)),
+        _p(f32(mod_delta)), C.c_uint32(len(mod_delta)), _p(X), C.c_uint32(stride), _p(u32(lens)),
+        _p(u32(zmax)), _p(f32(masses)), C.c_uint32(S), C.c_float(eps), C.c_uint32(topn),
+        _p(snm), _p(top), _p(fx), _p(cnt), _p(mp_pep), _p(mp_mod), _p(mp_uoff), _p(unrank),
+        C.c_uint64(cap_mpep), C.c_uint64(cap_unrank), C.byref(nmp), C.byref(lu),
+        C.c_int(nthreads))
+    assert rc == 0
+    m, l = nmp.value, lu.value
+    return dict(spec_num_mpep=snm, top_pos=top.reshape(S, topn), top_fx=fx.reshape(S, topn),
+                top_cnt=cnt, mpep_pep=mp_pep[:m].copy(), mpep_mod=mp_mod[:m].copy(),
+                mpep_unrank_off=mp_uoff[:m].copy(), unrank=unrank[:l].copy())
+
+
 def search_plain(ions_packed, r, tc, tn, mass26, idx_sorted, X, lens, zmax, masses, eps, topn,
                  want_all=False, nthreads=0):
     S, stride = X.shape
