@@ -164,6 +164,16 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_batch_create": (_I, [_P, C.POINTER(_P)]),
+    "hfx_batch_destroy": (_I, [_P]),
+    "hfx_batch_plan": (_I, [_P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _F, _U32, _U32, _P]),
+    "hfx_batch_run": (_I, [_P, _P]),
+    "hfx_batch_capture": (_I, [_P, _P]),
+    "hfx_batch_replay": (_I, [_P, _P]),
+    "hfx_batch_autotune": (_I, [_P, _P, _P]),
+    "hfx_batch_set_config": (_I, [_P, _U32, _U32, _U32]),
+    "hfx_batch_info": (_I, [_P, _P]),
+    "hfx_batch_outputs": (_I, [_P, _P]),
 }
 
 # symbols that may be absent from a partial build (none at present)

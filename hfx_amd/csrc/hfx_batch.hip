@@ -1,0 +1,650 @@
+/*
+ * hfx_batch.hip -- batched search: peptide ion index + persistent
+ * work-stealing multi-spectrum XCorr, hipGraph capture, autotuner.
+ *
+ * The reference scores each spectrum on its own: every (spectrum, candidate)
+ * pair rebuilds the candidate's theoretical spectrum (Search.hs:177-245).  A
+ * candidate's peaks depend only on the peptide and the fragment charges, and
+ * neighbouring precursors share most of their window, so a batch of thousands
+ * of spectra here
+ *   1. builds, once per batch, the sparse theoretical spectrum of every
+ *      peptide any window touches, for fragment charge levels 1..L (level l =
+ *      charges 1..l, max per bin): sorted uint16 entries bin | code << 14
+ *      (k_ix_build, one wave per peptide, from the same ladder/bin code as the
+ *      plain path);
+ *   2. sorts the spectra by precursor mass and groups G consecutive spectra
+ *      into blocks; a tile is (block, C consecutive candidates of the block's
+ *      union window); a persistent kernel pulls tiles from an atomic counter
+ *      (work stealing: heavy-tailed windows are cut into equal tiles), keeps
+ *      the block's G spectra in LDS, streams each candidate's entries once
+ *      (coalesced) and gathers every spectrum of the block that holds the
+ *      candidate in its window, reducing the G partial sums with one
+ *      reduce-scatter butterfly;
+ *   3. selects the top N with the exact re-score of hfx_topn (flagged spectra
+ *      completed in the same launch sequence, no host round trip).
+ * Scores land in the same layout as hfx_xcorr_sparse, so every result is
+ * checked against the same oracle.  Once planned, a batch runs without host
+ * synchronisation or allocation and can be captured in a hipGraph.
+ */
+#include "hfx_cand.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <tuple>
+#include <vector>
+
+#define IX_THREADS 256
+#define IX_WAVES (IX_THREADS / HFX_WAVE)
+#define BT_GMAX 8
+
+/* ------------------------------------------------------------------ */
+/* Ion index                                                             */
+/* ------------------------------------------------------------------ */
+
+/* capacity of position q at level l: every emitted peak fills at most one
+ * bin, 10 peaks per ladder position and charge, and no more than lcap bins */
+__global__ void k_ix_caps(uint32_t *caps, hfx_db db, uint32_t p0, uint32_t P, uint32_t L,
+                          uint32_t lcap)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P) return;
+    const uint32_t pep = db.d_idx_sorted[p0 + q];
+    const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
+    const uint32_t n = tn > tc ? tn - tc : 0;
+    for (uint32_t l = 1; l <= L; ++l) caps[(size_t)q * L + l - 1] = min(10u * n * l, lcap);
+}
+
+/* compact the map (bins ascending) into entries; returns the count */
+__device__ __forceinline__ uint32_t wave_compact(const uint32_t *map, uint32_t nw, uint16_t *out)
+{
+    const uint32_t lane = hfx_lane();
+    uint32_t base = 0;
+    for (uint32_t w0 = 0; w0 < nw; w0 += HFX_WAVE) {
+        const uint32_t w = w0 + lane;
+        const uint32_t word = w < nw ? map[w] : 0u;
+        const uint32_t nz = (word | (word >> 1) | (word >> 2) | (word >> 3)) & 0x11111111u;
+        const uint32_t c = (uint32_t)__builtin_popcount(nz);
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        uint32_t pos = base + incl - c;
+        uint32_t wd = word;
+        while (wd) {
+            const uint32_t sh = __builtin_ctz(wd) & ~3u;
+            const uint32_t nib = (wd >> sh) & 0xFu;
+            wd &= ~(0xFu << sh);
+            out[pos++] = (uint16_t)(((w << 3) + (sh >> 2)) | ((uint32_t)__builtin_popcount(nib) << 14));
+        }
+        base += (uint32_t)__shfl(incl, 63, 64);
+    }
+    return base;
+}
+
+__global__ void __launch_bounds__(IX_THREADS)
+    k_ix_build(uint16_t *ent, uint32_t *cnt, const uint64_t *off, hfx_db db, uint32_t p0, uint32_t P,
+               uint32_t L, uint32_t lcap)
+{
+    __shared__ uint32_t maps[IX_WAVES][HFX_MAP_WORDS];
+    __shared__ float mtab[32];
+    const uint32_t wave = threadIdx.x / HFX_WAVE, lane = hfx_lane();
+    uint32_t *map = maps[wave];
+    if (threadIdx.x < 32) mtab[threadIdx.x] = threadIdx.x < 26 ? db.d_mass[threadIdx.x] : 0.0f;
+    for (uint32_t w = lane; w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
+    __syncthreads();
+    const uint32_t nw = (lcap + 7) >> 3;
+    for (uint32_t q = blockIdx.x * IX_WAVES + wave; q < P; q += gridDim.x * IX_WAVES) {
+        const uint32_t pep = db.d_idx_sorted[p0 + q];
+        const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
+        const uint32_t n = tn > tc ? tn - tc : 0;
+        const float r = db.d_r[pep];
+        for (uint32_t l = 1; l <= L; ++l) {
+            /* level l = charges 1..l: add charge l to the map of level l-1 */
+            hfx_wave_peaks_z(map, n, r, l, l, lcap, HfxPlainMass{db.d_ions + tc, mtab});
+            hfx_wave_sync();
+            const size_t k = (size_t)q * L + l - 1;
+            const uint32_t c = wave_compact(map, nw, ent + off[k]);
+            if (lane == 0) cnt[k] = c;
+            hfx_wave_sync();
+        }
+        for (uint32_t w = lane; w < nw; w += HFX_WAVE) map[w] = 0;
+        hfx_wave_sync();
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Blocks, tiles                                                          */
+/* ------------------------------------------------------------------ */
+
+__global__ void k_bt_blocks(uint32_t *ubeg, uint32_t *uend, uint32_t *ntiles, const uint32_t *order,
+                            const uint32_t *begin, const uint32_t *num, uint32_t S, uint32_t G,
+                            uint32_t B, uint32_t C)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        const uint32_t i = b * G + g;
+        if (i >= S) break;
+        const uint32_t s = order[i];
+        if (num[s] == 0) continue;
+        lo = min(lo, begin[s]);
+        hi = max(hi, begin[s] + num[s]);
+    }
+    if (hi <= lo) { lo = 0; hi = 0; }
+    ubeg[b] = lo;
+    uend[b] = hi;
+    ntiles[b] = (hi - lo + C - 1) / C;
+}
+
+__global__ void k_bt_reset(uint32_t *counter) { *counter = 0; }
+
+/* zmax > 4: the reference launches nothing and the zero rows score 0 */
+__global__ void k_bt_zero_unscored(float *scores, const uint64_t *off, const uint32_t *zmax,
+                                   uint32_t S)
+{
+    const uint32_t s = blockIdx.x;
+    if (s >= S || hfx_zeff(zmax[s]) != 0) return;
+    for (uint64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x) scores[i] = 0.0f;
+}
+
+struct BtArgs {
+    float *scores;
+    const uint64_t *score_off;
+    const uint32_t *begin, *num, *order;
+    uint32_t S, G, B, C;
+    const uint32_t *ubeg, *uend;
+    const uint64_t *tile_off;
+    const uint16_t *ent;
+    const uint32_t *cnt;
+    const uint64_t *ent_off;
+    uint32_t p0, P, L, lstr;
+    uint32_t *counter;
+    hfx_spectra sp;
+};
+
+__device__ __forceinline__ float code_w(uint32_t c)
+{
+    return c == 3 ? 50.0f : (c == 2 ? 25.0f : (c == 1 ? 10.0f : 0.0f));
+}
+
+/* Sum each of the G accumulators over the wave: a reduce-scatter butterfly
+ * (halving the accumulators per lane at each of the first log2 G steps) then
+ * a plain butterfly.  Lane l ends with the sum of accumulator
+ * g = bit-reverse-free index ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1)
+ * for G = 8 (first bits only for smaller G). */
+template <int G>
+__device__ __forceinline__ float reduce_scatter(float (&acc)[G], uint32_t lane)
+{
+    if constexpr (G == 8) {
+        const bool up = lane & 32;
+        float a0 = acc[0] + __shfl_xor(acc[4], 32, 64), a1 = acc[1] + __shfl_xor(acc[5], 32, 64);
+        float a2 = acc[2] + __shfl_xor(acc[6], 32, 64), a3 = acc[3] + __shfl_xor(acc[7], 32, 64);
+        float b0 = acc[4] + __shfl_xor(acc[0], 32, 64), b1 = acc[5] + __shfl_xor(acc[1], 32, 64);
+        float b2 = acc[6] + __shfl_xor(acc[2], 32, 64), b3 = acc[7] + __shfl_xor(acc[3], 32, 64);
+        float c0 = up ? b0 : a0, c1 = up ? b1 : a1, c2 = up ? b2 : a2, c3 = up ? b3 : a3;
+        const bool u16 = lane & 16;
+        float d0 = c0 + __shfl_xor(c2, 16, 64), d1 = c1 + __shfl_xor(c3, 16, 64);
+        float e0 = c2 + __shfl_xor(c0, 16, 64), e1 = c3 + __shfl_xor(c1, 16, 64);
+        float f0 = u16 ? e0 : d0, f1 = u16 ? e1 : d1;
+        const bool u8 = lane & 8;
+        float h0 = f0 + __shfl_xor(f1, 8, 64), h1 = f1 + __shfl_xor(f0, 8, 64);
+        float v = u8 ? h1 : h0;
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 1, 64);
+        return v;
+    } else {
+        static_assert(G == 4, "G must be 4 or 8");
+        const bool up = lane & 32;
+        float a0 = acc[0] + __shfl_xor(acc[2], 32, 64), a1 = acc[1] + __shfl_xor(acc[3], 32, 64);
+        float b0 = acc[2] + __shfl_xor(acc[0], 32, 64), b1 = acc[3] + __shfl_xor(acc[1], 32, 64);
+        float c0 = up ? b0 : a0, c1 = up ? b1 : a1;
+        const bool u16 = lane & 16;
+        float d0 = c0 + __shfl_xor(c1, 16, 64), d1 = c1 + __shfl_xor(c0, 16, 64);
+        float v = u16 ? d1 : d0;
+        v += __shfl_xor(v, 8, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 1, 64);
+        return v;
+    }
+}
+
+template <int G>
+__device__ __forceinline__ uint32_t owner_of(uint32_t lane)
+{
+    if constexpr (G == 8) return ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    else return ((lane >> 5) & 1) * 2 + ((lane >> 4) & 1);
+}
+
+template <int G>
+__global__ void __launch_bounds__(512) k_batch(BtArgs a)
+{
+    extern __shared__ float bx[]; /* G spectra, stride lstr */
+    __shared__ uint32_t m_s[BT_GMAX], m_beg[BT_GMAX], m_end[BT_GMAX], m_z[BT_GMAX], m_len[BT_GMAX];
+    __shared__ uint64_t m_off[BT_GMAX];
+    __shared__ uint32_t sh_t, sh_blk, sh_lblk;
+    const uint32_t tid = threadIdx.x, lane = hfx_lane();
+    const uint32_t nwaves = blockDim.x / HFX_WAVE, wave = tid / HFX_WAVE;
+    const uint64_t ntiles = a.tile_off[a.B];
+    uint32_t cur = 0xffffffffu;
+    for (;;) {
+        if (tid == 0) sh_t = atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint64_t t = sh_t;
+        if (t >= ntiles) break;
+        /* block of the tile: largest b with tile_off[b] <= t */
+        uint32_t lo = 0, hi = a.B;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.tile_off[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t b = lo;
+        if (b != cur) {
+            /* stage the block's spectra (the previous tile's readers are done:
+             * the barrier above follows every wave's last read) */
+            if (tid < BT_GMAX) {
+                const uint32_t i = b * a.G + tid;
+                uint32_t s = 0xffffffffu, be = 0, en = 0, z = 0, ln = 0;
+                uint64_t of = 0;
+                if (tid < a.G && i < a.S) {
+                    s = a.order[i];
+                    be = a.begin[s];
+                    en = be + a.num[s];
+                    z = hfx_zeff(a.sp.d_zmax[s]);
+                    ln = min(a.sp.d_len[s], min(a.lstr, a.sp.x_stride));
+                    of = a.score_off[s];
+                }
+                m_s[tid] = s; m_beg[tid] = be; m_end[tid] = en; m_z[tid] = z; m_len[tid] = ln;
+                m_off[tid] = of;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t lb = 0;
+                for (int g = 0; g < G; ++g) lb = max(lb, m_z[g]);
+                sh_lblk = min(lb, a.L);
+            }
+            for (int g = 0; g < G; ++g) {
+                const uint32_t s = m_s[g];
+                if (s == 0xffffffffu) continue;
+                const float *xg = a.sp.d_x + (size_t)s * a.sp.x_stride;
+                for (uint32_t i = tid; i < m_len[g]; i += blockDim.x) bx[g * a.lstr + i] = xg[i];
+            }
+            cur = b;
+            __syncthreads();
+        }
+        const uint32_t lblk = sh_lblk;
+        const uint32_t chunk = (uint32_t)(t - a.tile_off[b]);
+        const uint32_t pa = a.ubeg[b] + chunk * a.C;
+        const uint32_t pb = min(pa + a.C, a.uend[b]);
+        for (uint32_t p = pa + wave; p < pb; p += nwaves) {
+            const uint32_t q = p - a.p0;
+            for (uint32_t l = 1; l <= lblk; ++l) {
+                bool act[G];
+                bool any = false;
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    act[g] = m_z[g] == l && m_beg[g] <= p && p < m_end[g];
+                    any |= act[g];
+                }
+                if (!any) continue;
+                const size_t k = (size_t)q * a.L + l - 1;
+                const uint16_t *e = a.ent + a.ent_off[k];
+                const uint32_t ne = a.cnt[k];
+                float acc[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) acc[g] = 0.0f;
+                for (uint32_t j = lane; j < ne; j += HFX_WAVE) {
+                    const uint32_t v = e[j];
+                    const uint32_t bin = v & 0x3fffu;
+                    const float w = code_w(v >> 14);
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+                        if (act[g] && bin < m_len[g]) acc[g] = acc[g] + w * bx[g * a.lstr + bin];
+                }
+                const float sum = reduce_scatter<G>(acc, lane);
+                const uint32_t g = owner_of<G>(lane);
+                if ((lane & 7) == 0 && act[g])
+                    a.scores[m_off[g] + (p - m_beg[g])] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* The batch object                                                       */
+/* ------------------------------------------------------------------ */
+
+struct hfx_batch {
+    hfx_ctx *ctx;
+    hfx_db db;
+    hfx_spectra sp;
+    float eps;
+    uint32_t N, margin;
+    uint32_t S, G, C, threads, B, L, lstr, lcap, p0, P;
+    uint64_t total, n_ent, tiles;
+    bool planned;
+    /* device buffers (owned) */
+    uint32_t *begin, *num, *order, *ubeg, *uend, *ntiles, *caps, *cnt, *counter;
+    uint64_t *score_off, *tile_off, *ent_off;
+    uint16_t *ent;
+    float *scores;
+    uint32_t *top_pos, *top_cnt, *flag;
+    double *top_exact;
+    float *top_f32;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+static void bt_free(hfx_batch *b)
+{
+    void *ps[] = {b->begin,    b->num,     b->order,  b->ubeg,      b->uend,    b->ntiles,
+                  b->caps,     b->cnt,     b->counter, b->score_off, b->tile_off, b->ent_off,
+                  b->ent,      b->scores,  b->top_pos, b->top_cnt,   b->flag,     b->top_exact,
+                  b->top_f32};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    hfx_ctx *ctx = b->ctx;
+    memset(b, 0, sizeof *b);
+    b->ctx = ctx;
+}
+
+extern "C" int hfx_batch_create(hfx_ctx *ctx, hfx_batch **out)
+{
+    HFX_REQUIRE(ctx && out, HFX_ERR_ARG, "hfx_batch_create: null argument");
+    hfx_batch *b = (hfx_batch *)calloc(1, sizeof(hfx_batch));
+    HFX_REQUIRE(b, HFX_ERR_NOMEM, "hfx_batch_create: out of host memory");
+    b->ctx = ctx;
+    *out = b;
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_destroy(hfx_batch *b)
+{
+    if (!b) return HFX_OK;
+    (void)hipDeviceSynchronize();
+    bt_free(b);
+    free(b);
+    return HFX_OK;
+}
+
+template <class T>
+static int dalloc(T **p, size_t n)
+{
+    hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        hfx_set_error("batch allocation of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+        return HFX_ERR_NOMEM;
+    }
+    return HFX_OK;
+}
+
+/* Shape-bucketed autotuner results of this process: (S bucket, P bucket, L)
+ * -> (G, C, threads). */
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t>> g_tune;
+
+static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
+{
+    auto lg = [](uint64_t v) { int k = 0; while (v > 1) { v >>= 1; ++k; } return k; };
+    return std::make_tuple(lg(b->S), lg(b->P), b->L);
+}
+
+extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float eps,
+                              uint32_t N, uint32_t margin, void *stream)
+{
+    HFX_REQUIRE(b && db && sp, HFX_ERR_ARG, "hfx_batch_plan: null argument");
+    HFX_REQUIRE(N >= 1 && N + margin <= 128 && N <= 64, HFX_ERR_ARG,
+                "hfx_batch_plan: N + margin must be <= 128 and N <= 64");
+    hipStream_t st = hfx_stream(stream);
+    bt_free(b);
+    b->db = *db;
+    b->sp = *sp;
+    b->eps = eps;
+    b->N = N;
+    b->margin = margin;
+    const uint32_t S = b->S = sp->num_spec;
+    int rc;
+    if ((rc = dalloc(&b->begin, S)) || (rc = dalloc(&b->num, S)) ||
+        (rc = dalloc(&b->score_off, (size_t)S + 1)) || (rc = dalloc(&b->order, S)))
+        return rc;
+    if ((rc = hfx_window(b->ctx, b->begin, b->num, db, sp->d_mass, S, eps, stream))) return rc;
+    if ((rc = hfx_scan_offsets(b->ctx, b->score_off, b->num, S, stream))) return rc;
+    /* host view of the windows, masses and charges (plan only) */
+    std::vector<uint32_t> hb(S), hn(S), hz(S), hl(S);
+    std::vector<float> hm(S);
+    HFX_CHECK_HIP(hipMemcpyAsync(hb.data(), b->begin, 4ull * S, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipMemcpyAsync(hn.data(), b->num, 4ull * S, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipMemcpyAsync(hz.data(), sp->d_zmax, 4ull * S, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipMemcpyAsync(hl.data(), sp->d_len, 4ull * S, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipMemcpyAsync(hm.data(), sp->d_mass, 4ull * S, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipMemcpyAsync(&b->total, b->score_off + S, 8, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipStreamSynchronize(st));
+    std::vector<uint32_t> ord(S);
+    std::iota(ord.begin(), ord.end(), 0u);
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return hm[x] < hm[y]; });
+    HFX_CHECK_HIP(hipMemcpyAsync(b->order, ord.data(), 4ull * S, hipMemcpyHostToDevice, st));
+    uint32_t p0 = 0xffffffffu, p1 = 0, L = 0, lmax = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        lmax = std::max(lmax, std::min(hl[s], std::min(sp->x_stride, (uint32_t)HFX_MAX_LEN)));
+        if (hn[s] == 0) continue;
+        p0 = std::min(p0, hb[s]);
+        p1 = std::max(p1, hb[s] + hn[s]);
+        uint32_t z = hz[s] <= HFX_MAX_CHARGE ? hz[s] : 0u;
+        L = std::max(L, z);
+    }
+    if (p1 <= p0) { p0 = 0; p1 = 0; }
+    b->p0 = p0;
+    b->P = p1 - p0;
+    b->L = std::max(L, 1u);
+    b->lcap = std::max(lmax, 8u);
+    b->lstr = (b->lcap + 15) & ~15u;
+    /* defaults, replaced by the autotuner's choice for this shape */
+    b->G = 8;
+    b->C = 64;
+    b->threads = 512;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(tune_key(b));
+        if (it != g_tune.end()) std::tie(b->G, b->C, b->threads) = it->second;
+    }
+    const uint32_t P = b->P, Lv = b->L;
+    /* index capacities and offsets */
+    if ((rc = dalloc(&b->caps, (size_t)P * Lv)) || (rc = dalloc(&b->ent_off, (size_t)P * Lv + 1)) ||
+        (rc = dalloc(&b->cnt, (size_t)P * Lv)))
+        return rc;
+    if (P) {
+        hipLaunchKernelGGL(k_ix_caps, dim3((P + 255) / 256), dim3(256), 0, st, b->caps, *db, p0, P,
+                           Lv, b->lcap);
+        HFX_REQUIRE((uint64_t)P * Lv < 0xffffffffull, HFX_ERR_ARG, "index too large");
+        if ((rc = hfx_scan_offsets(b->ctx, b->ent_off, b->caps, P * Lv, stream))) return rc;
+        HFX_CHECK_HIP(hipMemcpyAsync(&b->n_ent, b->ent_off + (size_t)P * Lv, 8,
+                                     hipMemcpyDeviceToHost, st));
+        HFX_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    if ((rc = dalloc(&b->ent, b->n_ent))) return rc;
+    if ((rc = dalloc(&b->scores, b->total)) || (rc = dalloc(&b->counter, 1)) ||
+        (rc = dalloc(&b->top_pos, (size_t)S * N)) || (rc = dalloc(&b->top_exact, (size_t)S * N)) ||
+        (rc = dalloc(&b->top_f32, (size_t)S * N)) || (rc = dalloc(&b->top_cnt, S)) ||
+        (rc = dalloc(&b->flag, S)))
+        return rc;
+    /* blocks are sized for the smallest G the tuner may pick (4) */
+    const uint32_t bmax = (S + 3) / 4;
+    if ((rc = dalloc(&b->ubeg, bmax)) || (rc = dalloc(&b->uend, bmax)) ||
+        (rc = dalloc(&b->ntiles, bmax)) || (rc = dalloc(&b->tile_off, (size_t)bmax + 1)))
+        return rc;
+    b->planned = true;
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+/* blocks + tiles for the current (G, C) */
+static int bt_tiles(hfx_batch *b, hipStream_t st)
+{
+    b->B = (b->S + b->G - 1) / b->G;
+    if (b->B == 0) return HFX_OK;
+    hipLaunchKernelGGL(k_bt_blocks, dim3((b->B + 255) / 256), dim3(256), 0, st, b->ubeg, b->uend,
+                       b->ntiles, b->order, b->begin, b->num, b->S, b->G, b->B, b->C);
+    return hfx_scan_offsets(b->ctx, b->tile_off, b->ntiles, b->B, (void *)st);
+}
+
+static int bt_score(hfx_batch *b, hipStream_t st)
+{
+    if (b->B == 0 || b->total == 0) return HFX_OK;
+    hipLaunchKernelGGL(k_bt_reset, dim3(1), dim3(1), 0, st, b->counter);
+    BtArgs a{b->scores, b->score_off, b->begin, b->num,  b->order,   b->S,     b->G,
+             b->B,      b->C,         b->ubeg,  b->uend, b->tile_off, b->ent,  b->cnt,
+             b->ent_off, b->p0,       b->P,     b->L,    b->lstr,    b->counter, b->sp};
+    const size_t smem = (size_t)b->G * b->lstr * 4;
+    const int waves_per_wg = (int)(b->threads / HFX_WAVE);
+    (void)waves_per_wg;
+    const uint32_t grid = (uint32_t)hfx_ctx_num_cus(b->ctx) * (b->G == 8 ? 2u : 4u) *
+                          (b->threads == 256 ? 2u : 1u);
+    if (b->G == 8)
+        hipLaunchKernelGGL(k_batch<8>, dim3(grid), dim3(b->threads), smem, st, a);
+    else
+        hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(b->threads), smem, st, a);
+    hipLaunchKernelGGL(k_bt_zero_unscored, dim3(b->S), dim3(256), 0, st, b->scores, b->score_off,
+                       b->sp.d_zmax, b->S);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+int hfx_topn_gated(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32, uint32_t *d_top_cnt,
+                   uint32_t *d_flag, uint32_t N, uint32_t margin, const float *d_scores,
+                   const uint64_t *d_off, const uint32_t *d_begin, const uint32_t *d_num,
+                   const hfx_db *db, const hfx_spectra *sp, void *stream);
+
+extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
+{
+    HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_run: batch not planned");
+    hipStream_t st = hfx_stream(stream);
+    int rc;
+    if (b->S == 0) return HFX_OK;
+    if ((rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, stream)))
+        return rc;
+    if ((rc = hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, stream))) return rc;
+    if (b->P) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((b->P + IX_WAVES - 1) / IX_WAVES,
+                                                           (uint64_t)hfx_ctx_num_cus(b->ctx) * 16);
+        hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, b->ent, b->cnt, b->ent_off,
+                           b->db, b->p0, b->P, b->L, b->lcap);
+    }
+    if ((rc = bt_tiles(b, st))) return rc;
+    if ((rc = bt_score(b, st))) return rc;
+    return hfx_topn_gated(b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag, b->N,
+                          b->margin, b->scores, b->score_off, b->begin, b->num, &b->db, &b->sp,
+                          stream);
+}
+
+extern "C" int hfx_batch_capture(hfx_batch *b, void *stream)
+{
+    HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_capture: batch not planned");
+    HFX_REQUIRE(stream != nullptr, HFX_ERR_ARG, "hfx_batch_capture: needs a non-null stream");
+    hipStream_t st = hfx_stream(stream);
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    HFX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = hfx_batch_run(b, stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    HFX_REQUIRE(rc == HFX_OK, rc, "hfx_batch_capture: run failed during capture: %s",
+                hfx_last_error());
+    HFX_CHECK_HIP(e);
+    b->graph = g;
+    HFX_CHECK_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_replay(hfx_batch *b, void *stream)
+{
+    HFX_REQUIRE(b && b->exec, HFX_ERR_ARG, "hfx_batch_replay: no captured graph");
+    HFX_CHECK_HIP(hipGraphLaunch(b->exec, hfx_stream(stream)));
+    return HFX_OK;
+}
+
+/* Times the scoring kernel of the planned batch for each (G, C, threads) in
+ * a small grid and keeps the fastest; the choice is remembered for batches of
+ * the same shape bucket in this process. */
+extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
+{
+    HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_autotune: batch not planned");
+    hipStream_t st = hfx_stream(stream);
+    int rc;
+    /* index and windows once */
+    if ((rc = hfx_batch_run(b, stream))) return rc;
+    hipEvent_t e0, e1;
+    HFX_CHECK_HIP(hipEventCreate(&e0));
+    HFX_CHECK_HIP(hipEventCreate(&e1));
+    const uint32_t Gs[2] = {4, 8}, Cs[4] = {32, 64, 128, 256}, Ts[2] = {256, 512};
+    float best = 1e30f;
+    uint32_t bG = b->G, bC = b->C, bT = b->threads;
+    for (uint32_t G : Gs)
+        for (uint32_t C : Cs)
+            for (uint32_t T : Ts) {
+                b->G = G;
+                b->C = C;
+                b->threads = T;
+                if ((rc = bt_tiles(b, st))) return rc;
+                if ((rc = bt_score(b, st))) return rc; /* warm */
+                HFX_CHECK_HIP(hipEventRecord(e0, st));
+                for (int k = 0; k < 2; ++k)
+                    if ((rc = bt_score(b, st))) return rc;
+                HFX_CHECK_HIP(hipEventRecord(e1, st));
+                HFX_CHECK_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                HFX_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; bG = G; bC = C; bT = T; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->G = bG;
+    b->C = bC;
+    b->threads = bT;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[tune_key(b)] = std::make_tuple(bG, bC, bT);
+    }
+    if (out_cfg) { out_cfg[0] = bG; out_cfg[1] = bC; out_cfg[2] = bT; }
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads)
+{
+    HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_config: batch not planned");
+    HFX_REQUIRE((G == 4 || G == 8) && C >= 1 && C <= 4096 && (threads == 256 || threads == 512),
+                HFX_ERR_ARG, "hfx_batch_set_config: G in {4,8}, C in 1..4096, threads in {256,512}");
+    b->G = G;
+    b->C = C;
+    b->threads = threads;
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_info(const hfx_batch *b, uint64_t *info /*[10]*/)
+{
+    HFX_REQUIRE(b && info, HFX_ERR_ARG, "hfx_batch_info: null argument");
+    info[0] = b->S; info[1] = b->total; info[2] = b->P; info[3] = b->L; info[4] = b->n_ent;
+    info[5] = b->G; info[6] = b->C; info[7] = b->threads; info[8] = b->lstr; info[9] = b->p0;
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_outputs(const hfx_batch *b, void **ptrs /*[10]*/)
+{
+    HFX_REQUIRE(b && ptrs, HFX_ERR_ARG, "hfx_batch_outputs: null argument");
+    ptrs[0] = b->top_pos; ptrs[1] = b->top_exact; ptrs[2] = b->top_f32; ptrs[3] = b->top_cnt;
+    ptrs[4] = b->flag; ptrs[5] = b->scores; ptrs[6] = b->score_off; ptrs[7] = b->begin;
+    ptrs[8] = b->num; ptrs[9] = b->ent;
+    return HFX_OK;
+}

@@ -171,13 +171,22 @@ __device__ __forceinline__ uint32_t hfx_lane() { return __lane_id(); }
  * of residue c0 + j for j < cn and 0 otherwise.  The b ladder is the
  * sequential FP32 prefix, reproduced exactly by broadcasting each residue mass
  * to every lane and letting lane k add masses 0..k in order.  For each ladder
- * position and each charge 1..zmax the peaks of addIonsAB/addIonsY
+ * position and each charge zlo..zmax the peaks of addIonsAB/addIonsY
  * (ion_series.cu:111-156) are OR-ed into `map` (bins >= len dropped, max per
  * bin through the unary weight code).
  */
 template <class MassFn>
+__device__ __forceinline__ void hfx_wave_peaks_z(uint32_t *map, uint32_t n, float r, uint32_t zlo,
+                                                 uint32_t zmax, uint32_t len, MassFn &&mass_of);
+template <class MassFn>
 __device__ __forceinline__ void hfx_wave_peaks(uint32_t *map, uint32_t n, float r,
                                                uint32_t zmax, uint32_t len, MassFn &&mass_of)
+{
+    hfx_wave_peaks_z(map, n, r, 1u, zmax, len, mass_of);
+}
+template <class MassFn>
+__device__ __forceinline__ void hfx_wave_peaks_z(uint32_t *map, uint32_t n, float r, uint32_t zlo,
+                                                 uint32_t zmax, uint32_t len, MassFn &&mass_of)
 {
     const uint32_t lane = hfx_lane();
     float carry = 0.0f;
@@ -192,7 +201,7 @@ __device__ __forceinline__ void hfx_wave_peaks(uint32_t *map, uint32_t n, float 
         carry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), cn - 1));
         if (lane < cn) {
             const float y = r - b;
-            for (uint32_t z = 1; z <= zmax; ++z) {
+            for (uint32_t z = zlo; z <= zmax; ++z) {
                 const float fz = (float)z;
                 const float zh = HFX_DEV_H * fz;
                 int32_t x;

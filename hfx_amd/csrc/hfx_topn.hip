@@ -216,8 +216,10 @@ __global__ void __launch_bounds__(TN_THREADS)
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn_exact(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
-                 uint32_t N, const uint32_t *list, Cands c, hfx_spectra sp)
+                 uint32_t N, const uint32_t *list, const uint32_t *gate, Cands c, hfx_spectra sp)
 {
+    /* list == nullptr: block b is spectrum b; gate: only spectra flagged there */
+    if (gate && !gate[list ? list[blockIdx.x] : blockIdx.x]) return;
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
     __shared__ float mtab[32];
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(TN_THREADS)
     __shared__ uint32_t lpos[TN_WAVES * TX_MAXN];
     __shared__ float lf32[TN_WAVES * TX_MAXN];
     __shared__ uint32_t lcnt[TN_WAVES];
-    const uint32_t s = list[blockIdx.x];
+    const uint32_t s = list ? list[blockIdx.x] : blockIdx.x;
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
     const uint32_t n = c.count(s);
     const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
@@ -318,7 +320,27 @@ static int launch_topn_exact(uint32_t *d_top_pos, double *d_top_exact, float *d_
                 TX_MAXN);
     if (num_list == 0) return HFX_OK;
     hipLaunchKernelGGL(k_topn_exact<Cands>, dim3(num_list), dim3(TN_THREADS), 0, hfx_stream(stream),
-                       d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_list, c, *sp);
+                       d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_list,
+                       (const uint32_t *)nullptr, c, *sp);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+/* hfx_topn followed, in the same stream and without a host round trip, by
+ * the exact fallback for the spectra it flags (used by the batched path). */
+int hfx_topn_gated(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32, uint32_t *d_top_cnt,
+                   uint32_t *d_flag, uint32_t N, uint32_t margin, const float *d_scores,
+                   const uint64_t *d_off, const uint32_t *d_begin, const uint32_t *d_num,
+                   const hfx_db *db, const hfx_spectra *sp, void *stream)
+{
+    HfxPlainCands c{*db, d_begin, d_num, d_off};
+    int rc = launch_topn(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores,
+                         c, sp, stream);
+    if (rc != HFX_OK || sp->num_spec == 0) return rc;
+    HFX_REQUIRE(N <= TX_MAXN, HFX_ERR_ARG, "N must be <= %d", TX_MAXN);
+    hipLaunchKernelGGL(k_topn_exact<HfxPlainCands>, dim3(sp->num_spec), dim3(TN_THREADS), 0,
+                       hfx_stream(stream), d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
+                       (const uint32_t *)nullptr, (const uint32_t *)d_flag, c, *sp);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }

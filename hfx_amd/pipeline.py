@@ -242,6 +242,7 @@ class DeviceSpectra:
         self.t_mass = dev(masses)
         self.desc = HfxSpectra(ptr(self.t_x), x_stride, ptr(self.t_len), ptr(self.t_zmax),
                                ptr(self.t_mass), n)
+        sync()  # later work may run on other (non-blocking) streams
 
     def lengths(self) -> np.ndarray:
         return host(self.t_len, np.uint32, self.num_spec)
@@ -350,6 +351,97 @@ class PlainSearch:
             flagged=flagged,
             scores=host(self.t_scores, np.float32, self.total).copy() if keep_scores else None,
         )
+
+
+class Stream:
+    """A non-blocking HIP stream (hfx_stream_create)."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().hfx_stream_create(C.byref(p)), "hfx_stream_create")
+        self.p = p.value
+
+    def sync(self):
+        check(lib().hfx_stream_sync(self.p), "hfx_stream_sync")
+
+    def __del__(self):
+        try:
+            lib().hfx_stream_destroy(self.p)
+        except Exception:
+            pass
+
+
+class BatchSearch:
+    """The batched path (hfx_batch_*): plan once, then run or replay a
+    captured graph; results in PlainResult form."""
+
+    INFO = ("spectra", "scores", "indexed", "levels", "entries", "G", "C", "threads", "lstr", "p0")
+
+    def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, eps: float, topn: int,
+                 margin: int = 16, stream: Stream | None = None):
+        L = lib()
+        self.ctx, self.db, self.sp, self.topn = ctx, db, sp, int(topn)
+        self.stream = stream or Stream()
+        h = C.c_void_p()
+        check(L.hfx_batch_create(ctx.h, C.byref(h)), "hfx_batch_create")
+        self.h = h.value
+        check(L.hfx_batch_plan(self.h, C.byref(db.desc), C.byref(sp.desc), float(eps), self.topn,
+                               int(margin), self.stream.p), "hfx_batch_plan")
+        self.captured = False
+
+    def info(self) -> dict:
+        a = np.zeros(10, np.uint64)
+        check(lib().hfx_batch_info(self.h, a.ctypes.data), "hfx_batch_info")
+        return {k: int(v) for k, v in zip(self.INFO, a)}
+
+    def autotune(self):
+        cfg = np.zeros(3, np.uint32)
+        check(lib().hfx_batch_autotune(self.h, self.stream.p, cfg.ctypes.data), "hfx_batch_autotune")
+        self.captured = False
+        return tuple(int(v) for v in cfg)
+
+    def set_config(self, G: int, Cn: int, threads: int):
+        check(lib().hfx_batch_set_config(self.h, G, Cn, threads), "hfx_batch_set_config")
+
+    def run(self):
+        check(lib().hfx_batch_run(self.h, self.stream.p), "hfx_batch_run")
+
+    def capture(self):
+        check(lib().hfx_batch_capture(self.h, self.stream.p), "hfx_batch_capture")
+        self.captured = True
+
+    def replay(self):
+        check(lib().hfx_batch_replay(self.h, self.stream.p), "hfx_batch_replay")
+
+    def results(self, keep_scores: bool = False) -> PlainResult:
+        self.stream.sync()
+        ptrs = (C.c_void_p * 10)()
+        check(lib().hfx_batch_outputs(self.h, ptrs), "hfx_batch_outputs")
+        S, N = self.sp.num_spec, self.topn
+        inf = self.info()
+
+        def get(i, dtype, n):
+            out = np.empty(n, dtype)
+            if n:
+                check(lib().hfx_memcpy_d2h(out.ctypes.data, ptrs[i], out.nbytes, self.stream.p),
+                      "d2h")
+            return out
+
+        r = PlainResult(
+            begin=get(7, np.uint32, S), num=get(8, np.uint32, S), off=get(6, np.uint64, S + 1),
+            top_pos=get(0, np.uint32, S * N).reshape(S, N),
+            top_exact=get(1, np.float64, S * N).reshape(S, N),
+            top_f32=get(2, np.float32, S * N).reshape(S, N), top_cnt=get(3, np.uint32, S),
+            flagged=np.nonzero(get(4, np.uint32, S))[0].astype(np.uint32),
+            scores=get(5, np.float32, inf["scores"]) if keep_scores else None)
+        self.stream.sync()
+        return r
+
+    def __del__(self):
+        try:
+            lib().hfx_batch_destroy(self.h)
+        except Exception:
+            pass
 
 
 @dataclass

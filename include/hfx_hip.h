@@ -310,6 +310,35 @@ int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                        uint32_t num_list, const uint64_t *d_mp_off, const hfx_modset *ms,
                        const hfx_db *db, const hfx_spectra *sp, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Batched search (new; Main.hs:129-142 scores 16 spectra per group).    */
+/* A batch of any number of spectra is planned once (windows, index      */
+/* sizes, every buffer allocated; synchronises), then run without host   */
+/* synchronisation or allocation -- directly or as a captured hipGraph.  */
+/* Results use the layouts of hfx_xcorr_sparse / hfx_topn.               */
+/* ------------------------------------------------------------------ */
+typedef struct hfx_batch hfx_batch;
+int hfx_batch_create(hfx_ctx *ctx, hfx_batch **out);
+int hfx_batch_destroy(hfx_batch *b);
+/* db and sp must stay valid and unchanged while the batch is used */
+int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float eps, uint32_t N,
+                   uint32_t margin, void *stream);
+/* windows -> peptide ion index -> persistent multi-spectrum XCorr -> top-N */
+int hfx_batch_run(hfx_batch *b, void *stream);
+/* capture hfx_batch_run on a non-null stream into a graph; replay it */
+int hfx_batch_capture(hfx_batch *b, void *stream);
+int hfx_batch_replay(hfx_batch *b, void *stream);
+/* time the scoring kernel over (spectra per block G, tile length C,
+ * workgroup threads) and keep the fastest; out_cfg = {G, C, threads} */
+int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg);
+int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads);
+/* info = {spectra, scores, indexed peptides, charge levels, index entries,
+ *         G, C, threads, LDS stride, first indexed position} */
+int hfx_batch_info(const hfx_batch *b, uint64_t *info);
+/* device pointers {top_pos, top_exact, top_f32, top_cnt, flag, scores,
+ *                  score offsets, window begin, window size, index entries} */
+int hfx_batch_outputs(const hfx_batch *b, void **ptrs);
+
 #ifdef __cplusplus
 }
 #endif

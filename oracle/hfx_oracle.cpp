@@ -831,7 +831,9 @@ int orc_search_plain(const uint8_t *ions, const float *r, const uint32_t *tc, co
                 uint32_t n = tn[pep] > tc[pep] ? tn[pep] - tc[pep] : 0;
                 m.resize(n);
                 for (uint32_t k = 0; k < n; ++k) m[k] = mass26[acid_idx(ions[tc[pep] + k])];
-                dense_from_masses(w.data(), L, m.data(), n, r[pep], zmax[s]);
+                /* ion_series.cu:334-342: no addIons launch for max charge > 4,
+                 * the zeroed rows score 0 */
+                dense_from_masses(w.data(), L, m.data(), n, r[pep], zmax[s] <= 4 ? zmax[s] : 0);
                 int64_t fx = 0;
                 float f32 = 0.0f;
                 for (uint32_t q = 0; q < L; ++q) {
@@ -942,7 +944,9 @@ int orc_search_mods(const uint8_t *ions, const float *r, const uint32_t *tc, con
                 mod_masses(m, ions, tc[p], tn[p], mass26, mod_ma_count + (size_t)md * num_ma, ma,
                            ma_mass, num_ma, unrank.data() + mp_scan[i]);
                 float rr = r[p] + mod_delta[md];
-                dense_from_masses(w.data(), L, m.data(), (uint32_t)m.size(), rr, zmax[s]);
+                /* mion_series.cu:335-556 dispatches max charges 1..10 only */
+                dense_from_masses(w.data(), L, m.data(), (uint32_t)m.size(), rr,
+                                  zmax[s] <= 10 ? zmax[s] : 0);
                 int64_t fx = 0;
                 for (uint32_t q = 0; q < L; ++q)
                     if (w[q]) fx += fx_term(w[q], xs[q]);
