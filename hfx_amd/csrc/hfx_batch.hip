@@ -512,10 +512,17 @@ static int bt_score(hfx_batch *b, hipStream_t st)
     (void)waves_per_wg;
     const uint32_t grid = (uint32_t)hfx_ctx_num_cus(b->ctx) * (b->G == 8 ? 2u : 4u) *
                           (b->threads == 256 ? 2u : 1u);
-    if (b->G == 8)
+    /* gfx950 gives a workgroup up to 160 KiB of LDS; ask for what this
+     * configuration stages (G spectra of lstr floats) */
+    if (b->G == 8) {
+        HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_batch<8>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         hipLaunchKernelGGL(k_batch<8>, dim3(grid), dim3(b->threads), smem, st, a);
-    else
+    } else {
+        HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_batch<4>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(b->threads), smem, st, a);
+    }
     hipLaunchKernelGGL(k_bt_zero_unscored, dim3(b->S), dim3(256), 0, st, b->scores, b->score_off,
                        b->sp.d_zmax, b->S);
     HFX_CHECK_LAUNCH();
