@@ -7,11 +7,16 @@ fragments), +-3 Da precursor window, top-10 per spectrum.
 
 A step is one pass of the scoring stage over the whole batch with the
 database and the preprocessed spectra resident in HBM: precursor windows ->
-sparse XCorr of every (spectrum, candidate) pair -> per-spectrum top-N with
-exact re-score.  value = scored pairs per second over the timed steps.
+theoretical peaks -> XCorr of every (spectrum, candidate) pair -> per-spectrum
+top-N with exact re-score.  Two forms are timed:
+  plain    hfx_window + hfx_xcorr_sparse + hfx_topn (the drop-in, one
+           theoretical spectrum per (spectrum, candidate) pair);
+  batched  hfx_batch_* (per-batch peptide ion index + persistent
+           work-stealing multi-spectrum XCorr + top-N), autotuned, captured in
+           a hipGraph and replayed.
+value = scored pairs per second of the batched form (whole job, one GPU).
 
-Prints one JSON line (see the fields below).  --gpus N > 1 exits: this build
-runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -52,7 +57,6 @@ def setup(workload: str):
     ctx = pipeline.Ctx(0)
     db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
     dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, 2112)
-    pipeline.sync()
     st = dsp.status()
     if (st != 0).any():
         raise SystemExit(f"preprocessing failed for {int((st != 0).sum())} spectra")
@@ -61,18 +65,27 @@ def setup(workload: str):
 
 
 def algorithmic_bytes(env, res) -> float:
-    """Compulsory HBM bytes of one sparse-XCorr launch: per scored pair the
-    candidate's residues + 4 index/terminal/residual words + the 4-byte score
-    written, plus each spectrum read once (4 bytes per bin)."""
-    r, tc, tn = env["r"], env["tc"], env["tn"]
+    """Compulsory HBM bytes of the scoring stage for the scored pairs: per
+    pair the candidate's residues + 16 bytes of index/terminal/residual words
+    + 8 bytes out (score + index), plus each spectrum read once (4 B/bin)."""
+    tc, tn = env["tc"], env["tn"]
     idx = env["db"].idx_sorted()
     nres = (tn.astype(np.int64) - tc.astype(np.int64)).clip(min=0)
-    per_pos = nres[idx] + 16 + 4
+    per_pos = nres[idx] + 16 + 8
     cs = np.concatenate([[0], np.cumsum(per_pos)])
     b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
     cand = float(np.sum(cs[b + n] - cs[b]))
     spec = float(4 * env["dsp"].lengths().astype(np.int64).sum())
     return cand + spec
+
+
+def same_topn(a, b, n=None) -> bool:
+    S = len(a.top_cnt) if n is None else n
+    if not (a.top_cnt[:S] == b.top_cnt[:S]).all():
+        return False
+    return all(a.top_pos[s, : a.top_cnt[s]].tolist() == b.top_pos[s, : b.top_cnt[s]].tolist()
+               and (a.top_exact[s, : a.top_cnt[s]] == b.top_exact[s, : b.top_cnt[s]]).all()
+               for s in range(S))
 
 
 def cpu_baseline(env, res, topn, target_s=12.0):
@@ -85,25 +98,27 @@ def cpu_baseline(env, res, topn, target_s=12.0):
     idx = env["db"].idx_sorted()
     ions = env["db"].packed_ions()
     S = dsp.num_spec
-    # calibrate on a few spectra, then time a bounded sample of ~target_s
-    n = 8
+    args = (ions, env["r"], env["tc"], env["tn"], env["mass"], idx)
+    n = threads * 4
     t0 = time.time()
-    orc.search_plain(ions, env["r"], env["tc"], env["tn"], env["mass"], idx, X[:n], lens[:n],
-                     dsp.zmax[:n], dsp.mass[:n], 3.0, topn, nthreads=threads)
+    orc.search_plain(*args, X[:n], lens[:n], dsp.zmax[:n], dsp.mass[:n], 3.0, topn,
+                     nthreads=threads)
     dt = max(time.time() - t0, 1e-3)
-    n = int(min(S, max(threads * 2, n * target_s / dt)))
+    n = int(min(S, max(threads * 4, n * target_s / dt)))
     t0 = time.time()
-    ref = orc.search_plain(ions, env["r"], env["tc"], env["tn"], env["mass"], idx, X[:n], lens[:n],
-                           dsp.zmax[:n], dsp.mass[:n], 3.0, topn, nthreads=threads)
+    ref = orc.search_plain(*args, X[:n], lens[:n], dsp.zmax[:n], dsp.mass[:n], 3.0, topn,
+                           nthreads=threads)
     dt = time.time() - t0
     scores = int(ref["num"].sum())
-    # the GPU's top-N on the same spectra must be identical
     same = bool((res.top_cnt[:n] == ref["top_cnt"]).all()) and all(
         res.top_pos[s, : res.top_cnt[s]].tolist() == ref["top_pos"][s, : ref["top_cnt"][s]].tolist()
+        and (res.top_exact[s, : res.top_cnt[s]] ==
+             orc.fx_to_double(ref["top_fx"][s, : ref["top_cnt"][s]])).all()
         for s in range(n))
     return dict(value=scores / dt, unit="scores/s", cores=threads, kind="port",
-                sample=f"first {n} of {S} spectra ({scores} scored pairs), oracle dense-row "
-                       f"scoring with exact top-{topn}, {dt:.1f}s", topn_identical_to_gpu=same)
+                sample=f"first {n} of {S} spectra ({scores} scored pairs): oracle dense-row "
+                       f"scoring with exact top-{topn}, OpenMP over spectra, {dt:.1f}s",
+                topn_identical_to_gpu=same)
 
 
 def calib_read(mib: int):
@@ -126,28 +141,9 @@ def calib_read(mib: int):
                       "launches": 3}), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="tryptic-20k-10k")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",
-                    help="only run a streaming read of MIB MiB (k_mvm_rows over a "
-                         "row matrix) to calibrate HBM byte counters, then exit")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
-        raise SystemExit(2)
-    if args.calib_read:
-        calib_read(args.calib_read)
-        return
-
+def bench_plain(env, topn, steps, warmup):
     from hfx_amd import pipeline
 
-    env = setup(args.workload)
-    topn = env["w"]["topn"]
     srch = pipeline.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn, margin=16)
 
     def step(ev=None):
@@ -160,62 +156,107 @@ def main():
         srch.select()
         return srch.fallback()
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     pipeline.sync()
-    # HIP events on the stream the kernels are launched on
-    evs = [(pipeline.Event(), pipeline.Event()) for _ in range(args.steps)]
+    evs = [(pipeline.Event(), pipeline.Event()) for _ in range(steps)]
     flagged = 0
-    pipeline.sync()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         flagged += len(step(evs[k]))
     pipeline.sync()
     elapsed = time.perf_counter() - t0
     kern_ms = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
-    res = srch.run()  # results of one more pass, for the checks below
-    scores_per_step = int(res.off[-1])
-    value = scores_per_step * args.steps / elapsed
+    res = srch.run()
+    return res, elapsed, kern_ms, flagged / steps
+
+
+def bench_batched(env, topn, steps, warmup):
+    from hfx_amd import pipeline
+
+    bs = pipeline.BatchSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn, margin=16)
+    t0 = time.time()
+    cfg = bs.autotune()
+    log(f"autotune: G={cfg[0]} C={cfg[1]} threads={cfg[2]} ({time.time() - t0:.1f}s)")
+    phases = [bs.time_phases() for _ in range(3)][-1]
+    bs.capture()
+    for _ in range(warmup):
+        bs.replay()
+    bs.stream.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        bs.replay()
+    bs.stream.sync()
+    elapsed = time.perf_counter() - t0
+    res = bs.results()
+    return res, elapsed, phases, cfg, bs.info()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="tryptic-20k-10k")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--path", choices=("both", "plain", "batched"), default="both")
+    ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",
+                    help="only run a streaming read of MIB MiB (k_mvm_rows over a "
+                         "row matrix) to calibrate HBM byte counters, then exit")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        raise SystemExit(2)
+    if args.calib_read:
+        calib_read(args.calib_read)
+        return
+
+    env = setup(args.workload)
+    topn = env["w"]["topn"]
+    out = {"metric": METRIC, "unit": "scores/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    paths = {}
+    plain_res = None
+    if args.path in ("both", "plain"):
+        plain_res, el, kern_ms, flagged = bench_plain(env, topn, args.steps, args.warmup)
+        sc = int(plain_res.off[-1])
+        ab = algorithmic_bytes(env, plain_res)
+        paths["plain_fp32_sparse"] = dict(
+            value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
+            kernel="k_xcorr<HfxPlainCands>", kernel_ms=kern_ms,
+            kernel_gbs=ab / (kern_ms * 1e-3) / 1e9, flagged_spectra_per_step=flagged)
+        log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s")
+    batch_res = None
+    if args.path in ("both", "batched"):
+        batch_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup)
+        sc = int(batch_res.off[-1])
+        ab = algorithmic_bytes(env, batch_res)
+        paths["batched_fp32_sparse"] = dict(
+            value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
+            kernel="k_batch", kernel_ms=phases["score"], phases_ms=phases,
+            config=dict(G=cfg[0], C=cfg[1], threads=cfg[2]), index=info,
+            graph="captured once, replayed per step",
+            flagged_spectra=int(len(batch_res.flagged)))
+        log(f"batched: {paths['batched_fp32_sparse']['value']:.3e} scores/s {phases}")
+    if plain_res is not None and batch_res is not None:
+        paths["batched_topn_identical_to_plain"] = same_topn(plain_res, batch_res)
+    main_key = "batched_fp32_sparse" if "batched_fp32_sparse" in paths else "plain_fp32_sparse"
+    res = batch_res if batch_res is not None else plain_res
+    p = paths[main_key]
     ab = algorithmic_bytes(env, res)
-    achieved = ab / (kern_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "scores/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": args.workload,
-            "proteins": env["w"]["proteins"],
-            "peptides": int(len(env["r"])),
-            "spectra": int(env["dsp"].num_spec),
-            "window_da": 3.0,
-            "bin_width": 1.0005079,
-            "max_len": int(env["dsp"].lengths().max()),
-            "topn": topn,
-            "path": "plain-fp32-sparse",
-            "scores_per_step": scores_per_step,
-            "flagged_spectra_per_step": flagged / args.steps,
-        },
-        "roofline": {
-            "kernel": "k_xcorr_plain",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel_ms": kern_ms,
-            "algorithmic_bytes": ab,
-        },
-    }
+    achieved = ab / (p["kernel_ms"] * 1e-3) / 1e9
+    out.update(
+        value=p["value"], ms_per_step=p["ms_per_step"],
+        config={"workload": args.workload, "proteins": env["w"]["proteins"],
+                "peptides": int(len(env["r"])), "spectra": int(env["dsp"].num_spec),
+                "window_da": 3.0, "bin_width": 1.0005079,
+                "max_len": int(env["dsp"].lengths().max()), "topn": topn, "path": main_key,
+                "scores_per_step": p["scores_per_step"]},
+        roofline={"kernel": p["kernel"], "bound": "hbm", "achieved": achieved,
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                  "traffic": None, "kernel_ms": p["kernel_ms"], "algorithmic_bytes": ab},
+        paths=paths)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(env, res, topn)

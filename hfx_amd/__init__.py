@@ -172,6 +172,7 @@ _SIGS = {
     "hfx_batch_replay": (_I, [_P, _P]),
     "hfx_batch_autotune": (_I, [_P, _P, _P]),
     "hfx_batch_set_config": (_I, [_P, _U32, _U32, _U32]),
+    "hfx_batch_time": (_I, [_P, _P, _P]),
     "hfx_batch_info": (_I, [_P, _P]),
     "hfx_batch_outputs": (_I, [_P, _P]),
 }

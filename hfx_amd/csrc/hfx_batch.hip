@@ -628,6 +628,41 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     return HFX_OK;
 }
 
+/* Runs the planned batch phase by phase with events between the phases:
+ * ms[0] windows + offsets, ms[1] ion index, ms[2] blocks/tiles + scoring
+ * kernel, ms[3] top-N (+ gated exact fallback). */
+extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
+{
+    HFX_REQUIRE(b && b->planned && ms, HFX_ERR_ARG, "hfx_batch_time: batch not planned");
+    hipStream_t st = hfx_stream(stream);
+    hipEvent_t ev[5];
+    for (auto &e : ev) HFX_CHECK_HIP(hipEventCreate(&e));
+    int rc = HFX_OK;
+    HFX_CHECK_HIP(hipEventRecord(ev[0], st));
+    if (!rc) rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, stream);
+    if (!rc) rc = hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, stream);
+    HFX_CHECK_HIP(hipEventRecord(ev[1], st));
+    if (!rc && b->P) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((b->P + IX_WAVES - 1) / IX_WAVES,
+                                                           (uint64_t)hfx_ctx_num_cus(b->ctx) * 16);
+        hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, b->ent, b->cnt, b->ent_off,
+                           b->db, b->p0, b->P, b->L, b->lcap);
+    }
+    HFX_CHECK_HIP(hipEventRecord(ev[2], st));
+    if (!rc) rc = bt_tiles(b, st);
+    if (!rc) rc = bt_score(b, st);
+    HFX_CHECK_HIP(hipEventRecord(ev[3], st));
+    if (!rc)
+        rc = hfx_topn_gated(b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag, b->N,
+                            b->margin, b->scores, b->score_off, b->begin, b->num, &b->db, &b->sp,
+                            stream);
+    HFX_CHECK_HIP(hipEventRecord(ev[4], st));
+    HFX_CHECK_HIP(hipEventSynchronize(ev[4]));
+    for (int i = 0; i < 4; ++i) HFX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
 extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_config: batch not planned");

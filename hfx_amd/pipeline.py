@@ -400,6 +400,11 @@ class BatchSearch:
         self.captured = False
         return tuple(int(v) for v in cfg)
 
+    def time_phases(self) -> dict:
+        ms = np.zeros(4, np.float32)
+        check(lib().hfx_batch_time(self.h, self.stream.p, ms.ctypes.data), "hfx_batch_time")
+        return dict(zip(("window", "index", "score", "topn"), (float(v) for v in ms)))
+
     def set_config(self, G: int, Cn: int, threads: int):
         check(lib().hfx_batch_set_config(self.h, G, Cn, threads), "hfx_batch_set_config")
 

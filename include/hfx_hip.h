@@ -332,6 +332,9 @@ int hfx_batch_replay(hfx_batch *b, void *stream);
  * workgroup threads) and keep the fastest; out_cfg = {G, C, threads} */
 int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg);
 int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads);
+/* one run, phase by phase, with event timings in ms: {windows, ion index,
+ * scoring, top-N} */
+int hfx_batch_time(hfx_batch *b, void *stream, float *ms);
 /* info = {spectra, scores, indexed peptides, charge levels, index entries,
  *         G, C, threads, LDS stride, first indexed position} */
 int hfx_batch_info(const hfx_batch *b, uint64_t *info);
