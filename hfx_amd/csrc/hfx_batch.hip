@@ -183,39 +183,28 @@ __device__ __forceinline__ float code_w(uint32_t c)
 template <int G>
 __device__ __forceinline__ float reduce_scatter(float (&acc)[G], uint32_t lane)
 {
-    if constexpr (G == 8) {
-        const bool up = lane & 32;
-        float a0 = acc[0] + __shfl_xor(acc[4], 32, 64), a1 = acc[1] + __shfl_xor(acc[5], 32, 64);
-        float a2 = acc[2] + __shfl_xor(acc[6], 32, 64), a3 = acc[3] + __shfl_xor(acc[7], 32, 64);
-        float b0 = acc[4] + __shfl_xor(acc[0], 32, 64), b1 = acc[5] + __shfl_xor(acc[1], 32, 64);
-        float b2 = acc[6] + __shfl_xor(acc[2], 32, 64), b3 = acc[7] + __shfl_xor(acc[3], 32, 64);
-        float c0 = up ? b0 : a0, c1 = up ? b1 : a1, c2 = up ? b2 : a2, c3 = up ? b3 : a3;
-        const bool u16 = lane & 16;
-        float d0 = c0 + __shfl_xor(c2, 16, 64), d1 = c1 + __shfl_xor(c3, 16, 64);
-        float e0 = c2 + __shfl_xor(c0, 16, 64), e1 = c3 + __shfl_xor(c1, 16, 64);
-        float f0 = u16 ? e0 : d0, f1 = u16 ? e1 : d1;
-        const bool u8 = lane & 8;
-        float h0 = f0 + __shfl_xor(f1, 8, 64), h1 = f1 + __shfl_xor(f0, 8, 64);
-        float v = u8 ? h1 : h0;
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 1, 64);
-        return v;
-    } else {
-        static_assert(G == 4, "G must be 4 or 8");
-        const bool up = lane & 32;
-        float a0 = acc[0] + __shfl_xor(acc[2], 32, 64), a1 = acc[1] + __shfl_xor(acc[3], 32, 64);
-        float b0 = acc[2] + __shfl_xor(acc[0], 32, 64), b1 = acc[3] + __shfl_xor(acc[1], 32, 64);
-        float c0 = up ? b0 : a0, c1 = up ? b1 : a1;
-        const bool u16 = lane & 16;
-        float d0 = c0 + __shfl_xor(c1, 16, 64), d1 = c1 + __shfl_xor(c0, 16, 64);
-        float v = u16 ? d1 : d0;
-        v += __shfl_xor(v, 8, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 1, 64);
-        return v;
+    static_assert(G == 4 || G == 8, "G must be 4 or 8");
+    float v[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) v[k] = acc[k];
+    /* each step pairs lanes across `mask`: the lower lane keeps the lower half
+     * of its live accumulators, the upper lane the upper half, each adding
+     * the partner's copy of the accumulators it keeps */
+    int mask = 32;
+#pragma unroll
+    for (int n = G; n > 1; n >>= 1, mask >>= 1) {
+        const bool up = lane & (uint32_t)mask;
+#pragma unroll
+        for (int k = 0; k < n / 2; ++k) {
+            const float send = up ? v[k] : v[k + n / 2];
+            const float keep = up ? v[k + n / 2] : v[k];
+            v[k] = keep + __shfl_xor(send, mask, 64);
+        }
     }
+    float s = v[0];
+#pragma unroll
+    for (; mask > 0; mask >>= 1) s += __shfl_xor(s, mask, 64);
+    return s;
 }
 
 template <int G>
