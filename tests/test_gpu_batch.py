@@ -20,9 +20,13 @@ def benv(orc):
     r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=2)
     sp = synth.make_spectra(prot, r, tc, tn, 300, seed=42, charges=(2, 3), charge_p=(0.6, 0.4),
                             static_mods={"C": 57.02146}, double_for_3plus=True)
-    # two spectra with max fragment charge > 4 (7+ precursors): scored 0
-    sp.charge[5] = 7.0
-    sp.charge[17] = 8.0
+    # two spectra with max fragment charge > 4 (7+ and 8+ precursors of the
+    # same peptides, same window): scored 0
+    H = 1.007276
+    for s, z in ((5, 7.0), (17, 8.0)):
+        z0 = float(sp.charge[s])
+        sp.pcr[s] = np.float32((float(sp.pcr[s]) * z0 - z0 * H + z * H) / z)
+        sp.charge[s] = z
     ctx = pipeline.Ctx(0)
     db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
     dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE)
