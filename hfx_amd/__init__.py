@@ -142,12 +142,13 @@ _SIGS = {
     ),
     "hfx_topn": (
         _I,
-        [_P, _P, _P, _P, _P, _P, _U32, _U32, _P, _P, _P, _P, C.POINTER(HfxDb),
+        [_P, _P, _P, _P, _P, _P, _U32, _U32, _P, _P, _U64, _P, _P, C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
     "hfx_topn_exact": (
         _I,
-        [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
+        [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, _U64, _P, _P, C.POINTER(HfxDb),
+         C.POINTER(HfxSpectra), _P],
     ),
     "hfx_mpep_offsets": (_I, [_P, _P, _P, _U32, _P, _U32, _P]),
     "hfx_xcorr_sparse_mod": (
@@ -161,7 +162,7 @@ _SIGS = {
     ),
     "hfx_topn_exact_mod": (
         _I,
-        [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
+        [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
     "hfx_batch_create": (_I, [_P, C.POINTER(_P)]),

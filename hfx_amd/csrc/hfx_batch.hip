@@ -330,6 +330,7 @@ struct hfx_batch {
     uint32_t *top_pos, *top_cnt, *flag;
     double *top_exact;
     float *top_f32;
+    uint8_t *tx_scratch; /* top-N exact route */
     hipGraph_t graph;
     hipGraphExec_t exec;
 };
@@ -339,7 +340,7 @@ static void bt_free(hfx_batch *b)
     void *ps[] = {b->begin,    b->num,     b->order,  b->ubeg,      b->uend,    b->ntiles,
                   b->caps,     b->cnt,     b->counter, b->score_off, b->tile_off, b->ent_off,
                   b->ent,      b->scores,  b->top_pos, b->top_cnt,   b->flag,     b->top_exact,
-                  b->top_f32};
+                  b->top_f32,  b->tx_scratch};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -469,6 +470,9 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
         (rc = dalloc(&b->top_f32, (size_t)S * N)) || (rc = dalloc(&b->top_cnt, S)) ||
         (rc = dalloc(&b->flag, S)))
         return rc;
+    /* the top-N exact route's scratch is owned by the batch, so that a
+     * captured graph never allocates and never sees it move */
+    if ((rc = dalloc(&b->tx_scratch, hfx_topn_scratch_bytes(S, b->total)))) return rc;
     /* blocks are sized for the smallest G the tuner may pick (4) */
     const uint32_t bmax = (S + 3) / 4;
     if ((rc = dalloc(&b->ubeg, bmax)) || (rc = dalloc(&b->uend, bmax)) ||
@@ -518,11 +522,6 @@ static int bt_score(hfx_batch *b, hipStream_t st)
     return HFX_OK;
 }
 
-int hfx_topn_gated(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32, uint32_t *d_top_cnt,
-                   uint32_t *d_flag, uint32_t N, uint32_t margin, const float *d_scores,
-                   const uint64_t *d_off, const uint32_t *d_begin, const uint32_t *d_num,
-                   const hfx_db *db, const hfx_spectra *sp, void *stream);
-
 extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_run: batch not planned");
@@ -540,9 +539,9 @@ extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
     }
     if ((rc = bt_tiles(b, st))) return rc;
     if ((rc = bt_score(b, st))) return rc;
-    return hfx_topn_gated(b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag, b->N,
-                          b->margin, b->scores, b->score_off, b->begin, b->num, &b->db, &b->sp,
-                          stream);
+    return hfx_topn_device(b->ctx, b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag,
+                           b->N, b->margin, b->scores, b->score_off, b->total, b->begin, b->num,
+                           &b->db, &b->sp, b->tx_scratch, stream);
 }
 
 extern "C" int hfx_batch_capture(hfx_batch *b, void *stream)
@@ -642,9 +641,9 @@ extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
     if (!rc) rc = bt_score(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[3], st));
     if (!rc)
-        rc = hfx_topn_gated(b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag, b->N,
-                            b->margin, b->scores, b->score_off, b->begin, b->num, &b->db, &b->sp,
-                            stream);
+        rc = hfx_topn_device(b->ctx, b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag,
+                             b->N, b->margin, b->scores, b->score_off, b->total, b->begin, b->num,
+                             &b->db, &b->sp, b->tx_scratch, stream);
     HFX_CHECK_HIP(hipEventRecord(ev[4], st));
     HFX_CHECK_HIP(hipEventSynchronize(ev[4]));
     for (int i = 0; i < 4; ++i) HFX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));

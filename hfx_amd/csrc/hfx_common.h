@@ -83,6 +83,7 @@ enum {
     HFX_SLOT_BATCH = 8,   /* batched path                        */
     HFX_SLOT_BATCH2 = 9,
     HFX_SLOT_DENSE = 10,  /* BF16 dense path                     */
+    HFX_SLOT_FLAGS = 11,  /* spectrum lists turned into flags    */
 };
 /* Scratch owned by a context: grown on demand, never shrunk. */
 int hfx_ctx_scratch(hfx_ctx *ctx, int slot, size_t bytes, void **out);
@@ -93,6 +94,16 @@ hfx_ctx *hfx_default_ctx(void);
 uint32_t hfx_sum_u32_to_host(hfx_ctx *ctx, const uint32_t *d_a, uint32_t n);
 /* exclusive scan in place, one workgroup */
 int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream);
+/* device scratch the top-N exact route needs for S spectra, total scores */
+size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total);
+/* hfx_topn over the unmodified candidates (batched path; no host sync).
+ * scratch: hfx_topn_scratch_bytes(S, total) bytes of device memory, or null
+ * for the context's own slot */
+int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                    uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
+                    const float *d_scores, const uint64_t *d_off, uint64_t total,
+                    const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+                    const hfx_spectra *sp, void *scratch, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device helpers                                                        */

@@ -208,166 +208,288 @@ __global__ void __launch_bounds__(TN_THREADS)
 }
 
 /* ------------------------------------------------------------------ */
-/* Exact top-N over every candidate (fallback for flagged spectra)       */
+/* Exact route: every candidate of a flagged spectrum scored exactly     */
+/* (tile-parallel, persistent), then a 64-bit radix select of the top N. */
+/* Runs on the device after k_topn without a host round trip.            */
 /* ------------------------------------------------------------------ */
 
 #define TX_MAXN 64
+#define TX_CHUNK 64
 
 template <class Cands>
-__global__ void __launch_bounds__(TN_THREADS)
-    k_topn_exact(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
-                 uint32_t N, const uint32_t *list, const uint32_t *gate, Cands c, hfx_spectra sp)
+__global__ void k_flag_tiles(uint32_t *tiles, const uint32_t *flag, Cands c, uint32_t S)
 {
-    /* list == nullptr: block b is spectrum b; gate: only spectra flagged there */
-    if (gate && !gate[list ? list[blockIdx.x] : blockIdx.x]) return;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) tiles[s] = flag[s] ? (c.count(s) + TX_CHUNK - 1) / TX_CHUNK : 0u;
+}
+
+__global__ void k_tx_reset(uint32_t *counter) { *counter = 0; }
+
+__global__ void k_zero_flags(uint32_t *flag, uint32_t S)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S) flag[i] = 0;
+}
+__global__ void k_mark_list(uint32_t *flag, const uint32_t *list, uint32_t num_list, uint32_t S)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < num_list && list[i] < S) flag[list[i]] = 1;
+}
+
+__device__ __forceinline__ uint32_t tx_segment(const uint64_t *off, uint32_t n, uint64_t t)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* persistent: tiles of TX_CHUNK candidates of flagged spectra from a counter */
+template <class Cands>
+__global__ void __launch_bounds__(TN_THREADS)
+    k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, uint32_t *counter, Cands c,
+                  hfx_spectra sp)
+{
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
     __shared__ float mtab[32];
-    __shared__ int64_t lfx[TN_WAVES * TX_MAXN];
-    __shared__ uint32_t lpos[TN_WAVES * TX_MAXN];
-    __shared__ float lf32[TN_WAVES * TX_MAXN];
-    __shared__ uint32_t lcnt[TN_WAVES];
-    const uint32_t s = list ? list[blockIdx.x] : blockIdx.x;
+    __shared__ uint32_t sh_t;
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
-    const uint32_t n = c.count(s);
-    const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
+    const uint32_t S = sp.num_spec;
+    const uint64_t ntiles = tile_off[S];
     uint32_t *map = maps[wave];
     for (uint32_t w = lane; w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
-    if (lane == 0) lcnt[wave] = 0;
-    __syncthreads();
-    const uint32_t zeff = c.zeff(sp.d_zmax[s]);
-    int64_t *wf = lfx + wave * TX_MAXN;
-    uint32_t *wp = lpos + wave * TX_MAXN;
-    float *w32 = lf32 + wave * TX_MAXN;
-    for (uint32_t i = wave; i < n; i += TN_WAVES) {
-        float f32;
-        int64_t fx;
-        c.peaks(map, mtab, s, i, len, zeff);
-        hfx_wave_sync();
-        hfx_wave_dot_both(map, xs, len, &f32, &fx);
-        hfx_wave_sync();
-        if (lane == 0) {
-            /* positions arrive ascending per wave: equal scores keep order */
-            uint32_t cnum = lcnt[wave];
-            if (cnum < N || fx > wf[cnum - 1]) {
-                uint32_t j = cnum < N ? cnum++ : cnum - 1;
-                while (j > 0 && fx > wf[j - 1]) {
-                    wf[j] = wf[j - 1];
-                    wp[j] = wp[j - 1];
-                    w32[j] = w32[j - 1];
-                    --j;
-                }
-                wf[j] = fx;
-                wp[j] = i;
-                w32[j] = f32;
-                lcnt[wave] = cnum;
-            }
+    uint32_t cur = 0xffffffffu, len = 0;
+    for (;;) {
+        if (tid == 0) sh_t = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint64_t t = sh_t;
+        if (t >= ntiles) break;
+        const uint32_t s = tx_segment(tile_off, S, t);
+        if (s != cur) {
+            len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
+            cur = s;
+            __syncthreads();
         }
-        hfx_wave_sync();
+        const uint32_t first = (uint32_t)(t - tile_off[s]) * TX_CHUNK;
+        const uint32_t cnt = min((uint32_t)TX_CHUNK, c.count(s) - first);
+        const uint32_t zeff = c.zeff(sp.d_zmax[s]);
+        for (uint32_t i = wave; i < cnt; i += TN_WAVES) {
+            c.peaks(map, mtab, s, first + i, len, zeff);
+            hfx_wave_sync();
+            const int64_t fx = hfx_wave_dot_fx(map, xs, len);
+            hfx_wave_sync();
+            if (lane == 0) fxbuf[c.base(s) + first + i] = fx;
+        }
+        __syncthreads();
+    }
+}
+
+/* top N of a flagged spectrum by (exact desc, position asc), radix select
+ * over order-preserving 64-bit keys */
+template <class Cands>
+__global__ void __launch_bounds__(TN_THREADS)
+    k_topn_fx(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt, uint32_t N,
+              const uint32_t *flag, const int64_t *fxbuf, const float *scores, Cands c)
+{
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sel_p[TX_MAXN];
+    __shared__ int64_t sel_f[TX_MAXN];
+    __shared__ uint32_t wcnt[TN_WAVES];
+    __shared__ uint32_t sh_digit, sh_rem, sh_gt;
+    const uint32_t s = blockIdx.x;
+    if (!flag[s]) return;
+    const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
+    const uint32_t n = c.count(s);
+    const int64_t *fx = fxbuf + c.base(s);
+    const float *sc = scores + c.base(s);
+    const uint32_t want = min(N, n);
+    if (want == 0) {
+        if (tid == 0) top_cnt[s] = 0;
+        return;
+    }
+    auto key = [&](uint32_t i) { return (uint64_t)fx[i] ^ 0x8000000000000000ull; };
+    uint64_t prefix = 0, pmask = 0;
+    if (tid == 0) sh_rem = want;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < n; i += TN_THREADS) {
+            const uint64_t k = key(i);
+            if ((k & pmask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t rem = sh_rem, cum = 0, d = 0;
+            for (int dd = 255; dd >= 0; --dd) {
+                if (cum + hist[dd] >= rem) { d = (uint32_t)dd; break; }
+                cum += hist[dd];
+            }
+            sh_digit = d;
+            sh_rem = rem - cum;
+        }
+        __syncthreads();
+        prefix |= (uint64_t)sh_digit << shift;
+        pmask |= 255ull << shift;
+    }
+    const uint64_t K = prefix;
+    const uint32_t need_eq = sh_rem, n_gt = want - need_eq;
+    if (tid == 0) sh_gt = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += TN_THREADS) {
+        if (key(i) > K) {
+            const uint32_t slot = atomicAdd(&sh_gt, 1u);
+            if (slot < n_gt) { sel_p[slot] = i; sel_f[slot] = fx[i]; }
+        }
+    }
+    uint32_t eq_base = 0;
+    for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
+        const uint32_t i = base + tid;
+        const bool f = i < n && key(i) == K;
+        const uint64_t bal = __ballot(f);
+        const uint32_t wr = (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (uint32_t w = 0; w < TN_WAVES; ++w) {
+            if (w < wave) wpre += wcnt[w];
+            tot += wcnt[w];
+        }
+        const uint32_t r = eq_base + wpre + wr;
+        if (f && r < need_eq) { sel_p[n_gt + r] = i; sel_f[n_gt + r] = fx[i]; }
+        eq_base += tot;
+        __syncthreads();
     }
     __syncthreads();
-    const uint32_t outn = min(N, n);
-    for (uint32_t e = tid; e < TN_WAVES * TX_MAXN; e += TN_THREADS) {
-        const uint32_t w = e / TX_MAXN, k = e % TX_MAXN;
-        if (k >= lcnt[w]) continue;
-        const int64_t fi = lfx[e];
-        const uint32_t pi = lpos[e];
+    for (uint32_t i = tid; i < want; i += TN_THREADS) {
         uint32_t rk = 0;
-        for (uint32_t w2 = 0; w2 < TN_WAVES; ++w2)
-            for (uint32_t k2 = 0; k2 < lcnt[w2]; ++k2) {
-                const uint32_t e2 = w2 * TX_MAXN + k2;
-                rk += before_fx(lfx[e2], lpos[e2], fi, pi) ? 1u : 0u;
-            }
-        if (rk < outn) {
-            top_pos[(size_t)s * N + rk] = pi;
-            top_exact[(size_t)s * N + rk] = (double)fi * (1.0 / HFX_FX_SCALE);
-            top_f32[(size_t)s * N + rk] = lf32[e];
-        }
+        for (uint32_t j = 0; j < want; ++j)
+            rk += before_fx(sel_f[j], sel_p[j], sel_f[i], sel_p[i]) ? 1u : 0u;
+        top_pos[(size_t)s * N + rk] = sel_p[i];
+        top_exact[(size_t)s * N + rk] = (double)sel_f[i] * (1.0 / HFX_FX_SCALE);
+        top_f32[(size_t)s * N + rk] = sc[sel_p[i]];
     }
-    if (tid == 0) top_cnt[s] = outn;
+    if (tid == 0) top_cnt[s] = want;
 }
 
 /* ------------------------------------------------------------------ */
-/* Entry points                                                          */
+/* Launch helpers                                                        */
 /* ------------------------------------------------------------------ */
 
+size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total)
+{
+    return total * 8 + (size_t)S * 4 + ((size_t)S + 1) * 8 + 256;
+}
+
 template <class Cands>
-static int launch_topn(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                       uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_flag,
+                       const float *d_scores, const Cands &c, uint64_t total,
+                       const hfx_spectra *sp, void *scratch, hipStream_t st)
+{
+    const uint32_t S = sp->num_spec;
+    if (S == 0) return HFX_OK;
+    HFX_REQUIRE(N >= 1 && N <= TX_MAXN, HFX_ERR_ARG, "top-N: N must be in 1..%d", TX_MAXN);
+    void *p = scratch;
+    int rc;
+    if (!p && (rc = hfx_ctx_scratch(ctx, HFX_SLOT_TOPN, hfx_topn_scratch_bytes(S, total), &p)))
+        return rc;
+    int64_t *fxbuf = (int64_t *)p;
+    uint64_t *tile_off = (uint64_t *)(fxbuf + total);
+    uint32_t *tiles = (uint32_t *)(tile_off + S + 1);
+    uint32_t *counter = tiles + S;
+    hipLaunchKernelGGL(k_flag_tiles<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, tiles, d_flag,
+                       c, S);
+    if ((rc = hfx_scan_offsets(ctx, tile_off, tiles, S, (void *)st)) != HFX_OK) return rc;
+    hipLaunchKernelGGL(k_tx_reset, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 4),
+                       dim3(TN_THREADS), 0, st, fxbuf, tile_off, counter, c, *sp);
+    hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(S), dim3(TN_THREADS), 0, st, d_top_pos, d_top_exact,
+                       d_top_f32, d_top_cnt, N, d_flag, fxbuf, d_scores, c);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+template <class Cands>
+static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                        uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
-                       const float *d_scores, const Cands &c, const hfx_spectra *sp, void *stream)
+                       const float *d_scores, const Cands &c, uint64_t total,
+                       const hfx_spectra *sp, void *scratch, void *stream)
 {
-    HFX_REQUIRE(d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_flag && d_scores && sp,
+    HFX_REQUIRE(ctx && d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_flag && d_scores &&
+                    sp,
                 HFX_ERR_ARG, "hfx_topn: null argument");
-    HFX_REQUIRE(N >= 1 && N + margin <= TN_MAXT, HFX_ERR_ARG,
-                "hfx_topn: N + margin must be in 1..%d (N=%u margin=%u)", TN_MAXT, N, margin);
+    HFX_REQUIRE(N >= 1 && N <= TX_MAXN && N + margin <= TN_MAXT, HFX_ERR_ARG,
+                "hfx_topn: need 1 <= N <= %d and N + margin <= %d (N=%u margin=%u)", TX_MAXN,
+                TN_MAXT, N, margin);
     if (sp->num_spec == 0) return HFX_OK;
-    hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS), 0, hfx_stream(stream),
-                       d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c,
-                       *sp);
+    hipStream_t st = hfx_stream(stream);
+    hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS), 0, st, d_top_pos,
+                       d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c, *sp);
     HFX_CHECK_LAUNCH();
-    return HFX_OK;
+    return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
+                       total, sp, scratch, st);
 }
 
 template <class Cands>
-static int launch_topn_exact(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
-                             uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_list,
-                             uint32_t num_list, const Cands &c, const hfx_spectra *sp,
-                             void *stream)
+static int launch_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
+                             float *d_top_f32, uint32_t *d_top_cnt, uint32_t N,
+                             const uint32_t *d_list, uint32_t num_list, const float *d_scores,
+                             const Cands &c, uint64_t total, const hfx_spectra *sp, void *stream)
 {
-    HFX_REQUIRE(d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_list && sp, HFX_ERR_ARG,
-                "hfx_topn_exact: null argument");
-    HFX_REQUIRE(N >= 1 && N <= TX_MAXN, HFX_ERR_ARG, "hfx_topn_exact: N must be in 1..%d",
-                TX_MAXN);
-    if (num_list == 0) return HFX_OK;
-    hipLaunchKernelGGL(k_topn_exact<Cands>, dim3(num_list), dim3(TN_THREADS), 0, hfx_stream(stream),
-                       d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_list,
-                       (const uint32_t *)nullptr, c, *sp);
-    HFX_CHECK_LAUNCH();
-    return HFX_OK;
+    HFX_REQUIRE(ctx && d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_list && sp,
+                HFX_ERR_ARG, "hfx_topn_exact: null argument");
+    const uint32_t S = sp->num_spec;
+    if (S == 0 || num_list == 0) return HFX_OK;
+    hipStream_t st = hfx_stream(stream);
+    void *p;
+    int rc;
+    if ((rc = hfx_ctx_scratch(ctx, HFX_SLOT_FLAGS, 4ull * S + 64, &p)) != HFX_OK) return rc;
+    uint32_t *flag = (uint32_t *)p;
+    hipLaunchKernelGGL(k_zero_flags, dim3((S + 255) / 256), dim3(256), 0, st, flag, S);
+    hipLaunchKernelGGL(k_mark_list, dim3((num_list + 255) / 256), dim3(256), 0, st, flag, d_list,
+                       num_list, S);
+    return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, flag, d_scores, c,
+                       total, sp, nullptr, st);
 }
 
-/* hfx_topn followed, in the same stream and without a host round trip, by
- * the exact fallback for the spectra it flags (used by the batched path). */
-int hfx_topn_gated(uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32, uint32_t *d_top_cnt,
-                   uint32_t *d_flag, uint32_t N, uint32_t margin, const float *d_scores,
-                   const uint64_t *d_off, const uint32_t *d_begin, const uint32_t *d_num,
-                   const hfx_db *db, const hfx_spectra *sp, void *stream)
+/* used by the batched path: the whole top-N on the device */
+int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                    uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
+                    const float *d_scores, const uint64_t *d_off, uint64_t total,
+                    const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+                    const hfx_spectra *sp, void *scratch, void *stream)
 {
     HfxPlainCands c{*db, d_begin, d_num, d_off};
-    int rc = launch_topn(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores,
-                         c, sp, stream);
-    if (rc != HFX_OK || sp->num_spec == 0) return rc;
-    HFX_REQUIRE(N <= TX_MAXN, HFX_ERR_ARG, "N must be <= %d", TX_MAXN);
-    hipLaunchKernelGGL(k_topn_exact<HfxPlainCands>, dim3(sp->num_spec), dim3(TN_THREADS), 0,
-                       hfx_stream(stream), d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
-                       (const uint32_t *)nullptr, (const uint32_t *)d_flag, c, *sp);
-    HFX_CHECK_LAUNCH();
-    return HFX_OK;
+    return launch_topn(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
+                       d_scores, c, total, sp, scratch, stream);
 }
 
 extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                         uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
-                        const float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-                        const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp, void *stream)
+                        const float *d_scores, const uint64_t *d_off, uint64_t total,
+                        const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+                        const hfx_spectra *sp, void *stream)
 {
-    (void)ctx;
     HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn: null argument");
-    HfxPlainCands c{*db, d_begin, d_num, d_off};
-    return launch_topn(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c,
-                       sp, stream);
+    return hfx_topn_device(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
+                           d_scores, d_off, total, d_begin, d_num, db, sp, nullptr, stream);
 }
 
 extern "C" int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
                               float *d_top_f32, uint32_t *d_top_cnt, uint32_t N,
-                              const uint32_t *d_spec_list, uint32_t num_list, const uint32_t *d_begin,
-                              const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp,
-                              void *stream)
+                              const uint32_t *d_spec_list, uint32_t num_list,
+                              const float *d_scores, const uint64_t *d_off, uint64_t total,
+                              const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+                              const hfx_spectra *sp, void *stream)
 {
-    (void)ctx;
-    HFX_REQUIRE(d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn_exact: null argument");
-    HfxPlainCands c{*db, d_begin, d_num, nullptr};
-    return launch_topn_exact(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list, num_list,
-                             c, sp, stream);
+    HFX_REQUIRE(d_off && d_begin && d_num && db && d_scores, HFX_ERR_ARG,
+                "hfx_topn_exact: null argument");
+    HfxPlainCands c{*db, d_begin, d_num, d_off};
+    return launch_topn_exact(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list,
+                             num_list, d_scores, c, total, sp, stream);
 }
 
 extern "C" int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
@@ -376,24 +498,23 @@ extern "C" int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exa
                             const hfx_modset *ms, const hfx_db *db, const hfx_spectra *sp,
                             void *stream)
 {
-    (void)ctx;
     HFX_REQUIRE(d_mp_off && ms && db, HFX_ERR_ARG, "hfx_topn_mod: null argument");
     HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_topn_mod: num_ma %u", ms->num_ma);
     HfxModCands c{*db, *ms, d_mp_off};
-    return launch_topn(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c,
-                       sp, stream);
+    return launch_topn(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
+                       d_scores, c, ms->num_mpep, sp, nullptr, stream);
 }
 
 extern "C" int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
                                   float *d_top_f32, uint32_t *d_top_cnt, uint32_t N,
                                   const uint32_t *d_spec_list, uint32_t num_list,
-                                  const uint64_t *d_mp_off, const hfx_modset *ms, const hfx_db *db,
-                                  const hfx_spectra *sp, void *stream)
+                                  const float *d_scores, const uint64_t *d_mp_off,
+                                  const hfx_modset *ms, const hfx_db *db, const hfx_spectra *sp,
+                                  void *stream)
 {
-    (void)ctx;
-    HFX_REQUIRE(d_mp_off && ms && db, HFX_ERR_ARG, "hfx_topn_exact_mod: null argument");
+    HFX_REQUIRE(d_mp_off && ms && db && d_scores, HFX_ERR_ARG, "hfx_topn_exact_mod: null argument");
     HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_topn_exact_mod: num_ma %u", ms->num_ma);
     HfxModCands c{*db, *ms, d_mp_off};
-    return launch_topn_exact(d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list, num_list,
-                             c, sp, stream);
+    return launch_topn_exact(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_spec_list,
+                             num_list, d_scores, c, ms->num_mpep, sp, stream);
 }

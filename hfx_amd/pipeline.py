@@ -313,21 +313,24 @@ class PlainSearch:
         st = stream_handle()
         check(L.hfx_topn(self.ctx.h, ptr(self.t_pos), ptr(self.t_exact), ptr(self.t_f32),
                          ptr(self.t_cnt), ptr(self.t_flag), self.topn, self.margin,
-                         ptr(self.t_scores), ptr(self.t_off), ptr(self.t_begin), ptr(self.t_num),
-                         C.byref(self.db.desc), C.byref(self.sp.desc), st), "hfx_topn")
+                         ptr(self.t_scores), ptr(self.t_off), self.total, ptr(self.t_begin),
+                         ptr(self.t_num), C.byref(self.db.desc), C.byref(self.sp.desc), st),
+              "hfx_topn")
 
-    def fallback(self) -> np.ndarray:
-        S = self.sp.num_spec
-        flags = host(self.t_flag, np.uint32, S)
-        lst = np.nonzero(flags)[0].astype(np.uint32)
+    def exact(self, spectra: np.ndarray):
+        """hfx_topn_exact for the listed spectra (every candidate exact)."""
+        lst = np.asarray(spectra, np.uint32)
         if len(lst):
             t_list = dev(lst)
             check(lib().hfx_topn_exact(self.ctx.h, ptr(self.t_pos), ptr(self.t_exact),
                                        ptr(self.t_f32), ptr(self.t_cnt), self.topn, ptr(t_list),
-                                       len(lst), ptr(self.t_begin), ptr(self.t_num),
-                                       C.byref(self.db.desc), C.byref(self.sp.desc),
-                                       stream_handle()), "hfx_topn_exact")
-        return lst
+                                       len(lst), ptr(self.t_scores), ptr(self.t_off), self.total,
+                                       ptr(self.t_begin), ptr(self.t_num), C.byref(self.db.desc),
+                                       C.byref(self.sp.desc), stream_handle()), "hfx_topn_exact")
+
+    def flagged(self) -> np.ndarray:
+        """Spectra that took the exact route in the last select()."""
+        return np.nonzero(host(self.t_flag, np.uint32, self.sp.num_spec))[0].astype(np.uint32)
 
     def run(self, keep_scores: bool = False) -> PlainResult:
         S, N = self.sp.num_spec, self.topn
@@ -335,7 +338,7 @@ class PlainSearch:
         if self.total:
             self.score()
             self.select()
-            flagged = self.fallback()
+            flagged = self.flagged()
         else:
             flagged = np.zeros(0, np.uint32)
             self.t_cnt.zero()
@@ -578,19 +581,15 @@ class ModSearch:
             check(L.hfx_topn_mod(self.ctx.h, t_pos.p, t_ex.p, t_f32.p, t_cnt.p, t_flag.p, N,
                                  self.margin, t_scores.p, t_mpoff.p, C.byref(self.modset),
                                  C.byref(db.desc), C.byref(self.sp.desc), None), "hfx_topn_mod")
+            # flagged spectra were re-ranked exactly inside hfx_topn_mod
             flags = host(t_flag, np.uint32, S)
             flagged = np.nonzero(flags)[0].astype(np.uint32)
-            if len(flagged):
-                t_l = dev(flagged)
-                check(L.hfx_topn_exact_mod(self.ctx.h, t_pos.p, t_ex.p, t_f32.p, t_cnt.p, N, t_l.p,
-                                           len(flagged), t_mpoff.p, C.byref(self.modset),
-                                           C.byref(db.desc), C.byref(self.sp.desc), None),
-                      "hfx_topn_exact_mod")
         else:
             t_cnt.zero()
         sync()
         # keep the device arrays alive as long as the result references them
         self._keep = (t_mp, t_unrank, t_pep, t_pmod, t_pcnt, t_nmp, t_comb, t_mpoff)
+        self._scored = (t_scores, t_mpoff, nmp)
         return ModResult(
             spec_cand=spec_cand, num_valid=int(nv), num_mpep=int(nmp), len_unrank=int(lu),
             mp_off=mp_off, top_pos=host(t_pos, np.uint32, S * N).reshape(S, N),
@@ -598,6 +597,26 @@ class ModSearch:
             top_f32=host(t_f32, np.float32, S * N).reshape(S, N),
             top_cnt=host(t_cnt, np.uint32, S), flagged=flagged, stages=st,
             scores=host(t_scores, np.float32, nmp) if keep_scores else None)
+
+
+    def exact(self, spectra: np.ndarray):
+        """hfx_topn_exact_mod for the listed spectra after run(): every modified
+        candidate scored exactly.  Returns (top_pos, top_exact, top_cnt)."""
+        t_scores, t_mpoff, nmp = self._scored
+        S, N = self.sp.num_spec, self.topn
+        t_pos, t_ex = empty(4 * S * N), empty(8 * S * N)
+        t_f32, t_cnt = empty(4 * S * N), empty(4 * S)
+        t_cnt.zero()
+        lst = np.asarray(spectra, np.uint32)
+        if nmp and len(lst):
+            t_l = dev(lst)
+            check(lib().hfx_topn_exact_mod(self.ctx.h, t_pos.p, t_ex.p, t_f32.p, t_cnt.p, N, t_l.p,
+                                           len(lst), t_scores.p, t_mpoff.p, C.byref(self.modset),
+                                           C.byref(self.db.desc), C.byref(self.sp.desc), None),
+                  "hfx_topn_exact_mod")
+        sync()
+        return (host(t_pos, np.uint32, S * N).reshape(S, N),
+                host(t_ex, np.float64, S * N).reshape(S, N), host(t_cnt, np.uint32, S))
 
 
 def merge_rankings(plain: PlainResult, mods: ModResult, n: int):

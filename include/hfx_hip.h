@@ -270,18 +270,24 @@ int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
  * (int64 fixed point, 2^-36 resolution, reported as double), and writes the
  * best N ordered by (exact score desc, candidate position asc):
  * d_top_pos[s*N + k] (position within the spectrum's window), d_top_exact,
- * d_top_f32, d_top_cnt[s].  d_flag[s] = 1 when a candidate outside the
- * re-scored set could still rank in the top N (FP32 error bound); flagged
- * spectra are completed by hfx_topn_exact. */
+ * d_top_f32, d_top_cnt[s].  Every candidate whose FP32 score lies within the
+ * FP32 error bound of the N-th exact score is re-scored too; a spectrum with
+ * more such candidates than the re-score set holds (exact ties) takes the
+ * exact route on the device: all its candidates scored exactly and the top N
+ * selected on the exact scores; d_flag[s] = 1 marks those spectra.  `total`
+ * = sum_s d_num[s].  N <= 64, N + margin <= 128. */
 int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
              uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
-             const float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-             const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp, void *stream);
+             const float *d_scores, const uint64_t *d_off, uint64_t total,
+             const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+             const hfx_spectra *sp, void *stream);
 
-/* Exact top-N of the listed spectra (all candidates scored in fixed point). */
+/* Exact top-N of the listed spectra (every candidate scored in fixed point;
+ * d_top_f32 taken from d_scores). */
 int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                    uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_spec_list,
-                   uint32_t num_list, const uint32_t *d_begin, const uint32_t *d_num,
+                   uint32_t num_list, const float *d_scores, const uint64_t *d_off,
+                   uint64_t total, const uint32_t *d_begin, const uint32_t *d_num,
                    const hfx_db *db, const hfx_spectra *sp, void *stream);
 
 /* Search.hs:381-386 on the device: spectrum s owns the modified peptides of
@@ -307,8 +313,9 @@ int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *
                  const hfx_db *db, const hfx_spectra *sp, void *stream);
 int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                        uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_spec_list,
-                       uint32_t num_list, const uint64_t *d_mp_off, const hfx_modset *ms,
-                       const hfx_db *db, const hfx_spectra *sp, void *stream);
+                       uint32_t num_list, const float *d_scores, const uint64_t *d_mp_off,
+                       const hfx_modset *ms, const hfx_db *db, const hfx_spectra *sp,
+                       void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched search (new; Main.hs:129-142 scores 16 spectra per group).    */

@@ -148,3 +148,22 @@ def test_merged_ranking(menv, orc):
         for a, b in zip(items, items[1:]):
             if a[0] == b[0]:
                 assert (a[1], a[2]) < (b[1], b[2])
+
+
+def test_exact_route_matches(menv, orc):
+    """hfx_topn_exact_mod for every spectrum: the all-exact ranking equals the
+    oracle's (which ranks by exact score)."""
+    res, mi, dsp, ms = menv["res"], menv["mi"], menv["dsp"], menv["ms"]
+    idx, _ = orc.sort_idx(menv["r"])
+    ions = orc.prepare_ions(menv["prot"].seq, mi.ma)
+    ref = orc.search_mods(ions, menv["r"], menv["tc"], menv["tn"], menv["mass"], idx, mi.ma,
+                          mi.ma_mass, mi.mod_ma_count, mi.mod_sum, mi.mod_delta, dsp.x(),
+                          dsp.lengths(), dsp.zmax, dsp.mass, 3.0, 5)
+    pos, ex, cnt = ms.exact(np.arange(dsp.num_spec, dtype=np.uint32))
+    assert (cnt == ref["top_cnt"]).all()
+    for s in range(dsp.num_spec):
+        c = int(cnt[s])
+        assert pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), s
+        assert (ex[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
+        # the gated route agrees with the all-exact route
+        assert pos[s, :c].tolist() == res.top_pos[s, :c].tolist(), s

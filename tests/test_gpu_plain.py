@@ -231,13 +231,10 @@ def test_exact_fallback_matches(env, orc):
     ref = _oracle_plain(orc, env, 5, want_all=False)
     srch = pl.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=5)
     srch.window()
+    srch.score()
     S = env["dsp"].num_spec
-    lst = pl.dev(np.arange(S, dtype=np.uint32))
-    hfx_amd.check(L.hfx_topn_exact(env["ctx"].h, pl.ptr(srch.t_pos), pl.ptr(srch.t_exact),
-                                   pl.ptr(srch.t_f32), pl.ptr(srch.t_cnt), 5, pl.ptr(lst), S,
-                                   pl.ptr(srch.t_begin), pl.ptr(srch.t_num),
-                                   C.byref(env["db"].desc), C.byref(env["dsp"].desc),
-                                   pl.stream_handle()), "topn_exact")
+    # every spectrum through the exact route (exact score of every candidate)
+    srch.exact(np.arange(S, dtype=np.uint32))
     pl.sync()
     pos = pl.host(srch.t_pos, np.uint32, S * 5).reshape(S, 5)
     ex = pl.host(srch.t_exact, np.float64, S * 5).reshape(S, 5)
