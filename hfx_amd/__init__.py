@@ -150,6 +150,15 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, _U64, _P, _P, C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_topn_rel": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, _U32, _U32, _F, _F, _P, _P, _U64, _P, _P, C.POINTER(HfxDb),
+         C.POINTER(HfxSpectra), _P],
+    ),
+    "hfx_xcorr_dense_bf16x2": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
+    ),
     "hfx_mpep_offsets": (_I, [_P, _P, _P, _U32, _P, _U32, _P]),
     "hfx_xcorr_sparse_mod": (
         _I,
@@ -173,6 +182,7 @@ _SIGS = {
     "hfx_batch_replay": (_I, [_P, _P]),
     "hfx_batch_autotune": (_I, [_P, _P, _P]),
     "hfx_batch_set_config": (_I, [_P, _U32, _U32, _U32]),
+    "hfx_batch_set_precision": (_I, [_P, _U32]),
     "hfx_batch_time": (_I, [_P, _P, _P]),
     "hfx_batch_info": (_I, [_P, _P]),
     "hfx_batch_outputs": (_I, [_P, _P]),

@@ -331,6 +331,9 @@ struct hfx_batch {
     double *top_exact;
     float *top_f32;
     uint8_t *tx_scratch; /* top-N exact route */
+    uint32_t precision;  /* HFX_BATCH_FP32_SPARSE or HFX_BATCH_BF16X2_DENSE */
+    uint32_t CH;         /* dense: candidates per work item */
+    uint8_t *dn_scratch; /* dense: hi/lo spectra, blocks (null if unsupported) */
     hipGraph_t graph;
     hipGraphExec_t exec;
 };
@@ -340,7 +343,7 @@ static void bt_free(hfx_batch *b)
     void *ps[] = {b->begin,    b->num,     b->order,  b->ubeg,      b->uend,    b->ntiles,
                   b->caps,     b->cnt,     b->counter, b->score_off, b->tile_off, b->ent_off,
                   b->ent,      b->scores,  b->top_pos, b->top_cnt,   b->flag,     b->top_exact,
-                  b->top_f32,  b->tx_scratch};
+                  b->top_f32,  b->tx_scratch, b->dn_scratch};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -385,6 +388,7 @@ static int dalloc(T **p, size_t n)
  * -> (G, C, threads). */
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t>> g_tune;
+static std::map<std::tuple<int, int, uint32_t>, uint32_t> g_tune_dense;
 
 static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
 {
@@ -424,7 +428,13 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     HFX_CHECK_HIP(hipStreamSynchronize(st));
     std::vector<uint32_t> ord(S);
     std::iota(ord.begin(), ord.end(), 0u);
-    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return hm[x] < hm[y]; });
+    /* by fragment charge level, then precursor mass: blocks of neighbouring
+     * spectra share one ladder level and most of their windows */
+    auto zlev = [&](uint32_t s) { return hz[s] <= HFX_MAX_CHARGE ? hz[s] : 0u; };
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+        const uint32_t zx = zlev(x), zy = zlev(y);
+        return zx != zy ? zx < zy : hm[x] < hm[y];
+    });
     HFX_CHECK_HIP(hipMemcpyAsync(b->order, ord.data(), 4ull * S, hipMemcpyHostToDevice, st));
     uint32_t p0 = 0xffffffffu, p1 = 0, L = 0, lmax = 0;
     for (uint32_t s = 0; s < S; ++s) {
@@ -445,10 +455,14 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     b->G = 8;
     b->C = 64;
     b->threads = 512;
+    b->CH = 256;
+    b->precision = HFX_BATCH_FP32_SPARSE;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(tune_key(b));
         if (it != g_tune.end()) std::tie(b->G, b->C, b->threads) = it->second;
+        auto jt = g_tune_dense.find(tune_key(b));
+        if (jt != g_tune_dense.end()) b->CH = jt->second;
     }
     const uint32_t P = b->P, Lv = b->L;
     /* index capacities and offsets */
@@ -473,6 +487,9 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     /* the top-N exact route's scratch is owned by the batch, so that a
      * captured graph never allocates and never sees it move */
     if ((rc = dalloc(&b->tx_scratch, hfx_topn_scratch_bytes(S, b->total)))) return rc;
+    if (sp->x_stride <= hfx_dense_max_len() &&
+        (rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(S))))
+        return rc;
     /* blocks are sized for the smallest G the tuner may pick (4) */
     const uint32_t bmax = (S + 3) / 4;
     if ((rc = dalloc(&b->ubeg, bmax)) || (rc = dalloc(&b->uend, bmax)) ||
@@ -522,26 +539,78 @@ static int bt_score(hfx_batch *b, hipStream_t st)
     return HFX_OK;
 }
 
+/* the phases of one batched search */
+static int bt_windows(hfx_batch *b, hipStream_t st)
+{
+    int rc;
+    if ((rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, (void *)st)))
+        return rc;
+    return hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, (void *)st);
+}
+
+static int bt_index(hfx_batch *b, hipStream_t st)
+{
+    if (b->precision != HFX_BATCH_FP32_SPARSE || b->P == 0) return HFX_OK;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((b->P + IX_WAVES - 1) / IX_WAVES,
+                                                       (uint64_t)hfx_ctx_num_cus(b->ctx) * 16);
+    hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, b->ent, b->cnt, b->ent_off,
+                       b->db, b->p0, b->P, b->L, b->lcap);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+static int bt_scores(hfx_batch *b, hipStream_t st)
+{
+    int rc;
+    if (b->precision == HFX_BATCH_BF16X2_DENSE) {
+        if (b->total == 0) return HFX_OK;
+        return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order, &b->db,
+                                &b->sp, b->CH, b->dn_scratch, (void *)st);
+    }
+    if ((rc = bt_tiles(b, st))) return rc;
+    return bt_score(b, st);
+}
+
+/* |dense - exact| <= (2^-16 + 210 * 2^-24) * 50 * sum|x|: the hi/lo split
+ * (2^-16 per bin) plus at most 100 FP32 additions on any term's path (32 in
+ * one MFMA, one per k-step after it) and the hi + lo sum, each allowed 2 ulp */
+#define BT_REL_DENSE (1.52587890625e-05f + 210.0f * 5.9604644775390625e-08f)
+#define BT_REL_F32 (80.0f * 5.9604644775390625e-08f)
+
+static int bt_topn(hfx_batch *b, hipStream_t st)
+{
+    const float rel0 = b->precision == HFX_BATCH_BF16X2_DENSE ? BT_REL_DENSE : BT_REL_F32;
+    return hfx_topn_device(b->ctx, b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag,
+                           b->N, b->margin, b->scores, b->score_off, b->total, b->begin, b->num,
+                           &b->db, &b->sp, rel0, 0.0f, b->tx_scratch, (void *)st);
+}
+
 extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_run: batch not planned");
     hipStream_t st = hfx_stream(stream);
     int rc;
     if (b->S == 0) return HFX_OK;
-    if ((rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, stream)))
-        return rc;
-    if ((rc = hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, stream))) return rc;
-    if (b->P) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((b->P + IX_WAVES - 1) / IX_WAVES,
-                                                           (uint64_t)hfx_ctx_num_cus(b->ctx) * 16);
-        hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, b->ent, b->cnt, b->ent_off,
-                           b->db, b->p0, b->P, b->L, b->lcap);
+    if ((rc = bt_windows(b, st)) || (rc = bt_index(b, st)) || (rc = bt_scores(b, st))) return rc;
+    return bt_topn(b, st);
+}
+
+extern "C" int hfx_batch_set_precision(hfx_batch *b, uint32_t precision)
+{
+    HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_precision: batch not planned");
+    HFX_REQUIRE(precision == HFX_BATCH_FP32_SPARSE || precision == HFX_BATCH_BF16X2_DENSE,
+                HFX_ERR_ARG, "hfx_batch_set_precision: unknown precision %u", precision);
+    HFX_REQUIRE(precision != HFX_BATCH_BF16X2_DENSE || b->dn_scratch, HFX_ERR_ARG,
+                "hfx_batch_set_precision: the dense form takes spectra of up to %u bins "
+                "(x_stride %u)", hfx_dense_max_len(), b->sp.x_stride);
+    if (precision != b->precision && b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
     }
-    if ((rc = bt_tiles(b, st))) return rc;
-    if ((rc = bt_score(b, st))) return rc;
-    return hfx_topn_device(b->ctx, b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag,
-                           b->N, b->margin, b->scores, b->score_off, b->total, b->begin, b->num,
-                           &b->db, &b->sp, b->tx_scratch, stream);
+    b->precision = precision;
+    return HFX_OK;
 }
 
 extern "C" int hfx_batch_capture(hfx_batch *b, void *stream)
@@ -583,42 +652,68 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     hipEvent_t e0, e1;
     HFX_CHECK_HIP(hipEventCreate(&e0));
     HFX_CHECK_HIP(hipEventCreate(&e1));
-    const uint32_t Gs[2] = {4, 8}, Cs[4] = {32, 64, 128, 256}, Ts[2] = {256, 512};
+    auto time2 = [&](float *ms) -> int {
+        int r;
+        if ((r = bt_scores(b, st))) return r; /* warm */
+        HFX_CHECK_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < 2; ++k)
+            if ((r = bt_scores(b, st))) return r;
+        HFX_CHECK_HIP(hipEventRecord(e1, st));
+        HFX_CHECK_HIP(hipEventSynchronize(e1));
+        HFX_CHECK_HIP(hipEventElapsedTime(ms, e0, e1));
+        return HFX_OK;
+    };
     float best = 1e30f;
-    uint32_t bG = b->G, bC = b->C, bT = b->threads;
-    for (uint32_t G : Gs)
-        for (uint32_t C : Cs)
-            for (uint32_t T : Ts) {
-                b->G = G;
-                b->C = C;
-                b->threads = T;
-                if ((rc = bt_tiles(b, st))) return rc;
-                if ((rc = bt_score(b, st))) return rc; /* warm */
-                HFX_CHECK_HIP(hipEventRecord(e0, st));
-                for (int k = 0; k < 2; ++k)
-                    if ((rc = bt_score(b, st))) return rc;
-                HFX_CHECK_HIP(hipEventRecord(e1, st));
-                HFX_CHECK_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                HFX_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; bG = G; bC = C; bT = T; }
-            }
+    uint32_t bG = b->G, bC = b->C, bT = b->threads, bCH = b->CH;
+    if (b->precision == HFX_BATCH_BF16X2_DENSE) {
+        const uint32_t CHs[5] = {64, 128, 256, 512, 1024};
+        for (uint32_t CH : CHs) {
+            b->CH = CH;
+            float ms = 0;
+            if ((rc = time2(&ms))) return rc;
+            if (ms < best) { best = ms; bCH = CH; }
+        }
+    } else {
+        const uint32_t Gs[2] = {4, 8}, Cs[4] = {32, 64, 128, 256}, Ts[2] = {256, 512};
+        for (uint32_t G : Gs)
+            for (uint32_t C : Cs)
+                for (uint32_t T : Ts) {
+                    b->G = G;
+                    b->C = C;
+                    b->threads = T;
+                    float ms = 0;
+                    if ((rc = time2(&ms))) return rc;
+                    if (ms < best) { best = ms; bG = G; bC = C; bT = T; }
+                }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->G = bG;
     b->C = bC;
     b->threads = bT;
+    b->CH = bCH;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[tune_key(b)] = std::make_tuple(bG, bC, bT);
+        if (b->precision == HFX_BATCH_BF16X2_DENSE) g_tune_dense[tune_key(b)] = bCH;
+        else g_tune[tune_key(b)] = std::make_tuple(bG, bC, bT);
     }
-    if (out_cfg) { out_cfg[0] = bG; out_cfg[1] = bC; out_cfg[2] = bT; }
+    if (b->exec) {
+        /* the captured graph holds the old launch shapes */
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    if (out_cfg) {
+        out_cfg[0] = bG; out_cfg[1] = bC; out_cfg[2] = bT;
+        if (b->precision == HFX_BATCH_BF16X2_DENSE) { out_cfg[0] = 0; out_cfg[1] = bCH; out_cfg[2] = 256; }
+    }
     return HFX_OK;
 }
 
 /* Runs the planned batch phase by phase with events between the phases:
- * ms[0] windows + offsets, ms[1] ion index, ms[2] blocks/tiles + scoring
- * kernel, ms[3] top-N (+ gated exact fallback). */
+ * ms[0] windows + offsets, ms[1] ion index (sparse form only), ms[2] blocks,
+ * tiles + scoring kernel, ms[3] top-N (+ exact route). */
 extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
 {
     HFX_REQUIRE(b && b->planned && ms, HFX_ERR_ARG, "hfx_batch_time: batch not planned");
@@ -627,23 +722,13 @@ extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
     for (auto &e : ev) HFX_CHECK_HIP(hipEventCreate(&e));
     int rc = HFX_OK;
     HFX_CHECK_HIP(hipEventRecord(ev[0], st));
-    if (!rc) rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, stream);
-    if (!rc) rc = hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, stream);
+    if (!rc) rc = bt_windows(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[1], st));
-    if (!rc && b->P) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((b->P + IX_WAVES - 1) / IX_WAVES,
-                                                           (uint64_t)hfx_ctx_num_cus(b->ctx) * 16);
-        hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, b->ent, b->cnt, b->ent_off,
-                           b->db, b->p0, b->P, b->L, b->lcap);
-    }
+    if (!rc) rc = bt_index(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[2], st));
-    if (!rc) rc = bt_tiles(b, st);
-    if (!rc) rc = bt_score(b, st);
+    if (!rc) rc = bt_scores(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[3], st));
-    if (!rc)
-        rc = hfx_topn_device(b->ctx, b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag,
-                             b->N, b->margin, b->scores, b->score_off, b->total, b->begin, b->num,
-                             &b->db, &b->sp, b->tx_scratch, stream);
+    if (!rc) rc = bt_topn(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[4], st));
     HFX_CHECK_HIP(hipEventSynchronize(ev[4]));
     for (int i = 0; i < 4; ++i) HFX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
@@ -654,6 +739,13 @@ extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
 extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_config: batch not planned");
+    if (b->precision == HFX_BATCH_BF16X2_DENSE) {
+        /* dense form: C = candidates per work item (G, threads fixed) */
+        HFX_REQUIRE(C >= 16 && C % 16 == 0 && C <= 8192, HFX_ERR_ARG,
+                    "hfx_batch_set_config: dense chunk must be a multiple of 16 in 16..8192");
+        b->CH = C;
+        return HFX_OK;
+    }
     HFX_REQUIRE((G == 4 || G == 8) && C >= 1 && C <= 4096 && (threads == 256 || threads == 512),
                 HFX_ERR_ARG, "hfx_batch_set_config: G in {4,8}, C in 1..4096, threads in {256,512}");
     b->G = G;
@@ -662,11 +754,12 @@ extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32
     return HFX_OK;
 }
 
-extern "C" int hfx_batch_info(const hfx_batch *b, uint64_t *info /*[10]*/)
+extern "C" int hfx_batch_info(const hfx_batch *b, uint64_t *info /*[12]*/)
 {
     HFX_REQUIRE(b && info, HFX_ERR_ARG, "hfx_batch_info: null argument");
     info[0] = b->S; info[1] = b->total; info[2] = b->P; info[3] = b->L; info[4] = b->n_ent;
     info[5] = b->G; info[6] = b->C; info[7] = b->threads; info[8] = b->lstr; info[9] = b->p0;
+    info[10] = b->precision; info[11] = b->CH;
     return HFX_OK;
 }
 

@@ -96,14 +96,24 @@ uint32_t hfx_sum_u32_to_host(hfx_ctx *ctx, const uint32_t *d_a, uint32_t n);
 int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream);
 /* device scratch the top-N exact route needs for S spectra, total scores */
 size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total);
-/* hfx_topn over the unmodified candidates (batched path; no host sync).
+/* hfx_topn over the unmodified candidates (batched path; no host sync), with
+ * the error bound (rel0 + rel1 * fragment charges) * 50 * sum|x| + 1e-6.
  * scratch: hfx_topn_scratch_bytes(S, total) bytes of device memory, or null
  * for the context's own slot */
 int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                     uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
                     const float *d_scores, const uint64_t *d_off, uint64_t total,
                     const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
-                    const hfx_spectra *sp, void *scratch, void *stream);
+                    const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream);
+/* dense BF16 hi+lo scores (same layout as hfx_xcorr_sparse); spectra are
+ * blocked eight at a time in d_order (null: index order), CH candidates per
+ * work item; scratch: hfx_dense_scratch_bytes(S) bytes or null for the
+ * context's slot */
+size_t hfx_dense_scratch_bytes(uint32_t S);
+uint32_t hfx_dense_max_len(void);
+int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
+                     const uint32_t *d_num, const uint32_t *d_order, const hfx_db *db,
+                     const hfx_spectra *sp, uint32_t CH, void *scratch, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device helpers                                                        */

@@ -1,0 +1,476 @@
+/*
+ * hfx_dense.hip -- XCorr of spectra that share a precursor window as a dense
+ * BF16 MFMA contraction (precision form), FP32 accumulation.
+ *
+ * The reference builds, per (spectrum, candidate), a dense theoretical row of
+ * len bins (weights 50/25/10, max per bin; ion_series.cu:111-232) and takes
+ * its dot product with the spectrum through cuBLAS sgemv (Search.hs:242-245).
+ * Here eight spectra that are neighbours in precursor mass form a block; every
+ * candidate of the block's union window gets ONE theoretical row, and the
+ * rows meet all eight spectra in one matrix product on the MFMA units:
+ *
+ *   A [16 candidates x K bins]   theoretical weights, exact in BF16
+ *   B [K bins x 16 columns]      column 2g = hi(x_g), 2g+1 = lo(x_g), where
+ *                                hi = bf16_rne(x), lo = bf16_rne(x - hi)
+ *   C [16 x 16] FP32             score(c, g) = C[c][2g] + C[c][2g+1]
+ *
+ * with v_mfma_f32_16x16x32_bf16 (gfx950).  Error: |x - hi - lo| <= 2^-16 |x|
+ * per bin, products are exact in FP32, and a zero term adds exactly, so
+ * |score - exact| <= (2^-16 + (m + 2) 2^-24) * sum_b w_b |x_b| with m the
+ * number of non-zero bins of the row (<= 10 * residues * fragment charges);
+ * the top-N stage re-scores exactly with that bound.
+ *
+ * Layout and work split (one workgroup of four waves per CU, persistent):
+ *   - k_dn_split writes every spectrum once as hi/lo BF16 rows of DN_MAXK bins
+ *     (zero beyond its length), bins permuted inside each group of eight so
+ *     that the A fragment built below needs no shuffles;
+ *   - a work item is (block, fragment charge level, chunk of CH candidates);
+ *     the workgroup stages the block's 16 B columns in LDS (once per block);
+ *   - each wave takes 16 candidates: lanes 0-15 run the sequential FP32 b
+ *     ladder of one candidate each (bit-exact prefix), then every lane takes
+ *     one ladder position and ORs the unary weight codes of its peaks into the
+ *     wave's LDS nibble map (OR of codes = max weight); the map is stored
+ *     [row][k-group][k-step] so that one 128-bit LDS read gives a lane its A
+ *     words for four k-steps;
+ *   - A words (8 nibbles) become eight BF16 weights with four v_perm lookups
+ *     (codes 0/1/3/7 select the high and low bytes of 0, 10, 25, 50) and four
+ *     byte interleaves; the map is cleared as it is read.
+ * Bins use the same canonical arithmetic as the sparse path: the divisions
+ * by the bin width and by the fragment charge are computed as
+ * q = x*R, r = fma(-q, d, x), q + r*R, which equals the correctly rounded
+ * x / d for every x the ladders produce (checked exhaustively on the CPU).
+ */
+#include "hfx_cand.h"
+
+#include <algorithm>
+
+#define DN_NS 8                     /* spectra per block */
+#define DN_COLS 16                  /* MFMA N: hi and lo of each spectrum */
+#define DN_WAVES 4
+#define DN_THREADS (DN_WAVES * HFX_WAVE)
+#define DN_ROWS 16                  /* candidates per wave tile (MFMA M) */
+#define DN_MAXK 2176                /* bins of a row (multiple of 128) */
+#define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
+#define DN_MSTR (4 * DN_NKS)        /* map words per row */
+#define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
+#define DN_ITEMS 768                /* ladder positions per wave pass */
+#define DN_NONE 0xffffffffu
+
+#define DN_B_BYTES (DN_COLS * DN_CSTR * 2)
+#define DN_MAP_BYTES (DN_WAVES * DN_ROWS * DN_MSTR * 4)
+#define DN_ITEMB_BYTES (DN_WAVES * DN_ITEMS * 4)
+#define DN_ITEMC_BYTES (DN_WAVES * DN_ITEMS)
+#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES)
+
+typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t dn_u32x4 __attribute__((ext_vector_type(4)));
+
+/* ------------------------------------------------------------------ */
+/* Arithmetic                                                            */
+/* ------------------------------------------------------------------ */
+
+/* x / d correctly rounded, given R = RN(1/d): q = RN(x R) is within an ulp,
+ * the residual x - q d is exact in one FMA, and one Newton step rounds to
+ * the nearest quotient.  Holds for every |x| in [1e-3, 65536) for the bin
+ * width and d = 1..4 (exhaustive CPU check); below 1e-3 both sides round to
+ * bin 0, above 65536 every bin lies beyond the row. */
+__device__ __forceinline__ float dn_div(float x, float d, float R)
+{
+    const float q = x * R;
+    const float r = __builtin_fmaf(-q, d, x);
+    return __builtin_fmaf(r, R, q);
+}
+
+__device__ __forceinline__ int32_t dn_bin(float mz)
+{
+    const float R = 1.0f / HFX_BIN_WIDTH_MONO;
+    return (int32_t)__builtin_rintf(dn_div(mz, HFX_BIN_WIDTH_MONO, R));
+}
+
+/* round-to-nearest-even FP32 -> BF16 (finite inputs) */
+__device__ __forceinline__ uint32_t dn_bf16(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+/* Eight unary weight codes (one nibble per bin, bins 0..7 of a group) to the
+ * eight BF16 of an A fragment, in slot order bins 0,2,4,6,1,3,5,7.  Byte pools
+ * for v_perm: code c selects byte c; lo bytes of (0, 10, 25, 50) = 00 20 C8 48,
+ * hi bytes = 00 41 41 42. */
+__device__ __forceinline__ dn_u32x4 dn_weights(uint32_t w)
+{
+    const uint32_t e = w & 0x0f0f0f0fu;
+    const uint32_t o = (w >> 4) & 0x0f0f0f0fu;
+    const uint32_t lo_e = __builtin_amdgcn_perm(0x48000000u, 0xc8002000u, e);
+    const uint32_t hi_e = __builtin_amdgcn_perm(0x42000000u, 0x41004100u, e);
+    const uint32_t lo_o = __builtin_amdgcn_perm(0x48000000u, 0xc8002000u, o);
+    const uint32_t hi_o = __builtin_amdgcn_perm(0x42000000u, 0x41004100u, o);
+    dn_u32x4 a;
+    a.x = __builtin_amdgcn_perm(hi_e, lo_e, 0x05010400u);
+    a.y = __builtin_amdgcn_perm(hi_e, lo_e, 0x07030602u);
+    a.z = __builtin_amdgcn_perm(hi_o, lo_o, 0x05010400u);
+    a.w = __builtin_amdgcn_perm(hi_o, lo_o, 0x07030602u);
+    return a;
+}
+
+/* ------------------------------------------------------------------ */
+/* Spectra as hi/lo BF16 rows                                            */
+/* ------------------------------------------------------------------ */
+
+/* xb[s][0][:] = hi, xb[s][1][:] = lo, DN_MAXK bins each; within each group of
+ * eight bins slot j holds bin {0,2,4,6,1,3,5,7}[j]. */
+__global__ void k_dn_split(uint16_t *xb, hfx_spectra sp)
+{
+    const uint32_t s = blockIdx.x;
+    const uint32_t len = min(sp.d_len[s], min(sp.x_stride, (uint32_t)DN_MAXK));
+    const float *x = sp.d_x + (size_t)s * sp.x_stride;
+    uint16_t *hi = xb + (size_t)s * 2 * DN_MAXK, *lo = hi + DN_MAXK;
+    for (uint32_t q = threadIdx.x; q < DN_MAXK / 8; q += blockDim.x) {
+        uint32_t h[8], l[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t bin = q * 8 + j;
+            const float v = bin < len ? x[bin] : 0.0f;
+            h[j] = dn_bf16(v);
+            l[j] = dn_bf16(v - __uint_as_float(h[j] << 16));
+        }
+        uint4 vh, vl;
+        vh.x = h[0] | h[2] << 16; vh.y = h[4] | h[6] << 16;
+        vh.z = h[1] | h[3] << 16; vh.w = h[5] | h[7] << 16;
+        vl.x = l[0] | l[2] << 16; vl.y = l[4] | l[6] << 16;
+        vl.z = l[1] | l[3] << 16; vl.w = l[5] | l[7] << 16;
+        *(uint4 *)(hi + q * 8) = vh;
+        *(uint4 *)(lo + q * 8) = vl;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Blocks                                                                 */
+/* ------------------------------------------------------------------ */
+
+__device__ __forceinline__ uint32_t dn_order(const uint32_t *order, uint32_t i)
+{
+    return order ? order[i] : i;
+}
+
+/* block b = spectra order[8b .. 8b+7]: union window, fragment charge levels
+ * present (bit z), row length kb (multiple of 128), work items */
+__global__ void k_dn_blocks(uint32_t *ubeg, uint32_t *uend, uint32_t *binfo, uint32_t *ntiles,
+                            const uint32_t *order, const uint32_t *begin, const uint32_t *num,
+                            hfx_spectra sp, uint32_t B, uint32_t CH)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    uint32_t lo = 0xffffffffu, hi = 0, mask = 0, kmax = 0;
+    for (uint32_t g = 0; g < DN_NS; ++g) {
+        const uint32_t i = b * DN_NS + g;
+        if (i >= sp.num_spec) break;
+        const uint32_t s = dn_order(order, i);
+        const uint32_t z = hfx_zeff(sp.d_zmax[s]);
+        if (z == 0 || num[s] == 0) continue;
+        lo = min(lo, begin[s]);
+        hi = max(hi, begin[s] + num[s]);
+        mask |= 1u << z;
+        kmax = max(kmax, min(sp.d_len[s], min(sp.x_stride, (uint32_t)DN_MAXK)));
+    }
+    if (hi <= lo) { lo = 0; hi = 0; mask = 0; }
+    const uint32_t kb = min((kmax + 127u) & ~127u, (uint32_t)DN_MAXK);
+    ubeg[b] = lo;
+    uend[b] = hi;
+    binfo[b] = kb | mask << 16;
+    ntiles[b] = ((hi - lo + CH - 1) / CH) * (uint32_t)__builtin_popcount(mask);
+}
+
+__global__ void k_dn_reset(uint32_t *counter) { *counter = 0; }
+
+/* spectra whose fragment charge is not generated score 0 (no launch in the
+ * reference, ion_series.cu:334-342) */
+__global__ void k_dn_zero_unscored(float *scores, const uint64_t *off, const uint32_t *zmax,
+                                   uint32_t S)
+{
+    const uint32_t s = blockIdx.x;
+    if (s >= S || hfx_zeff(zmax[s]) != 0) return;
+    for (uint64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x) scores[i] = 0.0f;
+}
+
+/* ------------------------------------------------------------------ */
+/* The contraction                                                        */
+/* ------------------------------------------------------------------ */
+
+struct DnArgs {
+    float *scores;
+    const uint64_t *score_off;
+    const uint32_t *begin, *num, *order;
+    uint32_t B, CH;
+    const uint32_t *ubeg, *uend, *binfo;
+    const uint64_t *tile_off;
+    const uint16_t *xb;
+    uint32_t *counter;
+    hfx_db db;
+    hfx_spectra sp;
+};
+
+struct DnZ {
+    float fz, zh, R, h2o, nh3;
+};
+
+/* peaks of one ladder position (b, y) at charge z into a map row: the ion
+ * series of ion_series.cu:111-156 with the FP32 operation order of the
+ * sparse path */
+__device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, const DnZ &q,
+                                         uint32_t kb)
+{
+#define DN_PUT(xx, code)                                                                \
+    do {                                                                                \
+        const int32_t x_ = (xx);                                                        \
+        if ((uint32_t)x_ < kb)                                                          \
+            atomicOr(&row[(((uint32_t)x_ >> 3) & 3u) * DN_NKS + ((uint32_t)x_ >> 5)],   \
+                     (code) << (((uint32_t)x_ & 7u) * 4u));                             \
+    } while (0)
+    DN_PUT(dn_bin(dn_div((b - HFX_DEV_CO) + q.zh, q.fz, q.R)), HFX_CODE10);
+    const float mb = dn_div(b + q.zh, q.fz, q.R);
+    const int32_t xb = dn_bin(mb);
+    DN_PUT(xb, HFX_CODE50);
+    DN_PUT(xb + 1, HFX_CODE25);
+    DN_PUT(xb - 1, HFX_CODE25);
+    DN_PUT(dn_bin(mb - q.h2o), HFX_CODE10);
+    DN_PUT(dn_bin(mb - q.nh3), HFX_CODE10);
+    const float my = dn_div((y + HFX_DEV_H2O) + q.zh, q.fz, q.R);
+    const int32_t xy = dn_bin(my);
+    DN_PUT(xy, HFX_CODE50);
+    DN_PUT(xy + 1, HFX_CODE25);
+    DN_PUT(xy - 1, HFX_CODE25);
+    DN_PUT(dn_bin(my - q.nh3), HFX_CODE10);
+#undef DN_PUT
+}
+
+__global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
+{
+    extern __shared__ __align__(16) unsigned char dn_lds[];
+    uint16_t *Bl = (uint16_t *)dn_lds;
+    uint32_t *maps = (uint32_t *)(dn_lds + DN_B_BYTES);
+    float *itemb_all = (float *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES);
+    uint8_t *itemc_all = dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES;
+    __shared__ float mtab[32];
+    __shared__ uint32_t m_s[DN_NS], m_beg[DN_NS], m_end[DN_NS], m_z[DN_NS];
+    __shared__ uint64_t m_off[DN_NS];
+    __shared__ float rr_all[DN_WAVES][DN_ROWS];
+    __shared__ uint32_t sh_t;
+
+    const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
+    uint32_t *map = maps + wave * DN_ROWS * DN_MSTR;
+    float *itemb = itemb_all + wave * DN_ITEMS;
+    uint8_t *itemc = itemc_all + wave * DN_ITEMS;
+    float *rr = rr_all[wave];
+    if (tid < 32) mtab[tid] = tid < 26 ? a.db.d_mass[tid] : 0.0f;
+    for (uint32_t i = tid; i < DN_WAVES * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
+    const uint64_t ntiles = a.tile_off[a.B];
+    const uint32_t row = lane & 15u, kg = lane >> 4;
+    uint32_t cur = DN_NONE;
+    for (;;) {
+        __syncthreads(); /* every wave is done with the previous item */
+        if (tid == 0) sh_t = atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint64_t t = sh_t;
+        if (t >= ntiles) break;
+        uint32_t lo = 0, hi = a.B;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.tile_off[mid] <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t b = lo;
+        const uint32_t info = a.binfo[b];
+        const uint32_t kb = info & 0xffffu, mask = info >> 16;
+        if (b != cur) {
+            if (tid < DN_NS) {
+                const uint32_t i = b * DN_NS + tid;
+                uint32_t s = DN_NONE, be = 0, en = 0, z = 0;
+                uint64_t of = 0;
+                if (i < a.sp.num_spec) {
+                    s = dn_order(a.order, i);
+                    be = a.begin[s];
+                    en = be + a.num[s];
+                    z = hfx_zeff(a.sp.d_zmax[s]);
+                    of = a.score_off[s];
+                }
+                m_s[tid] = s; m_beg[tid] = be; m_end[tid] = en; m_z[tid] = z; m_off[tid] = of;
+            }
+            __syncthreads();
+            const uint32_t nvec = kb / 8;
+            for (uint32_t i = tid; i < DN_COLS * nvec; i += DN_THREADS) {
+                const uint32_t col = i / nvec, v = i - col * nvec, s = m_s[col >> 1];
+                uint4 val = make_uint4(0u, 0u, 0u, 0u);
+                if (s != DN_NONE)
+                    val = *(const uint4 *)(a.xb + ((size_t)s * 2 + (col & 1u)) * DN_MAXK + v * 8);
+                *(uint4 *)(Bl + col * DN_CSTR + v * 8) = val;
+            }
+            cur = b;
+            __syncthreads();
+        }
+        const uint32_t beg = a.ubeg[b], end = a.uend[b];
+        const uint32_t nch = (end - beg + a.CH - 1) / a.CH;
+        const uint32_t local = (uint32_t)(t - a.tile_off[b]);
+        const uint32_t li = local / nch, chunk = local - li * nch;
+        uint32_t m = mask;
+        for (uint32_t q = 0; q < li; ++q) m &= m - 1u;
+        const uint32_t lvl = (uint32_t)__builtin_ctz(m);
+        DnZ zc[HFX_MAX_CHARGE];
+#pragma unroll
+        for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z) {
+            const float fz = (float)z;
+            zc[z - 1] = DnZ{fz, HFX_DEV_H * fz, 1.0f / fz, HFX_DEV_H2O / fz, HFX_DEV_NH3 / fz};
+        }
+        const uint32_t cbeg = beg + chunk * a.CH, cend = min(cbeg + a.CH, end);
+        for (uint32_t p0 = cbeg + wave * DN_ROWS; p0 < cend; p0 += DN_WAVES * DN_ROWS) {
+            const uint32_t nrows = min((uint32_t)DN_ROWS, cend - p0);
+            /* candidates of the tile (lanes 0..nrows-1) */
+            uint32_t tc = 0, n = 0;
+            float r = 0.0f;
+            if (lane < nrows) {
+                const uint32_t pep = a.db.d_idx_sorted[p0 + lane];
+                tc = a.db.d_tc[pep];
+                const uint32_t tn = a.db.d_tn[pep];
+                n = tn > tc ? min(tn - tc, (uint32_t)DN_ITEMS) : 0u;
+                r = a.db.d_r[pep];
+            }
+            if (lane < DN_ROWS) rr[lane] = r;
+            uint32_t incl = n;
+#pragma unroll
+            for (uint32_t o = 1; o < DN_ROWS; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            /* ladder positions in passes of at most DN_ITEMS */
+            uint32_t c_lo = 0;
+            while (c_lo < nrows) {
+                const uint32_t base = c_lo ? __shfl(incl, c_lo - 1, 64) : 0u;
+                const bool fits = lane >= c_lo && lane < nrows && incl - base <= DN_ITEMS;
+                const uint64_t bal = __ballot(fits);
+                const uint32_t c_hi = bal ? 64u - (uint32_t)__builtin_clzll(bal) : c_lo + 1;
+                const uint32_t nitems = __shfl(incl, c_hi - 1, 64) - base;
+                if (lane >= c_lo && lane < c_hi) {
+                    /* sequential FP32 b ladder of candidate `lane` */
+                    uint32_t o = incl - n - base;
+                    float bsum = 0.0f;
+                    const uint8_t *ion = a.db.d_ions + tc;
+                    for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+                        uint32_t v[8];
+#pragma unroll
+                        for (uint32_t j = 0; j < 8; ++j) v[j] = k0 + j < n ? ion[k0 + j] : 0u;
+#pragma unroll
+                        for (uint32_t j = 0; j < 8; ++j) {
+                            if (k0 + j < n) {
+                                bsum = bsum + mtab[HFX_ACID_IDX(v[j])];
+                                itemb[o] = bsum;
+                                itemc[o] = (uint8_t)lane;
+                                ++o;
+                            }
+                        }
+                    }
+                }
+                hfx_wave_sync();
+                for (uint32_t i = lane; i < nitems; i += HFX_WAVE) {
+                    const float bb = itemb[i];
+                    const uint32_t c = itemc[i];
+                    const float y = rr[c] - bb;
+                    uint32_t *mrow = map + c * DN_MSTR;
+#pragma unroll
+                    for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z)
+                        if (z <= lvl) dn_peaks(mrow, bb, y, zc[z - 1], kb);
+                }
+                hfx_wave_sync();
+                c_lo = c_hi;
+            }
+            /* 16 x 16 x kb contraction, clearing the map */
+            dn_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            uint32_t *arow = map + row * DN_MSTR + kg * DN_NKS;
+            const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
+            const uint32_t nks = kb >> 5;
+            for (uint32_t ks = 0; ks < nks; ks += 4) {
+                const dn_u32x4 a4 = *(const dn_u32x4 *)(arow + ks);
+                *(dn_u32x4 *)(arow + ks) = dn_u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const dn_u32x4 av = dn_weights(a4[u]);
+                    const dn_u32x4 bv = *(const dn_u32x4 *)(bcol + (ks + u) * 32);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, av), __builtin_bit_cast(dn_bf16x8, bv), acc,
+                        0, 0, 0);
+                }
+            }
+            hfx_wave_sync();
+            /* C[row 4*(lane/16)+i][col lane%16]; hi + lo of spectrum col/2 */
+#pragma unroll
+            for (uint32_t i = 0; i < 4; ++i) {
+                const float v = acc[i] + __shfl_xor(acc[i], 1, 64);
+                const uint32_t rr_i = 4u * kg + i, g = row >> 1, p = p0 + rr_i;
+                if ((lane & 1u) == 0 && rr_i < nrows && m_z[g] == lvl && m_beg[g] <= p &&
+                    p < m_end[g])
+                    a.scores[m_off[g] + (p - m_beg[g])] = v;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Launch                                                                 */
+/* ------------------------------------------------------------------ */
+
+size_t hfx_dense_scratch_bytes(uint32_t S)
+{
+    const size_t B = (S + DN_NS - 1) / DN_NS;
+    return (size_t)S * 2 * DN_MAXK * 2 + B * 4 * 4 + (B + 1) * 8 + 256;
+}
+
+uint32_t hfx_dense_max_len(void) { return DN_MAXK; }
+
+int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
+                     const uint32_t *d_num, const uint32_t *d_order, const hfx_db *db,
+                     const hfx_spectra *sp, uint32_t CH, void *scratch, void *stream)
+{
+    HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && db && sp, HFX_ERR_ARG,
+                "dense XCorr: null argument");
+    HFX_REQUIRE(sp->x_stride <= DN_MAXK, HFX_ERR_ARG,
+                "dense XCorr: spectra of up to %d bins (x_stride %u)", DN_MAXK, sp->x_stride);
+    HFX_REQUIRE(CH >= DN_ROWS && CH % DN_ROWS == 0 && CH <= 8192, HFX_ERR_ARG,
+                "dense XCorr: chunk %u must be a multiple of %d in %d..8192", CH, DN_ROWS, DN_ROWS);
+    const uint32_t S = sp->num_spec;
+    if (S == 0) return HFX_OK;
+    hipStream_t st = hfx_stream(stream);
+    void *p = scratch;
+    int rc;
+    if (!p && (rc = hfx_ctx_scratch(ctx, HFX_SLOT_DENSE, hfx_dense_scratch_bytes(S), &p)))
+        return rc;
+    const uint32_t B = (S + DN_NS - 1) / DN_NS;
+    uint16_t *xb = (uint16_t *)p;
+    uint32_t *ubeg = (uint32_t *)(xb + (size_t)S * 2 * DN_MAXK);
+    uint32_t *uend = ubeg + B, *binfo = uend + B, *ntiles = binfo + B;
+    uint64_t *tile_off = (uint64_t *)(((uintptr_t)(ntiles + B) + 7) & ~(uintptr_t)7);
+    uint32_t *counter = (uint32_t *)(tile_off + B + 1);
+    hipLaunchKernelGGL(k_dn_split, dim3(S), dim3(256), 0, st, xb, *sp);
+    hipLaunchKernelGGL(k_dn_blocks, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend, binfo,
+                       ntiles, d_order, d_begin, d_num, *sp, B, CH);
+    if ((rc = hfx_scan_offsets(ctx, tile_off, ntiles, B, (void *)st))) return rc;
+    hipLaunchKernelGGL(k_dn_reset, dim3(1), dim3(1), 0, st, counter);
+    DnArgs args{d_scores, d_off, d_begin, d_num, d_order, B, CH, ubeg, uend, binfo, tile_off, xb,
+                counter, *db, *sp};
+    HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_dense,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, DN_LDS_BYTES));
+    hipLaunchKernelGGL(k_dense, dim3((uint32_t)hfx_ctx_num_cus(ctx)), dim3(DN_THREADS),
+                       DN_LDS_BYTES, st, args);
+    hipLaunchKernelGGL(k_dn_zero_unscored, dim3(S), dim3(256), 0, st, d_scores, d_off, sp->d_zmax,
+                       S);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+extern "C" int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                                      const uint32_t *d_begin, const uint32_t *d_num,
+                                      const uint32_t *d_order, const hfx_db *db,
+                                      const hfx_spectra *sp, void *stream)
+{
+    return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, db, sp, 256, nullptr,
+                            stream);
+}

@@ -18,9 +18,13 @@
  *   4. orders the re-scored set by (exact desc, position asc), writes the best
  *      N.  If more than TN_EXT candidates fall within B, the spectrum is
  *      flagged and hfx_topn_exact scores all of its candidates exactly.
- * Bound: |f32 - exact| <= 80 * 2^-24 * sum|w*x| (<= 71 roundings per score)
- * and sum|w*x| <= 50 * sum_b |x_b| (each bin counts once, weights <= 50).
+ * Bound: |score - exact| <= (rel0 + rel1 * z) * 50 * sum_b |x_b| + 1e-6, with
+ * z the spectrum's fragment charge count and sum|w*x| <= 50 * sum_b |x_b|
+ * (each bin counts once, weights <= 50).  The sparse FP32 scores use
+ * rel0 = 80 * 2^-24 (<= 71 roundings per score), rel1 = 0; the dense BF16
+ * scores pass their own (split error + one rounding per non-zero bin).
  */
+#define HFX_TOPN_REL_F32 (80.0f * 5.9604644775390625e-08f)
 #include "hfx_cand.h"
 
 #define TN_THREADS 256
@@ -38,7 +42,7 @@ template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
            uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
-           hfx_spectra sp)
+           hfx_spectra sp, float rel0, float rel1)
 {
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
@@ -156,7 +160,8 @@ __global__ void __launch_bounds__(TN_THREADS)
         __syncthreads();
         float xa = 0.0f;
         for (uint32_t w = 0; w < TN_WAVES; ++w) xa += sh_x1[w];
-        const double bound = 80.0 * 5.9604644775390625e-08 * 50.0 * (double)xa * 1.001 + 1e-6;
+        const double rel = (double)rel0 + (double)rel1 * (double)zeff;
+        const double bound = rel * 50.0 * (double)xa * 1.001 + 1e-6;
         const double tau = (double)sh_exN * (1.0 / HFX_FX_SCALE) - bound;
         const uint32_t last = sh_last;
         for (uint32_t i = tid; i < n; i += TN_THREADS) {
@@ -416,7 +421,7 @@ template <class Cands>
 static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                        uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
                        const float *d_scores, const Cands &c, uint64_t total,
-                       const hfx_spectra *sp, void *scratch, void *stream)
+                       const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream)
 {
     HFX_REQUIRE(ctx && d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_flag && d_scores &&
                     sp,
@@ -427,7 +432,8 @@ static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     if (sp->num_spec == 0) return HFX_OK;
     hipStream_t st = hfx_stream(stream);
     hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS), 0, st, d_top_pos,
-                       d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c, *sp);
+                       d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c, *sp,
+                       rel0, rel1);
     HFX_CHECK_LAUNCH();
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
                        total, sp, scratch, st);
@@ -460,11 +466,13 @@ int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, floa
                     uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
                     const float *d_scores, const uint64_t *d_off, uint64_t total,
                     const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
-                    const hfx_spectra *sp, void *scratch, void *stream)
+                    const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream)
 {
+    HFX_REQUIRE(rel0 >= 0.0f && rel1 >= 0.0f && rel0 < 1.0f && rel1 < 1.0f, HFX_ERR_ARG,
+                "top-N: error bound factors must lie in [0, 1) (rel0=%g rel1=%g)", rel0, rel1);
     HfxPlainCands c{*db, d_begin, d_num, d_off};
     return launch_topn(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-                       d_scores, c, total, sp, scratch, stream);
+                       d_scores, c, total, sp, rel0, rel1, scratch, stream);
 }
 
 extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
@@ -475,7 +483,21 @@ extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, 
 {
     HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn: null argument");
     return hfx_topn_device(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-                           d_scores, d_off, total, d_begin, d_num, db, sp, nullptr, stream);
+                           d_scores, d_off, total, d_begin, d_num, db, sp, HFX_TOPN_REL_F32, 0.0f,
+                           nullptr, stream);
+}
+
+extern "C" int hfx_topn_rel(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
+                            float *d_top_f32, uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N,
+                            uint32_t margin, float rel0, float rel1, const float *d_scores,
+                            const uint64_t *d_off, uint64_t total, const uint32_t *d_begin,
+                            const uint32_t *d_num, const hfx_db *db, const hfx_spectra *sp,
+                            void *stream)
+{
+    HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn_rel: null argument");
+    return hfx_topn_device(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
+                           d_scores, d_off, total, d_begin, d_num, db, sp, rel0, rel1, nullptr,
+                           stream);
 }
 
 extern "C" int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
@@ -502,7 +524,7 @@ extern "C" int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exa
     HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_topn_mod: num_ma %u", ms->num_ma);
     HfxModCands c{*db, *ms, d_mp_off};
     return launch_topn(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-                       d_scores, c, ms->num_mpep, sp, nullptr, stream);
+                       d_scores, c, ms->num_mpep, sp, HFX_TOPN_REL_F32, 0.0f, nullptr, stream);
 }
 
 extern "C" int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,

@@ -317,6 +317,31 @@ class PlainSearch:
                          ptr(self.t_num), C.byref(self.db.desc), C.byref(self.sp.desc), st),
               "hfx_topn")
 
+    # |score - exact| <= REL * 50 * sum|x| for hfx_xcorr_dense_bf16x2 scores
+    REL_DENSE = float(np.float32(2.0**-16 + 210 * 2.0**-24))
+    REL_F32 = float(np.float32(80 * 2.0**-24))
+
+    def score_dense(self, order: np.ndarray | None = None):
+        """Scores by hfx_xcorr_dense_bf16x2 (blocks of eight spectra in
+        `order`, index order if None), same layout as score()."""
+        t_order = dev(np.asarray(order, np.uint32)) if order is not None else None
+        if self.total:
+            check(lib().hfx_xcorr_dense_bf16x2(self.ctx.h, ptr(self.t_scores), ptr(self.t_off),
+                                               ptr(self.t_begin), ptr(self.t_num),
+                                               ptr(t_order) if t_order is not None else None,
+                                               C.byref(self.db.desc), C.byref(self.sp.desc),
+                                               stream_handle()), "hfx_xcorr_dense_bf16x2")
+        sync()
+
+    def select_rel(self, rel0: float, rel1: float = 0.0):
+        """hfx_topn_rel: top-N for scores within the stated error bound."""
+        check(lib().hfx_topn_rel(self.ctx.h, ptr(self.t_pos), ptr(self.t_exact), ptr(self.t_f32),
+                                 ptr(self.t_cnt), ptr(self.t_flag), self.topn, self.margin,
+                                 float(rel0), float(rel1), ptr(self.t_scores), ptr(self.t_off),
+                                 self.total, ptr(self.t_begin), ptr(self.t_num),
+                                 C.byref(self.db.desc), C.byref(self.sp.desc), stream_handle()),
+              "hfx_topn_rel")
+
     def exact(self, spectra: np.ndarray):
         """hfx_topn_exact for the listed spectra (every candidate exact)."""
         lst = np.asarray(spectra, np.uint32)
@@ -332,12 +357,17 @@ class PlainSearch:
         """Spectra that took the exact route in the last select()."""
         return np.nonzero(host(self.t_flag, np.uint32, self.sp.num_spec))[0].astype(np.uint32)
 
-    def run(self, keep_scores: bool = False) -> PlainResult:
+    def run(self, keep_scores: bool = False, dense: bool = False) -> PlainResult:
+        """window -> XCorr (FP32 sparse, or dense BF16 hi+lo) -> top-N."""
         S, N = self.sp.num_spec, self.topn
         self.window()
         if self.total:
-            self.score()
-            self.select()
+            if dense:
+                self.score_dense()
+                self.select_rel(self.REL_DENSE)
+            else:
+                self.score()
+                self.select()
             flagged = self.flagged()
         else:
             flagged = np.zeros(0, np.uint32)
@@ -378,7 +408,10 @@ class BatchSearch:
     """The batched path (hfx_batch_*): plan once, then run or replay a
     captured graph; results in PlainResult form."""
 
-    INFO = ("spectra", "scores", "indexed", "levels", "entries", "G", "C", "threads", "lstr", "p0")
+    INFO = ("spectra", "scores", "indexed", "levels", "entries", "G", "C", "threads", "lstr", "p0",
+            "precision", "chunk")
+    FP32_SPARSE = 0
+    BF16X2_DENSE = 1
 
     def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, eps: float, topn: int,
                  margin: int = 16, stream: Stream | None = None):
@@ -393,7 +426,7 @@ class BatchSearch:
         self.captured = False
 
     def info(self) -> dict:
-        a = np.zeros(10, np.uint64)
+        a = np.zeros(12, np.uint64)
         check(lib().hfx_batch_info(self.h, a.ctypes.data), "hfx_batch_info")
         return {k: int(v) for k, v in zip(self.INFO, a)}
 
@@ -410,6 +443,11 @@ class BatchSearch:
 
     def set_config(self, G: int, Cn: int, threads: int):
         check(lib().hfx_batch_set_config(self.h, G, Cn, threads), "hfx_batch_set_config")
+
+    def set_precision(self, precision: int):
+        """FP32_SPARSE (ion index + gathers) or BF16X2_DENSE (MFMA)."""
+        check(lib().hfx_batch_set_precision(self.h, int(precision)), "hfx_batch_set_precision")
+        self.captured = False
 
     def run(self):
         check(lib().hfx_batch_run(self.h, self.stream.p), "hfx_batch_run")

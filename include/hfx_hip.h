@@ -282,6 +282,30 @@ int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_to
              const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
              const hfx_spectra *sp, void *stream);
 
+/* hfx_topn for scores with a caller-stated error bound: every score of
+ * spectrum s is within (rel0 + rel1 * z_s) * 50 * sum_b |x_s[b]| + 1e-6 of its
+ * exact value, z_s = fragment charges generated for s.  hfx_topn is
+ * hfx_topn_rel with rel0 = 80 * 2^-24, rel1 = 0 (FP32 sparse sums); scores of
+ * hfx_xcorr_dense_bf16x2 need rel0 = 2^-16 + 210 * 2^-24, rel1 = 0. */
+int hfx_topn_rel(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                 uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin, float rel0,
+                 float rel1, const float *d_scores, const uint64_t *d_off, uint64_t total,
+                 const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+                 const hfx_spectra *sp, void *stream);
+
+/* Precision form of the XCorr (new; replaces addIons + sgemv like
+ * hfx_xcorr_sparse, same output layout): spectra are taken eight at a time in
+ * the order d_order (null: index order; sort by fragment charge, then
+ * precursor mass, for speed), every candidate of the eight windows gets one
+ * theoretical row, and rows x spectra are contracted on the MFMA units with
+ * the weights in BF16 (exact) and each spectrum split into hi + lo BF16,
+ * FP32 accumulation.  |score - exact| <= (2^-16 + 210 * 2^-24) * 50 *
+ * sum_b |x[b]|.  Needs sp->x_stride <= 2176; peptides of at most 768
+ * residues. */
+int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                           const uint32_t *d_begin, const uint32_t *d_num, const uint32_t *d_order,
+                           const hfx_db *db, const hfx_spectra *sp, void *stream);
+
 /* Exact top-N of the listed spectra (every candidate scored in fixed point;
  * d_top_f32 taken from d_scores). */
 int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
@@ -330,20 +354,26 @@ int hfx_batch_destroy(hfx_batch *b);
 /* db and sp must stay valid and unchanged while the batch is used */
 int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float eps, uint32_t N,
                    uint32_t margin, void *stream);
-/* windows -> peptide ion index -> persistent multi-spectrum XCorr -> top-N */
+/* windows -> peptide ion index -> persistent multi-spectrum XCorr -> top-N
+ * (FP32 sparse form), or windows -> dense BF16 hi+lo XCorr -> top-N */
 int hfx_batch_run(hfx_batch *b, void *stream);
+#define HFX_BATCH_FP32_SPARSE 0u  /* default after hfx_batch_plan */
+#define HFX_BATCH_BF16X2_DENSE 1u /* hfx_xcorr_dense_bf16x2 scores */
+int hfx_batch_set_precision(hfx_batch *b, uint32_t precision);
 /* capture hfx_batch_run on a non-null stream into a graph; replay it */
 int hfx_batch_capture(hfx_batch *b, void *stream);
 int hfx_batch_replay(hfx_batch *b, void *stream);
 /* time the scoring kernel over (spectra per block G, tile length C,
- * workgroup threads) and keep the fastest; out_cfg = {G, C, threads} */
+ * workgroup threads) -- dense form: candidates per work item C -- and keep
+ * the fastest; out_cfg = {G, C, threads} ({0, C, 256} for the dense form) */
 int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg);
 int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads);
 /* one run, phase by phase, with event timings in ms: {windows, ion index,
  * scoring, top-N} */
 int hfx_batch_time(hfx_batch *b, void *stream, float *ms);
 /* info = {spectra, scores, indexed peptides, charge levels, index entries,
- *         G, C, threads, LDS stride, first indexed position} */
+ *         G, C, threads, LDS stride, first indexed position, precision,
+ *         dense chunk} (12 entries) */
 int hfx_batch_info(const hfx_batch *b, uint64_t *info);
 /* device pointers {top_pos, top_exact, top_f32, top_cnt, flag, scores,
  *                  score offsets, window begin, window size, index entries} */

@@ -695,6 +695,27 @@ float orc_score_f32(const uint8_t *w, const float *x, uint32_t len)
 
 double orc_fx_to_double(int64_t fx) { return std::ldexp((double)fx, -ORC_FX_SHIFT); }
 
+uint64_t orc_check_fast_div(float d, float lo, float hi, int corrected)
+{
+    volatile float dv = d; /* keep 1/d a run-time IEEE division */
+    const float R = 1.0f / dv;
+    uint32_t a, b;
+    std::memcpy(&a, &lo, 4);
+    std::memcpy(&b, &hi, 4);
+    uint64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t u = a; u < (int64_t)b; ++u) {
+        const uint32_t bits = (uint32_t)u;
+        float x;
+        std::memcpy(&x, &bits, 4);
+        const float q = x * R;
+        const float r = std::fma(-q, d, x);
+        const float q2 = corrected ? std::fma(r, R, q) : q;
+        if (q2 != x / d) ++bad;
+    }
+    return bad;
+}
+
 /* ---------------------------------------------------------------------
  * Spectrum preprocessing (Correlation.hs:29-109)
  * ------------------------------------------------------------------- */

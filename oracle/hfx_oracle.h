@@ -111,6 +111,14 @@ int64_t orc_score_fx(const uint8_t *w, const float *x, uint32_t len);
 float   orc_score_f32(const uint8_t *w, const float *x, uint32_t len);
 double  orc_fx_to_double(int64_t fx);
 
+/* ---- arithmetic checks ---------------------------------------------- */
+/* Counts the floats x in [lo, hi) (lo, hi >= 0) for which
+ * q = x*R, r = fma(-q, d, x), fma(r, R, q) with R = RN(1/d) differs from the
+ * correctly rounded x / d (the division the GPU ladders use); exhaustive.
+ * corrected = 0 checks q = x*R alone (a deliberately wrong formula, so that
+ * tests can see the checker catch it). */
+uint64_t orc_check_fast_div(float d, float lo, float hi, int corrected);
+
 /* ---- spectrum preprocessing ----------------------------------------- */
 /* mean_mode 0: reference FP32 sequential window sum; 1: FP64 window sum */
 int orc_preprocess(float *out, uint32_t cap, uint32_t *out_len, float pcr, float charge,

@@ -48,6 +48,8 @@ def lib() -> C.CDLL:
         L.orc_calc_total_mod_cands.restype = C.c_uint32
         L.orc_prepare_gen_mod.restype = C.c_uint32
         L.orc_peaks_list.restype = C.c_uint32
+        L.orc_check_fast_div.restype = C.c_uint64
+        L.orc_check_fast_div.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int]
         _lib = L
     return _lib
 
@@ -366,3 +368,10 @@ def search_plain(ions_packed, r, tc, tn, mass26, idx_sorted, X, lens, zmax, mass
     if want_all:
         res.update(off=off, all_f32=all32, all_fx=allfx)
     return res
+
+
+def check_fast_div(d: float, lo: float, hi: float, corrected: bool = True) -> int:
+    """Mismatches of the FMA-corrected reciprocal division over [lo, hi)
+    (corrected=False: of the uncorrected x * RN(1/d))."""
+    return int(lib().orc_check_fast_div(C.c_float(d), C.c_float(lo), C.c_float(hi),
+                                        int(corrected)))

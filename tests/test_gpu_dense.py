@@ -1,0 +1,140 @@
+"""GPU parity of the precision form (dense BF16 hi+lo MFMA XCorr) against the
+CPU oracle.
+
+Every dense score lies within the stated bound of its exact score,
+|dense - exact| <= (2^-16 + 210 * 2^-24) * 50 * sum|x|, and within 1e-4 of
+the exact score relative to max(|exact|, 100); top-N positions and exact
+scores equal the oracle's; the result does not depend on how spectra are
+blocked; the batched dense form (graph-replayed, autotuned) gives the same
+rankings.  Spectra cover fragment charges 1-4 (precursor 2+ to 5+), two
+spectra whose fragment charge exceeds 4 (scored 0) and an empty window.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STRIDE = 2112
+REL = 2.0**-16 + 210 * 2.0**-24
+
+
+@pytest.fixture(scope="module")
+def denv(orc):
+    from hfx_amd import pipeline, synth
+
+    prot = synth.make_proteins(3000, seed=51)
+    mass = orc.mass_table({"C": 57.02146})
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=2)
+    sp = synth.make_spectra(prot, r, tc, tn, 400, seed=52, charges=(2, 3, 4, 5),
+                            charge_p=(0.4, 0.3, 0.2, 0.1), static_mods={"C": 57.02146},
+                            double_for_3plus=True)
+    H = 1.007276
+    for s, z in ((7, 7.0), (19, 9.0)):
+        z0 = float(sp.charge[s])
+        sp.pcr[s] = np.float32((float(sp.pcr[s]) * z0 - z0 * H + z * H) / z)
+        sp.charge[s] = z
+    sp.pcr[3] = np.float32(90.0)  # far below every peptide: empty window
+    ctx = pipeline.Ctx(0)
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
+    dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE)
+    idx, _ = orc.sort_idx(r)
+    ref = orc.search_plain(orc.prepare_ions(prot.seq), r, tc, tn, mass, idx, dsp.x(), dsp.lengths(),
+                           dsp.zmax, dsp.mass, 3.0, 10, want_all=True)
+    return dict(ctx=ctx, db=db, dsp=dsp, ref=ref, pl=pipeline)
+
+
+def _bound(dsp, res):
+    """per-score error bound of the dense form"""
+    X, lens = dsp.x(), dsp.lengths()
+    x1 = np.array([np.abs(X[s, : lens[s]].astype(np.float64)).sum() for s in range(dsp.num_spec)])
+    per_spec = REL * 50.0 * x1 * 1.001 + 1e-6
+    return np.repeat(per_spec, np.diff(res.off.astype(np.int64)))
+
+
+def check_scores(res, ref, dsp, orc):
+    assert (res.begin == ref["begin"]).all() and (res.num == ref["num"]).all()
+    exact = orc.fx_to_double(ref["all_fx"][: len(res.scores)])
+    err = np.abs(res.scores.astype(np.float64) - exact)
+    bound = _bound(dsp, res)
+    ratio = float((err / bound).max()) if len(err) else 0.0
+    print(f"dense: {len(err)} scores, max |err| = {err.max():.3g}, max err/bound = {ratio:.3g}")
+    assert np.all(err <= bound), ratio
+    assert np.all(err <= 1e-4 * np.maximum(np.abs(exact), 100.0)), float(err.max())
+
+
+def check_topn(res, ref, orc):
+    assert (res.top_cnt == ref["top_cnt"]).all()
+    for s in range(len(res.top_cnt)):
+        c = int(res.top_cnt[s])
+        assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), s
+        assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all(), s
+
+
+def test_dense_scores_within_bound(denv, orc):
+    pl = denv["pl"]
+    srch = pl.PlainSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
+    res = srch.run(keep_scores=True, dense=True)
+    check_scores(res, denv["ref"], denv["dsp"], orc)
+    check_topn(res, denv["ref"], orc)
+    assert res.num[3] == 0
+    for s in (7, 19):
+        a, b = int(res.off[s]), int(res.off[s + 1])
+        assert b > a and np.all(res.scores[a:b] == 0)
+
+
+def test_dense_blocking_does_not_change_scores(denv, orc):
+    """Each score is one (row, column) of the contraction: the block a spectrum
+    lands in changes neither its row values nor its column, so any spectrum
+    order gives identical scores."""
+    pl, dsp = denv["pl"], denv["dsp"]
+    srch = pl.PlainSearch(denv["ctx"], denv["db"], dsp, eps=3.0, topn=10)
+    srch.window()
+    srch.score_dense(None)
+    a = pl.host(srch.t_scores, np.float32, srch.total).copy()
+    order = np.lexsort((dsp.mass, dsp.zmax)).astype(np.uint32)
+    srch.score_dense(order)
+    b = pl.host(srch.t_scores, np.float32, srch.total).copy()
+    rev = order[::-1].copy()
+    srch.score_dense(rev)
+    c = pl.host(srch.t_scores, np.float32, srch.total).copy()
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_batch_dense_matches_oracle(denv, orc):
+    pl = denv["pl"]
+    bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
+    bs.set_precision(bs.BF16X2_DENSE)
+    assert bs.info()["precision"] == 1
+    bs.run()
+    res = bs.results(keep_scores=True)
+    check_scores(res, denv["ref"], denv["dsp"], orc)
+    check_topn(res, denv["ref"], orc)
+
+
+@pytest.mark.parametrize("chunk", [16, 48, 512, 4096])
+def test_batch_dense_chunks_agree(denv, orc, chunk):
+    pl = denv["pl"]
+    bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
+    bs.set_precision(bs.BF16X2_DENSE)
+    bs.set_config(0, chunk, 256)
+    assert bs.info()["chunk"] == chunk
+    bs.run()
+    res = bs.results(keep_scores=True)
+    check_topn(res, denv["ref"], orc)
+
+
+def test_batch_dense_graph_and_autotune(denv, orc):
+    pl = denv["pl"]
+    bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
+    bs.set_precision(bs.BF16X2_DENSE)
+    G, Cn, T = bs.autotune()
+    assert G == 0 and Cn in (64, 128, 256, 512, 1024) and T == 256
+    bs.run()
+    first = bs.results(keep_scores=True)
+    check_topn(first, denv["ref"], orc)
+    bs.capture()
+    for _ in range(3):
+        bs.replay()
+        again = bs.results(keep_scores=True)
+        assert np.array_equal(again.scores, first.scores)
+        check_topn(again, denv["ref"], orc)

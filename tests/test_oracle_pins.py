@@ -349,3 +349,22 @@ def test_digest_hand_example(orc):
     r2, tc2, tn2 = orc.digest(seq, seg, mass, rule=1, missed=0, min_mass=0, max_mass=1e9,
                               p_rule=1)
     assert tc2.tolist() == [0, 5] and tn2.tolist() == [4, 5]
+
+
+@pytest.mark.parametrize("d", [1.0005079, 1.0, 2.0, 3.0, 4.0])
+def test_fast_division_is_exact(orc, d):
+    """The dense GPU kernel divides by the bin width and the fragment charge
+    as q = x*R, r = fma(-q, d, x), fma(r, R, q) with R = RN(1/d).  Every float
+    x in [1e-3, 65536) must give the correctly rounded x / d (exhaustive:
+    ~2.2e8 values per divisor); sign symmetry covers negative x, and outside
+    that range every bin rounds to 0 or lies beyond the last bin."""
+    assert orc.check_fast_div(np.float32(d), np.float32(1e-3), np.float32(65536.0)) == 0
+
+
+def test_fast_division_check_detects_a_wrong_reciprocal(orc):
+    """Planted mistake: without the FMA correction, q = x * RN(1/d) is off by
+    an ulp for a large share of x, and the checker must count them."""
+    for d in (1.0005079, 3.0):
+        assert orc.check_fast_div(np.float32(d), np.float32(1.0), np.float32(2.0), False) > 1000
+    # powers of two have exact reciprocals: nothing to correct
+    assert orc.check_fast_div(np.float32(2.0), np.float32(1.0), np.float32(2.0), False) == 0
