@@ -8,13 +8,18 @@ fragments), +-3 Da precursor window, top-10 per spectrum.
 A step is one pass of the scoring stage over the whole batch with the
 database and the preprocessed spectra resident in HBM: precursor windows ->
 theoretical peaks -> XCorr of every (spectrum, candidate) pair -> per-spectrum
-top-N with exact re-score.  Two forms are timed:
-  plain    hfx_window + hfx_xcorr_sparse + hfx_topn (the drop-in, one
-           theoretical spectrum per (spectrum, candidate) pair);
-  batched  hfx_batch_* (per-batch peptide ion index + persistent
-           work-stealing multi-spectrum XCorr + top-N), autotuned, captured in
-           a hipGraph and replayed.
-value = scored pairs per second of the batched form (whole job, one GPU).
+top-N with exact re-score.  Three forms are timed:
+  plain          hfx_window + hfx_xcorr_sparse + hfx_topn (the drop-in, one
+                 theoretical spectrum per (spectrum, candidate) pair, FP32);
+  batched fp32   hfx_batch_* in its FP32 sparse form (per-batch peptide ion
+                 index + persistent work-stealing multi-spectrum XCorr);
+  batched bf16x2 hfx_batch_* in its dense form (blocks of eight spectra
+                 contracted with their candidates' theoretical rows on the
+                 MFMA units, BF16 weights x hi+lo BF16 spectra, FP32
+                 accumulation) + top-N with the exact re-score;
+both batched forms autotuned, captured in a hipGraph and replayed.
+value = scored pairs per second of the batched BF16-split form (whole job,
+one GPU); every form's top-N must equal the plain form's.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -33,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "XCorr peptide-spectrum scores/sec, plain FP32 vs batched BF16-split, top-5 match"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA, no sparsity
 
 
 def log(msg):
@@ -170,13 +176,40 @@ def bench_plain(env, topn, steps, warmup):
     return res, elapsed, kern_ms, flagged
 
 
-def bench_batched(env, topn, steps, warmup):
+def dense_flops(env, res) -> tuple[float, float]:
+    """(executed, useful) MFMA FLOPs of one dense pass: blocks of eight
+    spectra in (fragment charge, precursor mass) order; per block and charge
+    level present, ceil(union window / 16) row tiles x 16 columns x K bins x 2
+    (K = longest spectrum of the block rounded up to 128); useful = 4 len_s
+    per scored pair (hi and lo columns)."""
+    dsp = env["dsp"]
+    z = dsp.zmax.astype(np.int64)
+    z = np.where(z <= 4, z, 0)
+    lens = np.minimum(dsp.lengths().astype(np.int64), 2176)
+    order = np.lexsort((dsp.mass, z))
+    b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
+    executed = 0.0
+    for i in range(0, len(order), 8):
+        blk = order[i:i + 8]
+        blk = blk[(z[blk] > 0) & (n[blk] > 0)]
+        if len(blk) == 0:
+            continue
+        rows = int((b[blk] + n[blk]).max() - b[blk].min())
+        kb = min(2176, (int(lens[blk].max()) + 127) // 128 * 128)
+        executed += len(set(z[blk].tolist())) * ((rows + 15) // 16) * 16 * 16 * kb * 2.0
+    useful = float(np.sum(4.0 * lens * n * (z > 0)))
+    return executed, useful
+
+
+def bench_batched(env, topn, steps, warmup, precision=0):
     from hfx_amd import pipeline
 
     bs = pipeline.BatchSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn, margin=16)
+    bs.set_precision(precision)
     t0 = time.time()
     cfg = bs.autotune()
-    log(f"autotune: G={cfg[0]} C={cfg[1]} threads={cfg[2]} ({time.time() - t0:.1f}s)")
+    log(f"autotune ({'dense' if precision else 'sparse'}): G={cfg[0]} C={cfg[1]} "
+        f"threads={cfg[2]} ({time.time() - t0:.1f}s)")
     phases = [bs.time_phases() for _ in range(3)][-1]
     bs.capture()
     for _ in range(warmup):
@@ -198,7 +231,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tryptic-20k-10k")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--path", choices=("both", "plain", "batched"), default="both")
+    ap.add_argument("--path", choices=("all", "plain", "batched", "dense"), default="all")
     ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",
                     help="only run a streaming read of MIB MiB (k_mvm_rows over a "
                          "row matrix) to calibrate HBM byte counters, then exit")
@@ -217,7 +250,7 @@ def main():
            "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
     paths = {}
     plain_res = None
-    if args.path in ("both", "plain"):
+    if args.path in ("all", "plain"):
         plain_res, el, kern_ms, flagged = bench_plain(env, topn, args.steps, args.warmup)
         sc = int(plain_res.off[-1])
         ab = algorithmic_bytes(env, plain_res)
@@ -227,35 +260,67 @@ def main():
             kernel_gbs=ab / (kern_ms * 1e-3) / 1e9, flagged_spectra_per_step=flagged)
         log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s")
     batch_res = None
-    if args.path in ("both", "batched"):
-        batch_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup)
+    if args.path in ("all", "batched"):
+        batch_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 0)
         sc = int(batch_res.off[-1])
         ab = algorithmic_bytes(env, batch_res)
         paths["batched_fp32_sparse"] = dict(
             value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
             kernel="k_batch", kernel_ms=phases["score"], phases_ms=phases,
+            kernel_gbs=ab / (phases["score"] * 1e-3) / 1e9,
             config=dict(G=cfg[0], C=cfg[1], threads=cfg[2]), index=info,
             graph="captured once, replayed per step",
             flagged_spectra=int(len(batch_res.flagged)))
-        log(f"batched: {paths['batched_fp32_sparse']['value']:.3e} scores/s {phases}")
-    if plain_res is not None and batch_res is not None:
-        paths["batched_topn_identical_to_plain"] = same_topn(plain_res, batch_res)
-    main_key = "batched_fp32_sparse" if "batched_fp32_sparse" in paths else "plain_fp32_sparse"
-    res = batch_res if batch_res is not None else plain_res
+        log(f"batched fp32: {paths['batched_fp32_sparse']['value']:.3e} scores/s {phases}")
+    dense_res = None
+    if args.path in ("all", "dense"):
+        dense_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 1)
+        sc = int(dense_res.off[-1])
+        ex, useful = dense_flops(env, dense_res)
+        paths["batched_bf16x2_dense"] = dict(
+            value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
+            kernel="k_dense", kernel_ms=phases["score"], phases_ms=phases,
+            mfma_flops_executed=ex, mfma_flops_useful=useful,
+            config=dict(chunk=cfg[1], threads=cfg[2], spectra_per_block=8),
+            graph="captured once, replayed per step",
+            flagged_spectra=int(len(dense_res.flagged)))
+        log(f"batched bf16x2: {paths['batched_bf16x2_dense']['value']:.3e} scores/s {phases}")
+    ref = plain_res
+    for key, r in (("batched_fp32_sparse", batch_res), ("batched_bf16x2_dense", dense_res)):
+        if r is not None and ref is not None:
+            paths[key]["topn_identical_to_plain"] = same_topn(ref, r)
+    if dense_res is not None:
+        main_key, res = "batched_bf16x2_dense", dense_res
+    elif batch_res is not None:
+        main_key, res = "batched_fp32_sparse", batch_res
+    else:
+        main_key, res = "plain_fp32_sparse", plain_res
     p = paths[main_key]
-    ab = algorithmic_bytes(env, res)
-    achieved = ab / (p["kernel_ms"] * 1e-3) / 1e9
+    if main_key == "batched_bf16x2_dense":
+        achieved = p["mfma_flops_executed"] / (p["kernel_ms"] * 1e-3) / 1e12
+        roofline = {"kernel": p["kernel"], "bound": "mfma", "achieved": achieved,
+                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                    "kernel_ms": p["kernel_ms"], "flops_executed": p["mfma_flops_executed"],
+                    "flops_useful": p["mfma_flops_useful"]}
+        dtype = "bf16x2"
+    else:
+        ab = algorithmic_bytes(env, res)
+        achieved = ab / (p["kernel_ms"] * 1e-3) / 1e9
+        roofline = {"kernel": p["kernel"], "bound": "hbm", "achieved": achieved,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": None, "kernel_ms": p["kernel_ms"], "algorithmic_bytes": ab}
+        dtype = "f32"
     out.update(
-        value=p["value"], ms_per_step=p["ms_per_step"],
+        value=p["value"], ms_per_step=p["ms_per_step"], dtype=dtype,
         config={"workload": args.workload, "proteins": env["w"]["proteins"],
                 "peptides": int(len(env["r"])), "spectra": int(env["dsp"].num_spec),
                 "window_da": 3.0, "bin_width": 1.0005079,
                 "max_len": int(env["dsp"].lengths().max()), "topn": topn, "path": main_key,
                 "scores_per_step": p["scores_per_step"]},
-        roofline={"kernel": p["kernel"], "bound": "hbm", "achieved": achieved,
-                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                  "traffic": None, "kernel_ms": p["kernel_ms"], "algorithmic_bytes": ab},
-        paths=paths)
+        roofline=roofline, paths=paths)
+    if "plain_fp32_sparse" in paths:
+        out["batched_over_plain"] = p["value"] / paths["plain_fp32_sparse"]["value"]
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(env, res, topn)
