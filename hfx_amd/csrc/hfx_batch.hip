@@ -666,7 +666,7 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     float best = 1e30f;
     uint32_t bG = b->G, bC = b->C, bT = b->threads, bCH = b->CH;
     if (b->precision == HFX_BATCH_BF16X2_DENSE) {
-        const uint32_t CHs[5] = {64, 128, 256, 512, 1024};
+        const uint32_t CHs[5] = {256, 512, 1024, 2048, 4096};
         for (uint32_t CH : CHs) {
             b->CH = CH;
             float ms = 0;

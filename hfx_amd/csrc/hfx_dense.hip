@@ -54,13 +54,16 @@
 #define DN_MSTR (4 * DN_NKS)        /* map words per row */
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
 #define DN_ITEMS 768                /* ladder positions per wave pass */
+#define DN_RMAX 64                  /* residues per candidate staged in LDS */
+#define DN_RSTR 17                  /* dwords per staged row (3 + DN_RMAX bytes) */
 #define DN_NONE 0xffffffffu
 
 #define DN_B_BYTES (DN_COLS * DN_CSTR * 2)
 #define DN_MAP_BYTES (DN_WAVES * DN_ROWS * DN_MSTR * 4)
 #define DN_ITEMB_BYTES (DN_WAVES * DN_ITEMS * 4)
 #define DN_ITEMC_BYTES (DN_WAVES * DN_ITEMS)
-#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES)
+#define DN_RES_BYTES (DN_WAVES * DN_ROWS * DN_RSTR * 4)
+#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES)
 
 typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
@@ -150,6 +153,14 @@ __global__ void k_dn_split(uint16_t *xb, hfx_spectra sp)
 /* Blocks                                                                 */
 /* ------------------------------------------------------------------ */
 
+/* read-only, wave-uniform tables: scalar loads through the constant address
+ * space (nothing in the kernel writes them) */
+template <class T>
+__device__ __forceinline__ T dn_sload(const T *p, uint32_t i)
+{
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
 __device__ __forceinline__ uint32_t dn_order(const uint32_t *order, uint32_t i)
 {
     return order ? order[i] : i;
@@ -216,34 +227,80 @@ struct DnZ {
     float fz, zh, R, h2o, nh3;
 };
 
+/* one weight code at bin x of a map row; a bin outside [0, kq) ORs 0 into
+ * a word private to the lane instead (no branch, no shared address) */
+__device__ __forceinline__ void dn_put(uint32_t *row, int32_t x, uint32_t code, uint32_t kq,
+                                       uint32_t lane)
+{
+    const bool ok = (uint32_t)x < kq;
+    const uint32_t w = ok ? (((uint32_t)x >> 3) & 3u) * DN_NKS + ((uint32_t)x >> 5) : lane;
+    atomicOr(&row[w], ok ? code << (((uint32_t)x & 7u) * 4u) : 0u);
+}
+
 /* peaks of one ladder position (b, y) at charge z into a map row: the ion
  * series of ion_series.cu:111-156 with the FP32 operation order of the
  * sparse path */
 __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, const DnZ &q,
-                                         uint32_t kb)
+                                         uint32_t kq, uint32_t lane)
 {
-#define DN_PUT(xx, code)                                                                \
-    do {                                                                                \
-        const int32_t x_ = (xx);                                                        \
-        if ((uint32_t)x_ < kb)                                                          \
-            atomicOr(&row[(((uint32_t)x_ >> 3) & 3u) * DN_NKS + ((uint32_t)x_ >> 5)],   \
-                     (code) << (((uint32_t)x_ & 7u) * 4u));                             \
-    } while (0)
-    DN_PUT(dn_bin(dn_div((b - HFX_DEV_CO) + q.zh, q.fz, q.R)), HFX_CODE10);
+    dn_put(row, dn_bin(dn_div((b - HFX_DEV_CO) + q.zh, q.fz, q.R)), HFX_CODE10, kq, lane);
     const float mb = dn_div(b + q.zh, q.fz, q.R);
     const int32_t xb = dn_bin(mb);
-    DN_PUT(xb, HFX_CODE50);
-    DN_PUT(xb + 1, HFX_CODE25);
-    DN_PUT(xb - 1, HFX_CODE25);
-    DN_PUT(dn_bin(mb - q.h2o), HFX_CODE10);
-    DN_PUT(dn_bin(mb - q.nh3), HFX_CODE10);
+    dn_put(row, xb, HFX_CODE50, kq, lane);
+    dn_put(row, xb + 1, HFX_CODE25, kq, lane);
+    dn_put(row, xb - 1, HFX_CODE25, kq, lane);
+    dn_put(row, dn_bin(mb - q.h2o), HFX_CODE10, kq, lane);
+    dn_put(row, dn_bin(mb - q.nh3), HFX_CODE10, kq, lane);
     const float my = dn_div((y + HFX_DEV_H2O) + q.zh, q.fz, q.R);
     const int32_t xy = dn_bin(my);
-    DN_PUT(xy, HFX_CODE50);
-    DN_PUT(xy + 1, HFX_CODE25);
-    DN_PUT(xy - 1, HFX_CODE25);
-    DN_PUT(dn_bin(my - q.nh3), HFX_CODE10);
-#undef DN_PUT
+    dn_put(row, xy, HFX_CODE50, kq, lane);
+    dn_put(row, xy + 1, HFX_CODE25, kq, lane);
+    dn_put(row, xy - 1, HFX_CODE25, kq, lane);
+    dn_put(row, dn_bin(my - q.nh3), HFX_CODE10, kq, lane);
+}
+
+/* residues of the tile's candidates staged in LDS: row c holds the ion bytes
+ * from the 4-byte aligned a0 <= tc on, DN_RMAX residues at most */
+struct DnStage {
+    uint32_t v[5];
+};
+
+/* lane (c = lane & 15, q = lane >> 4) loads dwords q, q+4, ... of candidate
+ * c's staged row; every byte read lies in [0, num_ions) */
+__device__ __forceinline__ void dn_stage_load(DnStage &st, const uint8_t *ions, uint64_t num_ions,
+                                              uint32_t tc, uint32_t n, uint32_t lane)
+{
+    const uint32_t c = lane & 15u, q = lane >> 4;
+    const uint32_t tcc = __shfl(tc, c, 64), nc = __shfl(n, c, 64);
+    const uint32_t mis = (uint32_t)(((uintptr_t)ions + tcc) & 3u);
+    const int64_t a0 = (int64_t)tcc - mis;
+    const uint32_t ndw = (mis + min(nc, (uint32_t)DN_RMAX) + 3u) >> 2;
+#pragma unroll
+    for (uint32_t i = 0; i < 5; ++i) {
+        const uint32_t d = q + 4u * i;
+        uint32_t w = 0;
+        if (d < ndw) {
+            const int64_t at = a0 + 4 * (int64_t)d;
+            if (at >= 0 && at + 3 < (int64_t)num_ions) {
+                w = *(const uint32_t *)(ions + at);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (at + k >= 0 && at + k < (int64_t)num_ions) w |= (uint32_t)ions[at + k] << (8 * k);
+            }
+        }
+        st.v[i] = w;
+    }
+}
+
+__device__ __forceinline__ void dn_stage_store(const DnStage &st, uint32_t *res, uint32_t lane)
+{
+    const uint32_t c = lane & 15u, q = lane >> 4;
+#pragma unroll
+    for (uint32_t i = 0; i < 5; ++i) {
+        const uint32_t d = q + 4u * i;
+        if (d < DN_RSTR) res[c * DN_RSTR + d] = st.v[i];
+    }
 }
 
 __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
@@ -253,35 +310,50 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     uint32_t *maps = (uint32_t *)(dn_lds + DN_B_BYTES);
     float *itemb_all = (float *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES);
     uint8_t *itemc_all = dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES;
+    uint32_t *res_all = (uint32_t *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES +
+                                     DN_ITEMC_BYTES);
     __shared__ float mtab[32];
     __shared__ uint32_t m_s[DN_NS], m_beg[DN_NS], m_end[DN_NS], m_z[DN_NS];
     __shared__ uint64_t m_off[DN_NS];
     __shared__ float rr_all[DN_WAVES][DN_ROWS];
-    __shared__ uint32_t sh_t;
+    __shared__ uint32_t sh_t[2];
 
     const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
     uint32_t *map = maps + wave * DN_ROWS * DN_MSTR;
     float *itemb = itemb_all + wave * DN_ITEMS;
     uint8_t *itemc = itemc_all + wave * DN_ITEMS;
+    uint32_t *res = res_all + wave * DN_ROWS * DN_RSTR;
+    const uint8_t *resb = (const uint8_t *)res;
     float *rr = rr_all[wave];
+    const uint8_t *ions = a.db.d_ions;
     if (tid < 32) mtab[tid] = tid < 26 ? a.db.d_mass[tid] : 0.0f;
     for (uint32_t i = tid; i < DN_WAVES * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
-    const uint64_t ntiles = a.tile_off[a.B];
+    const uint64_t ntiles = dn_sload(a.tile_off, a.B);
     const uint32_t row = lane & 15u, kg = lane >> 4;
+    DnZ zc[HFX_MAX_CHARGE];
+#pragma unroll
+    for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z) {
+        const float fz = (float)z;
+        zc[z - 1] = DnZ{fz, HFX_DEV_H * fz, 1.0f / fz, HFX_DEV_H2O / fz, HFX_DEV_NH3 / fz};
+    }
     uint32_t cur = DN_NONE;
-    for (;;) {
+    /* work items are claimed one ahead: the claim of item k+1 is in flight
+     * while item k is processed */
+    if (tid == 0) sh_t[0] = atomicAdd(a.counter, 1u);
+    uint32_t next = 0; /* thread 0: the claim of the following item */
+    for (uint32_t slot = 0;; slot ^= 1u) {
         __syncthreads(); /* every wave is done with the previous item */
-        if (tid == 0) sh_t = atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const uint64_t t = sh_t;
+        /* wave-uniform from here: block metadata comes through scalar loads */
+        const uint32_t t = __builtin_amdgcn_readfirstlane(sh_t[slot]);
         if (t >= ntiles) break;
+        if (tid == 0) next = atomicAdd(a.counter, 1u);
         uint32_t lo = 0, hi = a.B;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (a.tile_off[mid] <= t) lo = mid; else hi = mid;
+            if (dn_sload(a.tile_off, mid) <= t) lo = mid; else hi = mid;
         }
         const uint32_t b = lo;
-        const uint32_t info = a.binfo[b];
+        const uint32_t info = dn_sload(a.binfo, b);
         const uint32_t kb = info & 0xffffu, mask = info >> 16;
         if (b != cur) {
             if (tid < DN_NS) {
@@ -298,43 +370,97 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                 m_s[tid] = s; m_beg[tid] = be; m_end[tid] = en; m_z[tid] = z; m_off[tid] = of;
             }
             __syncthreads();
+            /* all loads of the block's B columns in flight before any store */
             const uint32_t nvec = kb / 8;
-            for (uint32_t i = tid; i < DN_COLS * nvec; i += DN_THREADS) {
-                const uint32_t col = i / nvec, v = i - col * nvec, s = m_s[col >> 1];
-                uint4 val = make_uint4(0u, 0u, 0u, 0u);
-                if (s != DN_NONE)
-                    val = *(const uint4 *)(a.xb + ((size_t)s * 2 + (col & 1u)) * DN_MAXK + v * 8);
-                *(uint4 *)(Bl + col * DN_CSTR + v * 8) = val;
+            constexpr uint32_t PER = (DN_COLS * (DN_MAXK / 8) + DN_THREADS - 1) / DN_THREADS;
+            uint4 val[PER];
+#pragma unroll
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t i = tid + k * DN_THREADS;
+                val[k] = make_uint4(0u, 0u, 0u, 0u);
+                if (i < DN_COLS * nvec) {
+                    const uint32_t col = i / nvec, v = i - col * nvec, s = m_s[col >> 1];
+                    if (s != DN_NONE)
+                        val[k] = *(const uint4 *)(a.xb + ((size_t)s * 2 + (col & 1u)) * DN_MAXK +
+                                                  v * 8);
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t i = tid + k * DN_THREADS;
+                if (i < DN_COLS * nvec) {
+                    const uint32_t col = i / nvec, v = i - col * nvec;
+                    *(uint4 *)(Bl + col * DN_CSTR + v * 8) = val[k];
+                }
             }
             cur = b;
             __syncthreads();
         }
-        const uint32_t beg = a.ubeg[b], end = a.uend[b];
+        /* the claim made above is published for the next round here, long
+         * after it was issued (sh_t[slot ^ 1] is not read before the next
+         * barrier) */
+        if (tid == 0) sh_t[slot ^ 1u] = next;
+        const uint32_t beg = dn_sload(a.ubeg, b), end = dn_sload(a.uend, b);
         const uint32_t nch = (end - beg + a.CH - 1) / a.CH;
-        const uint32_t local = (uint32_t)(t - a.tile_off[b]);
+        const uint32_t local = (uint32_t)(t - dn_sload(a.tile_off, b));
         const uint32_t li = local / nch, chunk = local - li * nch;
         uint32_t m = mask;
         for (uint32_t q = 0; q < li; ++q) m &= m - 1u;
         const uint32_t lvl = (uint32_t)__builtin_ctz(m);
-        DnZ zc[HFX_MAX_CHARGE];
-#pragma unroll
-        for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z) {
-            const float fz = (float)z;
-            zc[z - 1] = DnZ{fz, HFX_DEV_H * fz, 1.0f / fz, HFX_DEV_H2O / fz, HFX_DEV_NH3 / fz};
-        }
         const uint32_t cbeg = beg + chunk * a.CH, cend = min(cbeg + a.CH, end);
-        for (uint32_t p0 = cbeg + wave * DN_ROWS; p0 < cend; p0 += DN_WAVES * DN_ROWS) {
-            const uint32_t nrows = min((uint32_t)DN_ROWS, cend - p0);
-            /* candidates of the tile (lanes 0..nrows-1) */
-            uint32_t tc = 0, n = 0;
-            float r = 0.0f;
-            if (lane < nrows) {
-                const uint32_t pep = a.db.d_idx_sorted[p0 + lane];
-                tc = a.db.d_tc[pep];
-                const uint32_t tn = a.db.d_tn[pep];
+        const uint32_t step = DN_WAVES * DN_ROWS;
+        uint32_t p0 = cbeg + wave * DN_ROWS;
+        if (p0 >= cend) continue;
+
+        /* software pipeline over this wave's tiles p0, p0 + 64, ...: every
+         * global load is issued at the start of a tile and consumed at its
+         * end, before the tile's score stores (vmcnt counts stores too), so
+         * each wait falls after a full tile of LDS and MFMA work:
+         *   registers  tile j: tc, n, r   tile j+1: tc1, n1, r1, pep of j+2
+         *   LDS        residues of tile j
+         *   in flight  residues of j+1, metadata of j+2, peptide of j+3 */
+        auto rows_of = [&](uint32_t q0) -> uint32_t {
+            return q0 < cend ? min((uint32_t)DN_ROWS, cend - q0) : 0u;
+        };
+        auto load_pep = [&](uint32_t q0) -> uint32_t {
+            return lane < rows_of(q0) ? a.db.d_idx_sorted[q0 + lane] : DN_NONE;
+        };
+        uint32_t tc = 0, n = 0, tc1 = 0, n1 = 0, pep2 = DN_NONE;
+        float r = 0.0f, r1 = 0.0f;
+        {
+            const uint32_t pep0 = load_pep(p0), pep1 = load_pep(p0 + step);
+            pep2 = load_pep(p0 + 2 * step);
+            if (pep0 != DN_NONE) {
+                tc = a.db.d_tc[pep0];
+                const uint32_t tn = a.db.d_tn[pep0];
                 n = tn > tc ? min(tn - tc, (uint32_t)DN_ITEMS) : 0u;
-                r = a.db.d_r[pep];
+                r = a.db.d_r[pep0];
             }
+            if (pep1 != DN_NONE) {
+                tc1 = a.db.d_tc[pep1];
+                const uint32_t tn = a.db.d_tn[pep1];
+                n1 = tn > tc1 ? min(tn - tc1, (uint32_t)DN_ITEMS) : 0u;
+                r1 = a.db.d_r[pep1];
+            }
+            DnStage st0;
+            dn_stage_load(st0, ions, a.db.num_ions, tc, n, lane);
+            dn_stage_store(st0, res, lane);
+            hfx_wave_sync();
+        }
+        for (; p0 < cend; p0 += step) {
+            const uint32_t nrows = rows_of(p0);
+            /* issue: residues of j+1, metadata of j+2, peptide of j+3 */
+            DnStage st1;
+            dn_stage_load(st1, ions, a.db.num_ions, tc1, n1, lane);
+            uint32_t tc2 = 0, tn2 = 0;
+            float r2 = 0.0f;
+            if (pep2 != DN_NONE) {
+                tc2 = a.db.d_tc[pep2];
+                tn2 = a.db.d_tn[pep2];
+                r2 = a.db.d_r[pep2];
+            }
+            const uint32_t pep3 = load_pep(p0 + 3 * step);
+
             if (lane < DN_ROWS) rr[lane] = r;
             uint32_t incl = n;
 #pragma unroll
@@ -342,8 +468,12 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                 const uint32_t y = __shfl_up(incl, o, 64);
                 if (lane >= o) incl += y;
             }
+            /* heaviest fragment of the tile: every peak bin is below
+             * m / 1.0005 + 2 for the largest b or y + water + proton m */
+            float mmax = lane < nrows ? r : 0.0f;
             /* ladder positions in passes of at most DN_ITEMS */
             uint32_t c_lo = 0;
+            uint32_t kq = kb;
             while (c_lo < nrows) {
                 const uint32_t base = c_lo ? __shfl(incl, c_lo - 1, 64) : 0u;
                 const bool fits = lane >= c_lo && lane < nrows && incl - base <= DN_ITEMS;
@@ -354,14 +484,16 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                     /* sequential FP32 b ladder of candidate `lane` */
                     uint32_t o = incl - n - base;
                     float bsum = 0.0f;
-                    const uint8_t *ion = a.db.d_ions + tc;
-                    for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+                    const uint32_t mis = (uint32_t)(((uintptr_t)ions + tc) & 3u);
+                    const uint8_t *rs = resb + lane * DN_RSTR * 4 + mis;
+                    const uint32_t ns = min(n, (uint32_t)DN_RMAX);
+                    for (uint32_t k0 = 0; k0 < ns; k0 += 8) {
                         uint32_t v[8];
 #pragma unroll
-                        for (uint32_t j = 0; j < 8; ++j) v[j] = k0 + j < n ? ion[k0 + j] : 0u;
+                        for (uint32_t j = 0; j < 8; ++j) v[j] = k0 + j < ns ? rs[k0 + j] : 0u;
 #pragma unroll
                         for (uint32_t j = 0; j < 8; ++j) {
-                            if (k0 + j < n) {
+                            if (k0 + j < ns) {
                                 bsum = bsum + mtab[HFX_ACID_IDX(v[j])];
                                 itemb[o] = bsum;
                                 itemc[o] = (uint8_t)lane;
@@ -369,47 +501,99 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                             }
                         }
                     }
+                    for (uint32_t k = ns; k < n; ++k) { /* beyond the staged residues */
+                        bsum = bsum + mtab[HFX_ACID_IDX(ions[tc + k])];
+                        itemb[o] = bsum;
+                        itemc[o] = (uint8_t)lane;
+                        ++o;
+                    }
+                    mmax = fmaxf(mmax, bsum);
                 }
-                hfx_wave_sync();
-                for (uint32_t i = lane; i < nitems; i += HFX_WAVE) {
-                    const float bb = itemb[i];
-                    const uint32_t c = itemc[i];
-                    const float y = rr[c] - bb;
-                    uint32_t *mrow = map + c * DN_MSTR;
+                /* (every pass sees the heaviest b of the candidates so far;
+                 * the bound uses the residual, which the b ladder of any
+                 * later pass cannot exceed by more than rounding) */
+                float mm = mmax;
 #pragma unroll
-                    for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z)
-                        if (z <= lvl) dn_peaks(mrow, bb, y, zc[z - 1], kb);
+                for (uint32_t o2 = 32; o2 > 0; o2 >>= 1) mm = fmaxf(mm, __shfl_xor(mm, o2, 64));
+                kq = min(kb, (((uint32_t)(mm + 24.0f) + 4u) + 127u) & ~127u);
+                hfx_wave_sync();
+                /* two ladder positions per lane per round; a lane past the
+                 * end takes position 0 of the pass with kq = 0, so that every
+                 * code it makes goes to its private dummy word as 0 */
+                for (uint32_t i0 = 0; i0 < nitems; i0 += 2 * HFX_WAVE) {
+                    const uint32_t ia = i0 + lane, ib = ia + HFX_WAVE;
+                    const bool va = ia < nitems, vb = ib < nitems;
+                    const float ba = itemb[va ? ia : 0], bbv = itemb[vb ? ib : 0];
+                    const uint32_t ca = itemc[va ? ia : 0], cb = itemc[vb ? ib : 0];
+                    const float ya = rr[ca] - ba, yb = rr[cb] - bbv;
+                    uint32_t *ra = map + ca * DN_MSTR, *rb = map + cb * DN_MSTR;
+                    const uint32_t ka = va ? kq : 0u, kbq = vb ? kq : 0u;
+#pragma unroll
+                    for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z) {
+                        if (z <= lvl) {
+                            dn_peaks(ra, ba, ya, zc[z - 1], ka, lane);
+                            dn_peaks(rb, bbv, yb, zc[z - 1], kbq, lane);
+                        }
+                    }
                 }
                 hfx_wave_sync();
                 c_lo = c_hi;
             }
-            /* 16 x 16 x kb contraction, clearing the map */
-            dn_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            /* 16 x 16 x kq contraction, clearing the map; two accumulator
+             * chains, the A words of the next four k-steps read ahead */
+            dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
             uint32_t *arow = map + row * DN_MSTR + kg * DN_NKS;
             const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
-            const uint32_t nks = kb >> 5;
+            const uint32_t nks = kq >> 5;
+            dn_u32x4 a4 = *(const dn_u32x4 *)arow;
+            dn_u32x4 n0 = *(const dn_u32x4 *)(bcol + 0 * 32), n1v = *(const dn_u32x4 *)(bcol + 1 * 32);
+            dn_u32x4 n2 = *(const dn_u32x4 *)(bcol + 2 * 32), n3 = *(const dn_u32x4 *)(bcol + 3 * 32);
             for (uint32_t ks = 0; ks < nks; ks += 4) {
-                const dn_u32x4 a4 = *(const dn_u32x4 *)(arow + ks);
+                const dn_u32x4 cw = a4;
+                const dn_u32x4 b0 = n0, b1 = n1v, b2 = n2, b3 = n3;
                 *(dn_u32x4 *)(arow + ks) = dn_u32x4{0u, 0u, 0u, 0u};
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    const dn_u32x4 av = dn_weights(a4[u]);
-                    const dn_u32x4 bv = *(const dn_u32x4 *)(bcol + (ks + u) * 32);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, av), __builtin_bit_cast(dn_bf16x8, bv), acc,
-                        0, 0, 0);
+                if (ks + 4 < nks) {
+                    a4 = *(const dn_u32x4 *)(arow + ks + 4);
+                    n0 = *(const dn_u32x4 *)(bcol + (ks + 4) * 32);
+                    n1v = *(const dn_u32x4 *)(bcol + (ks + 5) * 32);
+                    n2 = *(const dn_u32x4 *)(bcol + (ks + 6) * 32);
+                    n3 = *(const dn_u32x4 *)(bcol + (ks + 7) * 32);
                 }
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)), __builtin_bit_cast(dn_bf16x8, b0),
+                    acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)), __builtin_bit_cast(dn_bf16x8, b1),
+                    acc1, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)), __builtin_bit_cast(dn_bf16x8, b2),
+                    acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)), __builtin_bit_cast(dn_bf16x8, b3),
+                    acc1, 0, 0, 0);
             }
             hfx_wave_sync();
+            /* consume what was issued at the start of the tile: the next
+             * tile's residues go to LDS, the pipeline registers advance */
+            dn_stage_store(st1, res, lane);
+            tc = tc1;
+            n = n1;
+            r = r1;
+            tc1 = tc2;
+            n1 = tn2 > tc2 ? min(tn2 - tc2, (uint32_t)DN_ITEMS) : 0u;
+            r1 = r2;
+            pep2 = pep3;
             /* C[row 4*(lane/16)+i][col lane%16]; hi + lo of spectrum col/2 */
 #pragma unroll
             for (uint32_t i = 0; i < 4; ++i) {
-                const float v = acc[i] + __shfl_xor(acc[i], 1, 64);
+                const float c4 = acc0[i] + acc1[i];
+                const float v = c4 + __shfl_xor(c4, 1, 64);
                 const uint32_t rr_i = 4u * kg + i, g = row >> 1, p = p0 + rr_i;
                 if ((lane & 1u) == 0 && rr_i < nrows && m_z[g] == lvl && m_beg[g] <= p &&
                     p < m_end[g])
                     a.scores[m_off[g] + (p - m_beg[g])] = v;
             }
+            hfx_wave_sync();
         }
     }
 }

@@ -144,7 +144,7 @@ def test_batch_dense_graph_and_autotune(denv, orc):
     bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
     bs.set_precision(bs.BF16X2_DENSE)
     G, Cn, T = bs.autotune()
-    assert G == 0 and Cn in (64, 128, 256, 512, 1024) and T == 256
+    assert G == 0 and Cn in (256, 512, 1024, 2048, 4096) and T == 256
     bs.run()
     first = bs.results(keep_scores=True)
     check_topn(first, denv["ref"], orc)
