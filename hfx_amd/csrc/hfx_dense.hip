@@ -29,9 +29,8 @@
  *   - each wave takes 16 candidates: lanes 0-15 run the sequential FP32 b
  *     ladder of one candidate each (bit-exact prefix), then every lane takes
  *     one ladder position and ORs the unary weight codes of its peaks into the
- *     wave's LDS nibble map (OR of codes = max weight); the map is stored
- *     [row][k-group][k-step] so that one 128-bit LDS read gives a lane its A
- *     words for four k-steps;
+ *     wave's LDS nibble map (OR of codes = max weight), bin x in word x/8
+ *     of its row, so a lane's A word at k-step ks is word 4 ks + k-group;
  *   - A words (8 nibbles) become eight BF16 weights with four v_perm lookups
  *     (codes 0/1/3/7 select the high and low bytes of 0, 10, 25, 50) and four
  *     byte interleaves; the map is cleared as it is read.
@@ -51,9 +50,9 @@
 #define DN_ROWS 16                  /* candidates per wave tile (MFMA M) */
 #define DN_MAXK 2176                /* bins of a row (multiple of 128) */
 #define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
-#define DN_MSTR (4 * DN_NKS)        /* map words per row */
+#define DN_MSTR (DN_MAXK / 8 + 4)   /* map words per row: bin x in word x/8 (padded) */
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
-#define DN_ITEMS 768                /* ladder positions per wave pass */
+#define DN_ITEMS 704                /* ladder positions per wave pass */
 #define DN_RMAX 64                  /* residues per candidate staged in LDS */
 #define DN_RSTR 17                  /* dwords per staged row (3 + DN_RMAX bytes) */
 #define DN_NONE 0xffffffffu
@@ -223,40 +222,57 @@ struct DnArgs {
     hfx_spectra sp;
 };
 
-struct DnZ {
-    float fz, zh, R, h2o, nh3;
-};
-
 /* one weight code at bin x of a map row; a bin outside [0, kq) ORs 0 into
  * a word private to the lane instead (no branch, no shared address) */
 __device__ __forceinline__ void dn_put(uint32_t *row, int32_t x, uint32_t code, uint32_t kq,
                                        uint32_t lane)
 {
-    const bool ok = (uint32_t)x < kq;
-    const uint32_t w = ok ? (((uint32_t)x >> 3) & 3u) * DN_NKS + ((uint32_t)x >> 5) : lane;
-    atomicOr(&row[w], ok ? code << (((uint32_t)x & 7u) * 4u) : 0u);
+    const uint32_t v = (uint32_t)x < kq ? code << (((uint32_t)x & 7u) << 2) : 0u;
+    atomicOr(&row[v ? (uint32_t)x >> 3 : lane], v);
 }
 
-/* peaks of one ladder position (b, y) at charge z into a map row: the ion
- * series of ion_series.cu:111-156 with the FP32 operation order of the
- * sparse path */
-__device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, const DnZ &q,
-                                         uint32_t kq, uint32_t lane)
+/* 25 at x-1, 50 at x, 25 at x+1 (each kept only below kq): the nibbles
+ * 3, 7, 3 span at most two words; the word of x-1 gets them shifted into
+ * place, the word of x+1 gets whatever spilled over (the same value again
+ * when all three share a word) */
+__device__ __forceinline__ void dn_put3(uint32_t *row, int32_t x, uint32_t kq, uint32_t lane)
 {
-    dn_put(row, dn_bin(dn_div((b - HFX_DEV_CO) + q.zh, q.fz, q.R)), HFX_CODE10, kq, lane);
-    const float mb = dn_div(b + q.zh, q.fz, q.R);
-    const int32_t xb = dn_bin(mb);
-    dn_put(row, xb, HFX_CODE50, kq, lane);
-    dn_put(row, xb + 1, HFX_CODE25, kq, lane);
-    dn_put(row, xb - 1, HFX_CODE25, kq, lane);
-    dn_put(row, dn_bin(mb - q.h2o), HFX_CODE10, kq, lane);
-    dn_put(row, dn_bin(mb - q.nh3), HFX_CODE10, kq, lane);
-    const float my = dn_div((y + HFX_DEV_H2O) + q.zh, q.fz, q.R);
-    const int32_t xy = dn_bin(my);
-    dn_put(row, xy, HFX_CODE50, kq, lane);
-    dn_put(row, xy + 1, HFX_CODE25, kq, lane);
-    dn_put(row, xy - 1, HFX_CODE25, kq, lane);
-    dn_put(row, dn_bin(my - q.nh3), HFX_CODE10, kq, lane);
+    const int32_t xm = x - 1;
+    const uint32_t sh = ((uint32_t)xm & 7u) << 2;
+    const uint32_t t = ((uint32_t)xm < kq ? HFX_CODE25 : 0u) |
+                       ((uint32_t)x < kq ? HFX_CODE50 << 4 : 0u) |
+                       ((uint32_t)(x + 1) < kq ? HFX_CODE25 << 8 : 0u);
+    const uint32_t v1 = t << sh;
+    const uint32_t v2 = sh >= 24u ? t >> (32u - sh) : v1;
+    atomicOr(&row[v1 ? (uint32_t)xm >> 3 : lane], v1);
+    atomicOr(&row[v2 ? (uint32_t)(x + 1) >> 3 : lane], v2);
+}
+
+/* division by the fragment charge Z: exact as a product for Z = 1, 2, 4 */
+template <int Z>
+__device__ __forceinline__ float dn_zdiv(float x)
+{
+    if constexpr (Z == 3) return dn_div(x, 3.0f, 1.0f / 3.0f);
+    else return x * (1.0f / (float)Z);
+}
+
+/* peaks of one ladder position (b, y) at charge Z into a map row: the ion
+ * series of ion_series.cu:111-156 with the FP32 operation order of the
+ * sparse path (zh = H * Z, H2O / Z and NH3 / Z fold to the same FP32 values) */
+template <int Z>
+__device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32_t kq,
+                                         uint32_t lane)
+{
+    constexpr float zh = HFX_DEV_H * (float)Z;
+    constexpr float h2o = HFX_DEV_H2O / (float)Z, nh3 = HFX_DEV_NH3 / (float)Z;
+    dn_put(row, dn_bin(dn_zdiv<Z>((b - HFX_DEV_CO) + zh)), HFX_CODE10, kq, lane);
+    const float mb = dn_zdiv<Z>(b + zh);
+    dn_put3(row, dn_bin(mb), kq, lane);
+    dn_put(row, dn_bin(mb - h2o), HFX_CODE10, kq, lane);
+    dn_put(row, dn_bin(mb - nh3), HFX_CODE10, kq, lane);
+    const float my = dn_zdiv<Z>((y + HFX_DEV_H2O) + zh);
+    dn_put3(row, dn_bin(my), kq, lane);
+    dn_put(row, dn_bin(my - nh3), HFX_CODE10, kq, lane);
 }
 
 /* residues of the tile's candidates staged in LDS: row c holds the ion bytes
@@ -330,12 +346,6 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     for (uint32_t i = tid; i < DN_WAVES * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
     const uint64_t ntiles = dn_sload(a.tile_off, a.B);
     const uint32_t row = lane & 15u, kg = lane >> 4;
-    DnZ zc[HFX_MAX_CHARGE];
-#pragma unroll
-    for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z) {
-        const float fz = (float)z;
-        zc[z - 1] = DnZ{fz, HFX_DEV_H * fz, 1.0f / fz, HFX_DEV_H2O / fz, HFX_DEV_NH3 / fz};
-    }
     uint32_t cur = DN_NONE;
     /* work items are claimed one ahead: the claim of item k+1 is in flight
      * while item k is processed */
@@ -528,12 +538,19 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                     const float ya = rr[ca] - ba, yb = rr[cb] - bbv;
                     uint32_t *ra = map + ca * DN_MSTR, *rb = map + cb * DN_MSTR;
                     const uint32_t ka = va ? kq : 0u, kbq = vb ? kq : 0u;
-#pragma unroll
-                    for (uint32_t z = 1; z <= HFX_MAX_CHARGE; ++z) {
-                        if (z <= lvl) {
-                            dn_peaks(ra, ba, ya, zc[z - 1], ka, lane);
-                            dn_peaks(rb, bbv, yb, zc[z - 1], kbq, lane);
-                        }
+                    dn_peaks<1>(ra, ba, ya, ka, lane);
+                    dn_peaks<1>(rb, bbv, yb, kbq, lane);
+                    if (lvl >= 2) {
+                        dn_peaks<2>(ra, ba, ya, ka, lane);
+                        dn_peaks<2>(rb, bbv, yb, kbq, lane);
+                    }
+                    if (lvl >= 3) {
+                        dn_peaks<3>(ra, ba, ya, ka, lane);
+                        dn_peaks<3>(rb, bbv, yb, kbq, lane);
+                    }
+                    if (lvl >= 4) {
+                        dn_peaks<4>(ra, ba, ya, ka, lane);
+                        dn_peaks<4>(rb, bbv, yb, kbq, lane);
                     }
                 }
                 hfx_wave_sync();
@@ -542,18 +559,26 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
             /* 16 x 16 x kq contraction, clearing the map; two accumulator
              * chains, the A words of the next four k-steps read ahead */
             dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
-            uint32_t *arow = map + row * DN_MSTR + kg * DN_NKS;
+            /* lane (row, kg) takes word 4 ks + kg of its row at k-step ks */
+            uint32_t *arow = map + row * DN_MSTR + kg;
             const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
             const uint32_t nks = kq >> 5;
-            dn_u32x4 a4 = *(const dn_u32x4 *)arow;
+            uint32_t w0 = arow[0], w1 = arow[4], w2 = arow[8], w3 = arow[12];
             dn_u32x4 n0 = *(const dn_u32x4 *)(bcol + 0 * 32), n1v = *(const dn_u32x4 *)(bcol + 1 * 32);
             dn_u32x4 n2 = *(const dn_u32x4 *)(bcol + 2 * 32), n3 = *(const dn_u32x4 *)(bcol + 3 * 32);
             for (uint32_t ks = 0; ks < nks; ks += 4) {
-                const dn_u32x4 cw = a4;
+                const dn_u32x4 cw = dn_u32x4{w0, w1, w2, w3};
                 const dn_u32x4 b0 = n0, b1 = n1v, b2 = n2, b3 = n3;
-                *(dn_u32x4 *)(arow + ks) = dn_u32x4{0u, 0u, 0u, 0u};
+                uint32_t *aw = arow + 4 * ks;
+                aw[0] = 0u;
+                aw[4] = 0u;
+                aw[8] = 0u;
+                aw[12] = 0u;
                 if (ks + 4 < nks) {
-                    a4 = *(const dn_u32x4 *)(arow + ks + 4);
+                    w0 = aw[16];
+                    w1 = aw[20];
+                    w2 = aw[24];
+                    w3 = aw[28];
                     n0 = *(const dn_u32x4 *)(bcol + (ks + 4) * 32);
                     n1v = *(const dn_u32x4 *)(bcol + (ks + 5) * 32);
                     n2 = *(const dn_u32x4 *)(bcol + (ks + 6) * 32);
