@@ -45,24 +45,27 @@
 
 #define DN_NS 8                     /* spectra per block */
 #define DN_COLS 16                  /* MFMA N: hi and lo of each spectrum */
-#define DN_WAVES 4
+#define DN_WAVES 8                  /* four pairs of waves */
+#define DN_PAIRS 4
 #define DN_THREADS (DN_WAVES * HFX_WAVE)
 #define DN_ROWS 16                  /* candidates per wave tile (MFMA M) */
 #define DN_MAXK 2176                /* bins of a row (multiple of 128) */
 #define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
 #define DN_MSTR (DN_MAXK / 8 + 4)   /* map words per row: bin x in word x/8 (padded) */
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
-#define DN_ITEMS 704                /* ladder positions per wave pass */
+#define DN_ITEMS 256                /* ladder positions per wave pass */
 #define DN_RMAX 64                  /* residues per candidate staged in LDS */
 #define DN_RSTR 17                  /* dwords per staged row (3 + DN_RMAX bytes) */
 #define DN_NONE 0xffffffffu
 
 #define DN_B_BYTES (DN_COLS * DN_CSTR * 2)
-#define DN_MAP_BYTES (DN_WAVES * DN_ROWS * DN_MSTR * 4)
+#define DN_MAP_BYTES (DN_PAIRS * DN_ROWS * DN_MSTR * 4)
 #define DN_ITEMB_BYTES (DN_WAVES * DN_ITEMS * 4)
 #define DN_ITEMC_BYTES (DN_WAVES * DN_ITEMS)
-#define DN_RES_BYTES (DN_WAVES * DN_ROWS * DN_RSTR * 4)
-#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES)
+#define DN_RES_BYTES (DN_PAIRS * DN_ROWS * DN_RSTR * 4)
+#define DN_COMB_BYTES (DN_PAIRS * HFX_WAVE * 16)
+#define DN_LDS_BYTES                                                                         \
+    (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES + DN_COMB_BYTES)
 
 typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
@@ -278,22 +281,23 @@ __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32
 /* residues of the tile's candidates staged in LDS: row c holds the ion bytes
  * from the 4-byte aligned a0 <= tc on, DN_RMAX residues at most */
 struct DnStage {
-    uint32_t v[5];
+    uint32_t v[3];
 };
 
-/* lane (c = lane & 15, q = lane >> 4) loads dwords q, q+4, ... of candidate
- * c's staged row; every byte read lies in [0, num_ions) */
+/* half h of a wave pair stages candidates 8h..8h+7: lane (c = 8h + lane % 8,
+ * q = lane / 8) loads dwords q, q+8, q+16 of candidate c's row; every byte
+ * read lies in [0, num_ions) */
 __device__ __forceinline__ void dn_stage_load(DnStage &st, const uint8_t *ions, uint64_t num_ions,
-                                              uint32_t tc, uint32_t n, uint32_t lane)
+                                              uint32_t tc, uint32_t n, uint32_t lane, uint32_t h)
 {
-    const uint32_t c = lane & 15u, q = lane >> 4;
+    const uint32_t c = 8u * h + (lane & 7u), q = lane >> 3;
     const uint32_t tcc = __shfl(tc, c, 64), nc = __shfl(n, c, 64);
     const uint32_t mis = (uint32_t)(((uintptr_t)ions + tcc) & 3u);
     const int64_t a0 = (int64_t)tcc - mis;
     const uint32_t ndw = (mis + min(nc, (uint32_t)DN_RMAX) + 3u) >> 2;
 #pragma unroll
-    for (uint32_t i = 0; i < 5; ++i) {
-        const uint32_t d = q + 4u * i;
+    for (uint32_t i = 0; i < 3; ++i) {
+        const uint32_t d = q + 8u * i;
         uint32_t w = 0;
         if (d < ndw) {
             const int64_t at = a0 + 4 * (int64_t)d;
@@ -309,12 +313,13 @@ __device__ __forceinline__ void dn_stage_load(DnStage &st, const uint8_t *ions, 
     }
 }
 
-__device__ __forceinline__ void dn_stage_store(const DnStage &st, uint32_t *res, uint32_t lane)
+__device__ __forceinline__ void dn_stage_store(const DnStage &st, uint32_t *res, uint32_t lane,
+                                               uint32_t h)
 {
-    const uint32_t c = lane & 15u, q = lane >> 4;
+    const uint32_t c = 8u * h + (lane & 7u), q = lane >> 3;
 #pragma unroll
-    for (uint32_t i = 0; i < 5; ++i) {
-        const uint32_t d = q + 4u * i;
+    for (uint32_t i = 0; i < 3; ++i) {
+        const uint32_t d = q + 8u * i;
         if (d < DN_RSTR) res[c * DN_RSTR + d] = st.v[i];
     }
 }
@@ -328,22 +333,28 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     uint8_t *itemc_all = dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES;
     uint32_t *res_all = (uint32_t *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES +
                                      DN_ITEMC_BYTES);
+    dn_f32x4 *comb_all = (dn_f32x4 *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES +
+                                      DN_ITEMC_BYTES + DN_RES_BYTES);
     __shared__ float mtab[32];
     __shared__ uint32_t m_s[DN_NS], m_beg[DN_NS], m_end[DN_NS], m_z[DN_NS];
     __shared__ uint64_t m_off[DN_NS];
-    __shared__ float rr_all[DN_WAVES][DN_ROWS];
+    __shared__ float rr_all[DN_PAIRS][DN_ROWS];
     __shared__ uint32_t sh_t[2];
 
+    /* wave pair q = wave % 4 (halves h = 0, 1) shares a map, its ladder
+     * positions and staged residues */
     const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
-    uint32_t *map = maps + wave * DN_ROWS * DN_MSTR;
+    const uint32_t q = wave % DN_PAIRS, h = wave / DN_PAIRS;
+    uint32_t *map = maps + q * DN_ROWS * DN_MSTR;
     float *itemb = itemb_all + wave * DN_ITEMS;
     uint8_t *itemc = itemc_all + wave * DN_ITEMS;
-    uint32_t *res = res_all + wave * DN_ROWS * DN_RSTR;
+    uint32_t *res = res_all + q * DN_ROWS * DN_RSTR;
     const uint8_t *resb = (const uint8_t *)res;
-    float *rr = rr_all[wave];
+    dn_f32x4 *comb = comb_all + q * HFX_WAVE;
+    float *rr = rr_all[q];
     const uint8_t *ions = a.db.d_ions;
     if (tid < 32) mtab[tid] = tid < 26 ? a.db.d_mass[tid] : 0.0f;
-    for (uint32_t i = tid; i < DN_WAVES * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
+    for (uint32_t i = tid; i < DN_PAIRS * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
     const uint64_t ntiles = dn_sload(a.tile_off, a.B);
     const uint32_t row = lane & 15u, kg = lane >> 4;
     uint32_t cur = DN_NONE;
@@ -415,19 +426,20 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         const uint32_t local = (uint32_t)(t - dn_sload(a.tile_off, b));
         const uint32_t li = local / nch, chunk = local - li * nch;
         uint32_t m = mask;
-        for (uint32_t q = 0; q < li; ++q) m &= m - 1u;
+        for (uint32_t qq = 0; qq < li; ++qq) m &= m - 1u;
         const uint32_t lvl = (uint32_t)__builtin_ctz(m);
         const uint32_t cbeg = beg + chunk * a.CH, cend = min(cbeg + a.CH, end);
-        const uint32_t step = DN_WAVES * DN_ROWS;
-        uint32_t p0 = cbeg + wave * DN_ROWS;
-        if (p0 >= cend) continue;
+        const uint32_t step = DN_PAIRS * DN_ROWS;
+        /* every pair runs the same number of tiles (barriers below are
+         * workgroup-wide); tiles past the chunk are empty */
+        const uint32_t ntile = (cend - cbeg + step - 1) / step;
+        uint32_t p0 = cbeg + q * DN_ROWS;
 
-        /* software pipeline over this wave's tiles p0, p0 + 64, ...: every
+        /* software pipeline over the pair's tiles p0, p0 + 64, ...: every
          * global load is issued at the start of a tile and consumed at its
-         * end, before the tile's score stores (vmcnt counts stores too), so
-         * each wait falls after a full tile of LDS and MFMA work:
+         * end, before the tile's score stores (vmcnt counts stores too):
          *   registers  tile j: tc, n, r   tile j+1: tc1, n1, r1, pep of j+2
-         *   LDS        residues of tile j
+         *   LDS        residues of tile j (each half its 8 candidates)
          *   in flight  residues of j+1, metadata of j+2, peptide of j+3 */
         auto rows_of = [&](uint32_t q0) -> uint32_t {
             return q0 < cend ? min((uint32_t)DN_ROWS, cend - q0) : 0u;
@@ -435,43 +447,42 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         auto load_pep = [&](uint32_t q0) -> uint32_t {
             return lane < rows_of(q0) ? a.db.d_idx_sorted[q0 + lane] : DN_NONE;
         };
-        uint32_t tc = 0, n = 0, tc1 = 0, n1 = 0, pep2 = DN_NONE;
-        float r = 0.0f, r1 = 0.0f;
+        auto meta = [&](uint32_t pep, uint32_t &tcv, uint32_t &nv, float &rv) {
+            tcv = 0;
+            nv = 0;
+            rv = 0.0f;
+            if (pep != DN_NONE) {
+                tcv = a.db.d_tc[pep];
+                const uint32_t tn = a.db.d_tn[pep];
+                nv = tn > tcv ? min(tn - tcv, (uint32_t)DN_ITEMS) : 0u;
+                rv = a.db.d_r[pep];
+            }
+        };
+        uint32_t tc, n, tc1, n1, pep2;
+        float r, r1;
         {
             const uint32_t pep0 = load_pep(p0), pep1 = load_pep(p0 + step);
             pep2 = load_pep(p0 + 2 * step);
-            if (pep0 != DN_NONE) {
-                tc = a.db.d_tc[pep0];
-                const uint32_t tn = a.db.d_tn[pep0];
-                n = tn > tc ? min(tn - tc, (uint32_t)DN_ITEMS) : 0u;
-                r = a.db.d_r[pep0];
-            }
-            if (pep1 != DN_NONE) {
-                tc1 = a.db.d_tc[pep1];
-                const uint32_t tn = a.db.d_tn[pep1];
-                n1 = tn > tc1 ? min(tn - tc1, (uint32_t)DN_ITEMS) : 0u;
-                r1 = a.db.d_r[pep1];
-            }
+            meta(pep0, tc, n, r);
+            meta(pep1, tc1, n1, r1);
             DnStage st0;
-            dn_stage_load(st0, ions, a.db.num_ions, tc, n, lane);
-            dn_stage_store(st0, res, lane);
-            hfx_wave_sync();
+            dn_stage_load(st0, ions, a.db.num_ions, tc, n, lane, h);
+            dn_stage_store(st0, res, lane, h);
         }
-        for (; p0 < cend; p0 += step) {
+        for (uint32_t j = 0; j < ntile; ++j, p0 += step) {
             const uint32_t nrows = rows_of(p0);
             /* issue: residues of j+1, metadata of j+2, peptide of j+3 */
             DnStage st1;
-            dn_stage_load(st1, ions, a.db.num_ions, tc1, n1, lane);
-            uint32_t tc2 = 0, tn2 = 0;
-            float r2 = 0.0f;
-            if (pep2 != DN_NONE) {
-                tc2 = a.db.d_tc[pep2];
-                tn2 = a.db.d_tn[pep2];
-                r2 = a.db.d_r[pep2];
-            }
+            dn_stage_load(st1, ions, a.db.num_ions, tc1, n1, lane, h);
+            uint32_t tc2, n2;
+            float r2;
+            meta(pep2, tc2, n2, r2);
             const uint32_t pep3 = load_pep(p0 + 3 * step);
 
-            if (lane < DN_ROWS) rr[lane] = r;
+            /* half h owns candidates 8h..8h+7 of the tile: their residual,
+             * ladder positions and map rows; nothing below the next barrier
+             * touches the other half's */
+            if (lane >= 8u * h && lane < 8u * h + 8u) rr[lane] = r;
             uint32_t incl = n;
 #pragma unroll
             for (uint32_t o = 1; o < DN_ROWS; o <<= 1) {
@@ -479,58 +490,58 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                 if (lane >= o) incl += y;
             }
             /* heaviest fragment of the tile: every peak bin is below
-             * m / 1.0005 + 2 for the largest b or y + water + proton m */
-            float mmax = lane < nrows ? r : 0.0f;
-            /* ladder positions in passes of at most DN_ITEMS */
-            uint32_t c_lo = 0;
-            uint32_t kq = kb;
-            while (c_lo < nrows) {
+             * m / 1.0005 + 2 for m = residual + water + proton (the b ladder
+             * sums the same residue masses as the residual) */
+            float mm = lane < nrows ? r : 0.0f;
+#pragma unroll
+            for (uint32_t o2 = 32; o2 > 0; o2 >>= 1) mm = fmaxf(mm, __shfl_xor(mm, o2, 64));
+            const uint32_t kq = nrows ? min(kb, (((uint32_t)(mm + 24.0f) + 4u) + 127u) & ~127u) : 0u;
+            const uint32_t c0 = 8u * h, c1 = min(c0 + 8u, nrows);
+            /* ladder positions of the half's candidates in passes of at most
+             * DN_ITEMS */
+            uint32_t c_lo = c0;
+            while (c_lo < c1) {
                 const uint32_t base = c_lo ? __shfl(incl, c_lo - 1, 64) : 0u;
-                const bool fits = lane >= c_lo && lane < nrows && incl - base <= DN_ITEMS;
+                const bool fits = lane >= c_lo && lane < c1 && incl - base <= DN_ITEMS;
                 const uint64_t bal = __ballot(fits);
-                const uint32_t c_hi = bal ? 64u - (uint32_t)__builtin_clzll(bal) : c_lo + 1;
+                const uint32_t c_hi = bal ? 64u - (uint32_t)__builtin_clzll(bal) : c_lo + 1u;
                 const uint32_t nitems = __shfl(incl, c_hi - 1, 64) - base;
-                if (lane >= c_lo && lane < c_hi) {
-                    /* sequential FP32 b ladder of candidate `lane` */
-                    uint32_t o = incl - n - base;
+                /* sequential FP32 b ladder of candidate c = c0 + lane */
+                const uint32_t c = c0 + (lane & 7u);
+                const uint32_t tcc = __shfl(tc, c, 64), nc = __shfl(n, c, 64);
+                const uint32_t oc = __shfl(incl, c, 64) - nc - base;
+                if (lane < 8 && c >= c_lo && c < c_hi) {
+                    uint32_t o = oc;
                     float bsum = 0.0f;
-                    const uint32_t mis = (uint32_t)(((uintptr_t)ions + tc) & 3u);
-                    const uint8_t *rs = resb + lane * DN_RSTR * 4 + mis;
-                    const uint32_t ns = min(n, (uint32_t)DN_RMAX);
+                    const uint32_t mis = (uint32_t)(((uintptr_t)ions + tcc) & 3u);
+                    const uint8_t *rs = resb + c * DN_RSTR * 4 + mis;
+                    const uint32_t ns = min(nc, (uint32_t)DN_RMAX);
                     for (uint32_t k0 = 0; k0 < ns; k0 += 8) {
                         uint32_t v[8];
 #pragma unroll
-                        for (uint32_t j = 0; j < 8; ++j) v[j] = k0 + j < ns ? rs[k0 + j] : 0u;
+                        for (uint32_t jj = 0; jj < 8; ++jj) v[jj] = k0 + jj < ns ? rs[k0 + jj] : 0u;
 #pragma unroll
-                        for (uint32_t j = 0; j < 8; ++j) {
-                            if (k0 + j < ns) {
-                                bsum = bsum + mtab[HFX_ACID_IDX(v[j])];
+                        for (uint32_t jj = 0; jj < 8; ++jj) {
+                            if (k0 + jj < ns) {
+                                bsum = bsum + mtab[HFX_ACID_IDX(v[jj])];
                                 itemb[o] = bsum;
-                                itemc[o] = (uint8_t)lane;
+                                itemc[o] = (uint8_t)c;
                                 ++o;
                             }
                         }
                     }
-                    for (uint32_t k = ns; k < n; ++k) { /* beyond the staged residues */
-                        bsum = bsum + mtab[HFX_ACID_IDX(ions[tc + k])];
+                    for (uint32_t k = ns; k < nc; ++k) { /* beyond the staged residues */
+                        bsum = bsum + mtab[HFX_ACID_IDX(ions[tcc + k])];
                         itemb[o] = bsum;
-                        itemc[o] = (uint8_t)lane;
+                        itemc[o] = (uint8_t)c;
                         ++o;
                     }
-                    mmax = fmaxf(mmax, bsum);
                 }
-                /* (every pass sees the heaviest b of the candidates so far;
-                 * the bound uses the residual, which the b ladder of any
-                 * later pass cannot exceed by more than rounding) */
-                float mm = mmax;
-#pragma unroll
-                for (uint32_t o2 = 32; o2 > 0; o2 >>= 1) mm = fmaxf(mm, __shfl_xor(mm, o2, 64));
-                kq = min(kb, (((uint32_t)(mm + 24.0f) + 4u) + 127u) & ~127u);
                 hfx_wave_sync();
-                /* two ladder positions per lane per round; a lane past the
-                 * end takes position 0 of the pass with kq = 0, so that every
-                 * code it makes goes to its private dummy word as 0 */
-                for (uint32_t i0 = 0; i0 < nitems; i0 += 2 * HFX_WAVE) {
+                /* two positions per lane per round; a lane past the end takes
+                 * position 0 with kq = 0, so that every code it makes goes to
+                 * its private dummy word as 0 */
+                for (uint32_t i0 = 0; i0 < nitems; i0 += 2u * HFX_WAVE) {
                     const uint32_t ia = i0 + lane, ib = ia + HFX_WAVE;
                     const bool va = ia < nitems, vb = ib < nitems;
                     const float ba = itemb[va ? ia : 0], bbv = itemb[vb ? ib : 0];
@@ -556,69 +567,82 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                 hfx_wave_sync();
                 c_lo = c_hi;
             }
-            /* 16 x 16 x kq contraction, clearing the map; two accumulator
-             * chains, the A words of the next four k-steps read ahead */
+            __syncthreads(); /* (b) the tile's map rows are complete */
+
+            /* 16 x 16 x kq contraction, half h takes its half of the k-steps
+             * and clears those map words; two accumulator chains, operands of
+             * the next four k-steps read ahead.  Lane (row, kg) takes word
+             * 4 ks + kg of its row at k-step ks. */
             dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
-            /* lane (row, kg) takes word 4 ks + kg of its row at k-step ks */
+            const uint32_t nks = kq >> 5, mid = (nks >> 3) << 2;
+            const uint32_t ks_lo = h ? mid : 0u, ks_hi = h ? nks : mid;
             uint32_t *arow = map + row * DN_MSTR + kg;
             const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
-            const uint32_t nks = kq >> 5;
-            uint32_t w0 = arow[0], w1 = arow[4], w2 = arow[8], w3 = arow[12];
-            dn_u32x4 n0 = *(const dn_u32x4 *)(bcol + 0 * 32), n1v = *(const dn_u32x4 *)(bcol + 1 * 32);
-            dn_u32x4 n2 = *(const dn_u32x4 *)(bcol + 2 * 32), n3 = *(const dn_u32x4 *)(bcol + 3 * 32);
-            for (uint32_t ks = 0; ks < nks; ks += 4) {
-                const dn_u32x4 cw = dn_u32x4{w0, w1, w2, w3};
-                const dn_u32x4 b0 = n0, b1 = n1v, b2 = n2, b3 = n3;
-                uint32_t *aw = arow + 4 * ks;
-                aw[0] = 0u;
-                aw[4] = 0u;
-                aw[8] = 0u;
-                aw[12] = 0u;
-                if (ks + 4 < nks) {
-                    w0 = aw[16];
-                    w1 = aw[20];
-                    w2 = aw[24];
-                    w3 = aw[28];
-                    n0 = *(const dn_u32x4 *)(bcol + (ks + 4) * 32);
-                    n1v = *(const dn_u32x4 *)(bcol + (ks + 5) * 32);
-                    n2 = *(const dn_u32x4 *)(bcol + (ks + 6) * 32);
-                    n3 = *(const dn_u32x4 *)(bcol + (ks + 7) * 32);
+            if (ks_lo < ks_hi) {
+                uint32_t *aw0 = arow + 4 * ks_lo;
+                uint32_t w0 = aw0[0], w1 = aw0[4], w2 = aw0[8], w3 = aw0[12];
+                dn_u32x4 n0 = *(const dn_u32x4 *)(bcol + (ks_lo + 0) * 32);
+                dn_u32x4 n1v = *(const dn_u32x4 *)(bcol + (ks_lo + 1) * 32);
+                dn_u32x4 n2v = *(const dn_u32x4 *)(bcol + (ks_lo + 2) * 32);
+                dn_u32x4 n3 = *(const dn_u32x4 *)(bcol + (ks_lo + 3) * 32);
+                for (uint32_t ks = ks_lo; ks < ks_hi; ks += 4) {
+                    const dn_u32x4 cw = dn_u32x4{w0, w1, w2, w3};
+                    const dn_u32x4 b0 = n0, b1 = n1v, b2 = n2v, b3 = n3;
+                    uint32_t *aw = arow + 4 * ks;
+                    aw[0] = 0u;
+                    aw[4] = 0u;
+                    aw[8] = 0u;
+                    aw[12] = 0u;
+                    if (ks + 4 < ks_hi) {
+                        w0 = aw[16];
+                        w1 = aw[20];
+                        w2 = aw[24];
+                        w3 = aw[28];
+                        n0 = *(const dn_u32x4 *)(bcol + (ks + 4) * 32);
+                        n1v = *(const dn_u32x4 *)(bcol + (ks + 5) * 32);
+                        n2v = *(const dn_u32x4 *)(bcol + (ks + 6) * 32);
+                        n3 = *(const dn_u32x4 *)(bcol + (ks + 7) * 32);
+                    }
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
+                        __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
+                        __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
+                        __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
+                        __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
                 }
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)), __builtin_bit_cast(dn_bf16x8, b0),
-                    acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)), __builtin_bit_cast(dn_bf16x8, b1),
-                    acc1, 0, 0, 0);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)), __builtin_bit_cast(dn_bf16x8, b2),
-                    acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)), __builtin_bit_cast(dn_bf16x8, b3),
-                    acc1, 0, 0, 0);
             }
-            hfx_wave_sync();
+            const dn_f32x4 part = acc0 + acc1;
+            if (h == 1) comb[lane] = part;
             /* consume what was issued at the start of the tile: the next
              * tile's residues go to LDS, the pipeline registers advance */
-            dn_stage_store(st1, res, lane);
+            dn_stage_store(st1, res, lane, h);
             tc = tc1;
             n = n1;
             r = r1;
             tc1 = tc2;
-            n1 = tn2 > tc2 ? min(tn2 - tc2, (uint32_t)DN_ITEMS) : 0u;
+            n1 = n2;
             r1 = r2;
             pep2 = pep3;
-            /* C[row 4*(lane/16)+i][col lane%16]; hi + lo of spectrum col/2 */
+            __syncthreads(); /* (c) both halves' partial sums are in */
+            if (h == 0) {
+                const dn_f32x4 other = comb[lane];
+                /* C[row 4*(lane/16)+i][col lane%16]; hi + lo of spectrum col/2 */
 #pragma unroll
-            for (uint32_t i = 0; i < 4; ++i) {
-                const float c4 = acc0[i] + acc1[i];
-                const float v = c4 + __shfl_xor(c4, 1, 64);
-                const uint32_t rr_i = 4u * kg + i, g = row >> 1, p = p0 + rr_i;
-                if ((lane & 1u) == 0 && rr_i < nrows && m_z[g] == lvl && m_beg[g] <= p &&
-                    p < m_end[g])
-                    a.scores[m_off[g] + (p - m_beg[g])] = v;
+                for (uint32_t i = 0; i < 4; ++i) {
+                    const float c4 = part[i] + other[i];
+                    const float v = c4 + __shfl_xor(c4, 1, 64);
+                    const uint32_t rr_i = 4u * kg + i, g = row >> 1, p = p0 + rr_i;
+                    if ((lane & 1u) == 0 && rr_i < nrows && m_z[g] == lvl && m_beg[g] <= p &&
+                        p < m_end[g])
+                        a.scores[m_off[g] + (p - m_beg[g])] = v;
+                }
             }
-            hfx_wave_sync();
         }
     }
 }
