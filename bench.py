@@ -147,10 +147,11 @@ def calib_read(mib: int):
                       "launches": 3}), flush=True)
 
 
-def bench_plain(env, topn, steps, warmup):
+def bench_plain(env, topn, steps, warmup, margin=16):
     from hfx_amd import pipeline
 
-    srch = pipeline.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn, margin=16)
+    srch = pipeline.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn,
+                                margin=margin)
 
     def step(ev=None):
         srch.window()
@@ -201,10 +202,11 @@ def dense_flops(env, res) -> tuple[float, float]:
     return executed, useful
 
 
-def bench_batched(env, topn, steps, warmup, precision=0):
+def bench_batched(env, topn, steps, warmup, precision=0, margin=16):
     from hfx_amd import pipeline
 
-    bs = pipeline.BatchSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn, margin=16)
+    bs = pipeline.BatchSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn,
+                              margin=margin)
     bs.set_precision(precision)
     t0 = time.time()
     cfg = bs.autotune()
@@ -231,6 +233,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tryptic-20k-10k")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--margin", type=int, default=16,
+                    help="extra candidates re-scored exactly beyond the top N")
     ap.add_argument("--path", choices=("all", "plain", "batched", "dense"), default="all")
     ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",
                     help="only run a streaming read of MIB MiB (k_mvm_rows over a "
@@ -251,7 +255,8 @@ def main():
     paths = {}
     plain_res = None
     if args.path in ("all", "plain"):
-        plain_res, el, kern_ms, flagged = bench_plain(env, topn, args.steps, args.warmup)
+        plain_res, el, kern_ms, flagged = bench_plain(env, topn, args.steps, args.warmup,
+                                                      args.margin)
         sc = int(plain_res.off[-1])
         ab = algorithmic_bytes(env, plain_res)
         paths["plain_fp32_sparse"] = dict(
@@ -261,7 +266,8 @@ def main():
         log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s")
     batch_res = None
     if args.path in ("all", "batched"):
-        batch_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 0)
+        batch_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 0,
+                                                         args.margin)
         sc = int(batch_res.off[-1])
         ab = algorithmic_bytes(env, batch_res)
         paths["batched_fp32_sparse"] = dict(
@@ -274,7 +280,8 @@ def main():
         log(f"batched fp32: {paths['batched_fp32_sparse']['value']:.3e} scores/s {phases}")
     dense_res = None
     if args.path in ("all", "dense"):
-        dense_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 1)
+        dense_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 1,
+                                                         args.margin)
         sc = int(dense_res.off[-1])
         ex, useful = dense_flops(env, dense_res)
         paths["batched_bf16x2_dense"] = dict(
@@ -316,7 +323,8 @@ def main():
         config={"workload": args.workload, "proteins": env["w"]["proteins"],
                 "peptides": int(len(env["r"])), "spectra": int(env["dsp"].num_spec),
                 "window_da": 3.0, "bin_width": 1.0005079,
-                "max_len": int(env["dsp"].lengths().max()), "topn": topn, "path": main_key,
+                "max_len": int(env["dsp"].lengths().max()), "topn": topn, "margin": args.margin,
+                "path": main_key,
                 "scores_per_step": p["scores_per_step"]},
         roofline=roofline, paths=paths)
     if "plain_fp32_sparse" in paths:

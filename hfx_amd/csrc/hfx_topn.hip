@@ -38,6 +38,43 @@ __device__ __forceinline__ bool before_fx(int64_t fa, uint32_t pa, int64_t fb, u
     return fa > fb || (fa == fb && pa < pb);
 }
 
+/* One radix-select step on a 256-bucket histogram: the digit d holding the
+ * rem-th largest key (counting from bucket 255 down) and the rank left inside
+ * it.  Wave 0 does it in parallel: lane l sums buckets 255-4l .. 252-4l, an
+ * inclusive scan over the lanes finds the lane whose range holds rank rem,
+ * and that lane walks its four buckets.  Every thread must call it (it
+ * contains the barriers); all get the same (digit, rem). */
+__device__ __forceinline__ void tn_select_digit(const uint32_t *hist, uint32_t *sh_digit,
+                                                uint32_t *sh_rem)
+{
+    const uint32_t tid = threadIdx.x, lane = hfx_lane();
+    if (tid < HFX_WAVE) {
+        const uint32_t rem = *sh_rem;
+        const uint32_t top = 255u - 4u * lane;
+        const uint32_t h0 = hist[top], h1 = hist[top - 1], h2 = hist[top - 2], h3 = hist[top - 3];
+        const uint32_t sum = h0 + h1 + h2 + h3;
+        uint32_t incl = sum;
+#pragma unroll
+        for (uint32_t o = 1; o < HFX_WAVE; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t excl = incl - sum;
+        if (excl < rem && rem <= incl) {
+            uint32_t cum = excl, d = top - 3u;
+            const uint32_t hh[4] = {h0, h1, h2, h3};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                if (cum + hh[k] >= rem) { d = top - k; break; }
+                cum += hh[k];
+            }
+            *sh_digit = d;
+            *sh_rem = rem - cum;
+        }
+    }
+    __syncthreads();
+}
+
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
@@ -79,16 +116,7 @@ __global__ void __launch_bounds__(TN_THREADS)
             if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t rem = sh_rem, cum = 0, d = 0;
-            for (int dd = 255; dd >= 0; --dd) {
-                if (cum + hist[dd] >= rem) { d = (uint32_t)dd; break; }
-                cum += hist[dd];
-            }
-            sh_digit = d;
-            sh_rem = rem - cum;
-        }
-        __syncthreads();
+        tn_select_digit(hist, &sh_digit, &sh_rem);
         prefix |= sh_digit << shift;
         pmask |= 255u << shift;
     }
@@ -326,16 +354,7 @@ __global__ void __launch_bounds__(TN_THREADS)
             if ((k & pmask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t rem = sh_rem, cum = 0, d = 0;
-            for (int dd = 255; dd >= 0; --dd) {
-                if (cum + hist[dd] >= rem) { d = (uint32_t)dd; break; }
-                cum += hist[dd];
-            }
-            sh_digit = d;
-            sh_rem = rem - cum;
-        }
-        __syncthreads();
+        tn_select_digit(hist, &sh_digit, &sh_rem);
         prefix |= (uint64_t)sh_digit << shift;
         pmask |= 255ull << shift;
     }
