@@ -79,7 +79,7 @@ template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
            uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
-           hfx_spectra sp, float rel0, float rel1)
+           hfx_spectra sp, float rel0, float rel1, double *tau_out)
 {
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
@@ -204,7 +204,9 @@ __global__ void __launch_bounds__(TN_THREADS)
         __syncthreads();
         const uint32_t ne = sh_ext;
         if (ne > TN_EXT) {
+            /* the exact route re-scores every candidate at or above tau */
             fl = 1;
+            if (tid == 0) tau_out[s] = tau;
         } else {
             for (uint32_t i = T + wave; i < T + ne; i += TN_WAVES) {
                 c.peaks(map, mtab, s, cp[i], len, zeff);
@@ -263,10 +265,14 @@ __global__ void k_zero_flags(uint32_t *flag, uint32_t S)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S) flag[i] = 0;
 }
-__global__ void k_mark_list(uint32_t *flag, const uint32_t *list, uint32_t num_list, uint32_t S)
+__global__ void k_mark_list(uint32_t *flag, double *tau, const uint32_t *list, uint32_t num_list,
+                            uint32_t S)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < num_list && list[i] < S) flag[list[i]] = 1;
+    if (i < num_list && list[i] < S) {
+        flag[list[i]] = 1;
+        tau[list[i]] = -INFINITY; /* every candidate scored exactly */
+    }
 }
 
 __device__ __forceinline__ uint32_t tx_segment(const uint64_t *off, uint32_t n, uint64_t t)
@@ -283,7 +289,7 @@ __device__ __forceinline__ uint32_t tx_segment(const uint64_t *off, uint32_t n, 
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, uint32_t *counter, Cands c,
-                  hfx_spectra sp)
+                  hfx_spectra sp, const float *scores, const double *tau)
 {
     __shared__ float xs[HFX_MAX_LEN];
     __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
@@ -309,12 +315,20 @@ __global__ void __launch_bounds__(TN_THREADS)
         const uint32_t first = (uint32_t)(t - tile_off[s]) * TX_CHUNK;
         const uint32_t cnt = min((uint32_t)TX_CHUNK, c.count(s) - first);
         const uint32_t zeff = c.zeff(sp.d_zmax[s]);
+        const double ts = tau[s];
         for (uint32_t i = wave; i < cnt; i += TN_WAVES) {
+            /* a candidate below tau cannot reach the top N: its exact score
+             * is below the N-th exact score (it keeps the smallest key) */
+            const uint64_t at = c.base(s) + first + i;
+            if ((double)scores[at] < ts) {
+                if (lane == 0) fxbuf[at] = INT64_MIN;
+                continue;
+            }
             c.peaks(map, mtab, s, first + i, len, zeff);
             hfx_wave_sync();
             const int64_t fx = hfx_wave_dot_fx(map, xs, len);
             hfx_wave_sync();
-            if (lane == 0) fxbuf[c.base(s) + first + i] = fx;
+            if (lane == 0) fxbuf[at] = fx;
         }
         __syncthreads();
     }
@@ -404,7 +418,31 @@ __global__ void __launch_bounds__(TN_THREADS)
 
 size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total)
 {
-    return total * 8 + (size_t)S * 4 + ((size_t)S + 1) * 8 + 256;
+    return total * 8 + ((size_t)S + 1) * 8 + (size_t)S * 8 + (size_t)S * 4 + 256;
+}
+
+/* exact-route scratch: fxbuf[total] | tile_off[S+1] | tau[S] | tiles[S] | counter */
+struct TxScratch {
+    int64_t *fxbuf;
+    uint64_t *tile_off;
+    double *tau;
+    uint32_t *tiles, *counter;
+};
+static TxScratch tx_carve(void *p, uint32_t S, uint64_t total)
+{
+    TxScratch x;
+    x.fxbuf = (int64_t *)p;
+    x.tile_off = (uint64_t *)(x.fxbuf + total);
+    x.tau = (double *)(x.tile_off + S + 1);
+    x.tiles = (uint32_t *)(x.tau + S);
+    x.counter = x.tiles + S;
+    return x;
+}
+static int tx_scratch(hfx_ctx *ctx, void *given, uint32_t S, uint64_t total, void **out)
+{
+    *out = given;
+    if (given) return HFX_OK;
+    return hfx_ctx_scratch(ctx, HFX_SLOT_TOPN, hfx_topn_scratch_bytes(S, total), out);
 }
 
 template <class Cands>
@@ -416,22 +454,19 @@ static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     const uint32_t S = sp->num_spec;
     if (S == 0) return HFX_OK;
     HFX_REQUIRE(N >= 1 && N <= TX_MAXN, HFX_ERR_ARG, "top-N: N must be in 1..%d", TX_MAXN);
-    void *p = scratch;
+    void *p;
     int rc;
-    if (!p && (rc = hfx_ctx_scratch(ctx, HFX_SLOT_TOPN, hfx_topn_scratch_bytes(S, total), &p)))
-        return rc;
-    int64_t *fxbuf = (int64_t *)p;
-    uint64_t *tile_off = (uint64_t *)(fxbuf + total);
-    uint32_t *tiles = (uint32_t *)(tile_off + S + 1);
-    uint32_t *counter = tiles + S;
-    hipLaunchKernelGGL(k_flag_tiles<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, tiles, d_flag,
-                       c, S);
-    if ((rc = hfx_scan_offsets(ctx, tile_off, tiles, S, (void *)st)) != HFX_OK) return rc;
-    hipLaunchKernelGGL(k_tx_reset, dim3(1), dim3(1), 0, st, counter);
+    if ((rc = tx_scratch(ctx, scratch, S, total, &p))) return rc;
+    const TxScratch x = tx_carve(p, S, total);
+    hipLaunchKernelGGL(k_flag_tiles<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, x.tiles,
+                       d_flag, c, S);
+    if ((rc = hfx_scan_offsets(ctx, x.tile_off, x.tiles, S, (void *)st)) != HFX_OK) return rc;
+    hipLaunchKernelGGL(k_tx_reset, dim3(1), dim3(1), 0, st, x.counter);
     hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 4),
-                       dim3(TN_THREADS), 0, st, fxbuf, tile_off, counter, c, *sp);
+                       dim3(TN_THREADS), 0, st, x.fxbuf, x.tile_off, x.counter, c, *sp, d_scores,
+                       x.tau);
     hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(S), dim3(TN_THREADS), 0, st, d_top_pos, d_top_exact,
-                       d_top_f32, d_top_cnt, N, d_flag, fxbuf, d_scores, c);
+                       d_top_f32, d_top_cnt, N, d_flag, x.fxbuf, d_scores, c);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }
@@ -450,12 +485,16 @@ static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                 TN_MAXT, N, margin);
     if (sp->num_spec == 0) return HFX_OK;
     hipStream_t st = hfx_stream(stream);
+    void *p;
+    int rc;
+    if ((rc = tx_scratch(ctx, scratch, sp->num_spec, total, &p))) return rc;
+    const TxScratch x = tx_carve(p, sp->num_spec, total);
     hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS), 0, st, d_top_pos,
                        d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c, *sp,
-                       rel0, rel1);
+                       rel0, rel1, x.tau);
     HFX_CHECK_LAUNCH();
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
-                       total, sp, scratch, st);
+                       total, sp, p, st);
 }
 
 template <class Cands>
@@ -469,15 +508,17 @@ static int launch_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_ex
     const uint32_t S = sp->num_spec;
     if (S == 0 || num_list == 0) return HFX_OK;
     hipStream_t st = hfx_stream(stream);
-    void *p;
+    void *p, *q;
     int rc;
     if ((rc = hfx_ctx_scratch(ctx, HFX_SLOT_FLAGS, 4ull * S + 64, &p)) != HFX_OK) return rc;
+    if ((rc = tx_scratch(ctx, nullptr, S, total, &q))) return rc;
     uint32_t *flag = (uint32_t *)p;
+    const TxScratch x = tx_carve(q, S, total);
     hipLaunchKernelGGL(k_zero_flags, dim3((S + 255) / 256), dim3(256), 0, st, flag, S);
-    hipLaunchKernelGGL(k_mark_list, dim3((num_list + 255) / 256), dim3(256), 0, st, flag, d_list,
-                       num_list, S);
+    hipLaunchKernelGGL(k_mark_list, dim3((num_list + 255) / 256), dim3(256), 0, st, flag, x.tau,
+                       d_list, num_list, S);
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, flag, d_scores, c,
-                       total, sp, nullptr, st);
+                       total, sp, q, st);
 }
 
 /* used by the batched path: the whole top-N on the device */

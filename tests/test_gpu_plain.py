@@ -267,3 +267,56 @@ def test_empty_windows_and_high_charge(env, orc):
     assert res.num[0] == 0 and res.top_cnt[0] == 0
     a, b = int(res.off[1]), int(res.off[2])
     assert b > a and np.all(res.scores[a:b] == 0)
+
+
+def test_tie_heavy_window_takes_exact_route(orc):
+    """400 proteins carry the same tryptic peptide: a spectrum made from it has
+    400 candidates with one identical theoretical spectrum, so its N-th place
+    is an exact tie among far more candidates than the re-score set holds.
+    The spectrum goes down the on-device exact route (flagged), and the top N
+    (the tied copies in ascending position order, then the rest) equals the
+    oracle's.  Both scoring forms feed the same top-N."""
+    from hfx_amd import pipeline, synth
+
+    base = synth.make_proteins(600, seed=61)
+    motif = np.frombuffer(b"GASGTPAGEGSK", np.uint8)
+    rng = np.random.default_rng(62)
+    parts, seg = [base.seq], list(base.seg)
+    for _ in range(400):
+        head = np.frombuffer(b"MKR", np.uint8)
+        tail = synth.make_proteins(1, seed=int(rng.integers(1 << 30)), min_len=20, max_len=60).seq
+        p = np.concatenate([head, motif, tail])
+        parts.append(p)
+        seg.append(seg[-1] + len(p))
+    prot = synth.Proteins(seq=np.concatenate(parts).astype(np.uint8),
+                          seg=np.asarray(seg, np.uint32))
+    mass = orc.mass_table()
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=1)
+    sp = synth.make_spectra(prot, r, tc, tn, 24, seed=63, charges=(2,))
+    # spectrum 0 from the shared peptide
+    hit = [i for i in range(len(r))
+           if synth.peptide_residues(prot, int(tc[i]), int(tn[i])) == motif.tobytes()]
+    assert len(hit) == 400
+    one = synth.make_spectra(prot, r[hit[0]:hit[0] + 1], tc[hit[0]:hit[0] + 1],
+                             tn[hit[0]:hit[0] + 1], 1, seed=64, charges=(2,))
+    n0 = int(one.off[1])
+    mz = np.concatenate([one.mz[:n0], sp.mz[sp.off[1]:]])
+    inten = np.concatenate([one.inten[:n0], sp.inten[sp.off[1]:]])
+    off = np.concatenate([[0], n0 + (sp.off[2:] - sp.off[1])]).astype(np.uint32)
+    pcr = np.concatenate([one.pcr[:1], sp.pcr[1:]]).astype(np.float32)
+    ch = np.concatenate([one.charge[:1], sp.charge[1:]]).astype(np.float32)
+
+    ctx = pipeline.Ctx(0)
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
+    dsp = pipeline.DeviceSpectra(ctx, pcr, ch, mz, inten, off, 2112)
+    idx, _ = orc.sort_idx(r)
+    ref = orc.search_plain(orc.prepare_ions(prot.seq), r, tc, tn, mass, idx, dsp.x(),
+                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, 10)
+    for dense in (False, True):
+        res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=10).run(dense=dense)
+        assert 0 in res.flagged.tolist(), dense
+        assert (res.top_cnt == ref["top_cnt"]).all()
+        for s in range(dsp.num_spec):
+            c = int(res.top_cnt[s])
+            assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), (dense, s)
+            assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
