@@ -225,8 +225,12 @@ struct DnArgs {
     hfx_spectra sp;
 };
 
-/* one weight code at bin x of a map row; a bin outside [0, kq) ORs 0 into
- * a word private to the lane instead (no branch, no shared address) */
+/* Map rows: bin x lives in word 1 + x/8 of its row (word 0 is a margin that
+ * the contraction never reads, so a bin in [-8, 0) needs no check).  `row`
+ * below points at word 1. */
+
+/* checked form: a bin outside [0, kq) ORs 0 into a word private to the lane
+ * (no branch, no shared address) */
 __device__ __forceinline__ void dn_put(uint32_t *row, int32_t x, uint32_t code, uint32_t kq,
                                        uint32_t lane)
 {
@@ -234,9 +238,9 @@ __device__ __forceinline__ void dn_put(uint32_t *row, int32_t x, uint32_t code, 
     atomicOr(&row[v ? (uint32_t)x >> 3 : lane], v);
 }
 
-/* 25 at x-1, 50 at x, 25 at x+1 (each kept only below kq): the nibbles
- * 3, 7, 3 span at most two words; the word of x-1 gets them shifted into
- * place, the word of x+1 gets whatever spilled over (the same value again
+/* checked 25 at x-1, 50 at x, 25 at x+1 (each kept only below kq): the
+ * nibbles 3, 7, 3 span at most two words; the word of x-1 gets them shifted
+ * into place, the word of x+1 whatever spilled over (the same value again
  * when all three share a word) */
 __device__ __forceinline__ void dn_put3(uint32_t *row, int32_t x, uint32_t kq, uint32_t lane)
 {
@@ -251,6 +255,21 @@ __device__ __forceinline__ void dn_put3(uint32_t *row, int32_t x, uint32_t kq, u
     atomicOr(&row[v2 ? (uint32_t)(x + 1) >> 3 : lane], v2);
 }
 
+/* unchecked forms, for passes whose every bin lies below kq: a negative bin
+ * goes to the margin word (word index clamped at -1) */
+__device__ __forceinline__ void dn_put_u(uint32_t *row, int32_t x, uint32_t code)
+{
+    atomicOr(&row[max(x >> 3, -1)], code << (((uint32_t)x & 7u) << 2));
+}
+__device__ __forceinline__ void dn_put3_u(uint32_t *row, int32_t x)
+{
+    const int32_t xm = x - 1;
+    const uint32_t sh = ((uint32_t)xm & 7u) << 2;
+    const uint32_t t = HFX_CODE25 | HFX_CODE50 << 4 | HFX_CODE25 << 8;
+    atomicOr(&row[max(xm >> 3, -1)], t << sh);
+    atomicOr(&row[max((x + 1) >> 3, -1)], sh >= 24u ? t >> (32u - sh) : t << sh);
+}
+
 /* division by the fragment charge Z: exact as a product for Z = 1, 2, 4 */
 template <int Z>
 __device__ __forceinline__ float dn_zdiv(float x)
@@ -261,21 +280,44 @@ __device__ __forceinline__ float dn_zdiv(float x)
 
 /* peaks of one ladder position (b, y) at charge Z into a map row: the ion
  * series of ion_series.cu:111-156 with the FP32 operation order of the
- * sparse path (zh = H * Z, H2O / Z and NH3 / Z fold to the same FP32 values) */
-template <int Z>
+ * sparse path (zh = H * Z, H2O / Z and NH3 / Z fold to the same FP32 values);
+ * CHECK keeps only bins in [0, kq) */
+template <int Z, bool CHECK>
 __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32_t kq,
                                          uint32_t lane)
 {
     constexpr float zh = HFX_DEV_H * (float)Z;
     constexpr float h2o = HFX_DEV_H2O / (float)Z, nh3 = HFX_DEV_NH3 / (float)Z;
-    dn_put(row, dn_bin(dn_zdiv<Z>((b - HFX_DEV_CO) + zh)), HFX_CODE10, kq, lane);
+    const int32_t xa = dn_bin(dn_zdiv<Z>((b - HFX_DEV_CO) + zh));
     const float mb = dn_zdiv<Z>(b + zh);
-    dn_put3(row, dn_bin(mb), kq, lane);
-    dn_put(row, dn_bin(mb - h2o), HFX_CODE10, kq, lane);
-    dn_put(row, dn_bin(mb - nh3), HFX_CODE10, kq, lane);
+    const int32_t xb = dn_bin(mb), xw = dn_bin(mb - h2o), xn = dn_bin(mb - nh3);
     const float my = dn_zdiv<Z>((y + HFX_DEV_H2O) + zh);
-    dn_put3(row, dn_bin(my), kq, lane);
-    dn_put(row, dn_bin(my - nh3), HFX_CODE10, kq, lane);
+    const int32_t xy = dn_bin(my), xyn = dn_bin(my - nh3);
+    if constexpr (CHECK) {
+        dn_put(row, xa, HFX_CODE10, kq, lane);
+        dn_put3(row, xb, kq, lane);
+        dn_put(row, xw, HFX_CODE10, kq, lane);
+        dn_put(row, xn, HFX_CODE10, kq, lane);
+        dn_put3(row, xy, kq, lane);
+        dn_put(row, xyn, HFX_CODE10, kq, lane);
+    } else {
+        dn_put_u(row, xa, HFX_CODE10);
+        dn_put3_u(row, xb);
+        dn_put_u(row, xw, HFX_CODE10);
+        dn_put_u(row, xn, HFX_CODE10);
+        dn_put3_u(row, xy);
+        dn_put_u(row, xyn, HFX_CODE10);
+    }
+}
+
+template <bool CHECK>
+__device__ __forceinline__ void dn_peaks_all(uint32_t *row, float b, float y, uint32_t kq,
+                                             uint32_t lane, uint32_t lvl)
+{
+    dn_peaks<1, CHECK>(row, b, y, kq, lane);
+    if (lvl >= 2) dn_peaks<2, CHECK>(row, b, y, kq, lane);
+    if (lvl >= 3) dn_peaks<3, CHECK>(row, b, y, kq, lane);
+    if (lvl >= 4) dn_peaks<4, CHECK>(row, b, y, kq, lane);
 }
 
 /* residues of the tile's candidates staged in LDS: row c holds the ion bytes
@@ -510,6 +552,8 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                 const uint32_t c = c0 + (lane & 7u);
                 const uint32_t tcc = __shfl(tc, c, 64), nc = __shfl(n, c, 64);
                 const uint32_t oc = __shfl(incl, c, 64) - nc - base;
+                const float rc = __shfl(r, c, 64);
+                float mloc = 0.0f;
                 if (lane < 8 && c >= c_lo && c < c_hi) {
                     uint32_t o = oc;
                     float bsum = 0.0f;
@@ -536,32 +580,27 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                         itemc[o] = (uint8_t)c;
                         ++o;
                     }
+                    mloc = fmaxf(rc, bsum);
                 }
+                /* every bin of the pass lies below m / 1.0005 + 2 for the
+                 * heaviest b + proton or y + water + proton (y <= residual):
+                 * with that below kq, the unchecked writes apply */
+#pragma unroll
+                for (uint32_t o2 = 32; o2 > 0; o2 >>= 1) mloc = fmaxf(mloc, __shfl_xor(mloc, o2, 64));
+                const bool fast = (uint32_t)(mloc + 24.0f) + 4u <= kq;
                 hfx_wave_sync();
-                /* two positions per lane per round; a lane past the end takes
-                 * position 0 with kq = 0, so that every code it makes goes to
-                 * its private dummy word as 0 */
-                for (uint32_t i0 = 0; i0 < nitems; i0 += 2u * HFX_WAVE) {
-                    const uint32_t ia = i0 + lane, ib = ia + HFX_WAVE;
-                    const bool va = ia < nitems, vb = ib < nitems;
-                    const float ba = itemb[va ? ia : 0], bbv = itemb[vb ? ib : 0];
-                    const uint32_t ca = itemc[va ? ia : 0], cb = itemc[vb ? ib : 0];
-                    const float ya = rr[ca] - ba, yb = rr[cb] - bbv;
-                    uint32_t *ra = map + ca * DN_MSTR, *rb = map + cb * DN_MSTR;
-                    const uint32_t ka = va ? kq : 0u, kbq = vb ? kq : 0u;
-                    dn_peaks<1>(ra, ba, ya, ka, lane);
-                    dn_peaks<1>(rb, bbv, yb, kbq, lane);
-                    if (lvl >= 2) {
-                        dn_peaks<2>(ra, ba, ya, ka, lane);
-                        dn_peaks<2>(rb, bbv, yb, kbq, lane);
+                /* one ladder position per lane per round */
+                if (fast) {
+                    for (uint32_t i = lane; i < nitems; i += HFX_WAVE) {
+                        const float bb = itemb[i];
+                        const uint32_t ci = itemc[i];
+                        dn_peaks_all<false>(map + ci * DN_MSTR + 1, bb, rr[ci] - bb, kq, lane, lvl);
                     }
-                    if (lvl >= 3) {
-                        dn_peaks<3>(ra, ba, ya, ka, lane);
-                        dn_peaks<3>(rb, bbv, yb, kbq, lane);
-                    }
-                    if (lvl >= 4) {
-                        dn_peaks<4>(ra, ba, ya, ka, lane);
-                        dn_peaks<4>(rb, bbv, yb, kbq, lane);
+                } else {
+                    for (uint32_t i = lane; i < nitems; i += HFX_WAVE) {
+                        const float bb = itemb[i];
+                        const uint32_t ci = itemc[i];
+                        dn_peaks_all<true>(map + ci * DN_MSTR + 1, bb, rr[ci] - bb, kq, lane, lvl);
                     }
                 }
                 hfx_wave_sync();
@@ -576,45 +615,51 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
             dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
             const uint32_t nks = kq >> 5, mid = (nks >> 3) << 2;
             const uint32_t ks_lo = h ? mid : 0u, ks_hi = h ? nks : mid;
-            uint32_t *arow = map + row * DN_MSTR + kg;
+            uint32_t *arow = map + row * DN_MSTR + 1 + kg;
             const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
+            /* groups of four k-steps, read one group ahead; the read past
+             * the last group stays inside the LDS allocation and is unused */
+            auto mma4 = [&](const dn_u32x4 &cw, const dn_u32x4 &b0, const dn_u32x4 &b1,
+                            const dn_u32x4 &b2, const dn_u32x4 &b3) {
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
+                    __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
+                    __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
+                    __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
+                    __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
+            };
             if (ks_lo < ks_hi) {
-                uint32_t *aw0 = arow + 4 * ks_lo;
-                uint32_t w0 = aw0[0], w1 = aw0[4], w2 = aw0[8], w3 = aw0[12];
-                dn_u32x4 n0 = *(const dn_u32x4 *)(bcol + (ks_lo + 0) * 32);
-                dn_u32x4 n1v = *(const dn_u32x4 *)(bcol + (ks_lo + 1) * 32);
-                dn_u32x4 n2v = *(const dn_u32x4 *)(bcol + (ks_lo + 2) * 32);
-                dn_u32x4 n3 = *(const dn_u32x4 *)(bcol + (ks_lo + 3) * 32);
-                for (uint32_t ks = ks_lo; ks < ks_hi; ks += 4) {
-                    const dn_u32x4 cw = dn_u32x4{w0, w1, w2, w3};
-                    const dn_u32x4 b0 = n0, b1 = n1v, b2 = n2v, b3 = n3;
-                    uint32_t *aw = arow + 4 * ks;
-                    aw[0] = 0u;
-                    aw[4] = 0u;
-                    aw[8] = 0u;
-                    aw[12] = 0u;
-                    if (ks + 4 < ks_hi) {
-                        w0 = aw[16];
-                        w1 = aw[20];
-                        w2 = aw[24];
-                        w3 = aw[28];
-                        n0 = *(const dn_u32x4 *)(bcol + (ks + 4) * 32);
-                        n1v = *(const dn_u32x4 *)(bcol + (ks + 5) * 32);
-                        n2v = *(const dn_u32x4 *)(bcol + (ks + 6) * 32);
-                        n3 = *(const dn_u32x4 *)(bcol + (ks + 7) * 32);
-                    }
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
-                        __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
-                        __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
-                        __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
-                        __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
+                uint32_t *aw = arow + 4 * ks_lo;
+                const uint16_t *bw = bcol + ks_lo * 32;
+                dn_u32x4 ca = {aw[0], aw[4], aw[8], aw[12]};
+                dn_u32x4 a0 = *(const dn_u32x4 *)(bw + 0), a1 = *(const dn_u32x4 *)(bw + 32);
+                dn_u32x4 a2 = *(const dn_u32x4 *)(bw + 64), a3 = *(const dn_u32x4 *)(bw + 96);
+                for (uint32_t ks = ks_lo; ks < ks_hi; ks += 8) {
+                    /* group ks (in ca, a0..a3); read group ks + 4 */
+                    aw[0] = 0u; aw[4] = 0u; aw[8] = 0u; aw[12] = 0u;
+                    const dn_u32x4 cb = {aw[16], aw[20], aw[24], aw[28]};
+                    const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + 128);
+                    const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + 160);
+                    const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + 192);
+                    const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + 224);
+                    mma4(ca, a0, a1, a2, a3);
+                    if (ks + 4 >= ks_hi) break;
+                    /* group ks + 4 (in cb, e0..e3); read group ks + 8 */
+                    aw[16] = 0u; aw[20] = 0u; aw[24] = 0u; aw[28] = 0u;
+                    ca = dn_u32x4{aw[32], aw[36], aw[40], aw[44]};
+                    a0 = *(const dn_u32x4 *)(bw + 256);
+                    a1 = *(const dn_u32x4 *)(bw + 288);
+                    a2 = *(const dn_u32x4 *)(bw + 320);
+                    a3 = *(const dn_u32x4 *)(bw + 352);
+                    mma4(cb, e0, e1, e2, e3);
+                    aw += 32;
+                    bw += 256;
                 }
             }
             const dn_f32x4 part = acc0 + acc1;
