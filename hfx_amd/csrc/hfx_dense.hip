@@ -53,7 +53,7 @@
 #define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
 #define DN_MSTR (DN_MAXK / 8 + 4)   /* map words per row: bin x in word x/8 (padded) */
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
-#define DN_ITEMS 256                /* ladder positions per wave pass */
+#define DN_ITEMS 352                /* ladder positions per wave pass */
 #define DN_RMAX 64                  /* residues per candidate staged in LDS */
 #define DN_RSTR 17                  /* dwords per staged row (3 + DN_RMAX bytes) */
 #define DN_NONE 0xffffffffu
@@ -63,9 +63,8 @@
 #define DN_ITEMB_BYTES (DN_WAVES * DN_ITEMS * 4)
 #define DN_ITEMC_BYTES (DN_WAVES * DN_ITEMS)
 #define DN_RES_BYTES (DN_PAIRS * DN_ROWS * DN_RSTR * 4)
-#define DN_COMB_BYTES (DN_PAIRS * HFX_WAVE * 16)
-#define DN_LDS_BYTES                                                                         \
-    (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES + DN_COMB_BYTES)
+#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES)
+static_assert(DN_ITEMS * 4 >= HFX_WAVE * 16, "the pair's partial sums live in half 0's items");
 
 typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
@@ -375,8 +374,6 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     uint8_t *itemc_all = dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES;
     uint32_t *res_all = (uint32_t *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES +
                                      DN_ITEMC_BYTES);
-    dn_f32x4 *comb_all = (dn_f32x4 *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES +
-                                      DN_ITEMC_BYTES + DN_RES_BYTES);
     __shared__ float mtab[32];
     __shared__ uint32_t m_s[DN_NS], m_beg[DN_NS], m_end[DN_NS], m_z[DN_NS];
     __shared__ uint64_t m_off[DN_NS];
@@ -392,7 +389,10 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     uint8_t *itemc = itemc_all + wave * DN_ITEMS;
     uint32_t *res = res_all + q * DN_ROWS * DN_RSTR;
     const uint8_t *resb = (const uint8_t *)res;
-    dn_f32x4 *comb = comb_all + q * HFX_WAVE;
+    /* half 1's partial sums go to half 0's ladder positions: half 0 is done
+     * with them before barrier (b) and reads the sums after barrier (c),
+     * before its next ladder */
+    dn_f32x4 *comb = (dn_f32x4 *)(itemb_all + q * DN_ITEMS);
     float *rr = rr_all[q];
     const uint8_t *ions = a.db.d_ions;
     if (tid < 32) mtab[tid] = tid < 26 ? a.db.d_mass[tid] : 0.0f;
