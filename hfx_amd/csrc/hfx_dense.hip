@@ -53,7 +53,7 @@
 #define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
 #define DN_MSTR (DN_MAXK / 8 + 4)   /* map words per row: bin x in word x/8 (padded) */
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
-#define DN_ITEMS 352                /* ladder positions per wave pass */
+#define DN_ITEMS 456                /* ladder positions per wave pass */
 #define DN_RMAX 64                  /* residues per candidate staged in LDS */
 #define DN_RSTR 17                  /* dwords per staged row (3 + DN_RMAX bytes) */
 #define DN_NONE 0xffffffffu
@@ -65,6 +65,11 @@
 #define DN_RES_BYTES (DN_PAIRS * DN_ROWS * DN_RSTR * 4)
 #define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES)
 static_assert(DN_ITEMS * 4 >= HFX_WAVE * 16, "the pair's partial sums live in half 0's items");
+/* the kernel's fixed __shared__ arrays (mtab, m_*, rr_all, sh_t; 592 bytes
+ * with the compiler's alignment) share the CU's 160 KiB with the dynamic part */
+#define DN_FIXED_LDS (32 * 4 + DN_NS * 4 * 4 + DN_NS * 8 + DN_PAIRS * DN_ROWS * 4 + 2 * 4 + 16)
+static_assert(DN_LDS_BYTES + DN_FIXED_LDS <= 160 * 1024, "dense kernel LDS exceeds a CU");
+static_assert(DN_ITEMS % 4 == 0, "ladder buffers keep 16-byte alignment");
 
 typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
