@@ -124,6 +124,45 @@ __device__ __forceinline__ dn_u32x4 dn_weights(uint32_t w)
     return a;
 }
 
+/* Lane moves inside rows of 16 lanes through DPP (no LDS round trip):
+ * row_shr:n -- lane i takes lane i - n of its row, lanes below n keep `old`;
+ * row_ror:n -- lane i takes lane (i + n) mod 16 of its row (n = 8 swaps the
+ * row's halves either way round); quad_perm [1,0,3,2] swaps neighbours. */
+#define DN_SHR(n) (0x110 + (n))
+#define DN_ROR(n) (0x120 + (n))
+#define DN_SWAP1 0xb1
+template <int CTRL>
+__device__ __forceinline__ uint32_t dn_dpp(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dn_dppf(float v)
+{
+    return __uint_as_float(dn_dpp<CTRL>(__float_as_uint(v), __float_as_uint(v)));
+}
+
+/* inclusive sum over each row of 16 lanes */
+__device__ __forceinline__ uint32_t dn_row_scan(uint32_t v)
+{
+    v += dn_dpp<DN_SHR(1)>(0u, v);
+    v += dn_dpp<DN_SHR(2)>(0u, v);
+    v += dn_dpp<DN_SHR(4)>(0u, v);
+    v += dn_dpp<DN_SHR(8)>(0u, v);
+    return v;
+}
+
+/* maximum over each row of 16 lanes, left in every lane of the row (the
+ * rotations by 8, 4, 2, 1 reach all 16 lanes whichever way they turn) */
+__device__ __forceinline__ float dn_row_max(float v)
+{
+    v = fmaxf(v, dn_dppf<DN_ROR(8)>(v));
+    v = fmaxf(v, dn_dppf<DN_ROR(4)>(v));
+    v = fmaxf(v, dn_dppf<DN_ROR(2)>(v));
+    v = fmaxf(v, dn_dppf<DN_ROR(1)>(v));
+    return v;
+}
+
 /* ------------------------------------------------------------------ */
 /* Spectra as hi/lo BF16 rows                                            */
 /* ------------------------------------------------------------------ */
@@ -331,13 +370,12 @@ struct DnStage {
 };
 
 /* half h of a wave pair stages candidates 8h..8h+7: lane (c = 8h + lane % 8,
- * q = lane / 8) loads dwords q, q+8, q+16 of candidate c's row; every byte
- * read lies in [0, num_ions) */
+ * q = lane / 8) loads dwords q, q+8, q+16 of candidate c's row, given c's
+ * first ion tcc and residue count nc; every byte read lies in [0, num_ions) */
 __device__ __forceinline__ void dn_stage_load(DnStage &st, const uint8_t *ions, uint64_t num_ions,
-                                              uint32_t tc, uint32_t n, uint32_t lane, uint32_t h)
+                                              uint32_t tcc, uint32_t nc, uint32_t lane)
 {
-    const uint32_t c = 8u * h + (lane & 7u), q = lane >> 3;
-    const uint32_t tcc = __shfl(tc, c, 64), nc = __shfl(n, c, 64);
+    const uint32_t q = lane >> 3;
     const uint32_t mis = (uint32_t)(((uintptr_t)ions + tcc) & 3u);
     const int64_t a0 = (int64_t)tcc - mis;
     const uint32_t ndw = (mis + min(nc, (uint32_t)DN_RMAX) + 3u) >> 2;
@@ -491,8 +529,18 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         auto rows_of = [&](uint32_t q0) -> uint32_t {
             return q0 < cend ? min((uint32_t)DN_ROWS, cend - q0) : 0u;
         };
+        /* lane l < 16 holds candidate l of a tile; lanes 16.. hold candidate
+         * 8h + l % 8 (the half's own, for staging its residues) */
+        const uint32_t cidx = lane < 16u ? lane : 8u * h + (lane & 7u);
         auto load_pep = [&](uint32_t q0) -> uint32_t {
-            return lane < rows_of(q0) ? a.db.d_idx_sorted[q0 + lane] : DN_NONE;
+            return cidx < rows_of(q0) ? a.db.d_idx_sorted[q0 + cidx] : DN_NONE;
+        };
+        /* the half's candidate 8h + l % 8 in every lane l: row 0 swaps its
+         * halves where lane l holds the other half's candidate */
+        const bool swap8 = lane < 16u && ((lane >> 3) & 1u) != h;
+        auto stage = [&](DnStage &st, uint32_t tcv, uint32_t nv) {
+            const uint32_t tr = dn_dpp<DN_ROR(8)>(tcv, tcv), nr = dn_dpp<DN_ROR(8)>(nv, nv);
+            dn_stage_load(st, ions, a.db.num_ions, swap8 ? tr : tcv, swap8 ? nr : nv, lane);
         };
         auto meta = [&](uint32_t pep, uint32_t &tcv, uint32_t &nv, float &rv) {
             tcv = 0;
@@ -513,14 +561,14 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
             meta(pep0, tc, n, r);
             meta(pep1, tc1, n1, r1);
             DnStage st0;
-            dn_stage_load(st0, ions, a.db.num_ions, tc, n, lane, h);
+            stage(st0, tc, n);
             dn_stage_store(st0, res, lane, h);
         }
         for (uint32_t j = 0; j < ntile; ++j, p0 += step) {
             const uint32_t nrows = rows_of(p0);
             /* issue: residues of j+1, metadata of j+2, peptide of j+3 */
             DnStage st1;
-            dn_stage_load(st1, ions, a.db.num_ions, tc1, n1, lane, h);
+            stage(st1, tc1, n1);
             uint32_t tc2, n2;
             float r2;
             meta(pep2, tc2, n2, r2);
@@ -530,34 +578,32 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
              * ladder positions and map rows; nothing below the next barrier
              * touches the other half's */
             if (lane >= 8u * h && lane < 8u * h + 8u) rr[lane] = r;
-            uint32_t incl = n;
-#pragma unroll
-            for (uint32_t o = 1; o < DN_ROWS; o <<= 1) {
-                const uint32_t y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-            }
+            /* lane c < 16: ladder positions of candidates 0..c */
+            const uint32_t incl = dn_row_scan(n);
             /* heaviest fragment of the tile: every peak bin is below
              * m / 1.0005 + 2 for m = residual + water + proton (the b ladder
              * sums the same residue masses as the residual) */
-            float mm = lane < nrows ? r : 0.0f;
-#pragma unroll
-            for (uint32_t o2 = 32; o2 > 0; o2 >>= 1) mm = fmaxf(mm, __shfl_xor(mm, o2, 64));
+            const float mm = __uint_as_float(__builtin_amdgcn_readfirstlane(
+                __float_as_uint(dn_row_max(lane < nrows ? r : 0.0f))));
             const uint32_t kq = nrows ? min(kb, (((uint32_t)(mm + 24.0f) + 4u) + 127u) & ~127u) : 0u;
             const uint32_t c0 = 8u * h, c1 = min(c0 + 8u, nrows);
             /* ladder positions of the half's candidates in passes of at most
              * DN_ITEMS */
             uint32_t c_lo = c0;
             while (c_lo < c1) {
-                const uint32_t base = c_lo ? __shfl(incl, c_lo - 1, 64) : 0u;
+                const uint32_t base = c_lo ? __builtin_amdgcn_readlane(incl, c_lo - 1) : 0u;
                 const bool fits = lane >= c_lo && lane < c1 && incl - base <= DN_ITEMS;
                 const uint64_t bal = __ballot(fits);
                 const uint32_t c_hi = bal ? 64u - (uint32_t)__builtin_clzll(bal) : c_lo + 1u;
-                const uint32_t nitems = __shfl(incl, c_hi - 1, 64) - base;
-                /* sequential FP32 b ladder of candidate c = c0 + lane */
+                const uint32_t nitems = __builtin_amdgcn_readlane(incl, c_hi - 1) - base;
+                /* sequential FP32 b ladder of candidate c = c0 + lane, lane < 8:
+                 * half 1 takes row 0's upper lanes */
                 const uint32_t c = c0 + (lane & 7u);
-                const uint32_t tcc = __shfl(tc, c, 64), nc = __shfl(n, c, 64);
-                const uint32_t oc = __shfl(incl, c, 64) - nc - base;
-                const float rc = __shfl(r, c, 64);
+                const uint32_t excl = incl - n;
+                const uint32_t tcc = h ? dn_dpp<DN_ROR(8)>(tc, tc) : tc;
+                const uint32_t nc = h ? dn_dpp<DN_ROR(8)>(n, n) : n;
+                const uint32_t oc = (h ? dn_dpp<DN_ROR(8)>(excl, excl) : excl) - base;
+                const float rc = h ? dn_dppf<DN_ROR(8)>(r) : r;
                 float mloc = 0.0f;
                 if (lane < 8 && c >= c_lo && c < c_hi) {
                     uint32_t o = oc;
@@ -590,8 +636,7 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                 /* every bin of the pass lies below m / 1.0005 + 2 for the
                  * heaviest b + proton or y + water + proton (y <= residual):
                  * with that below kq, the unchecked writes apply */
-#pragma unroll
-                for (uint32_t o2 = 32; o2 > 0; o2 >>= 1) mloc = fmaxf(mloc, __shfl_xor(mloc, o2, 64));
+                mloc = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(dn_row_max(mloc))));
                 const bool fast = (uint32_t)(mloc + 24.0f) + 4u <= kq;
                 hfx_wave_sync();
                 /* one ladder position per lane per round */
@@ -687,7 +732,7 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
 #pragma unroll
                 for (uint32_t i = 0; i < 4; ++i) {
                     const float c4 = part[i] + other[i];
-                    const float v = c4 + __shfl_xor(c4, 1, 64);
+                    const float v = c4 + dn_dppf<DN_SWAP1>(c4);
                     const uint32_t rr_i = 4u * kg + i, g = row >> 1, p = p0 + rr_i;
                     if ((lane & 1u) == 0 && rr_i < nrows && m_z[g] == lvl && m_beg[g] <= p &&
                         p < m_end[g])
