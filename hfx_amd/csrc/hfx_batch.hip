@@ -334,6 +334,8 @@ struct hfx_batch {
     uint32_t precision;  /* HFX_BATCH_FP32_SPARSE or HFX_BATCH_BF16X2_DENSE */
     uint32_t CH;         /* dense: candidates per work item */
     uint8_t *dn_scratch; /* dense: hi/lo spectra, blocks (null if unsupported) */
+    void *dn_mem[2];     /* dense: the ladders of the plan's windows */
+    hfx_dn_ladder dn_lad;
     hipGraph_t graph;
     hipGraphExec_t exec;
 };
@@ -343,7 +345,7 @@ static void bt_free(hfx_batch *b)
     void *ps[] = {b->begin,    b->num,     b->order,  b->ubeg,      b->uend,    b->ntiles,
                   b->caps,     b->cnt,     b->counter, b->score_off, b->tile_off, b->ent_off,
                   b->ent,      b->scores,  b->top_pos, b->top_cnt,   b->flag,     b->top_exact,
-                  b->top_f32,  b->tx_scratch, b->dn_scratch};
+                  b->top_f32,  b->tx_scratch, b->dn_scratch, b->dn_mem[0], b->dn_mem[1]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -487,9 +489,13 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     /* the top-N exact route's scratch is owned by the batch, so that a
      * captured graph never allocates and never sees it move */
     if ((rc = dalloc(&b->tx_scratch, hfx_topn_scratch_bytes(S, b->total)))) return rc;
-    if (sp->x_stride <= hfx_dense_max_len() &&
-        (rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(S))))
-        return rc;
+    if (sp->x_stride <= hfx_dense_max_len()) {
+        if ((rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(S)))) return rc;
+        /* b ladders of every candidate of the plan, built once */
+        if ((rc = hfx_dense_ladder_build(b->ctx, db, b->p0, b->P, 1, &b->dn_lad, b->dn_mem,
+                                         stream)))
+            return rc;
+    }
     /* blocks are sized for the smallest G the tuner may pick (4) */
     const uint32_t bmax = (S + 3) / 4;
     if ((rc = dalloc(&b->ubeg, bmax)) || (rc = dalloc(&b->uend, bmax)) ||
@@ -564,8 +570,8 @@ static int bt_scores(hfx_batch *b, hipStream_t st)
     int rc;
     if (b->precision == HFX_BATCH_BF16X2_DENSE) {
         if (b->total == 0) return HFX_OK;
-        return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order, &b->db,
-                                &b->sp, b->CH, b->dn_scratch, (void *)st);
+        return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order,
+                                &b->dn_lad, &b->sp, b->CH, b->dn_scratch, (void *)st);
     }
     if ((rc = bt_tiles(b, st))) return rc;
     return bt_score(b, st);

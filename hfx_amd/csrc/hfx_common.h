@@ -84,6 +84,8 @@ enum {
     HFX_SLOT_BATCH2 = 9,
     HFX_SLOT_DENSE = 10,  /* BF16 dense path                     */
     HFX_SLOT_FLAGS = 11,  /* spectrum lists turned into flags    */
+    HFX_SLOT_LADDER = 12, /* dense ladders: offsets, tops, counts */
+    HFX_SLOT_LADDER2 = 13, /* dense ladders: positions           */
 };
 /* Scratch owned by a context: grown on demand, never shrunk. */
 int hfx_ctx_scratch(hfx_ctx *ctx, int slot, size_t bytes, void **out);
@@ -105,14 +107,35 @@ int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, floa
                     const float *d_scores, const uint64_t *d_off, uint64_t total,
                     const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
                     const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream);
+/* b ladders of the peptides in residual-sorted positions [q0, q0 + nq), for
+ * the dense path.  Position k < tn - tc of sorted peptide q is
+ * pos[off[q - q0] + k] = {b_k, r - b_k, q}, with b_k the sequential FP32 sum
+ * of its residue masses 0..k; top[q - q0] = max(r, b_last) bounds every
+ * fragment mass of the peptide. */
+struct hfx_dn_pos {
+    float b, y;
+    uint32_t q;
+};
+struct hfx_dn_ladder {
+    const hfx_dn_pos *pos;
+    const uint64_t *off; /* [nq + 1] */
+    const float *top;    /* [nq] */
+    uint32_t q0, nq;
+};
+/* builds the ladder of sorted peptides [q0, q0 + nq) on the stream and waits
+ * for it.  own != 0: two buffers from hipMalloc returned in mem[0..1], which
+ * the caller frees; own == 0: the context's ladder slots (valid until the
+ * next build on the context). */
+int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t nq, int own,
+                           hfx_dn_ladder *lad, void **mem, void *stream);
 /* dense BF16 hi+lo scores (same layout as hfx_xcorr_sparse); spectra are
  * blocked eight at a time in d_order (null: index order), CH candidates per
- * work item; scratch: hfx_dense_scratch_bytes(S) bytes or null for the
- * context's slot */
+ * work item; every window lies inside the ladder's range; scratch:
+ * hfx_dense_scratch_bytes(S) bytes or null for the context's slot */
 size_t hfx_dense_scratch_bytes(uint32_t S);
 uint32_t hfx_dense_max_len(void);
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-                     const uint32_t *d_num, const uint32_t *d_order, const hfx_db *db,
+                     const uint32_t *d_num, const uint32_t *d_order, const hfx_dn_ladder *lad,
                      const hfx_spectra *sp, uint32_t CH, void *scratch, void *stream);
 
 /* ------------------------------------------------------------------ */

@@ -20,20 +20,26 @@
  * number of non-zero bins of the row (<= 10 * residues * fragment charges);
  * the top-N stage re-scores exactly with that bound.
  *
- * Layout and work split (one workgroup of four waves per CU, persistent):
+ * Layout and work split (one persistent workgroup of eight waves per CU):
  *   - k_dn_split writes every spectrum once as hi/lo BF16 rows of DN_MAXK bins
  *     (zero beyond its length), bins permuted inside each group of eight so
  *     that the A fragment built below needs no shuffles;
+ *   - the b ladders of the candidates (sequential FP32 sums of residue masses,
+ *     bit-exact with the sparse path) are built once, in residual-sorted
+ *     order, so that the ladder positions of a tile of consecutive
+ *     candidates are one contiguous range;
  *   - a work item is (block, fragment charge level, chunk of CH candidates);
  *     the workgroup stages the block's 16 B columns in LDS (once per block);
- *   - each wave takes 16 candidates: lanes 0-15 run the sequential FP32 b
- *     ladder of one candidate each (bit-exact prefix), then every lane takes
- *     one ladder position and ORs the unary weight codes of its peaks into the
- *     wave's LDS nibble map (OR of codes = max weight), bin x in word x/8
- *     of its row, so a lane's A word at k-step ks is word 4 ks + k-group;
+ *   - a pair of waves takes 16 candidates: each lane takes one ladder
+ *     position (of its half's eight candidates) and ORs the unary weight
+ *     codes of its peaks into the pair's LDS nibble map (OR of codes = max
+ *     weight), bin x in word x/8 of its row, so a lane's A word at k-step ks
+ *     is word 4 ks + k-group; ladder positions are loaded a tile ahead;
  *   - A words (8 nibbles) become eight BF16 weights with four v_perm lookups
  *     (codes 0/1/3/7 select the high and low bytes of 0, 10, 25, 50) and four
- *     byte interleaves; the map is cleared as it is read.
+ *     byte interleaves; the map is cleared as it is read; each half of the
+ *     pair contracts every other group of 128 bins and the halves add their
+ *     partial sums.
  * Bins use the same canonical arithmetic as the sparse path: the divisions
  * by the bin width and by the fragment charge are computed as
  * q = x*R, r = fma(-q, d, x), q + r*R, which equals the correctly rounded
@@ -42,6 +48,7 @@
 #include "hfx_cand.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #define DN_NS 8                     /* spectra per block */
 #define DN_COLS 16                  /* MFMA N: hi and lo of each spectrum */
@@ -53,23 +60,17 @@
 #define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
 #define DN_MSTR (DN_MAXK / 8 + 4)   /* map words per row: bin x in word x/8 (padded) */
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
-#define DN_ITEMS 456                /* ladder positions per wave pass */
-#define DN_RMAX 64                  /* residues per candidate staged in LDS */
-#define DN_RSTR 17                  /* dwords per staged row (3 + DN_RMAX bytes) */
+#define DN_PRE 4                    /* ladder positions per lane loaded a tile ahead */
 #define DN_NONE 0xffffffffu
 
 #define DN_B_BYTES (DN_COLS * DN_CSTR * 2)
 #define DN_MAP_BYTES (DN_PAIRS * DN_ROWS * DN_MSTR * 4)
-#define DN_ITEMB_BYTES (DN_WAVES * DN_ITEMS * 4)
-#define DN_ITEMC_BYTES (DN_WAVES * DN_ITEMS)
-#define DN_RES_BYTES (DN_PAIRS * DN_ROWS * DN_RSTR * 4)
-#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES + DN_ITEMC_BYTES + DN_RES_BYTES)
-static_assert(DN_ITEMS * 4 >= HFX_WAVE * 16, "the pair's partial sums live in half 0's items");
-/* the kernel's fixed __shared__ arrays (mtab, m_*, rr_all, sh_t; 592 bytes
- * with the compiler's alignment) share the CU's 160 KiB with the dynamic part */
-#define DN_FIXED_LDS (32 * 4 + DN_NS * 4 * 4 + DN_NS * 8 + DN_PAIRS * DN_ROWS * 4 + 2 * 4 + 16)
+#define DN_COMB_BYTES (DN_PAIRS * 2 * HFX_WAVE * 16)
+#define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_COMB_BYTES)
+/* the kernel's fixed __shared__ arrays (m_*, sh_t) share the CU's 160 KiB
+ * with the dynamic part */
+#define DN_FIXED_LDS (DN_NS * 4 * 4 + DN_NS * 8 + 2 * 4 + 16)
 static_assert(DN_LDS_BYTES + DN_FIXED_LDS <= 160 * 1024, "dense kernel LDS exceeds a CU");
-static_assert(DN_ITEMS % 4 == 0, "ladder buffers keep 16-byte alignment");
 
 typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
@@ -125,10 +126,8 @@ __device__ __forceinline__ dn_u32x4 dn_weights(uint32_t w)
 }
 
 /* Lane moves inside rows of 16 lanes through DPP (no LDS round trip):
- * row_shr:n -- lane i takes lane i - n of its row, lanes below n keep `old`;
- * row_ror:n -- lane i takes lane (i + n) mod 16 of its row (n = 8 swaps the
- * row's halves either way round); quad_perm [1,0,3,2] swaps neighbours. */
-#define DN_SHR(n) (0x110 + (n))
+ * row_ror:n -- lane i takes lane (i + n) mod 16 of its row; quad_perm
+ * [1,0,3,2] swaps neighbours. */
 #define DN_ROR(n) (0x120 + (n))
 #define DN_SWAP1 0xb1
 template <int CTRL>
@@ -140,16 +139,6 @@ template <int CTRL>
 __device__ __forceinline__ float dn_dppf(float v)
 {
     return __uint_as_float(dn_dpp<CTRL>(__float_as_uint(v), __float_as_uint(v)));
-}
-
-/* inclusive sum over each row of 16 lanes */
-__device__ __forceinline__ uint32_t dn_row_scan(uint32_t v)
-{
-    v += dn_dpp<DN_SHR(1)>(0u, v);
-    v += dn_dpp<DN_SHR(2)>(0u, v);
-    v += dn_dpp<DN_SHR(4)>(0u, v);
-    v += dn_dpp<DN_SHR(8)>(0u, v);
-    return v;
 }
 
 /* maximum over each row of 16 lanes, left in every lane of the row (the
@@ -255,6 +244,15 @@ __global__ void k_dn_zero_unscored(float *scores, const uint64_t *off, const uin
 /* The contraction                                                        */
 /* ------------------------------------------------------------------ */
 
+/* the pipeline registers of one tile */
+struct DnPipe {
+    hfx_dn_pos pos[DN_PRE]; /* the half's ladder positions, lane + 64 k */
+    uint64_t rng;           /* lanes 0, 1: the half's range in the tile after */
+    uint64_t L;             /* first position */
+    uint32_t np;            /* number of positions */
+    float top;              /* lane < 16: top of candidate lane */
+};
+
 struct DnArgs {
     float *scores;
     const uint64_t *score_off;
@@ -264,7 +262,8 @@ struct DnArgs {
     const uint64_t *tile_off;
     const uint16_t *xb;
     uint32_t *counter;
-    hfx_db db;
+    float *dummy; /* [64]: target of the score stores that have no score */
+    hfx_dn_ladder lad;
     hfx_spectra sp;
 };
 
@@ -363,82 +362,22 @@ __device__ __forceinline__ void dn_peaks_all(uint32_t *row, float b, float y, ui
     if (lvl >= 4) dn_peaks<4, CHECK>(row, b, y, kq, lane);
 }
 
-/* residues of the tile's candidates staged in LDS: row c holds the ion bytes
- * from the 4-byte aligned a0 <= tc on, DN_RMAX residues at most */
-struct DnStage {
-    uint32_t v[3];
-};
-
-/* half h of a wave pair stages candidates 8h..8h+7: lane (c = 8h + lane % 8,
- * q = lane / 8) loads dwords q, q+8, q+16 of candidate c's row, given c's
- * first ion tcc and residue count nc; every byte read lies in [0, num_ions) */
-__device__ __forceinline__ void dn_stage_load(DnStage &st, const uint8_t *ions, uint64_t num_ions,
-                                              uint32_t tcc, uint32_t nc, uint32_t lane)
-{
-    const uint32_t q = lane >> 3;
-    const uint32_t mis = (uint32_t)(((uintptr_t)ions + tcc) & 3u);
-    const int64_t a0 = (int64_t)tcc - mis;
-    const uint32_t ndw = (mis + min(nc, (uint32_t)DN_RMAX) + 3u) >> 2;
-#pragma unroll
-    for (uint32_t i = 0; i < 3; ++i) {
-        const uint32_t d = q + 8u * i;
-        uint32_t w = 0;
-        if (d < ndw) {
-            const int64_t at = a0 + 4 * (int64_t)d;
-            if (at >= 0 && at + 3 < (int64_t)num_ions) {
-                w = *(const uint32_t *)(ions + at);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (at + k >= 0 && at + k < (int64_t)num_ions) w |= (uint32_t)ions[at + k] << (8 * k);
-            }
-        }
-        st.v[i] = w;
-    }
-}
-
-__device__ __forceinline__ void dn_stage_store(const DnStage &st, uint32_t *res, uint32_t lane,
-                                               uint32_t h)
-{
-    const uint32_t c = 8u * h + (lane & 7u), q = lane >> 3;
-#pragma unroll
-    for (uint32_t i = 0; i < 3; ++i) {
-        const uint32_t d = q + 8u * i;
-        if (d < DN_RSTR) res[c * DN_RSTR + d] = st.v[i];
-    }
-}
-
 __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
 {
     extern __shared__ __align__(16) unsigned char dn_lds[];
     uint16_t *Bl = (uint16_t *)dn_lds;
     uint32_t *maps = (uint32_t *)(dn_lds + DN_B_BYTES);
-    float *itemb_all = (float *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES);
-    uint8_t *itemc_all = dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES;
-    uint32_t *res_all = (uint32_t *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES + DN_ITEMB_BYTES +
-                                     DN_ITEMC_BYTES);
-    __shared__ float mtab[32];
+    dn_f32x4 *comb_all = (dn_f32x4 *)(dn_lds + DN_B_BYTES + DN_MAP_BYTES);
     __shared__ uint32_t m_s[DN_NS], m_beg[DN_NS], m_end[DN_NS], m_z[DN_NS];
     __shared__ uint64_t m_off[DN_NS];
-    __shared__ float rr_all[DN_PAIRS][DN_ROWS];
     __shared__ uint32_t sh_t[2];
 
-    /* wave pair q = wave % 4 (halves h = 0, 1) shares a map, its ladder
-     * positions and staged residues */
+    /* wave pair q = wave % 4 (halves h = 0, 1) shares a map; the halves
+     * swap their partial sums through comb */
     const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
     const uint32_t q = wave % DN_PAIRS, h = wave / DN_PAIRS;
     uint32_t *map = maps + q * DN_ROWS * DN_MSTR;
-    float *itemb = itemb_all + wave * DN_ITEMS;
-    uint8_t *itemc = itemc_all + wave * DN_ITEMS;
-    uint32_t *res = res_all + q * DN_ROWS * DN_RSTR;
-    const uint8_t *resb = (const uint8_t *)res;
-    /* half 1's partial sums go to half 0's ladder positions: half 0 is done
-     * with them before barrier (b) and reads the sums after barrier (c),
-     * before its next ladder */
-    dn_f32x4 *comb = (dn_f32x4 *)(itemb_all + q * DN_ITEMS);
-    float *rr = rr_all[q];
-    const uint8_t *ions = a.db.d_ions;
-    if (tid < 32) mtab[tid] = tid < 26 ? a.db.d_mass[tid] : 0.0f;
+    dn_f32x4 *comb = comb_all + q * 2 * HFX_WAVE;
     for (uint32_t i = tid; i < DN_PAIRS * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
     const uint64_t ntiles = dn_sload(a.tile_off, a.B);
     const uint32_t row = lane & 15u, kg = lane >> 4;
@@ -507,6 +446,8 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
          * barrier) */
         if (tid == 0) sh_t[slot ^ 1u] = next;
         const uint32_t beg = dn_sload(a.ubeg, b), end = dn_sload(a.uend, b);
+        /* windows outside the ladder have no positions to read: no scores */
+        if (beg < a.lad.q0 || end > a.lad.q0 + a.lad.nq) continue;
         const uint32_t nch = (end - beg + a.CH - 1) / a.CH;
         const uint32_t local = (uint32_t)(t - dn_sload(a.tile_off, b));
         const uint32_t li = local / nch, chunk = local - li * nch;
@@ -516,250 +457,276 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         const uint32_t cbeg = beg + chunk * a.CH, cend = min(cbeg + a.CH, end);
         const uint32_t step = DN_PAIRS * DN_ROWS;
         /* every pair runs the same number of tiles (barriers below are
-         * workgroup-wide); tiles past the chunk are empty */
-        const uint32_t ntile = (cend - cbeg + step - 1) / step;
-        uint32_t p0 = cbeg + q * DN_ROWS;
+         * workgroup-wide), rounded up to even for the two-tile loop; tiles
+         * past the chunk are empty */
+        const uint32_t ntile = ((cend - cbeg + step - 1) / step + 1u) & ~1u;
+        const uint32_t pstart = cbeg + q * DN_ROWS;
+        /* the spectrum of this lane's C column (lane % 16 = hi or lo of
+         * spectrum g): charge level, window and score offset */
+        const uint32_t g = row >> 1;
+        const uint32_t zg = m_z[g], bg = m_beg[g], eg = m_end[g];
+        const uint64_t og = m_off[g];
+        float *const dummy = a.dummy + lane;
 
-        /* software pipeline over the pair's tiles p0, p0 + 64, ...: every
-         * global load is issued at the start of a tile and consumed at its
-         * end, before the tile's score stores (vmcnt counts stores too):
-         *   registers  tile j: tc, n, r   tile j+1: tc1, n1, r1, pep of j+2
-         *   LDS        residues of tile j (each half its 8 candidates)
-         *   in flight  residues of j+1, metadata of j+2, peptide of j+3 */
-        auto rows_of = [&](uint32_t q0) -> uint32_t {
-            return q0 < cend ? min((uint32_t)DN_ROWS, cend - q0) : 0u;
+        /* software pipeline over the pair's tiles: at the start of tile j the
+         * ladder range of tile j+2 and the ladder positions and tops of tile
+         * j+1 are issued, each consumed a tile later; loads and stores are
+         * unconditional (clamped indices, a dummy target), so the number in
+         * flight does not depend on the data and the waits stay exact.  Half
+         * h takes the positions of candidates 8h..8h+7. */
+        auto rows_of = [&](uint32_t t0) -> uint32_t {
+            return t0 < cend ? min((uint32_t)DN_ROWS, cend - t0) : 0u;
         };
-        /* lane l < 16 holds candidate l of a tile; lanes 16.. hold candidate
-         * 8h + l % 8 (the half's own, for staging its residues) */
-        const uint32_t cidx = lane < 16u ? lane : 8u * h + (lane & 7u);
-        auto load_pep = [&](uint32_t q0) -> uint32_t {
-            return cidx < rows_of(q0) ? a.db.d_idx_sorted[q0 + cidx] : DN_NONE;
+        /* lane 0: the half's first position in the tile at t0, lane 1: one
+         * past its last (both offset 0 for an empty tile) */
+        auto load_range = [&](uint32_t t0) -> uint64_t {
+            const uint32_t nr = rows_of(t0);
+            const uint32_t c = min(8u * h + (lane ? 8u : 0u), nr);
+            return a.lad.off[nr ? t0 + c - a.lad.q0 : 0u];
         };
-        /* the half's candidate 8h + l % 8 in every lane l: row 0 swaps its
-         * halves where lane l holds the other half's candidate */
-        const bool swap8 = lane < 16u && ((lane >> 3) & 1u) != h;
-        auto stage = [&](DnStage &st, uint32_t tcv, uint32_t nv) {
-            const uint32_t tr = dn_dpp<DN_ROR(8)>(tcv, tcv), nr = dn_dpp<DN_ROR(8)>(nv, nv);
-            dn_stage_load(st, ions, a.db.num_ions, swap8 ? tr : tcv, swap8 ? nr : nv, lane);
+        /* lane < 16: the top of candidate lane (meaningful below the tile's
+         * row count) */
+        auto load_top = [&](uint32_t t0) -> float {
+            return a.lad.top[lane < rows_of(t0) ? t0 + lane - a.lad.q0 : 0u];
         };
-        auto meta = [&](uint32_t pep, uint32_t &tcv, uint32_t &nv, float &rv) {
-            tcv = 0;
-            nv = 0;
-            rv = 0.0f;
-            if (pep != DN_NONE) {
-                tcv = a.db.d_tc[pep];
-                const uint32_t tn = a.db.d_tn[pep];
-                nv = tn > tcv ? min(tn - tcv, (uint32_t)DN_ITEMS) : 0u;
-                rv = a.db.d_r[pep];
+        auto lane64 = [&](uint64_t v, uint32_t l) -> uint64_t {
+            return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32;
+        };
+        /* lane l takes positions l, l + 64, ...; past the half's count it
+         * reloads the last one (or position 0) */
+        auto load_pos = [&](hfx_dn_pos(&pv)[DN_PRE], uint64_t L0, uint32_t np) {
+#pragma unroll
+            for (uint32_t k = 0; k < DN_PRE; ++k) {
+                const uint32_t i = lane + k * HFX_WAVE;
+                pv[k] = a.lad.pos[i < np ? L0 + i : (np ? L0 + np - 1u : 0ull)];
             }
         };
-        uint32_t tc, n, tc1, n1, pep2;
-        float r, r1;
+        DnPipe X, Y;
         {
-            const uint32_t pep0 = load_pep(p0), pep1 = load_pep(p0 + step);
-            pep2 = load_pep(p0 + 2 * step);
-            meta(pep0, tc, n, r);
-            meta(pep1, tc1, n1, r1);
-            DnStage st0;
-            stage(st0, tc, n);
-            dn_stage_store(st0, res, lane, h);
+            /* X.rng before X's positions and top, as in the loop, so that the
+             * wait for it at the loop head is the same on both entries */
+            const uint64_t r0 = load_range(pstart);
+            X.rng = load_range(pstart + step);
+            __builtin_amdgcn_sched_barrier(0);
+            X.L = lane64(r0, 0);
+            X.np = (uint32_t)(lane64(r0, 1) - X.L);
+            load_pos(X.pos, X.L, X.np);
+            X.top = load_top(pstart);
         }
-        for (uint32_t j = 0; j < ntile; ++j, p0 += step) {
+        auto tile = [&](DnPipe &cur, DnPipe &nxt, uint32_t p0) {
             const uint32_t nrows = rows_of(p0);
-            /* issue: residues of j+1, metadata of j+2, peptide of j+3 */
-            DnStage st1;
-            stage(st1, tc1, n1);
-            uint32_t tc2, n2;
-            float r2;
-            meta(pep2, tc2, n2, r2);
-            const uint32_t pep3 = load_pep(p0 + 3 * step);
+            /* issue: range of tile j+2, positions and tops of tile j+1 */
+            nxt.rng = load_range(p0 + 2 * step);
+            nxt.L = lane64(cur.rng, 0);
+            nxt.np = (uint32_t)(lane64(cur.rng, 1) - nxt.L);
+            load_pos(nxt.pos, nxt.L, nxt.np);
+            nxt.top = load_top(p0 + step);
 
-            /* half h owns candidates 8h..8h+7 of the tile: their residual,
-             * ladder positions and map rows; nothing below the next barrier
-             * touches the other half's */
-            if (lane >= 8u * h && lane < 8u * h + 8u) rr[lane] = r;
-            /* lane c < 16: ladder positions of candidates 0..c */
-            const uint32_t incl = dn_row_scan(n);
-            /* heaviest fragment of the tile: every peak bin is below
-             * m / 1.0005 + 2 for m = residual + water + proton (the b ladder
-             * sums the same residue masses as the residual) */
+            /* every peak bin of a position lies below top / 1.0005 + 2 for
+             * top = max(residual, last b) of its candidate: kq covers the
+             * tile, and where it is not cut at the spectrum's length the
+             * unchecked writes apply */
             const float mm = __uint_as_float(__builtin_amdgcn_readfirstlane(
-                __float_as_uint(dn_row_max(lane < nrows ? r : 0.0f))));
-            const uint32_t kq = nrows ? min(kb, (((uint32_t)(mm + 24.0f) + 4u) + 127u) & ~127u) : 0u;
-            const uint32_t c0 = 8u * h, c1 = min(c0 + 8u, nrows);
-            /* ladder positions of the half's candidates in passes of at most
-             * DN_ITEMS */
-            uint32_t c_lo = c0;
-            while (c_lo < c1) {
-                const uint32_t base = c_lo ? __builtin_amdgcn_readlane(incl, c_lo - 1) : 0u;
-                const bool fits = lane >= c_lo && lane < c1 && incl - base <= DN_ITEMS;
-                const uint64_t bal = __ballot(fits);
-                const uint32_t c_hi = bal ? 64u - (uint32_t)__builtin_clzll(bal) : c_lo + 1u;
-                const uint32_t nitems = __builtin_amdgcn_readlane(incl, c_hi - 1) - base;
-                /* sequential FP32 b ladder of candidate c = c0 + lane, lane < 8:
-                 * half 1 takes row 0's upper lanes */
-                const uint32_t c = c0 + (lane & 7u);
-                const uint32_t excl = incl - n;
-                const uint32_t tcc = h ? dn_dpp<DN_ROR(8)>(tc, tc) : tc;
-                const uint32_t nc = h ? dn_dpp<DN_ROR(8)>(n, n) : n;
-                const uint32_t oc = (h ? dn_dpp<DN_ROR(8)>(excl, excl) : excl) - base;
-                const float rc = h ? dn_dppf<DN_ROR(8)>(r) : r;
-                float mloc = 0.0f;
-                if (lane < 8 && c >= c_lo && c < c_hi) {
-                    uint32_t o = oc;
-                    float bsum = 0.0f;
-                    const uint32_t mis = (uint32_t)(((uintptr_t)ions + tcc) & 3u);
-                    const uint8_t *rs = resb + c * DN_RSTR * 4 + mis;
-                    const uint32_t ns = min(nc, (uint32_t)DN_RMAX);
-                    for (uint32_t k0 = 0; k0 < ns; k0 += 8) {
-                        uint32_t v[8];
+                __float_as_uint(dn_row_max(lane < nrows ? cur.top : 0.0f))));
+            const uint32_t kneed = (uint32_t)(mm + 24.0f) + 4u;
+            const uint32_t kq = nrows ? min(kb, (kneed + 127u) & ~127u) : 0u;
+            auto peaks = [&](auto chk) {
+                constexpr bool CHECK = decltype(chk)::value;
 #pragma unroll
-                        for (uint32_t jj = 0; jj < 8; ++jj) v[jj] = k0 + jj < ns ? rs[k0 + jj] : 0u;
-#pragma unroll
-                        for (uint32_t jj = 0; jj < 8; ++jj) {
-                            if (k0 + jj < ns) {
-                                bsum = bsum + mtab[HFX_ACID_IDX(v[jj])];
-                                itemb[o] = bsum;
-                                itemc[o] = (uint8_t)c;
-                                ++o;
-                            }
-                        }
-                    }
-                    for (uint32_t k = ns; k < nc; ++k) { /* beyond the staged residues */
-                        bsum = bsum + mtab[HFX_ACID_IDX(ions[tcc + k])];
-                        itemb[o] = bsum;
-                        itemc[o] = (uint8_t)c;
-                        ++o;
-                    }
-                    mloc = fmaxf(rc, bsum);
+                for (uint32_t k = 0; k < DN_PRE; ++k) {
+                    const hfx_dn_pos &v = cur.pos[k];
+                    if (lane + k * HFX_WAVE < cur.np)
+                        dn_peaks_all<CHECK>(map + (v.q - p0) * DN_MSTR + 1, v.b, v.y, kq, lane,
+                                            lvl);
                 }
-                /* every bin of the pass lies below m / 1.0005 + 2 for the
-                 * heaviest b + proton or y + water + proton (y <= residual):
-                 * with that below kq, the unchecked writes apply */
-                mloc = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(dn_row_max(mloc))));
-                const bool fast = (uint32_t)(mloc + 24.0f) + 4u <= kq;
-                hfx_wave_sync();
-                /* one ladder position per lane per round */
-                if (fast) {
-                    for (uint32_t i = lane; i < nitems; i += HFX_WAVE) {
-                        const float bb = itemb[i];
-                        const uint32_t ci = itemc[i];
-                        dn_peaks_all<false>(map + ci * DN_MSTR + 1, bb, rr[ci] - bb, kq, lane, lvl);
-                    }
-                } else {
-                    for (uint32_t i = lane; i < nitems; i += HFX_WAVE) {
-                        const float bb = itemb[i];
-                        const uint32_t ci = itemc[i];
-                        dn_peaks_all<true>(map + ci * DN_MSTR + 1, bb, rr[ci] - bb, kq, lane, lvl);
-                    }
+                for (uint32_t i = DN_PRE * HFX_WAVE + lane; i < cur.np; i += HFX_WAVE) {
+                    const hfx_dn_pos v = a.lad.pos[cur.L + i];
+                    dn_peaks_all<CHECK>(map + (v.q - p0) * DN_MSTR + 1, v.b, v.y, kq, lane, lvl);
                 }
-                hfx_wave_sync();
-                c_lo = c_hi;
-            }
+            };
+            if (kneed <= kq) peaks(std::false_type{});
+            else peaks(std::true_type{});
             __syncthreads(); /* (b) the tile's map rows are complete */
 
-            /* 16 x 16 x kq contraction over groups of four k-steps (128
-             * bins): half h takes groups h, h+2, ... and clears their map
-             * words.  The split depends only on the group index, so each
-             * half's sum over a row's non-zero k-steps -- and the score --
-             * is the same whichever tile the row falls in.  Two accumulator
-             * chains; the next group's operands are read ahead (a read past
-             * the last group stays inside the LDS allocation, unused).  Lane
-             * (row, kg) takes word 4 ks + kg of its row at k-step ks. */
-            dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
-            const uint32_t ngrp = kq >> 7;
-            uint32_t *arow = map + row * DN_MSTR + 1 + kg;
-            const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
-            auto mma4 = [&](const dn_u32x4 &cw, const dn_u32x4 &b0, const dn_u32x4 &b1,
-                            const dn_u32x4 &b2, const dn_u32x4 &b3) {
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
-                    __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
-                    __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
-                    __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
-                    __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
-            };
-            if (h < ngrp) {
-                uint32_t *aw = arow + 16 * h;
-                const uint16_t *bw = bcol + 128 * h;
-                dn_u32x4 ca = {aw[0], aw[4], aw[8], aw[12]};
-                dn_u32x4 a0 = *(const dn_u32x4 *)(bw + 0), a1 = *(const dn_u32x4 *)(bw + 32);
-                dn_u32x4 a2 = *(const dn_u32x4 *)(bw + 64), a3 = *(const dn_u32x4 *)(bw + 96);
-                for (uint32_t g = h; g < ngrp; g += 4) {
-                    /* group g (in ca, a0..a3); read group g + 2 */
-                    aw[0] = 0u; aw[4] = 0u; aw[8] = 0u; aw[12] = 0u;
-                    const dn_u32x4 cb = {aw[32], aw[36], aw[40], aw[44]};
-                    const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + 256);
-                    const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + 288);
-                    const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + 320);
-                    const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + 352);
-                    mma4(ca, a0, a1, a2, a3);
-                    if (g + 2 >= ngrp) break;
-                    /* group g + 2 (in cb, e0..e3); read group g + 4 */
-                    aw[32] = 0u; aw[36] = 0u; aw[40] = 0u; aw[44] = 0u;
-                    ca = dn_u32x4{aw[64], aw[68], aw[72], aw[76]};
-                    a0 = *(const dn_u32x4 *)(bw + 512);
-                    a1 = *(const dn_u32x4 *)(bw + 544);
-                    a2 = *(const dn_u32x4 *)(bw + 576);
-                    a3 = *(const dn_u32x4 *)(bw + 608);
-                    mma4(cb, e0, e1, e2, e3);
-                    aw += 64;
-                    bw += 512;
+                /* 16 x 16 x kq contraction over groups of four k-steps (128
+                 * bins): half h takes groups h, h+2, ... and clears their map
+                 * words.  The split depends only on the group index, so each
+                 * half's sum over a row's non-zero k-steps -- and the score --
+                 * is the same whichever tile the row falls in.  Two accumulator
+                 * chains; the next group's operands are read ahead (a read past
+                 * the last group stays inside the LDS allocation, unused).  Lane
+                 * (row, kg) takes word 4 ks + kg of its row at k-step ks. */
+                dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+                const uint32_t ngrp = kq >> 7;
+                uint32_t *arow = map + row * DN_MSTR + 1 + kg;
+                const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
+                auto mma4 = [&](const dn_u32x4 &cw, const dn_u32x4 &b0, const dn_u32x4 &b1,
+                                const dn_u32x4 &b2, const dn_u32x4 &b3) {
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
+                        __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
+                        __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
+                        __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
+                        __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
+                };
+                if (h < ngrp) {
+                    uint32_t *aw = arow + 16 * h;
+                    const uint16_t *bw = bcol + 128 * h;
+                    dn_u32x4 ca = {aw[0], aw[4], aw[8], aw[12]};
+                    dn_u32x4 a0 = *(const dn_u32x4 *)(bw + 0), a1 = *(const dn_u32x4 *)(bw + 32);
+                    dn_u32x4 a2 = *(const dn_u32x4 *)(bw + 64), a3 = *(const dn_u32x4 *)(bw + 96);
+                    for (uint32_t g = h; g < ngrp; g += 4) {
+                        /* group g (in ca, a0..a3); read group g + 2 */
+                        aw[0] = 0u; aw[4] = 0u; aw[8] = 0u; aw[12] = 0u;
+                        const dn_u32x4 cb = {aw[32], aw[36], aw[40], aw[44]};
+                        const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + 256);
+                        const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + 288);
+                        const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + 320);
+                        const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + 352);
+                        mma4(ca, a0, a1, a2, a3);
+                        if (g + 2 >= ngrp) break;
+                        /* group g + 2 (in cb, e0..e3); read group g + 4 */
+                        aw[32] = 0u; aw[36] = 0u; aw[40] = 0u; aw[44] = 0u;
+                        ca = dn_u32x4{aw[64], aw[68], aw[72], aw[76]};
+                        a0 = *(const dn_u32x4 *)(bw + 512);
+                        a1 = *(const dn_u32x4 *)(bw + 544);
+                        a2 = *(const dn_u32x4 *)(bw + 576);
+                        a3 = *(const dn_u32x4 *)(bw + 608);
+                        mma4(cb, e0, e1, e2, e3);
+                        aw += 64;
+                        bw += 512;
+                    }
                 }
-            }
             const dn_f32x4 part = acc0 + acc1;
-            if (h == 1) comb[lane] = part;
-            /* consume what was issued at the start of the tile: the next
-             * tile's residues go to LDS, the pipeline registers advance */
-            dn_stage_store(st1, res, lane, h);
-            tc = tc1;
-            n = n1;
-            r = r1;
-            tc1 = tc2;
-            n1 = n2;
-            r1 = r2;
-            pep2 = pep3;
+            comb[h * HFX_WAVE + lane] = part;
             __syncthreads(); /* (c) both halves' partial sums are in */
-            if (h == 0) {
-                const dn_f32x4 other = comb[lane];
-                /* C[row 4*(lane/16)+i][col lane%16]; hi + lo of spectrum col/2 */
+            const dn_f32x4 other = comb[(h ^ 1u) * HFX_WAVE + lane];
+            /* C[row 4 kg + i][column lane % 16]: half h stores rows i = 2h,
+             * 2h + 1; an even column's lane adds its odd neighbour (hi + lo) */
 #pragma unroll
-                for (uint32_t i = 0; i < 4; ++i) {
-                    const float c4 = part[i] + other[i];
-                    const float v = c4 + dn_dppf<DN_SWAP1>(c4);
-                    const uint32_t rr_i = 4u * kg + i, g = row >> 1, p = p0 + rr_i;
-                    if ((lane & 1u) == 0 && rr_i < nrows && m_z[g] == lvl && m_beg[g] <= p &&
-                        p < m_end[g])
-                        a.scores[m_off[g] + (p - m_beg[g])] = v;
-                }
+            for (uint32_t i2 = 0; i2 < 2; ++i2) {
+                const float mine = h ? part[2 + i2] : part[i2];
+                const float oth = h ? other[2 + i2] : other[i2];
+                const float c4 = h ? oth + mine : mine + oth;
+                const float v = c4 + dn_dppf<DN_SWAP1>(c4);
+                const uint32_t rr_i = 4u * kg + 2u * h + i2, p = p0 + rr_i;
+                const bool ok =
+                    (lane & 1u) == 0 && rr_i < nrows && zg == lvl && bg <= p && p < eg;
+                *(ok ? a.scores + og + (p - bg) : dummy) = v;
             }
+        };
+        for (uint32_t j = 0; j < ntile; j += 2) {
+            tile(X, Y, pstart + j * step);
+            tile(Y, X, pstart + (j + 1) * step);
         }
     }
 }
 
 /* ------------------------------------------------------------------ */
-/* Launch                                                                 */
+/* Ladders                                                                */
 /* ------------------------------------------------------------------ */
+
+__global__ void k_dn_lad_count(uint32_t *cnt, hfx_db db, uint32_t q0, uint32_t nq)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint32_t pep = db.d_idx_sorted[q0 + i];
+    const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
+    cnt[i] = tn > tc ? tn - tc : 0u;
+}
+
+/* one thread per peptide: the sequential FP32 b ladder of the sparse path
+ * (residue masses from the [26] table by index byte & 31, zero past 25) and
+ * y = r - b */
+__global__ void k_dn_lad_fill(hfx_dn_pos *pos, float *top, const uint64_t *off, hfx_db db,
+                              uint32_t q0, uint32_t nq)
+{
+    __shared__ float mt[32];
+    if (threadIdx.x < 32) mt[threadIdx.x] = threadIdx.x < 26 ? db.d_mass[threadIdx.x] : 0.0f;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint32_t pep = db.d_idx_sorted[q0 + i];
+    const uint32_t tc = db.d_tc[pep];
+    const float r = db.d_r[pep];
+    const uint64_t o = off[i];
+    const uint32_t n = (uint32_t)(off[i + 1] - o);
+    float b = 0.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+        b = b + mt[HFX_ACID_IDX(db.d_ions[(uint64_t)tc + k])];
+        pos[o + k] = hfx_dn_pos{b, r - b, q0 + i};
+    }
+    top[i] = fmaxf(r, b);
+}
+
+int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t nq, int own,
+                           hfx_dn_ladder *lad, void **mem, void *stream)
+{
+    HFX_REQUIRE(ctx && db && lad && (!own || mem), HFX_ERR_ARG, "dense ladder: null argument");
+    HFX_REQUIRE((uint64_t)q0 + nq <= db->num_pep, HFX_ERR_ARG,
+                "dense ladder: peptides %u..%u beyond the database's %u", q0, q0 + nq,
+                db->num_pep);
+    if (own) mem[0] = mem[1] = nullptr;
+    hipStream_t st = hfx_stream(stream);
+    const size_t meta = ((size_t)nq + 1) * 8 + (size_t)nq * 8 + 64;
+    void *m = nullptr;
+    int rc;
+    if (own) {
+        HFX_CHECK_HIP(hipMalloc(&m, meta));
+        mem[0] = m;
+    } else if ((rc = hfx_ctx_scratch(ctx, HFX_SLOT_LADDER, meta, &m))) {
+        return rc;
+    }
+    uint64_t *off = (uint64_t *)m;
+    float *top = (float *)(off + nq + 1);
+    uint32_t *cnt = (uint32_t *)(top + nq);
+    const uint32_t g = (nq + 255) / 256;
+    if (nq) hipLaunchKernelGGL(k_dn_lad_count, dim3(g), dim3(256), 0, st, cnt, *db, q0, nq);
+    if ((rc = hfx_scan_offsets(ctx, off, cnt, nq, stream))) return rc;
+    uint64_t total = 0;
+    HFX_CHECK_HIP(hipMemcpyAsync(&total, off + nq, 8, hipMemcpyDeviceToHost, st));
+    HFX_CHECK_HIP(hipStreamSynchronize(st));
+    /* at least one position: the kernel's surplus loads read position 0 */
+    const size_t pb = (size_t)std::max<uint64_t>(total, 1) * sizeof(hfx_dn_pos);
+    void *pp = nullptr;
+    if (own) {
+        HFX_CHECK_HIP(hipMalloc(&pp, pb));
+        mem[1] = pp;
+    } else if ((rc = hfx_ctx_scratch(ctx, HFX_SLOT_LADDER2, pb, &pp))) {
+        return rc;
+    }
+    if (nq)
+        hipLaunchKernelGGL(k_dn_lad_fill, dim3(g), dim3(256), 0, st, (hfx_dn_pos *)pp, top, off,
+                           *db, q0, nq);
+    HFX_CHECK_LAUNCH();
+    lad->pos = (const hfx_dn_pos *)pp;
+    lad->off = off;
+    lad->top = top;
+    lad->q0 = q0;
+    lad->nq = nq;
+    return HFX_OK;
+}
 
 size_t hfx_dense_scratch_bytes(uint32_t S)
 {
     const size_t B = (S + DN_NS - 1) / DN_NS;
-    return (size_t)S * 2 * DN_MAXK * 2 + B * 4 * 4 + (B + 1) * 8 + 256;
+    return (size_t)S * 2 * DN_MAXK * 2 + B * 4 * 4 + (B + 1) * 8 + 1024;
 }
 
 uint32_t hfx_dense_max_len(void) { return DN_MAXK; }
 
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-                     const uint32_t *d_num, const uint32_t *d_order, const hfx_db *db,
+                     const uint32_t *d_num, const uint32_t *d_order, const hfx_dn_ladder *lad,
                      const hfx_spectra *sp, uint32_t CH, void *scratch, void *stream)
 {
-    HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && db && sp, HFX_ERR_ARG,
+    HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && lad && sp, HFX_ERR_ARG,
                 "dense XCorr: null argument");
     HFX_REQUIRE(sp->x_stride <= DN_MAXK, HFX_ERR_ARG,
                 "dense XCorr: spectra of up to %d bins (x_stride %u)", DN_MAXK, sp->x_stride);
@@ -778,13 +745,14 @@ int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const
     uint32_t *uend = ubeg + B, *binfo = uend + B, *ntiles = binfo + B;
     uint64_t *tile_off = (uint64_t *)(((uintptr_t)(ntiles + B) + 7) & ~(uintptr_t)7);
     uint32_t *counter = (uint32_t *)(tile_off + B + 1);
+    float *dummy = (float *)(((uintptr_t)(counter + 1) + 255) & ~(uintptr_t)255);
     hipLaunchKernelGGL(k_dn_split, dim3(S), dim3(256), 0, st, xb, *sp);
     hipLaunchKernelGGL(k_dn_blocks, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend, binfo,
                        ntiles, d_order, d_begin, d_num, *sp, B, CH);
     if ((rc = hfx_scan_offsets(ctx, tile_off, ntiles, B, (void *)st))) return rc;
     hipLaunchKernelGGL(k_dn_reset, dim3(1), dim3(1), 0, st, counter);
-    DnArgs args{d_scores, d_off, d_begin, d_num, d_order, B, CH, ubeg, uend, binfo, tile_off, xb,
-                counter, *db, *sp};
+    DnArgs args{d_scores, d_off, d_begin, d_num, d_order, B,     CH,   ubeg,
+                uend,     binfo, tile_off, xb,  counter, dummy, *lad, *sp};
     HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_dense,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, DN_LDS_BYTES));
     hipLaunchKernelGGL(k_dense, dim3((uint32_t)hfx_ctx_num_cus(ctx)), dim3(DN_THREADS),
@@ -800,6 +768,14 @@ extern "C" int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint6
                                       const uint32_t *d_order, const hfx_db *db,
                                       const hfx_spectra *sp, void *stream)
 {
-    return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, db, sp, 256, nullptr,
+    HFX_REQUIRE(db, HFX_ERR_ARG, "hfx_xcorr_dense_bf16x2: null database");
+    if (sp && sp->num_spec == 0) return HFX_OK;
+    /* ladders of the whole database, rebuilt on every call (a batch plan
+     * builds those of its windows once) */
+    hfx_dn_ladder lad;
+    int rc;
+    if ((rc = hfx_dense_ladder_build(ctx, db, 0, db->num_pep, 0, &lad, nullptr, stream)))
+        return rc;
+    return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, &lad, sp, 256, nullptr,
                             stream);
 }
