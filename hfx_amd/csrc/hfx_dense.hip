@@ -549,61 +549,61 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
             else peaks(std::true_type{});
             __syncthreads(); /* (b) the tile's map rows are complete */
 
-                /* 16 x 16 x kq contraction over groups of four k-steps (128
-                 * bins): half h takes groups h, h+2, ... and clears their map
-                 * words.  The split depends only on the group index, so each
-                 * half's sum over a row's non-zero k-steps -- and the score --
-                 * is the same whichever tile the row falls in.  Two accumulator
-                 * chains; the next group's operands are read ahead (a read past
-                 * the last group stays inside the LDS allocation, unused).  Lane
-                 * (row, kg) takes word 4 ks + kg of its row at k-step ks. */
-                dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
-                const uint32_t ngrp = kq >> 7;
-                uint32_t *arow = map + row * DN_MSTR + 1 + kg;
-                const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
-                auto mma4 = [&](const dn_u32x4 &cw, const dn_u32x4 &b0, const dn_u32x4 &b1,
-                                const dn_u32x4 &b2, const dn_u32x4 &b3) {
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
-                        __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
-                        __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
-                        __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
-                        __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
-                };
-                if (h < ngrp) {
-                    uint32_t *aw = arow + 16 * h;
-                    const uint16_t *bw = bcol + 128 * h;
-                    dn_u32x4 ca = {aw[0], aw[4], aw[8], aw[12]};
-                    dn_u32x4 a0 = *(const dn_u32x4 *)(bw + 0), a1 = *(const dn_u32x4 *)(bw + 32);
-                    dn_u32x4 a2 = *(const dn_u32x4 *)(bw + 64), a3 = *(const dn_u32x4 *)(bw + 96);
-                    for (uint32_t g = h; g < ngrp; g += 4) {
-                        /* group g (in ca, a0..a3); read group g + 2 */
-                        aw[0] = 0u; aw[4] = 0u; aw[8] = 0u; aw[12] = 0u;
-                        const dn_u32x4 cb = {aw[32], aw[36], aw[40], aw[44]};
-                        const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + 256);
-                        const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + 288);
-                        const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + 320);
-                        const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + 352);
-                        mma4(ca, a0, a1, a2, a3);
-                        if (g + 2 >= ngrp) break;
-                        /* group g + 2 (in cb, e0..e3); read group g + 4 */
-                        aw[32] = 0u; aw[36] = 0u; aw[40] = 0u; aw[44] = 0u;
-                        ca = dn_u32x4{aw[64], aw[68], aw[72], aw[76]};
-                        a0 = *(const dn_u32x4 *)(bw + 512);
-                        a1 = *(const dn_u32x4 *)(bw + 544);
-                        a2 = *(const dn_u32x4 *)(bw + 576);
-                        a3 = *(const dn_u32x4 *)(bw + 608);
-                        mma4(cb, e0, e1, e2, e3);
-                        aw += 64;
-                        bw += 512;
-                    }
+            /* 16 x 16 x kq contraction over groups of four k-steps (128
+             * bins): half h takes groups h, h+2, ... and clears their map
+             * words.  The split depends only on the group index, so each
+             * half's sum over a row's non-zero k-steps -- and the score --
+             * is the same whichever tile the row falls in.  Two accumulator
+             * chains; the next group's operands are read ahead (a read past
+             * the last group stays inside the LDS allocation, unused).  Lane
+             * (row, kg) takes word 4 ks + kg of its row at k-step ks. */
+            dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+            const uint32_t ngrp = kq >> 7;
+            uint32_t *arow = map + row * DN_MSTR + 1 + kg;
+            const uint16_t *bcol = Bl + row * DN_CSTR + kg * 8;
+            auto mma4 = [&](const dn_u32x4 &cw, const dn_u32x4 &b0, const dn_u32x4 &b1,
+                            const dn_u32x4 &b2, const dn_u32x4 &b3) {
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.x)),
+                    __builtin_bit_cast(dn_bf16x8, b0), acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.y)),
+                    __builtin_bit_cast(dn_bf16x8, b1), acc1, 0, 0, 0);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.z)),
+                    __builtin_bit_cast(dn_bf16x8, b2), acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
+                    __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
+            };
+            if (h < ngrp) {
+                uint32_t *aw = arow + 16 * h;
+                const uint16_t *bw = bcol + 128 * h;
+                dn_u32x4 ca = {aw[0], aw[4], aw[8], aw[12]};
+                dn_u32x4 a0 = *(const dn_u32x4 *)(bw + 0), a1 = *(const dn_u32x4 *)(bw + 32);
+                dn_u32x4 a2 = *(const dn_u32x4 *)(bw + 64), a3 = *(const dn_u32x4 *)(bw + 96);
+                for (uint32_t g = h; g < ngrp; g += 4) {
+                    /* group g (in ca, a0..a3); read group g + 2 */
+                    aw[0] = 0u; aw[4] = 0u; aw[8] = 0u; aw[12] = 0u;
+                    const dn_u32x4 cb = {aw[32], aw[36], aw[40], aw[44]};
+                    const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + 256);
+                    const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + 288);
+                    const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + 320);
+                    const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + 352);
+                    mma4(ca, a0, a1, a2, a3);
+                    if (g + 2 >= ngrp) break;
+                    /* group g + 2 (in cb, e0..e3); read group g + 4 */
+                    aw[32] = 0u; aw[36] = 0u; aw[40] = 0u; aw[44] = 0u;
+                    ca = dn_u32x4{aw[64], aw[68], aw[72], aw[76]};
+                    a0 = *(const dn_u32x4 *)(bw + 512);
+                    a1 = *(const dn_u32x4 *)(bw + 544);
+                    a2 = *(const dn_u32x4 *)(bw + 576);
+                    a3 = *(const dn_u32x4 *)(bw + 608);
+                    mma4(cb, e0, e1, e2, e3);
+                    aw += 64;
+                    bw += 512;
                 }
+            }
             const dn_f32x4 part = acc0 + acc1;
             comb[h * HFX_WAVE + lane] = part;
             __syncthreads(); /* (c) both halves' partial sums are in */
