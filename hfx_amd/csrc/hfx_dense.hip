@@ -58,7 +58,10 @@
 #define DN_ROWS 16                  /* candidates per wave tile (MFMA M) */
 #define DN_MAXK 2176                /* bins of a row (multiple of 128) */
 #define DN_NKS (DN_MAXK / 32)       /* k-steps of 32 bins */
-#define DN_MSTR (DN_MAXK / 8 + 4)   /* map words per row: bin x in word x/8 (padded) */
+/* map words per row: bin x in word 1 + x/8; 290 = 2 mod 32, so the 16 rows x
+ * 2 k-groups that a half-wave reads or clears fall in 32 distinct banks */
+#define DN_MSTR 290
+static_assert(DN_MSTR >= DN_MAXK / 8 + 2 && DN_MSTR % 32 == 2, "map row stride");
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
 #define DN_PRE 4                    /* ladder positions per lane loaded a tile ahead */
 #define DN_NONE 0xffffffffu
