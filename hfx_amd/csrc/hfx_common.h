@@ -111,7 +111,8 @@ int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, floa
  * the dense path.  Position k < tn - tc of sorted peptide q is
  * pos[off[q - q0] + k] = {b_k, r - b_k, q}, with b_k the sequential FP32 sum
  * of its residue masses 0..k; top[q - q0] = max(r, b_last) bounds every
- * fragment mass of the peptide. */
+ * fragment mass of the peptide (1e9 when a first residue under 32 Da could
+ * put a peak below bin 1). */
 struct hfx_dn_pos {
     float b, y;
     uint32_t q;

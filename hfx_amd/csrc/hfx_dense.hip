@@ -300,19 +300,36 @@ __device__ __forceinline__ void dn_put3(uint32_t *row, int32_t x, uint32_t kq, u
     atomicOr(&row[v2 ? (uint32_t)(x + 1) >> 3 : lane], v2);
 }
 
-/* unchecked forms, for passes whose every bin lies below kq: a negative bin
- * goes to the margin word (word index clamped at -1) */
-__device__ __forceinline__ void dn_put_u(uint32_t *row, int32_t x, uint32_t code)
+/* Unchecked forms, for tiles whose every bin lies in [1, kq): a bin comes as
+ * the bits u of x + 1.5 * 2^23 -- the float addition rounds x to the nearest
+ * integer, ties to even (as rint), and leaves it in the low mantissa bits, so
+ * u = 0x4B400000 + rint(x) for |x| < 2^22.  The constant's low three bits are
+ * zero: u & 7 = x & 7 and u >> 3 = x >> 3 + DN_MAGIC_WORDS. */
+#define DN_MAGIC 12582912.0f
+#define DN_MAGIC_WORDS (0x4B400000u >> 3)
+
+__device__ __forceinline__ uint32_t dn_binu(float mz)
 {
-    atomicOr(&row[max(x >> 3, -1)], code << (((uint32_t)x & 7u) << 2));
+    const float R = 1.0f / HFX_BIN_WIDTH_MONO;
+    return __float_as_uint(dn_div(mz, HFX_BIN_WIDTH_MONO, R) + DN_MAGIC);
 }
-__device__ __forceinline__ void dn_put3_u(uint32_t *row, int32_t x)
+/* rowm: the row's word-0 address less DN_MAGIC_WORDS words (a 32-bit LDS
+ * address that wraps; only rowm + (u >> 3) is ever dereferenced) */
+__device__ __forceinline__ void dn_put_u(uint32_t *rowm, uint32_t u, uint32_t code)
 {
-    const int32_t xm = x - 1;
-    const uint32_t sh = ((uint32_t)xm & 7u) << 2;
-    const uint32_t t = HFX_CODE25 | HFX_CODE50 << 4 | HFX_CODE25 << 8;
-    atomicOr(&row[max(xm >> 3, -1)], t << sh);
-    atomicOr(&row[max((x + 1) >> 3, -1)], sh >= 24u ? t >> (32u - sh) : t << sh);
+    atomicOr(rowm + (u >> 3), code << ((u & 7u) << 2));
+}
+/* 25 at x-1, 50 at x, 25 at x+1: the nibbles 3, 7, 3 shifted into place as
+ * one 64-bit value over the word of x-1 and the next (zero when nothing
+ * spills) */
+__device__ __forceinline__ void dn_put3_u(uint32_t *rowm, uint32_t u)
+{
+    const uint32_t um = u - 1u;
+    const uint64_t t = (uint64_t)(HFX_CODE25 | HFX_CODE50 << 4 | HFX_CODE25 << 8)
+                       << ((um & 7u) << 2);
+    uint32_t *w = rowm + (um >> 3);
+    atomicOr(w, (uint32_t)t);
+    atomicOr(w + 1, (uint32_t)(t >> 32));
 }
 
 /* division by the fragment charge Z: exact as a product for Z = 1, 2, 4 */
@@ -333,25 +350,24 @@ __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32
 {
     constexpr float zh = HFX_DEV_H * (float)Z;
     constexpr float h2o = HFX_DEV_H2O / (float)Z, nh3 = HFX_DEV_NH3 / (float)Z;
-    const int32_t xa = dn_bin(dn_zdiv<Z>((b - HFX_DEV_CO) + zh));
+    const float ma = dn_zdiv<Z>((b - HFX_DEV_CO) + zh);
     const float mb = dn_zdiv<Z>(b + zh);
-    const int32_t xb = dn_bin(mb), xw = dn_bin(mb - h2o), xn = dn_bin(mb - nh3);
     const float my = dn_zdiv<Z>((y + HFX_DEV_H2O) + zh);
-    const int32_t xy = dn_bin(my), xyn = dn_bin(my - nh3);
     if constexpr (CHECK) {
-        dn_put(row, xa, HFX_CODE10, kq, lane);
-        dn_put3(row, xb, kq, lane);
-        dn_put(row, xw, HFX_CODE10, kq, lane);
-        dn_put(row, xn, HFX_CODE10, kq, lane);
-        dn_put3(row, xy, kq, lane);
-        dn_put(row, xyn, HFX_CODE10, kq, lane);
+        dn_put(row, dn_bin(ma), HFX_CODE10, kq, lane);
+        dn_put3(row, dn_bin(mb), kq, lane);
+        dn_put(row, dn_bin(mb - h2o), HFX_CODE10, kq, lane);
+        dn_put(row, dn_bin(mb - nh3), HFX_CODE10, kq, lane);
+        dn_put3(row, dn_bin(my), kq, lane);
+        dn_put(row, dn_bin(my - nh3), HFX_CODE10, kq, lane);
     } else {
-        dn_put_u(row, xa, HFX_CODE10);
-        dn_put3_u(row, xb);
-        dn_put_u(row, xw, HFX_CODE10);
-        dn_put_u(row, xn, HFX_CODE10);
-        dn_put3_u(row, xy);
-        dn_put_u(row, xyn, HFX_CODE10);
+        uint32_t *rowm = row - DN_MAGIC_WORDS;
+        dn_put_u(rowm, dn_binu(ma), HFX_CODE10);
+        dn_put3_u(rowm, dn_binu(mb));
+        dn_put_u(rowm, dn_binu(mb - h2o), HFX_CODE10);
+        dn_put_u(rowm, dn_binu(mb - nh3), HFX_CODE10);
+        dn_put3_u(rowm, dn_binu(my));
+        dn_put_u(rowm, dn_binu(my - nh3), HFX_CODE10);
     }
 }
 
@@ -661,12 +677,16 @@ __global__ void k_dn_lad_fill(hfx_dn_pos *pos, float *top, const uint64_t *off, 
     const float r = db.d_r[pep];
     const uint64_t o = off[i];
     const uint32_t n = (uint32_t)(off[i + 1] - o);
-    float b = 0.0f;
+    float b = 0.0f, b0 = 0.0f;
     for (uint32_t k = 0; k < n; ++k) {
         b = b + mt[HFX_ACID_IDX(db.d_ions[(uint64_t)tc + k])];
+        if (k == 0) b0 = b;
         pos[o + k] = hfx_dn_pos{b, r - b, q0 + i};
     }
-    top[i] = fmaxf(r, b);
+    /* with a first residue of 32 Da or more every peak bin is >= 1 (the
+     * lowest, y - NH3 at charge 4, is above 1.2); a lighter one (a zero-mass
+     * residue code) marks the peptide's tiles for the checked writes */
+    top[i] = n && b0 < 32.0f ? 1e9f : fmaxf(r, b);
 }
 
 int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t nq, int own,
