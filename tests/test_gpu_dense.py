@@ -154,3 +154,40 @@ def test_batch_dense_graph_and_autotune(denv, orc):
         again = bs.results(keep_scores=True)
         assert np.array_equal(again.scores, first.scores)
         check_topn(again, denv["ref"], orc)
+
+
+def test_dense_light_first_residue(orc):
+    """A static modification that makes glycine 20 Da: peptides that start
+    with it have a-ions below bin 0, so their tiles take the checked peak
+    writes (the ladder marks them); scores and rankings still follow the
+    oracle."""
+    from hfx_amd import pipeline, synth
+
+    prot = synth.make_proteins(800, seed=61)
+    static = {"C": 57.02146, "G": -37.0}
+    mass = orc.mass_table(static)
+    assert 19.0 < float(mass[ord("G") - 65]) < 21.0
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=2)
+    ions = orc.prepare_ions(prot.seq)
+    first_g = sum(1 for i in range(len(tc)) if ions[tc[i]] == ord("G") - 65)
+    assert first_g > 100
+    sp = synth.make_spectra(prot, r, tc, tn, 160, seed=62, charges=(2, 3, 4),
+                            charge_p=(0.5, 0.3, 0.2), static_mods=static)
+    ctx = pipeline.Ctx(0)
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
+    dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE)
+    idx, _ = orc.sort_idx(r)
+    ref = orc.search_plain(ions, r, tc, tn, mass, idx, dsp.x(), dsp.lengths(), dsp.zmax, dsp.mass,
+                           3.0, 10, want_all=True)
+    absx = orc.search_plain(ions, r, tc, tn, mass, idx, np.abs(dsp.x()), dsp.lengths(), dsp.zmax,
+                            dsp.mass, 3.0, 10, want_all=True)
+    ref["all_scale"] = orc.fx_to_double(absx["all_fx"])
+    res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=10).run(keep_scores=True, dense=True)
+    check_scores(res, ref, dsp, orc)
+    check_topn(res, ref, orc)
+    bs = pipeline.BatchSearch(ctx, db, dsp, eps=3.0, topn=10)
+    bs.set_precision(bs.BF16X2_DENSE)
+    bs.run()
+    res2 = bs.results(keep_scores=True)
+    check_scores(res2, ref, dsp, orc)
+    check_topn(res2, ref, orc)
