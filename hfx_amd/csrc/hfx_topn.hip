@@ -75,6 +75,24 @@ __device__ __forceinline__ void tn_select_digit(const uint32_t *hist, uint32_t *
     __syncthreads();
 }
 
+/* Histogram increment of digit d for the active lanes of a wave: lanes that
+ * share the first pending lane's digit add their count with one atomic (a
+ * few rounds: score keys crowd into few high-byte buckets), the rest add one
+ * each. */
+__device__ __forceinline__ void tn_hist_add(uint32_t *hist, uint32_t d, bool act)
+{
+    const uint32_t lane = hfx_lane();
+    uint64_t m = __ballot(act);
+    for (int it = 0; it < 4 && m; ++it) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, (int)leader);
+        const uint64_t same = __ballot(((m >> lane) & 1ull) && d == v);
+        if (lane == leader) atomicAdd(&hist[v], (uint32_t)__builtin_popcountll(same));
+        m &= ~same;
+    }
+    if ((m >> lane) & 1ull) atomicAdd(&hist[d], 1u);
+}
+
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
@@ -105,53 +123,92 @@ __global__ void __launch_bounds__(TN_THREADS)
     }
     const uint32_t T = min(N + margin, n);
 
-    /* 1. radix select of the T largest keys (MSB first, 8-bit digits) */
+    /* 1. the T largest keys (key desc, position asc).  Radix select over the
+     * key's two high bytes; then every key at or above that 16-bit prefix --
+     * M of them, known from the histograms -- is collected and ranked
+     * exactly.  With more than TN_MAXC of them (long runs of near-equal
+     * scores) the select goes on over the low bytes and the keys above the
+     * threshold, then the smallest positions equal to it, are collected. */
     uint32_t prefix = 0, pmask = 0;
-    if (tid == 0) sh_rem = T;
+    if (tid == 0) {
+        sh_rem = T;
+        sh_gt = 0;
+        sh_ext = 0;
+        sh_last = 0;
+    }
+    bool ranked = false;
     for (int shift = 24; shift >= 0; shift -= 8) {
         hist[tid] = 0;
         __syncthreads();
         for (uint32_t i = tid; i < n; i += TN_THREADS) {
             const uint32_t k = hfx_f2key(sc[i]);
-            if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+            tn_hist_add(hist, (k >> shift) & 255u, (k & pmask) == prefix);
         }
         __syncthreads();
         tn_select_digit(hist, &sh_digit, &sh_rem);
         prefix |= sh_digit << shift;
         pmask |= 255u << shift;
-    }
-    const uint32_t K = prefix;
-    const uint32_t need_eq = sh_rem; /* >= 1 */
-    const uint32_t n_gt = T - need_eq;
-    if (tid == 0) { sh_gt = 0; sh_ext = 0; sh_last = 0; }
-    __syncthreads();
-    for (uint32_t i = tid; i < n; i += TN_THREADS) {
-        if (hfx_f2key(sc[i]) > K) {
-            const uint32_t slot = atomicAdd(&sh_gt, 1u);
-            if (slot < n_gt) cp[slot] = i;
+        if (shift == 16) {
+            const uint32_t M = T - sh_rem + hist[sh_digit];
+            if (M <= TN_MAXC) {
+                /* (key, ~position) packed so that a larger value ranks first;
+                 * cfx holds them until the exact scores replace them */
+                uint64_t *kv = (uint64_t *)cfx;
+                for (uint32_t i = tid; i < n; i += TN_THREADS) {
+                    const uint32_t k = hfx_f2key(sc[i]);
+                    if ((k >> 16) >= (prefix >> 16))
+                        kv[atomicAdd(&sh_gt, 1u)] = (uint64_t)k << 32 | (0xffffffffu - i);
+                }
+                __syncthreads();
+                for (uint32_t a = tid; a < M; a += TN_THREADS) {
+                    const uint64_t va = kv[a];
+                    uint32_t rk = 0;
+                    for (uint32_t b = 0; b < M; ++b) rk += kv[b] > va ? 1u : 0u;
+                    const uint32_t pa = 0xffffffffu - (uint32_t)va;
+                    if (rk < T) cp[rk] = pa;
+                    if (rk == T - 1u) {
+                        sh_digit = (uint32_t)(va >> 32);
+                        sh_last = pa;
+                    }
+                }
+                __syncthreads();
+                ranked = true;
+                break;
+            }
         }
     }
-    /* keys equal to the threshold: the need_eq smallest positions */
-    uint32_t eq_base = 0;
-    for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
-        const uint32_t i = base + tid;
-        const bool f = i < n && hfx_f2key(sc[i]) == K;
-        const uint64_t bal = __ballot(f);
-        const uint32_t wr = (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-        for (uint32_t w = 0; w < TN_WAVES; ++w) {
-            if (w < wave) wpre += wcnt[w];
-            tot += wcnt[w];
+    const uint32_t K = ranked ? sh_digit : prefix;
+    if (!ranked) {
+        const uint32_t need_eq = sh_rem; /* >= 1 */
+        const uint32_t n_gt = T - need_eq;
+        for (uint32_t i = tid; i < n; i += TN_THREADS) {
+            if (hfx_f2key(sc[i]) > K) {
+                const uint32_t slot = atomicAdd(&sh_gt, 1u);
+                if (slot < n_gt) cp[slot] = i;
+            }
         }
-        const uint32_t r = eq_base + wpre + wr;
-        if (f && r < need_eq) {
-            cp[n_gt + r] = i;
-            if (r == need_eq - 1) sh_last = i;
+        /* keys equal to the threshold: the need_eq smallest positions */
+        uint32_t eq_base = 0;
+        for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
+            const uint32_t i = base + tid;
+            const bool f = i < n && hfx_f2key(sc[i]) == K;
+            const uint64_t bal = __ballot(f);
+            const uint32_t wr = (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
+            __syncthreads();
+            uint32_t wpre = 0, tot = 0;
+            for (uint32_t w = 0; w < TN_WAVES; ++w) {
+                if (w < wave) wpre += wcnt[w];
+                tot += wcnt[w];
+            }
+            const uint32_t r = eq_base + wpre + wr;
+            if (f && r < need_eq) {
+                cp[n_gt + r] = i;
+                if (r == need_eq - 1) sh_last = i;
+            }
+            eq_base += tot;
+            __syncthreads();
         }
-        eq_base += tot;
-        __syncthreads();
     }
 
     /* 2. exact re-score of the selection */

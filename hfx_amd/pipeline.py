@@ -218,6 +218,16 @@ class DeviceSpectra:
         L = lib()
         self.num_spec = n = len(pcr)
         self.x_stride = x_stride
+        # the kernel reads peaks off[s]..off[s+1] of every spectrum: check the
+        # layout here, where the host arrays are at hand
+        off_h = np.asarray(off)
+        if len(charge) != n or len(off_h) != n + 1:
+            raise ValueError(f"{n} spectra need {n} charges and {n + 1} offsets, "
+                             f"got {len(charge)} and {len(off_h)}")
+        if n and (off_h[0] != 0 or np.any(np.diff(off_h.astype(np.int64)) < 0)
+                  or int(off_h[-1]) > min(len(mz), len(inten))):
+            raise ValueError("peak offsets must start at 0, not decrease and stay within "
+                             "the m/z and intensity arrays")
         self.t_pcr = dev(np.asarray(pcr, np.float32))
         self.t_charge = dev(np.asarray(charge, np.float32))
         self.t_mz = dev(np.asarray(mz, np.float32))

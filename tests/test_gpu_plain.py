@@ -302,7 +302,7 @@ def test_tie_heavy_window_takes_exact_route(orc):
     n0 = int(one.off[1])
     mz = np.concatenate([one.mz[:n0], sp.mz[sp.off[1]:]])
     inten = np.concatenate([one.inten[:n0], sp.inten[sp.off[1]:]])
-    off = np.concatenate([[0], n0 + (sp.off[2:] - sp.off[1])]).astype(np.uint32)
+    off = np.concatenate([[0], n0 + (sp.off[1:] - sp.off[1])]).astype(np.uint32)
     pcr = np.concatenate([one.pcr[:1], sp.pcr[1:]]).astype(np.float32)
     ch = np.concatenate([one.charge[:1], sp.charge[1:]]).astype(np.float32)
 

@@ -94,3 +94,17 @@ def test_pipeline_refuses_without_gpu():
         pytest.skip("GPU present")
     with pytest.raises(RuntimeError):
         pipeline.empty(16)
+
+
+@pytest.mark.parametrize("case", ["short", "start", "decreasing", "beyond"])
+def test_spectra_offsets_checked_before_upload(case):
+    """Malformed peak offsets are refused on the host: the preprocessing
+    kernel reads peaks off[s]..off[s+1] of every spectrum."""
+    mz = np.linspace(200.0, 900.0, 30, dtype=np.float32)
+    it = np.ones(30, np.float32)
+    pcr = np.array([500.0, 600.0, 700.0], np.float32)
+    ch = np.array([2.0, 2.0, 3.0], np.float32)
+    off = {"short": [0, 10, 20], "start": [1, 10, 20, 30], "decreasing": [0, 20, 10, 30],
+           "beyond": [0, 10, 20, 31]}[case]
+    with pytest.raises(ValueError):
+        pipeline.DeviceSpectra(None, pcr, ch, mz, it, np.array(off, np.uint32))
