@@ -123,6 +123,22 @@ __global__ void __launch_bounds__(TN_THREADS)
     }
     const uint32_t T = min(N + margin, n);
 
+    /* every sweep below visits this thread's keys through for_keys(f), f(i,
+     * key), eight loads in flight at a time (clamped into the window, so
+     * that none is conditional); i >= n past the window, where all lanes of
+     * a wave still take part (for the ballots) */
+    auto for_keys = [&](auto &&f) {
+        constexpr uint32_t U = 8;
+        for (uint32_t base = 0; base < n; base += U * TN_THREADS) {
+            uint32_t kk[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                kk[u] = hfx_f2key(sc[min(base + u * TN_THREADS + tid, n - 1u)]);
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) f(base + u * TN_THREADS + tid, kk[u]);
+        }
+    };
+
     /* 1. the T largest keys (key desc, position asc).  Radix select over the
      * key's two high bytes; then every key at or above that 16-bit prefix --
      * M of them, known from the histograms -- is collected and ranked
@@ -140,10 +156,9 @@ __global__ void __launch_bounds__(TN_THREADS)
     for (int shift = 24; shift >= 0; shift -= 8) {
         hist[tid] = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += TN_THREADS) {
-            const uint32_t k = hfx_f2key(sc[i]);
-            tn_hist_add(hist, (k >> shift) & 255u, (k & pmask) == prefix);
-        }
+        for_keys([&](uint32_t i, uint32_t k) {
+            tn_hist_add(hist, (k >> shift) & 255u, i < n && (k & pmask) == prefix);
+        });
         __syncthreads();
         tn_select_digit(hist, &sh_digit, &sh_rem);
         prefix |= sh_digit << shift;
@@ -154,11 +169,10 @@ __global__ void __launch_bounds__(TN_THREADS)
                 /* (key, ~position) packed so that a larger value ranks first;
                  * cfx holds them until the exact scores replace them */
                 uint64_t *kv = (uint64_t *)cfx;
-                for (uint32_t i = tid; i < n; i += TN_THREADS) {
-                    const uint32_t k = hfx_f2key(sc[i]);
-                    if ((k >> 16) >= (prefix >> 16))
+                for_keys([&](uint32_t i, uint32_t k) {
+                    if (i < n && (k >> 16) >= (prefix >> 16))
                         kv[atomicAdd(&sh_gt, 1u)] = (uint64_t)k << 32 | (0xffffffffu - i);
-                }
+                });
                 __syncthreads();
                 for (uint32_t a = tid; a < M; a += TN_THREADS) {
                     const uint64_t va = kv[a];
@@ -181,12 +195,12 @@ __global__ void __launch_bounds__(TN_THREADS)
     if (!ranked) {
         const uint32_t need_eq = sh_rem; /* >= 1 */
         const uint32_t n_gt = T - need_eq;
-        for (uint32_t i = tid; i < n; i += TN_THREADS) {
-            if (hfx_f2key(sc[i]) > K) {
+        for_keys([&](uint32_t i, uint32_t k) {
+            if (i < n && k > K) {
                 const uint32_t slot = atomicAdd(&sh_gt, 1u);
                 if (slot < n_gt) cp[slot] = i;
             }
-        }
+        });
         /* keys equal to the threshold: the need_eq smallest positions */
         uint32_t eq_base = 0;
         for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
@@ -249,15 +263,13 @@ __global__ void __launch_bounds__(TN_THREADS)
         const double bound = rel * 50.0 * (double)xa * 1.001 + 1e-6;
         const double tau = (double)sh_exN * (1.0 / HFX_FX_SCALE) - bound;
         const uint32_t last = sh_last;
-        for (uint32_t i = tid; i < n; i += TN_THREADS) {
-            const float f = sc[i];
-            const uint32_t k = hfx_f2key(f);
+        for_keys([&](uint32_t i, uint32_t k) {
             const bool selected = k > K || (k == K && i <= last);
-            if (!selected && (double)f >= tau) {
+            if (i < n && !selected && (double)hfx_key2f(k) >= tau) {
                 const uint32_t slot = atomicAdd(&sh_ext, 1u);
                 if (slot < TN_EXT) cp[T + slot] = i;
             }
-        }
+        });
         __syncthreads();
         const uint32_t ne = sh_ext;
         if (ne > TN_EXT) {
