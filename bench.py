@@ -147,7 +147,7 @@ def calib_read(mib: int):
                       "launches": 3}), flush=True)
 
 
-def bench_plain(env, topn, steps, warmup, margin=16):
+def bench_plain(env, topn, steps, warmup, margin=0):
     from hfx_amd import pipeline
 
     srch = pipeline.PlainSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn,
@@ -178,31 +178,37 @@ def bench_plain(env, topn, steps, warmup, margin=16):
 
 
 def dense_flops(env, res) -> tuple[float, float]:
-    """(executed, useful) MFMA FLOPs of one dense pass: blocks of eight
+    """(executed, useful) MFMA FLOPs of one dense pass.  Blocks of eight
     spectra in (fragment charge, precursor mass) order; per block and charge
-    level present, ceil(union window / 16) row tiles x 16 columns x K bins x 2
-    (K = longest spectrum of the block rounded up to 128); useful = 4 len_s
-    per scored pair (hi and lo columns)."""
+    level present, the union window in tiles of 16 candidates; a tile is
+    contracted over kq bins, kq = min(K, heaviest candidate's residual + 28
+    rounded up to 128), K = the block's longest spectrum rounded up to 128:
+    16 rows x 16 columns x kq x 2.  Useful = 4 len_s per scored pair (hi and
+    lo columns over the spectrum's own length)."""
     dsp = env["dsp"]
     z = dsp.zmax.astype(np.int64)
     z = np.where(z <= 4, z, 0)
     lens = np.minimum(dsp.lengths().astype(np.int64), 2176)
     order = np.lexsort((dsp.mass, z))
     b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
+    r_sorted = env["r"][env["db"].idx_sorted()].astype(np.float32)
     executed = 0.0
     for i in range(0, len(order), 8):
         blk = order[i:i + 8]
         blk = blk[(z[blk] > 0) & (n[blk] > 0)]
         if len(blk) == 0:
             continue
-        rows = int((b[blk] + n[blk]).max() - b[blk].min())
+        u0, u1 = int(b[blk].min()), int((b[blk] + n[blk]).max())
         kb = min(2176, (int(lens[blk].max()) + 127) // 128 * 128)
-        executed += len(set(z[blk].tolist())) * ((rows + 15) // 16) * 16 * 16 * kb * 2.0
+        top = np.maximum.reduceat(r_sorted[u0:u1], np.arange(0, u1 - u0, 16))
+        kneed = (top + np.float32(24.0)).astype(np.int64) + 4
+        kq = np.minimum(kb, (kneed + 127) // 128 * 128)
+        executed += len(set(z[blk].tolist())) * float(kq.sum()) * 16 * 16 * 2.0
     useful = float(np.sum(4.0 * lens * n * (z > 0)))
     return executed, useful
 
 
-def bench_batched(env, topn, steps, warmup, precision=0, margin=16):
+def bench_batched(env, topn, steps, warmup, precision=0, margin=0):
     from hfx_amd import pipeline
 
     bs = pipeline.BatchSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn,
@@ -233,7 +239,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tryptic-20k-10k")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--margin", type=int, default=16,
+    ap.add_argument("--margin", type=int, default=0,
                     help="extra candidates re-scored exactly beyond the top N")
     ap.add_argument("--path", choices=("all", "plain", "batched", "dense"), default="all")
     ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",

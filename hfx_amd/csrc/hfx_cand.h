@@ -20,7 +20,18 @@ __device__ __forceinline__ uint32_t hfx_stage_spectrum(float *xs, float *mtab, c
 {
     const uint32_t len = min(sp.d_len[s], min(sp.x_stride, (uint32_t)HFX_MAX_LEN));
     const float *xg = sp.d_x + (size_t)s * sp.x_stride;
-    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) xs[i] = xg[i];
+    /* eight loads in flight per thread (clamped into the row) before the stores */
+    constexpr uint32_t U = 8;
+    for (uint32_t base = 0; base < len; base += U * blockDim.x) {
+        float v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) v[u] = xg[min(base + u * blockDim.x + threadIdx.x, len - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t i = base + u * blockDim.x + threadIdx.x;
+            if (i < len) xs[i] = v[u];
+        }
+    }
     if (threadIdx.x < 32) mtab[threadIdx.x] = threadIdx.x < 26 ? db.d_mass[threadIdx.x] : 0.0f;
     return len;
 }

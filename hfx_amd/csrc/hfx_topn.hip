@@ -427,15 +427,26 @@ __global__ void __launch_bounds__(TN_THREADS)
         return;
     }
     auto key = [&](uint32_t i) { return (uint64_t)fx[i] ^ 0x8000000000000000ull; };
+    /* f(i, key) over this thread's keys, eight loads in flight (clamped into
+     * the window); i >= n past it, where all lanes still take part */
+    auto for_keys = [&](auto &&f) {
+        constexpr uint32_t U = 8;
+        for (uint32_t base = 0; base < n; base += U * TN_THREADS) {
+            uint64_t kk[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) kk[u] = key(min(base + u * TN_THREADS + tid, n - 1u));
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) f(base + u * TN_THREADS + tid, kk[u]);
+        }
+    };
     uint64_t prefix = 0, pmask = 0;
     if (tid == 0) sh_rem = want;
     for (int shift = 56; shift >= 0; shift -= 8) {
         hist[tid] = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < n; i += TN_THREADS) {
-            const uint64_t k = key(i);
-            if ((k & pmask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
-        }
+        for_keys([&](uint32_t i, uint64_t k) {
+            tn_hist_add(hist, (uint32_t)(k >> shift) & 255u, i < n && (k & pmask) == prefix);
+        });
         __syncthreads();
         tn_select_digit(hist, &sh_digit, &sh_rem);
         prefix |= (uint64_t)sh_digit << shift;
@@ -445,12 +456,15 @@ __global__ void __launch_bounds__(TN_THREADS)
     const uint32_t need_eq = sh_rem, n_gt = want - need_eq;
     if (tid == 0) sh_gt = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < n; i += TN_THREADS) {
-        if (key(i) > K) {
+    for_keys([&](uint32_t i, uint64_t k) {
+        if (i < n && k > K) {
             const uint32_t slot = atomicAdd(&sh_gt, 1u);
-            if (slot < n_gt) { sel_p[slot] = i; sel_f[slot] = fx[i]; }
+            if (slot < n_gt) {
+                sel_p[slot] = i;
+                sel_f[slot] = (int64_t)(k ^ 0x8000000000000000ull);
+            }
         }
-    }
+    });
     uint32_t eq_base = 0;
     for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
         const uint32_t i = base + tid;

@@ -281,7 +281,7 @@ class PlainSearch:
     """searchWithoutMods for one batch; buffers reused across calls."""
 
     def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, eps: float, topn: int,
-                 margin: int = 16):
+                 margin: int = 0):
         self.ctx, self.db, self.sp = ctx, db, sp
         self.eps, self.topn, self.margin = float(eps), int(topn), int(margin)
         S = sp.num_spec
@@ -424,7 +424,7 @@ class BatchSearch:
     BF16X2_DENSE = 1
 
     def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, eps: float, topn: int,
-                 margin: int = 16, stream: Stream | None = None):
+                 margin: int = 0, stream: Stream | None = None):
         L = lib()
         self.ctx, self.db, self.sp, self.topn = ctx, db, sp, int(topn)
         self.stream = stream or Stream()
@@ -562,7 +562,7 @@ class ModSearch:
     prepareGenMod, genModCands) and the sparse modified-peptide scorer."""
 
     def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, mi: ModInfo, eps: float,
-                 topn: int, margin: int = 16):
+                 topn: int, margin: int = 0):
         self.ctx, self.db, self.sp, self.mi = ctx, db, sp, mi
         self.eps, self.topn, self.margin = float(eps), int(topn), int(margin)
         self.t_ma = dev(np.frombuffer(mi.ma, np.uint8))
