@@ -33,6 +33,17 @@
 #define TN_EXT 256
 #define TN_MAXC (TN_MAXT + TN_EXT)
 
+/* The staged spectrum and the waves' maps live in dynamic LDS sized by the
+ * spectra's row stride: bins of a row, rounded up to whole map words */
+__host__ __device__ inline uint32_t tn_xcap(uint32_t x_stride)
+{
+    return ((x_stride < HFX_MAX_LEN ? x_stride : HFX_MAX_LEN) + 7u) & ~7u;
+}
+__host__ __device__ inline size_t tn_dyn_bytes(uint32_t x_stride)
+{
+    return (size_t)tn_xcap(x_stride) * 4 + (size_t)TN_WAVES * (tn_xcap(x_stride) / 8) * 4;
+}
+
 __device__ __forceinline__ bool before_fx(int64_t fa, uint32_t pa, int64_t fb, uint32_t pb)
 {
     return fa > fb || (fa == fb && pa < pb);
@@ -99,8 +110,10 @@ __global__ void __launch_bounds__(TN_THREADS)
            uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
            hfx_spectra sp, float rel0, float rel1, double *tau_out)
 {
-    __shared__ float xs[HFX_MAX_LEN];
-    __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
+    extern __shared__ __align__(16) unsigned char tn_dyn[];
+    const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
+    float *xs = (float *)tn_dyn;
+    uint32_t *maps = (uint32_t *)(tn_dyn + (size_t)xcap * 4);
     __shared__ float mtab[32];
     __shared__ uint32_t hist[256];
     __shared__ uint32_t cp[TN_MAXC];
@@ -227,8 +240,8 @@ __global__ void __launch_bounds__(TN_THREADS)
 
     /* 2. exact re-score of the selection */
     const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
-    uint32_t *map = maps[wave];
-    for (uint32_t w = lane; w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
+    uint32_t *map = maps + wave * mwords;
+    for (uint32_t w = lane; w < mwords; w += HFX_WAVE) map[w] = 0;
     __syncthreads();
     const uint32_t zeff = c.zeff(sp.d_zmax[s]);
     for (uint32_t i = wave; i < T; i += TN_WAVES) {
@@ -360,15 +373,17 @@ __global__ void __launch_bounds__(TN_THREADS)
     k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, uint32_t *counter, Cands c,
                   hfx_spectra sp, const float *scores, const double *tau)
 {
-    __shared__ float xs[HFX_MAX_LEN];
-    __shared__ uint32_t maps[TN_WAVES][HFX_MAP_WORDS];
+    extern __shared__ __align__(16) unsigned char tn_dyn[];
+    const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
+    float *xs = (float *)tn_dyn;
+    uint32_t *maps = (uint32_t *)(tn_dyn + (size_t)xcap * 4);
     __shared__ float mtab[32];
     __shared__ uint32_t sh_t;
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
     const uint32_t S = sp.num_spec;
     const uint64_t ntiles = tile_off[S];
-    uint32_t *map = maps[wave];
-    for (uint32_t w = lane; w < HFX_MAP_WORDS; w += HFX_WAVE) map[w] = 0;
+    uint32_t *map = maps + wave * mwords;
+    for (uint32_t w = lane; w < mwords; w += HFX_WAVE) map[w] = 0;
     uint32_t cur = 0xffffffffu, len = 0;
     for (;;) {
         if (tid == 0) sh_t = atomicAdd(counter, 1u);
@@ -545,9 +560,9 @@ static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                        d_flag, c, S);
     if ((rc = hfx_scan_offsets(ctx, x.tile_off, x.tiles, S, (void *)st)) != HFX_OK) return rc;
     hipLaunchKernelGGL(k_tx_reset, dim3(1), dim3(1), 0, st, x.counter);
-    hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 4),
-                       dim3(TN_THREADS), 0, st, x.fxbuf, x.tile_off, x.counter, c, *sp, d_scores,
-                       x.tau);
+    hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 8),
+                       dim3(TN_THREADS), tn_dyn_bytes(sp->x_stride), st, x.fxbuf, x.tile_off,
+                       x.counter, c, *sp, d_scores, x.tau);
     hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(S), dim3(TN_THREADS), 0, st, d_top_pos, d_top_exact,
                        d_top_f32, d_top_cnt, N, d_flag, x.fxbuf, d_scores, c);
     HFX_CHECK_LAUNCH();
@@ -572,9 +587,9 @@ static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     int rc;
     if ((rc = tx_scratch(ctx, scratch, sp->num_spec, total, &p))) return rc;
     const TxScratch x = tx_carve(p, sp->num_spec, total);
-    hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS), 0, st, d_top_pos,
-                       d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin, d_scores, c, *sp,
-                       rel0, rel1, x.tau);
+    hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS),
+                       tn_dyn_bytes(sp->x_stride), st, d_top_pos, d_top_exact, d_top_f32,
+                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, x.tau);
     HFX_CHECK_LAUNCH();
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
                        total, sp, p, st);
