@@ -36,6 +36,15 @@ __device__ __forceinline__ uint32_t hfx_stage_spectrum(float *xs, float *mtab, c
     return len;
 }
 
+__device__ __forceinline__ uint32_t hfx_take(uint32_t v, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ float hfx_take(float v, uint32_t l)
+{
+    return __uint_as_float(hfx_take(__float_as_uint(v), l));
+}
+
 /* Max fragment charge actually generated: the reference launches nothing for
  * max_charge > 4 (ion_series.cu:334-342), leaving zero rows. */
 __device__ __forceinline__ uint32_t hfx_zeff(uint32_t zmax)
@@ -53,14 +62,32 @@ struct HfxPlainCands {
     __device__ __forceinline__ uint64_t base(uint32_t s) const { return off[s]; }
     __device__ __forceinline__ uint32_t zeff(uint32_t zmax) const { return hfx_zeff(zmax); }
 
+    /* what the peaks of candidate k of spectrum s need from global memory
+     * (prep: one lane's fetch; take: lane l's, to the whole wave) */
+    struct Prep {
+        uint32_t tc, n;
+        float r;
+    };
+    __device__ __forceinline__ Prep prep(uint32_t s, uint32_t k) const
+    {
+        const uint32_t pep = db.d_idx_sorted[begin[s] + k];
+        const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
+        return Prep{tc, tn > tc ? tn - tc : 0u, db.d_r[pep]};
+    }
+    __device__ __forceinline__ static Prep take(const Prep &p, uint32_t l)
+    {
+        return Prep{hfx_take(p.tc, l), hfx_take(p.n, l), hfx_take(p.r, l)};
+    }
+    __device__ __forceinline__ void peaks_prep(uint32_t *map, const float *mtab, const Prep &p,
+                                               uint32_t len, uint32_t zeff) const
+    {
+        hfx_wave_peaks(map, p.n, p.r, zeff, len, HfxPlainMass{db.d_ions + p.tc, mtab});
+    }
     /* peaks of candidate k of spectrum s into the wave's map */
     __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
                                           uint32_t len, uint32_t zeff) const
     {
-        const uint32_t pep = db.d_idx_sorted[begin[s] + k];
-        const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
-        const uint32_t n = tn > tc ? tn - tc : 0;
-        hfx_wave_peaks(map, n, db.d_r[pep], zeff, len, HfxPlainMass{db.d_ions + tc, mtab});
+        peaks_prep(map, mtab, prep(s, k), len, zeff);
     }
 };
 
@@ -77,17 +104,34 @@ struct HfxModCands {
     /* addModIons dispatches max charges 1..10 (mion_series.cu:335-556) */
     __device__ __forceinline__ uint32_t zeff(uint32_t zmax) const { return zmax <= 10 ? zmax : 0u; }
 
-    __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
-                                          uint32_t len, uint32_t zeff) const
+    struct Prep {
+        uint32_t tc, n, mod, uoff;
+        float r;
+    };
+    __device__ __forceinline__ Prep prep(uint32_t s, uint32_t k) const
     {
         const uint64_t i = off[s] + k;
         const uint32_t pep = ms.d_mpep_pep_idx[i], mod = ms.d_mpep_mod_idx[i];
         const uint32_t tc = db.d_tc[pep], tn = db.d_tn[pep];
-        const uint32_t n = tn > tc ? tn - tc : 0;
+        return Prep{tc, tn > tc ? tn - tc : 0u, mod, ms.d_mpep_unrank_off[i],
+                    db.d_r[pep] + ms.d_mod_delta[mod]};
+    }
+    __device__ __forceinline__ static Prep take(const Prep &p, uint32_t l)
+    {
+        return Prep{hfx_take(p.tc, l), hfx_take(p.n, l), hfx_take(p.mod, l), hfx_take(p.uoff, l),
+                    hfx_take(p.r, l)};
+    }
+    __device__ __forceinline__ void peaks_prep(uint32_t *map, const float *mtab, const Prep &p,
+                                               uint32_t len, uint32_t zeff) const
+    {
         const uint32_t na = min(ms.num_ma, (uint32_t)HFX_MAX_MA);
-        HfxModMass mm{db.d_ions + tc, mtab, ms.d_mod_ma_count + (size_t)mod * ms.num_ma,
-                      ms.d_unrank + ms.d_mpep_unrank_off[i], ms.d_ma_mass, na, {0, 0, 0, 0, 0, 0, 0}};
-        const float r = db.d_r[pep] + ms.d_mod_delta[mod];
-        hfx_wave_peaks(map, n, r, zeff, len, mm);
+        HfxModMass mm{db.d_ions + p.tc, mtab, ms.d_mod_ma_count + (size_t)p.mod * ms.num_ma,
+                      ms.d_unrank + p.uoff, ms.d_ma_mass, na, {0, 0, 0, 0, 0, 0, 0}};
+        hfx_wave_peaks(map, p.n, p.r, zeff, len, mm);
+    }
+    __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
+                                          uint32_t len, uint32_t zeff) const
+    {
+        peaks_prep(map, mtab, prep(s, k), len, zeff);
     }
 };

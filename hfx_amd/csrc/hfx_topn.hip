@@ -244,8 +244,12 @@ __global__ void __launch_bounds__(TN_THREADS)
     for (uint32_t w = lane; w < mwords; w += HFX_WAVE) map[w] = 0;
     __syncthreads();
     const uint32_t zeff = c.zeff(sp.d_zmax[s]);
-    for (uint32_t i = wave; i < T; i += TN_WAVES) {
-        c.peaks(map, mtab, s, cp[i], len, zeff);
+    /* the wave's candidates i = wave + 4 j: lane j fetches candidate i's
+     * metadata, all of them in one round */
+    typename Cands::Prep mine{};
+    if (wave + TN_WAVES * lane < T) mine = c.prep(s, cp[wave + TN_WAVES * lane]);
+    for (uint32_t i = wave, j = 0; i < T; i += TN_WAVES, ++j) {
+        c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
         hfx_wave_sync();
         const int64_t fx = hfx_wave_dot_fx(map, xs, len);
         hfx_wave_sync();
@@ -290,8 +294,12 @@ __global__ void __launch_bounds__(TN_THREADS)
             fl = 1;
             if (tid == 0) tau_out[s] = tau;
         } else {
-            for (uint32_t i = T + wave; i < T + ne; i += TN_WAVES) {
-                c.peaks(map, mtab, s, cp[i], len, zeff);
+            /* at most TN_EXT / 4 = 64 per wave: one lane each */
+            typename Cands::Prep ext{};
+            if (T + wave + TN_WAVES * lane < T + ne)
+                ext = c.prep(s, cp[T + wave + TN_WAVES * lane]);
+            for (uint32_t i = T + wave, j = 0; i < T + ne; i += TN_WAVES, ++j) {
+                c.peaks_prep(map, mtab, Cands::take(ext, j), len, zeff);
                 hfx_wave_sync();
                 const int64_t fx = hfx_wave_dot_fx(map, xs, len);
                 hfx_wave_sync();
@@ -332,6 +340,11 @@ __global__ void __launch_bounds__(TN_THREADS)
 
 #define TX_MAXN 64
 #define TX_CHUNK 64
+/* a wave fetches the metadata of its share of a candidate list one lane
+ * per candidate */
+static_assert(TN_MAXT / TN_WAVES <= HFX_WAVE && TN_EXT / TN_WAVES <= HFX_WAVE &&
+                  TX_CHUNK / TN_WAVES <= HFX_WAVE,
+              "one lane per candidate of a wave's share");
 
 template <class Cands>
 __global__ void k_flag_tiles(uint32_t *tiles, const uint32_t *flag, Cands c, uint32_t S)
@@ -400,7 +413,11 @@ __global__ void __launch_bounds__(TN_THREADS)
         const uint32_t cnt = min((uint32_t)TX_CHUNK, c.count(s) - first);
         const uint32_t zeff = c.zeff(sp.d_zmax[s]);
         const double ts = tau[s];
-        for (uint32_t i = wave; i < cnt; i += TN_WAVES) {
+        /* the wave's candidates i = wave + 4 j (j < 16): lane j fetches i's
+         * metadata */
+        typename Cands::Prep mine{};
+        if (wave + TN_WAVES * lane < cnt) mine = c.prep(s, first + wave + TN_WAVES * lane);
+        for (uint32_t i = wave, j = 0; i < cnt; i += TN_WAVES, ++j) {
             /* a candidate below tau cannot reach the top N: its exact score
              * is below the N-th exact score (it keeps the smallest key) */
             const uint64_t at = c.base(s) + first + i;
@@ -408,7 +425,7 @@ __global__ void __launch_bounds__(TN_THREADS)
                 if (lane == 0) fxbuf[at] = INT64_MIN;
                 continue;
             }
-            c.peaks(map, mtab, s, first + i, len, zeff);
+            c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
             hfx_wave_sync();
             const int64_t fx = hfx_wave_dot_fx(map, xs, len);
             hfx_wave_sync();
