@@ -144,15 +144,17 @@ __device__ __forceinline__ float dn_dppf(float v)
     return __uint_as_float(dn_dpp<CTRL>(__float_as_uint(v), __float_as_uint(v)));
 }
 
-/* maximum over each row of 16 lanes, left in every lane of the row (the
- * rotations by 8, 4, 2, 1 reach all 16 lanes whichever way they turn) */
-__device__ __forceinline__ float dn_row_max(float v)
+/* maximum over each row of 16 lanes of non-negative floats (ordered as their
+ * bit patterns), left in every lane of the row (the rotations by 8, 4, 2, 1
+ * reach all 16 lanes whichever way they turn) */
+__device__ __forceinline__ float dn_row_max(float f)
 {
-    v = fmaxf(v, dn_dppf<DN_ROR(8)>(v));
-    v = fmaxf(v, dn_dppf<DN_ROR(4)>(v));
-    v = fmaxf(v, dn_dppf<DN_ROR(2)>(v));
-    v = fmaxf(v, dn_dppf<DN_ROR(1)>(v));
-    return v;
+    uint32_t v = __float_as_uint(f);
+    v = max(v, dn_dpp<DN_ROR(8)>(v, v));
+    v = max(v, dn_dpp<DN_ROR(4)>(v, v));
+    v = max(v, dn_dpp<DN_ROR(2)>(v, v));
+    v = max(v, dn_dpp<DN_ROR(1)>(v, v));
+    return __uint_as_float(v);
 }
 
 /* ------------------------------------------------------------------ */
@@ -513,13 +515,13 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                    (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32;
         };
         /* lane l takes positions l, l + 64, ...; past the half's count it
-         * reloads the last one (or position 0) */
+         * reloads the last one (or position 0); a wave-uniform base and a
+         * 32-bit index per lane */
         auto load_pos = [&](hfx_dn_pos(&pv)[DN_PRE], uint64_t L0, uint32_t np) {
+            const hfx_dn_pos *base = np ? a.lad.pos + L0 : a.lad.pos;
+            const uint32_t last = np ? np - 1u : 0u;
 #pragma unroll
-            for (uint32_t k = 0; k < DN_PRE; ++k) {
-                const uint32_t i = lane + k * HFX_WAVE;
-                pv[k] = a.lad.pos[i < np ? L0 + i : (np ? L0 + np - 1u : 0ull)];
-            }
+            for (uint32_t k = 0; k < DN_PRE; ++k) pv[k] = base[min(lane + k * HFX_WAVE, last)];
         };
         DnPipe X, Y;
         {
