@@ -491,7 +491,9 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     if ((rc = dalloc(&b->tx_scratch, hfx_topn_scratch_bytes(S, b->total)))) return rc;
     if (sp->x_stride <= hfx_dense_max_len()) {
         if ((rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(S)))) return rc;
-        /* b ladders of every candidate of the plan, built once */
+        /* the spectra's hi/lo BF16 rows and the b ladders of every candidate
+         * of the plan, built once */
+        if ((rc = hfx_dense_split(sp, b->dn_scratch, stream))) return rc;
         if ((rc = hfx_dense_ladder_build(b->ctx, db, b->p0, b->P, 1, &b->dn_lad, b->dn_mem,
                                          stream)))
             return rc;
@@ -571,7 +573,7 @@ static int bt_scores(hfx_batch *b, hipStream_t st)
     if (b->precision == HFX_BATCH_BF16X2_DENSE) {
         if (b->total == 0) return HFX_OK;
         return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order,
-                                &b->dn_lad, &b->sp, b->CH, b->dn_scratch, (void *)st);
+                                &b->dn_lad, &b->sp, b->CH, b->dn_scratch, 1, (void *)st);
     }
     if ((rc = bt_tiles(b, st))) return rc;
     return bt_score(b, st);

@@ -747,9 +747,22 @@ size_t hfx_dense_scratch_bytes(uint32_t S)
 
 uint32_t hfx_dense_max_len(void) { return DN_MAXK; }
 
+int hfx_dense_split(const hfx_spectra *sp, void *scratch, void *stream)
+{
+    HFX_REQUIRE(sp && scratch, HFX_ERR_ARG, "dense split: null argument");
+    HFX_REQUIRE(sp->x_stride <= DN_MAXK, HFX_ERR_ARG,
+                "dense XCorr: spectra of up to %d bins (x_stride %u)", DN_MAXK, sp->x_stride);
+    if (sp->num_spec)
+        hipLaunchKernelGGL(k_dn_split, dim3(sp->num_spec), dim3(256), 0, hfx_stream(stream),
+                           (uint16_t *)scratch, *sp);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
                      const uint32_t *d_num, const uint32_t *d_order, const hfx_dn_ladder *lad,
-                     const hfx_spectra *sp, uint32_t CH, void *scratch, void *stream)
+                     const hfx_spectra *sp, uint32_t CH, void *scratch, int presplit,
+                     void *stream)
 {
     HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && lad && sp, HFX_ERR_ARG,
                 "dense XCorr: null argument");
@@ -771,7 +784,7 @@ int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const
     uint64_t *tile_off = (uint64_t *)(((uintptr_t)(ntiles + B) + 7) & ~(uintptr_t)7);
     uint32_t *counter = (uint32_t *)(tile_off + B + 1);
     float *dummy = (float *)(((uintptr_t)(counter + 1) + 255) & ~(uintptr_t)255);
-    hipLaunchKernelGGL(k_dn_split, dim3(S), dim3(256), 0, st, xb, *sp);
+    if (!presplit) hipLaunchKernelGGL(k_dn_split, dim3(S), dim3(256), 0, st, xb, *sp);
     hipLaunchKernelGGL(k_dn_blocks, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend, binfo,
                        ntiles, d_order, d_begin, d_num, *sp, B, CH);
     if ((rc = hfx_scan_offsets(ctx, tile_off, ntiles, B, (void *)st))) return rc;
@@ -802,5 +815,5 @@ extern "C" int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint6
     if ((rc = hfx_dense_ladder_build(ctx, db, 0, db->num_pep, 0, &lad, nullptr, stream)))
         return rc;
     return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, &lad, sp, 256, nullptr,
-                            stream);
+                            0, stream);
 }
