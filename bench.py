@@ -177,10 +177,11 @@ def bench_plain(env, topn, steps, warmup, margin=0):
     return res, elapsed, kern_ms, flagged
 
 
-def dense_flops(env, res) -> tuple[float, float]:
-    """(executed, useful) MFMA FLOPs of one dense pass.  Blocks of eight
-    spectra in (fragment charge, precursor mass) order; per block and charge
-    level present, the union window in tiles of 16 candidates; a tile is
+def dense_flops(env, res, bstart) -> tuple[float, float]:
+    """(executed, useful) MFMA FLOPs of one dense pass.  The plan's blocks
+    (bstart: block starts in (fragment charge, precursor mass) order, at most
+    eight spectra each); per block and charge level present, the union
+    window in tiles of 16 candidates; a tile is
     contracted over kq bins, kq = min(K, heaviest candidate's residual + 28
     rounded up to 128), K = the block's longest spectrum rounded up to 128:
     16 rows x 16 columns x kq x 2.  Useful = 4 len_s per scored pair (hi and
@@ -193,8 +194,8 @@ def dense_flops(env, res) -> tuple[float, float]:
     b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
     r_sorted = env["r"][env["db"].idx_sorted()].astype(np.float32)
     executed = 0.0
-    for i in range(0, len(order), 8):
-        blk = order[i:i + 8]
+    for i0, i1 in zip(bstart[:-1], bstart[1:]):
+        blk = order[int(i0):int(i1)]
         blk = blk[(z[blk] > 0) & (n[blk] > 0)]
         if len(blk) == 0:
             continue
@@ -229,7 +230,10 @@ def bench_batched(env, topn, steps, warmup, precision=0, margin=0):
     bs.stream.sync()
     elapsed = time.perf_counter() - t0
     res = bs.results()
-    return res, elapsed, phases, cfg, bs.info()
+    info = bs.info()
+    if precision:
+        info["dense_blocks"] = bs.dense_blocks()
+    return res, elapsed, phases, cfg, info
 
 
 def main():
@@ -289,12 +293,13 @@ def main():
         dense_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 1,
                                                          args.margin)
         sc = int(dense_res.off[-1])
-        ex, useful = dense_flops(env, dense_res)
+        ex, useful = dense_flops(env, dense_res, info["dense_blocks"])
         paths["batched_bf16x2_dense"] = dict(
             value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
             kernel="k_dense", kernel_ms=phases["score"], phases_ms=phases,
             mfma_flops_executed=ex, mfma_flops_useful=useful,
-            config=dict(chunk=cfg[1], threads=cfg[2], spectra_per_block=8),
+            config=dict(chunk=cfg[1], threads=cfg[2], spectra_per_block_max=8,
+                        blocks=int(len(info["dense_blocks"]) - 1)),
             graph="captured once, replayed per step",
             flagged_spectra=int(len(dense_res.flagged)))
         log(f"batched bf16x2: {paths['batched_bf16x2_dense']['value']:.3e} scores/s {phases}")

@@ -185,6 +185,7 @@ _SIGS = {
     "hfx_batch_set_precision": (_I, [_P, _U32]),
     "hfx_batch_time": (_I, [_P, _P, _P]),
     "hfx_batch_info": (_I, [_P, _P]),
+    "hfx_batch_dense_blocks": (_I, [_P, _P, _U32, _P]),
     "hfx_batch_outputs": (_I, [_P, _P]),
 }
 

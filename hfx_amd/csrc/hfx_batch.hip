@@ -336,6 +336,8 @@ struct hfx_batch {
     uint8_t *dn_scratch; /* dense: hi/lo spectra, blocks (null if unsupported) */
     void *dn_mem[2];     /* dense: the ladders of the plan's windows */
     hfx_dn_ladder dn_lad;
+    uint32_t *dn_bstart; /* dense: [DB + 1] block starts in order */
+    uint32_t DB;
     hipGraph_t graph;
     hipGraphExec_t exec;
 };
@@ -345,7 +347,8 @@ static void bt_free(hfx_batch *b)
     void *ps[] = {b->begin,    b->num,     b->order,  b->ubeg,      b->uend,    b->ntiles,
                   b->caps,     b->cnt,     b->counter, b->score_off, b->tile_off, b->ent_off,
                   b->ent,      b->scores,  b->top_pos, b->top_cnt,   b->flag,     b->top_exact,
-                  b->top_f32,  b->tx_scratch, b->dn_scratch, b->dn_mem[0], b->dn_mem[1]};
+                  b->top_f32,  b->tx_scratch, b->dn_scratch, b->dn_mem[0], b->dn_mem[1],
+                  b->dn_bstart};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -438,6 +441,42 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
         return zx != zy ? zx < zy : hm[x] < hm[y];
     });
     HFX_CHECK_HIP(hipMemcpyAsync(b->order, ord.data(), 4ull * S, hipMemcpyHostToDevice, st));
+    /* dense blocks: the order cut into runs of at most eight spectra that
+     * minimise the sum over blocks of (charge levels x rows of the hull of
+     * the windows) + a per-block constant -- a block spanning a gap in
+     * precursor mass would score every candidate in the gap */
+    std::vector<uint32_t> bcut;
+    {
+        constexpr uint64_t FIXED = 512;
+        std::vector<uint64_t> best(S + 1, UINT64_MAX);
+        std::vector<uint32_t> from(S + 1, 0);
+        best[0] = 0;
+        for (uint32_t j = 1; j <= S; ++j) {
+            uint32_t lo = 0xffffffffu, hi = 0, lv = 0;
+            for (uint32_t k = 1; k <= 8 && k <= j; ++k) {
+                const uint32_t x = ord[j - k], z = zlev(x);
+                if (z && hn[x]) {
+                    lo = std::min(lo, hb[x]);
+                    hi = std::max(hi, hb[x] + hn[x]);
+                    lv |= 1u << z;
+                }
+                const uint64_t rows = hi > lo ? hi - lo : 0;
+                const uint64_t c = best[j - k] + rows * (uint64_t)__builtin_popcount(lv) + FIXED;
+                if (c < best[j]) {
+                    best[j] = c;
+                    from[j] = j - k;
+                }
+            }
+        }
+        for (uint32_t j = S; j > 0; j = from[j]) bcut.push_back(from[j]);
+        std::reverse(bcut.begin(), bcut.end());
+        bcut.push_back(S);
+    }
+    b->DB = (uint32_t)bcut.size() - 1;
+    if ((rc = dalloc(&b->dn_bstart, bcut.size()))) return rc;
+    HFX_CHECK_HIP(hipMemcpyAsync(b->dn_bstart, bcut.data(), 4ull * bcut.size(),
+                                 hipMemcpyHostToDevice, st));
+    HFX_CHECK_HIP(hipStreamSynchronize(st));
     uint32_t p0 = 0xffffffffu, p1 = 0, L = 0, lmax = 0;
     for (uint32_t s = 0; s < S; ++s) {
         lmax = std::max(lmax, std::min(hl[s], std::min(sp->x_stride, (uint32_t)HFX_MAX_LEN)));
@@ -573,7 +612,8 @@ static int bt_scores(hfx_batch *b, hipStream_t st)
     if (b->precision == HFX_BATCH_BF16X2_DENSE) {
         if (b->total == 0) return HFX_OK;
         return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order,
-                                &b->dn_lad, &b->sp, b->CH, b->dn_scratch, 1, (void *)st);
+                                b->dn_bstart, b->DB, &b->dn_lad, &b->sp, b->CH, b->dn_scratch, 1,
+                                (void *)st);
     }
     if ((rc = bt_tiles(b, st))) return rc;
     return bt_score(b, st);
@@ -759,6 +799,19 @@ extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32
     b->G = G;
     b->C = C;
     b->threads = threads;
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_dense_blocks(const hfx_batch *b, uint32_t *h_bstart, uint32_t cap,
+                                      uint32_t *num_blocks)
+{
+    HFX_REQUIRE(b && num_blocks, HFX_ERR_ARG, "hfx_batch_dense_blocks: null argument");
+    HFX_REQUIRE(b->planned, HFX_ERR_ARG, "hfx_batch_dense_blocks: batch not planned");
+    *num_blocks = b->DB;
+    if (h_bstart && cap) {
+        const size_t n = std::min<size_t>(cap, (size_t)b->DB + 1);
+        HFX_CHECK_HIP(hipMemcpy(h_bstart, b->dn_bstart, 4 * n, hipMemcpyDeviceToHost));
+    }
     return HFX_OK;
 }
 

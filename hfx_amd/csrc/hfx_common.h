@@ -130,8 +130,10 @@ struct hfx_dn_ladder {
 int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t nq, int own,
                            hfx_dn_ladder *lad, void **mem, void *stream);
 /* dense BF16 hi+lo scores (same layout as hfx_xcorr_sparse); spectra are
- * blocked eight at a time in d_order (null: index order), CH candidates per
- * work item; every window lies inside the ladder's range; scratch:
+ * blocked in d_order (null: index order): block b holds order positions
+ * [d_bstart[b], d_bstart[b+1]) of nblocks blocks of at most eight, or eight
+ * at a time when d_bstart is null; CH candidates per work item; every
+ * window lies inside the ladder's range; scratch:
  * hfx_dense_scratch_bytes(S) bytes or null for the context's slot, whose
  * hi/lo rows hfx_dense_split writes (presplit != 0: already written for
  * these spectra, as a batch plan does once) */
@@ -139,9 +141,9 @@ size_t hfx_dense_scratch_bytes(uint32_t S);
 uint32_t hfx_dense_max_len(void);
 int hfx_dense_split(const hfx_spectra *sp, void *scratch, void *stream);
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-                     const uint32_t *d_num, const uint32_t *d_order, const hfx_dn_ladder *lad,
-                     const hfx_spectra *sp, uint32_t CH, void *scratch, int presplit,
-                     void *stream);
+                     const uint32_t *d_num, const uint32_t *d_order, const uint32_t *d_bstart,
+                     uint32_t nblocks, const hfx_dn_ladder *lad, const hfx_spectra *sp,
+                     uint32_t CH, void *scratch, int presplit, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Device helpers                                                        */

@@ -205,18 +205,30 @@ __device__ __forceinline__ uint32_t dn_order(const uint32_t *order, uint32_t i)
     return order ? order[i] : i;
 }
 
-/* block b = spectra order[8b .. 8b+7]: union window, fragment charge levels
- * present (bit z), row length kb (multiple of 128), work items */
+/* spectra (positions in order) of block b: [bstart[b], bstart[b+1]), at
+ * most DN_NS of them, or eight at a time without a table */
+__device__ __forceinline__ uint32_t dn_bfirst(const uint32_t *bstart, uint32_t b)
+{
+    return bstart ? bstart[b] : b * DN_NS;
+}
+__device__ __forceinline__ uint32_t dn_bend(const uint32_t *bstart, uint32_t b, uint32_t S)
+{
+    return bstart ? bstart[b + 1] : min(S, b * DN_NS + DN_NS);
+}
+
+/* block b: union window, fragment charge levels present (bit z), longest
+ * row kb (multiple of 128), work items */
 __global__ void k_dn_blocks(uint32_t *ubeg, uint32_t *uend, uint32_t *binfo, uint32_t *ntiles,
-                            const uint32_t *order, const uint32_t *begin, const uint32_t *num,
-                            hfx_spectra sp, uint32_t B, uint32_t CH)
+                            const uint32_t *order, const uint32_t *bstart, const uint32_t *begin,
+                            const uint32_t *num, hfx_spectra sp, uint32_t B, uint32_t CH)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     uint32_t lo = 0xffffffffu, hi = 0, mask = 0, kmax = 0;
+    const uint32_t i0 = dn_bfirst(bstart, b), i1 = dn_bend(bstart, b, sp.num_spec);
     for (uint32_t g = 0; g < DN_NS; ++g) {
-        const uint32_t i = b * DN_NS + g;
-        if (i >= sp.num_spec) break;
+        const uint32_t i = i0 + g;
+        if (i >= i1) break;
         const uint32_t s = dn_order(order, i);
         const uint32_t z = hfx_zeff(sp.d_zmax[s]);
         if (z == 0 || num[s] == 0) continue;
@@ -262,6 +274,7 @@ struct DnArgs {
     float *scores;
     const uint64_t *score_off;
     const uint32_t *begin, *num, *order;
+    const uint32_t *bstart; /* [B + 1] block starts in order, or null: eight a block */
     uint32_t B, CH;
     const uint32_t *ubeg, *uend, *binfo;
     const uint64_t *tile_off;
@@ -423,10 +436,10 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         const uint32_t kb = info & 0xffffu, mask = info >> 16;
         if (b != cur) {
             if (tid < DN_NS) {
-                const uint32_t i = b * DN_NS + tid;
+                const uint32_t i = dn_bfirst(a.bstart, b) + tid;
                 uint32_t s = DN_NONE, be = 0, en = 0, z = 0;
                 uint64_t of = 0;
-                if (i < a.sp.num_spec) {
+                if (i < dn_bend(a.bstart, b, a.sp.num_spec)) {
                     s = dn_order(a.order, i);
                     be = a.begin[s];
                     en = be + a.num[s];
@@ -741,7 +754,7 @@ int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t
 
 size_t hfx_dense_scratch_bytes(uint32_t S)
 {
-    const size_t B = (S + DN_NS - 1) / DN_NS;
+    const size_t B = S; /* blocks of at least one spectrum */
     return (size_t)S * 2 * DN_MAXK * 2 + B * 4 * 4 + (B + 1) * 8 + 1024;
 }
 
@@ -760,9 +773,9 @@ int hfx_dense_split(const hfx_spectra *sp, void *scratch, void *stream)
 }
 
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
-                     const uint32_t *d_num, const uint32_t *d_order, const hfx_dn_ladder *lad,
-                     const hfx_spectra *sp, uint32_t CH, void *scratch, int presplit,
-                     void *stream)
+                     const uint32_t *d_num, const uint32_t *d_order, const uint32_t *d_bstart,
+                     uint32_t nblocks, const hfx_dn_ladder *lad, const hfx_spectra *sp,
+                     uint32_t CH, void *scratch, int presplit, void *stream)
 {
     HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && lad && sp, HFX_ERR_ARG,
                 "dense XCorr: null argument");
@@ -777,7 +790,9 @@ int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const
     int rc;
     if (!p && (rc = hfx_ctx_scratch(ctx, HFX_SLOT_DENSE, hfx_dense_scratch_bytes(S), &p)))
         return rc;
-    const uint32_t B = (S + DN_NS - 1) / DN_NS;
+    HFX_REQUIRE(!d_bstart || (nblocks >= 1 && nblocks <= S), HFX_ERR_ARG,
+                "dense XCorr: %u blocks for %u spectra", nblocks, S);
+    const uint32_t B = d_bstart ? nblocks : (S + DN_NS - 1) / DN_NS;
     uint16_t *xb = (uint16_t *)p;
     uint32_t *ubeg = (uint32_t *)(xb + (size_t)S * 2 * DN_MAXK);
     uint32_t *uend = ubeg + B, *binfo = uend + B, *ntiles = binfo + B;
@@ -786,11 +801,12 @@ int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const
     float *dummy = (float *)(((uintptr_t)(counter + 1) + 255) & ~(uintptr_t)255);
     if (!presplit) hipLaunchKernelGGL(k_dn_split, dim3(S), dim3(256), 0, st, xb, *sp);
     hipLaunchKernelGGL(k_dn_blocks, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend, binfo,
-                       ntiles, d_order, d_begin, d_num, *sp, B, CH);
+                       ntiles, d_order, d_bstart, d_begin, d_num, *sp, B, CH);
     if ((rc = hfx_scan_offsets(ctx, tile_off, ntiles, B, (void *)st))) return rc;
     hipLaunchKernelGGL(k_dn_reset, dim3(1), dim3(1), 0, st, counter);
-    DnArgs args{d_scores, d_off, d_begin, d_num, d_order, B,     CH,   ubeg,
-                uend,     binfo, tile_off, xb,  counter, dummy, *lad, *sp};
+    DnArgs args{d_scores, d_off,    d_begin, d_num,   d_order, d_bstart, B,    CH,
+                ubeg,     uend,     binfo,   tile_off, xb,      counter,  dummy, *lad,
+                *sp};
     HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_dense,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, DN_LDS_BYTES));
     hipLaunchKernelGGL(k_dense, dim3((uint32_t)hfx_ctx_num_cus(ctx)), dim3(DN_THREADS),
@@ -814,6 +830,6 @@ extern "C" int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint6
     int rc;
     if ((rc = hfx_dense_ladder_build(ctx, db, 0, db->num_pep, 0, &lad, nullptr, stream)))
         return rc;
-    return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, &lad, sp, 256, nullptr,
-                            0, stream);
+    return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, nullptr, 0, &lad, sp,
+                            256, nullptr, 0, stream);
 }

@@ -440,6 +440,16 @@ class BatchSearch:
         check(lib().hfx_batch_info(self.h, a.ctypes.data), "hfx_batch_info")
         return {k: int(v) for k, v in zip(self.INFO, a)}
 
+    def dense_blocks(self) -> np.ndarray:
+        """block starts of the dense form as positions in the plan's (fragment
+        charge, precursor mass) order, the last one the spectrum count"""
+        nb = C.c_uint32(0)
+        check(lib().hfx_batch_dense_blocks(self.h, None, 0, C.byref(nb)), "hfx_batch_dense_blocks")
+        out = np.zeros(nb.value + 1, np.uint32)
+        check(lib().hfx_batch_dense_blocks(self.h, out.ctypes.data, len(out), C.byref(nb)),
+              "hfx_batch_dense_blocks")
+        return out
+
     def autotune(self):
         cfg = np.zeros(3, np.uint32)
         check(lib().hfx_batch_autotune(self.h, self.stream.p, cfg.ctypes.data), "hfx_batch_autotune")

@@ -375,6 +375,13 @@ int hfx_batch_time(hfx_batch *b, void *stream, float *ms);
  *         G, C, threads, LDS stride, first indexed position, precision,
  *         dense chunk} (12 entries) */
 int hfx_batch_info(const hfx_batch *b, uint64_t *info);
+/* blocks of the dense form (new): the plan cuts the spectra, in fragment
+ * charge then precursor mass order, into runs of at most eight whose
+ * precursor windows overlap most.  *num_blocks = their number; h_bstart
+ * (host, optional) receives min(cap, num_blocks + 1) block starts as
+ * positions in that order, the last one the spectrum count. */
+int hfx_batch_dense_blocks(const hfx_batch *b, uint32_t *h_bstart, uint32_t cap,
+                           uint32_t *num_blocks);
 /* device pointers {top_pos, top_exact, top_f32, top_cnt, flag, scores,
  *                  score offsets, window begin, window size, index entries} */
 int hfx_batch_outputs(const hfx_batch *b, void **ptrs);
