@@ -443,11 +443,12 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     HFX_CHECK_HIP(hipMemcpyAsync(b->order, ord.data(), 4ull * S, hipMemcpyHostToDevice, st));
     /* dense blocks: the order cut into runs of at most eight spectra that
      * minimise the sum over blocks of (charge levels x rows of the hull of
-     * the windows) + a per-block constant -- a block spanning a gap in
-     * precursor mass would score every candidate in the gap */
+     * the windows) -- a block spanning a gap in precursor mass would score
+     * every candidate in the gap.  A per-block constant (B staging, tile
+     * rounding) measured no better than none at tryptic-20k-10k. */
     std::vector<uint32_t> bcut;
     {
-        constexpr uint64_t FIXED = 512;
+        constexpr uint64_t FIXED = 0;
         std::vector<uint64_t> best(S + 1, UINT64_MAX);
         std::vector<uint32_t> from(S + 1, 0);
         best[0] = 0;
