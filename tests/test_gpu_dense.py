@@ -127,6 +127,43 @@ def test_batch_dense_matches_oracle(denv, orc):
     check_topn(res, denv["ref"], orc)
 
 
+def test_batch_dense_blocks_cover_the_order(denv):
+    """The plan's dense blocks are consecutive runs of one to eight spectra of
+    the (fragment charge, precursor mass) order covering every spectrum, and
+    their total cost -- charge levels x rows of the hull of the windows,
+    summed over blocks -- is the minimum over all such cuts (an independent
+    dynamic programme here; ties may cut differently)."""
+    pl, dsp = denv["pl"], denv["dsp"]
+    bs = pl.BatchSearch(denv["ctx"], denv["db"], dsp, eps=3.0, topn=10)
+    bs.set_precision(bs.BF16X2_DENSE)
+    bs.run()
+    res = bs.results()
+    st = bs.dense_blocks().astype(np.int64)
+    S = dsp.num_spec
+    assert st[0] == 0 and st[-1] == S
+    sizes = np.diff(st)
+    assert sizes.min() >= 1 and sizes.max() <= 8
+    z = dsp.zmax.astype(np.int64)
+    z = np.where(z <= 4, z, 0)
+    order = np.lexsort((dsp.mass, z))
+    b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
+
+    def cost(lo, hi):
+        blk = order[lo:hi]
+        blk = blk[(z[blk] > 0) & (n[blk] > 0)]
+        if len(blk) == 0:
+            return 0
+        return len(set(z[blk].tolist())) * int((b[blk] + n[blk]).max() - b[blk].min())
+
+    plan = sum(cost(int(a), int(c)) for a, c in zip(st[:-1], st[1:]))
+    best = [0] + [None] * S
+    for j in range(1, S + 1):
+        best[j] = min(best[j - k] + cost(j - k, j) for k in range(1, min(8, j) + 1))
+    assert plan == best[S]
+    # and no worse than fixed runs of eight
+    assert plan <= sum(cost(i, min(S, i + 8)) for i in range(0, S, 8))
+
+
 @pytest.mark.parametrize("chunk", [16, 48, 512, 4096])
 def test_batch_dense_chunks_agree(denv, orc, chunk):
     pl = denv["pl"]
