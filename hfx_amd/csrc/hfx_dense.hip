@@ -20,7 +20,7 @@
  * number of non-zero bins of the row (<= 10 * residues * fragment charges);
  * the top-N stage re-scores exactly with that bound.
  *
- * Layout and work split (one persistent workgroup of eight waves per CU):
+ * Layout and work split (one persistent workgroup of sixteen waves per CU):
  *   - k_dn_split writes every spectrum once as hi/lo BF16 rows of DN_MAXK bins
  *     (zero beyond its length), bins permuted inside each group of eight so
  *     that the A fragment built below needs no shuffles;
@@ -30,16 +30,16 @@
  *     candidates are one contiguous range;
  *   - a work item is (block, fragment charge level, chunk of CH candidates);
  *     the workgroup stages the block's 16 B columns in LDS (once per block);
- *   - a pair of waves takes 16 candidates: each lane takes one ladder
- *     position (of its half's eight candidates) and ORs the unary weight
- *     codes of its peaks into the pair's LDS nibble map (OR of codes = max
+ *   - four waves share a map and take 16 candidates: each lane takes one
+ *     ladder position (of its wave's four candidates) and ORs the unary
+ *     weight codes of its peaks into the map in LDS (OR of codes = max
  *     weight), bin x in word x/8 of its row, so a lane's A word at k-step ks
  *     is word 4 ks + k-group; ladder positions are loaded a tile ahead;
  *   - A words (8 nibbles) become eight BF16 weights with four v_perm lookups
  *     (codes 0/1/3/7 select the high and low bytes of 0, 10, 25, 50) and four
- *     byte interleaves; the map is cleared as it is read; each half of the
- *     pair contracts every other group of 128 bins and the halves add their
- *     partial sums.
+ *     byte interleaves; the map is cleared as it is read; each wave of the
+ *     map contracts every fourth group of 128 bins and the four partial sums
+ *     are added in wave order.
  * Bins use the same canonical arithmetic as the sparse path: the divisions
  * by the bin width and by the fragment charge are computed as
  * q = x*R, r = fma(-q, d, x), q + r*R, which equals the correctly rounded
@@ -52,8 +52,10 @@
 
 #define DN_NS 8                     /* spectra per block */
 #define DN_COLS 16                  /* MFMA N: hi and lo of each spectrum */
-#define DN_WAVES 8                  /* four pairs of waves */
-#define DN_PAIRS 4
+#define DN_PAIRS 4                  /* four maps, one 16-candidate tile each */
+#define DN_SPLIT 4                  /* waves sharing a map */
+#define DN_WAVES (DN_PAIRS * DN_SPLIT)
+#define DN_CPW (16 / DN_SPLIT)      /* candidates whose ladder positions a wave takes */
 #define DN_THREADS (DN_WAVES * HFX_WAVE)
 #define DN_ROWS 16                  /* candidates per wave tile (MFMA M) */
 #define DN_MAXK 2176                /* bins of a row (multiple of 128) */
@@ -68,12 +70,13 @@ static_assert(DN_MSTR >= DN_MAXK / 8 + 2 && DN_MSTR % 32 == 2, "map row stride")
 
 #define DN_B_BYTES (DN_COLS * DN_CSTR * 2)
 #define DN_MAP_BYTES (DN_PAIRS * DN_ROWS * DN_MSTR * 4)
-#define DN_COMB_BYTES (DN_PAIRS * 2 * HFX_WAVE * 16)
+#define DN_COMB_BYTES (DN_PAIRS * DN_SPLIT * HFX_WAVE * 16)
 #define DN_LDS_BYTES (DN_B_BYTES + DN_MAP_BYTES + DN_COMB_BYTES)
 /* the kernel's fixed __shared__ arrays (m_*, sh_t) share the CU's 160 KiB
  * with the dynamic part */
 #define DN_FIXED_LDS (DN_NS * 4 * 4 + DN_NS * 8 + 2 * 4 + 16)
 static_assert(DN_LDS_BYTES + DN_FIXED_LDS <= 160 * 1024, "dense kernel LDS exceeds a CU");
+static_assert(DN_SPLIT == 2 || DN_SPLIT == 4, "a wave takes 16 / DN_SPLIT candidates and 4 / DN_SPLIT rows of C");
 
 typedef __bf16 dn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float dn_f32x4 __attribute__((ext_vector_type(4)));
@@ -406,12 +409,12 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     __shared__ uint64_t m_off[DN_NS];
     __shared__ uint32_t sh_t[2];
 
-    /* wave pair q = wave % 4 (halves h = 0, 1) shares a map; the halves
-     * swap their partial sums through comb */
+    /* the DN_SPLIT waves q, q + 4, ... (h = wave / 4) share map q and pass
+     * their partial sums through comb */
     const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
     const uint32_t q = wave % DN_PAIRS, h = wave / DN_PAIRS;
     uint32_t *map = maps + q * DN_ROWS * DN_MSTR;
-    dn_f32x4 *comb = comb_all + q * 2 * HFX_WAVE;
+    dn_f32x4 *comb = comb_all + q * DN_SPLIT * HFX_WAVE;
     for (uint32_t i = tid; i < DN_PAIRS * DN_ROWS * DN_MSTR; i += DN_THREADS) maps[i] = 0;
     const uint64_t ntiles = dn_sload(a.tile_off, a.B);
     const uint32_t row = lane & 15u, kg = lane >> 4;
@@ -502,12 +505,13 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         const uint64_t og = m_off[g];
         float *const dummy = a.dummy + lane;
 
-        /* software pipeline over the pair's tiles: at the start of tile j the
+        /* software pipeline over the map's tiles: at the start of tile j the
          * ladder range of tile j+2 and the ladder positions and tops of tile
          * j+1 are issued, each consumed a tile later; loads and stores are
          * unconditional (clamped indices, a dummy target), so the number in
-         * flight does not depend on the data and the waits stay exact.  Half
-         * h takes the positions of candidates 8h..8h+7. */
+         * flight does not depend on the data and the waits stay exact.  Wave
+         * h of a map takes the positions of its candidates DN_CPW h ..
+         * DN_CPW (h + 1) - 1. */
         auto rows_of = [&](uint32_t t0) -> uint32_t {
             return t0 < cend ? min((uint32_t)DN_ROWS, cend - t0) : 0u;
         };
@@ -515,7 +519,7 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
          * past its last (both offset 0 for an empty tile) */
         auto load_range = [&](uint32_t t0) -> uint64_t {
             const uint32_t nr = rows_of(t0);
-            const uint32_t c = min(8u * h + (lane ? 8u : 0u), nr);
+            const uint32_t c = min(DN_CPW * h + (lane ? DN_CPW : 0u), nr);
             return a.lad.off[nr ? t0 + c - a.lad.q0 : 0u];
         };
         /* lane < 16: the top of candidate lane (meaningful below the tile's
@@ -584,13 +588,14 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
             __syncthreads(); /* (b) the tile's map rows are complete */
 
             /* 16 x 16 x kq contraction over groups of four k-steps (128
-             * bins): half h takes groups h, h+2, ... and clears their map
-             * words.  The split depends only on the group index, so each
-             * half's sum over a row's non-zero k-steps -- and the score --
-             * is the same whichever tile the row falls in.  Two accumulator
-             * chains; the next group's operands are read ahead (a read past
-             * the last group stays inside the LDS allocation, unused).  Lane
-             * (row, kg) takes word 4 ks + kg of its row at k-step ks. */
+             * bins): wave h of the map takes groups h, h + DN_SPLIT, ... and
+             * clears their map words.  The split depends only on the group
+             * index, so each wave's sum over a row's non-zero k-steps -- and
+             * the score -- is the same whichever tile the row falls in.  Two
+             * accumulator chains; the next group's operands are read ahead (a
+             * read past the last group stays inside the LDS allocation,
+             * unused).  Lane (row, kg) takes word 4 ks + kg of its row at
+             * k-step ks. */
             dn_f32x4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
             const uint32_t ngrp = kq >> 7;
             uint32_t *arow = map + row * DN_MSTR + 1 + kg;
@@ -610,47 +615,53 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
                     __builtin_bit_cast(dn_bf16x8, dn_weights(cw.w)),
                     __builtin_bit_cast(dn_bf16x8, b3), acc1, 0, 0, 0);
             };
+            /* group offsets: G = DN_SPLIT groups on, 16 words / 128 BF16 a group */
+            constexpr uint32_t GW = 16 * DN_SPLIT, GB = 128 * DN_SPLIT;
             if (h < ngrp) {
                 uint32_t *aw = arow + 16 * h;
                 const uint16_t *bw = bcol + 128 * h;
                 dn_u32x4 ca = {aw[0], aw[4], aw[8], aw[12]};
                 dn_u32x4 a0 = *(const dn_u32x4 *)(bw + 0), a1 = *(const dn_u32x4 *)(bw + 32);
                 dn_u32x4 a2 = *(const dn_u32x4 *)(bw + 64), a3 = *(const dn_u32x4 *)(bw + 96);
-                for (uint32_t g = h; g < ngrp; g += 4) {
-                    /* group g (in ca, a0..a3); read group g + 2 */
+                for (uint32_t g = h; g < ngrp; g += 2 * DN_SPLIT) {
+                    /* group g (in ca, a0..a3); read group g + G */
                     aw[0] = 0u; aw[4] = 0u; aw[8] = 0u; aw[12] = 0u;
-                    const dn_u32x4 cb = {aw[32], aw[36], aw[40], aw[44]};
-                    const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + 256);
-                    const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + 288);
-                    const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + 320);
-                    const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + 352);
+                    const dn_u32x4 cb = {aw[GW], aw[GW + 4], aw[GW + 8], aw[GW + 12]};
+                    const dn_u32x4 e0 = *(const dn_u32x4 *)(bw + GB);
+                    const dn_u32x4 e1 = *(const dn_u32x4 *)(bw + GB + 32);
+                    const dn_u32x4 e2 = *(const dn_u32x4 *)(bw + GB + 64);
+                    const dn_u32x4 e3 = *(const dn_u32x4 *)(bw + GB + 96);
                     mma4(ca, a0, a1, a2, a3);
-                    if (g + 2 >= ngrp) break;
-                    /* group g + 2 (in cb, e0..e3); read group g + 4 */
-                    aw[32] = 0u; aw[36] = 0u; aw[40] = 0u; aw[44] = 0u;
-                    ca = dn_u32x4{aw[64], aw[68], aw[72], aw[76]};
-                    a0 = *(const dn_u32x4 *)(bw + 512);
-                    a1 = *(const dn_u32x4 *)(bw + 544);
-                    a2 = *(const dn_u32x4 *)(bw + 576);
-                    a3 = *(const dn_u32x4 *)(bw + 608);
+                    if (g + DN_SPLIT >= ngrp) break;
+                    /* group g + G (in cb, e0..e3); read group g + 2G */
+                    aw[GW] = 0u; aw[GW + 4] = 0u; aw[GW + 8] = 0u; aw[GW + 12] = 0u;
+                    ca = dn_u32x4{aw[2 * GW], aw[2 * GW + 4], aw[2 * GW + 8], aw[2 * GW + 12]};
+                    a0 = *(const dn_u32x4 *)(bw + 2 * GB);
+                    a1 = *(const dn_u32x4 *)(bw + 2 * GB + 32);
+                    a2 = *(const dn_u32x4 *)(bw + 2 * GB + 64);
+                    a3 = *(const dn_u32x4 *)(bw + 2 * GB + 96);
                     mma4(cb, e0, e1, e2, e3);
-                    aw += 64;
-                    bw += 512;
+                    aw += 2 * GW;
+                    bw += 2 * GB;
                 }
             }
             const dn_f32x4 part = acc0 + acc1;
             comb[h * HFX_WAVE + lane] = part;
-            __syncthreads(); /* (c) both halves' partial sums are in */
-            const dn_f32x4 other = comb[(h ^ 1u) * HFX_WAVE + lane];
-            /* C[row 4 kg + i][column lane % 16]: half h stores rows i = 2h,
-             * 2h + 1; an even column's lane adds its odd neighbour (hi + lo) */
+            __syncthreads(); /* (c) every wave's partial sums are in */
+            /* C[row 4 kg + i][column lane % 16]: wave h stores rows
+             * i = (4 / DN_SPLIT) h + i2, summing the partials of waves
+             * 0, 1, ... in that order; an even column's lane adds its odd
+             * neighbour (hi + lo) */
+            constexpr uint32_t RPW = 4 / DN_SPLIT;
 #pragma unroll
-            for (uint32_t i2 = 0; i2 < 2; ++i2) {
-                const float mine = h ? part[2 + i2] : part[i2];
-                const float oth = h ? other[2 + i2] : other[i2];
-                const float c4 = h ? oth + mine : mine + oth;
+            for (uint32_t i2 = 0; i2 < RPW; ++i2) {
+                const uint32_t i = RPW * h + i2;
+                const float *cf = (const float *)comb;
+                float c4 = cf[(0 * HFX_WAVE + lane) * 4 + i];
+#pragma unroll
+                for (uint32_t w2 = 1; w2 < DN_SPLIT; ++w2) c4 = c4 + cf[(w2 * HFX_WAVE + lane) * 4 + i];
                 const float v = c4 + dn_dppf<DN_SWAP1>(c4);
-                const uint32_t rr_i = 4u * kg + 2u * h + i2, p = p0 + rr_i;
+                const uint32_t rr_i = 4u * kg + i, p = p0 + rr_i;
                 const bool ok =
                     (lane & 1u) == 0 && rr_i < nrows && zg == lvl && bg <= p && p < eg;
                 *(ok ? a.scores + og + (p - bg) : dummy) = v;
