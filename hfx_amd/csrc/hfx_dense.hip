@@ -65,7 +65,7 @@
 #define DN_MSTR 290
 static_assert(DN_MSTR >= DN_MAXK / 8 + 2 && DN_MSTR % 32 == 2, "map row stride");
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
-#define DN_PRE 4                    /* ladder positions per lane loaded a tile ahead */
+#define DN_PRE 2                    /* ladder positions per lane loaded a tile ahead */
 #define DN_NONE 0xffffffffu
 
 #define DN_B_BYTES (DN_COLS * DN_CSTR * 2)
