@@ -339,21 +339,39 @@ __global__ void __launch_bounds__(TN_THREADS)
 /* ------------------------------------------------------------------ */
 
 #define TX_MAXN 64
-#define TX_CHUNK 64
+#define TX_CHUNK 256
 /* a wave fetches the metadata of its share of a candidate list one lane
  * per candidate */
 static_assert(TN_MAXT / TN_WAVES <= HFX_WAVE && TN_EXT / TN_WAVES <= HFX_WAVE &&
                   TX_CHUNK / TN_WAVES <= HFX_WAVE,
               "one lane per candidate of a wave's share");
 
+/* tiles per flagged spectrum, and the flagged spectra listed (in any
+ * order) for the final select; counter[1] counts them */
 template <class Cands>
-__global__ void k_flag_tiles(uint32_t *tiles, const uint32_t *flag, Cands c, uint32_t S)
+__global__ void k_flag_tiles(uint32_t *tiles, uint32_t *list, uint32_t *counter,
+                             const uint32_t *flag, Cands c, uint32_t S)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S) tiles[s] = flag[s] ? (c.count(s) + TX_CHUNK - 1) / TX_CHUNK : 0u;
+    if (s >= S) return;
+    const bool f = flag[s] != 0u;
+    tiles[s] = f ? (c.count(s) + TX_CHUNK - 1) / TX_CHUNK : 0u;
+    if (f) list[atomicAdd(&counter[1], 1u)] = s;
 }
 
-__global__ void k_tx_reset(uint32_t *counter) { *counter = 0; }
+__global__ void k_tx_reset(uint32_t *counter)
+{
+    counter[0] = 0;
+    counter[1] = 0;
+}
+
+/* tile -> spectrum, one thread per spectrum writing its own tiles */
+__global__ void k_tile_spec(uint32_t *tile_spec, const uint64_t *tile_off, uint32_t S)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S)
+        for (uint64_t t = tile_off[s]; t < tile_off[s + 1]; ++t) tile_spec[t] = s;
+}
 
 __global__ void k_zero_flags(uint32_t *flag, uint32_t S)
 {
@@ -370,21 +388,15 @@ __global__ void k_mark_list(uint32_t *flag, double *tau, const uint32_t *list, u
     }
 }
 
-__device__ __forceinline__ uint32_t tx_segment(const uint64_t *off, uint32_t n, uint64_t t)
-{
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= t) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-/* persistent: tiles of TX_CHUNK candidates of flagged spectra from a counter */
+/* persistent: tiles of flagged spectra from a counter.  A spectrum's nt
+ * tiles interleave its candidates (tile k takes k, k + nt, k + 2 nt, ...):
+ * candidates near tau sit together in the window's order, and so are
+ * spread over the tiles. */
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
-    k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, uint32_t *counter, Cands c,
-                  hfx_spectra sp, const float *scores, const double *tau)
+    k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, const uint32_t *tile_spec,
+                  uint32_t *counter, Cands c, hfx_spectra sp, const float *scores,
+                  const double *tau)
 {
     extern __shared__ __align__(16) unsigned char tn_dyn[];
     const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
@@ -403,145 +415,322 @@ __global__ void __launch_bounds__(TN_THREADS)
         __syncthreads();
         const uint64_t t = sh_t;
         if (t >= ntiles) break;
-        const uint32_t s = tx_segment(tile_off, S, t);
+        const uint32_t s = tile_spec[t];
         if (s != cur) {
             len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
             cur = s;
             __syncthreads();
         }
-        const uint32_t first = (uint32_t)(t - tile_off[s]) * TX_CHUNK;
-        const uint32_t cnt = min((uint32_t)TX_CHUNK, c.count(s) - first);
+        const uint32_t k = (uint32_t)(t - tile_off[s]);
+        const uint32_t nt = (uint32_t)(tile_off[s + 1] - tile_off[s]);
+        const uint32_t n = c.count(s);
         const uint32_t zeff = c.zeff(sp.d_zmax[s]);
         const double ts = tau[s];
-        /* the wave's candidates i = wave + 4 j (j < 16): lane j fetches i's
-         * metadata */
+        /* the wave's candidates: tile slots m = wave + 4 j (j < 64), lane j
+         * reading candidate i = k + nt m's score.  A candidate below tau
+         * cannot reach the top N (its exact score is below the N-th exact
+         * score): it keeps the smallest key, stored by its own lane.  The
+         * others' metadata is fetched by their lanes, and the wave re-scores
+         * them one by one. */
+        const uint64_t tb = c.base(s);
+        const uint32_t mi = k + nt * (wave + TN_WAVES * lane);
+        bool live = false;
+        if (wave + TN_WAVES * lane < TX_CHUNK && mi < n) {
+            live = !((double)scores[tb + mi] < ts);
+            if (!live) fxbuf[tb + mi] = INT64_MIN;
+        }
         typename Cands::Prep mine{};
-        if (wave + TN_WAVES * lane < cnt) mine = c.prep(s, first + wave + TN_WAVES * lane);
-        for (uint32_t i = wave, j = 0; i < cnt; i += TN_WAVES, ++j) {
-            /* a candidate below tau cannot reach the top N: its exact score
-             * is below the N-th exact score (it keeps the smallest key) */
-            const uint64_t at = c.base(s) + first + i;
-            if ((double)scores[at] < ts) {
-                if (lane == 0) fxbuf[at] = INT64_MIN;
-                continue;
-            }
+        if (live) mine = c.prep(s, mi);
+        for (uint64_t m = __ballot(live); m; m &= m - 1ull) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(m);
             c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
             hfx_wave_sync();
             const int64_t fx = hfx_wave_dot_fx(map, xs, len);
             hfx_wave_sync();
-            if (lane == 0) fxbuf[at] = fx;
+            if (lane == 0) fxbuf[tb + hfx_take(mi, j)] = fx;
         }
         __syncthreads();
     }
 }
 
-/* top N of a flagged spectrum by (exact desc, position asc), radix select
- * over order-preserving 64-bit keys */
+/* top N of a flagged spectrum by (exact desc, position asc).
+ *
+ * Only the live candidates count: the exact route leaves INT64_MIN for every
+ * candidate below tau, and more than N are live.  They are gathered into LDS
+ * in one sweep (up to FX_CAP of them) and selected there by radix over the
+ * bits in which their 64-bit keys differ, 11 bits a pass, then, for keys
+ * tied in full, over the complemented position.  Windows with more live
+ * candidates (or fewer than N) take the 8-bit radix select over global
+ * memory.  A persistent grid walks the list of flagged spectra. */
+#define FX_CAP 4096
+#define FX_DBITS 11
+
+/* the digit holding the rem-th largest element of an nb-bucket histogram
+ * (counting from bucket nb - 1 down), and the rank left inside it: wave 0's
+ * lane l sums its run of buckets, a scan over the lanes finds the run that
+ * holds rank rem, and that lane walks it.  Every thread calls it. */
+__device__ __forceinline__ void fx_select_digit(const uint32_t *hist, uint32_t nb,
+                                                uint32_t *sh_digit, uint32_t *sh_rem)
+{
+    const uint32_t tid = threadIdx.x, lane = hfx_lane();
+    if (tid < HFX_WAVE) {
+        const uint32_t rem = *sh_rem;
+        const uint32_t per = (nb + HFX_WAVE - 1) / HFX_WAVE;
+        const int top = (int)nb - 1 - (int)(lane * per);
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < per; ++k)
+            if (top - (int)k >= 0) sum += hist[top - (int)k];
+        uint32_t incl = sum;
+#pragma unroll
+        for (uint32_t o = 1; o < HFX_WAVE; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t excl = incl - sum;
+        if (excl < rem && rem <= incl) {
+            uint32_t cum = excl;
+            for (uint32_t k = 0; k < per && top - (int)k >= 0; ++k) {
+                const uint32_t h = hist[top - (int)k];
+                if (cum + h >= rem) {
+                    *sh_digit = (uint32_t)(top - (int)k);
+                    *sh_rem = rem - cum;
+                    break;
+                }
+                cum += h;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn_fx(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt, uint32_t N,
-              const uint32_t *flag, const int64_t *fxbuf, const float *scores, Cands c)
+              const uint32_t *list, const uint32_t *list_cnt, const int64_t *fxbuf,
+              const float *scores, Cands c)
 {
-    __shared__ uint32_t hist[256];
+    __shared__ uint64_t lk[FX_CAP];
+    __shared__ uint32_t lp[FX_CAP];
+    __shared__ uint32_t hist[1u << FX_DBITS];
     __shared__ uint32_t sel_p[TX_MAXN];
     __shared__ int64_t sel_f[TX_MAXN];
     __shared__ uint32_t wcnt[TN_WAVES];
-    __shared__ uint32_t sh_digit, sh_rem, sh_gt;
-    const uint32_t s = blockIdx.x;
-    if (!flag[s]) return;
+    __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_cnt;
+    __shared__ uint64_t sh_diff;
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
-    const uint32_t n = c.count(s);
-    const int64_t *fx = fxbuf + c.base(s);
-    const float *sc = scores + c.base(s);
-    const uint32_t want = min(N, n);
-    if (want == 0) {
-        if (tid == 0) top_cnt[s] = 0;
-        return;
-    }
-    auto key = [&](uint32_t i) { return (uint64_t)fx[i] ^ 0x8000000000000000ull; };
-    /* f(i, key) over this thread's keys, eight loads in flight (clamped into
-     * the window); i >= n past it, where all lanes still take part */
-    auto for_keys = [&](auto &&f) {
-        constexpr uint32_t U = 8;
-        for (uint32_t base = 0; base < n; base += U * TN_THREADS) {
-            uint64_t kk[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u) kk[u] = key(min(base + u * TN_THREADS + tid, n - 1u));
-#pragma unroll
-            for (uint32_t u = 0; u < U; ++u) f(base + u * TN_THREADS + tid, kk[u]);
+    const uint32_t F = *list_cnt;
+
+    for (uint32_t q = blockIdx.x; q < F; q += gridDim.x) {
+        const uint32_t s = list[q];
+        const uint32_t n = c.count(s);
+        const int64_t *fx = fxbuf + c.base(s);
+        const float *sc = scores + c.base(s);
+        const uint32_t want = min(N, n);
+        if (want == 0) {
+            if (tid == 0) top_cnt[s] = 0;
+            continue;
         }
-    };
-    uint64_t prefix = 0, pmask = 0;
-    if (tid == 0) sh_rem = want;
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        hist[tid] = 0;
+        auto key = [&](uint32_t i) { return (uint64_t)fx[i] ^ 0x8000000000000000ull; };
+        /* f(i, key) over this thread's keys, eight loads in flight
+         * (clamped into the window); i >= n past it, where all lanes
+         * still take part */
+        auto for_keys = [&](auto &&f) {
+            constexpr uint32_t U = 8;
+            for (uint32_t base = 0; base < n; base += U * TN_THREADS) {
+                uint64_t kk[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    kk[u] = key(min(base + u * TN_THREADS + tid, n - 1u));
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) f(base + u * TN_THREADS + tid, kk[u]);
+            }
+        };
+
+        /* 1. the live keys (key 0 is INT64_MIN) into LDS, one LDS atomic
+         * per wave and batch */
+        if (tid == 0) {
+            sh_cnt = 0;
+            sh_gt = 0;
+            sh_diff = 0;
+            sh_rem = want;
+        }
         __syncthreads();
         for_keys([&](uint32_t i, uint64_t k) {
-            tn_hist_add(hist, (uint32_t)(k >> shift) & 255u, i < n && (k & pmask) == prefix);
+            const bool live = i < n && k != 0ull;
+            const uint64_t b = __ballot(live);
+            if (b) {
+                const uint32_t leader = (uint32_t)__builtin_ctzll(b);
+                uint32_t wb = 0;
+                if (lane == leader) wb = atomicAdd(&sh_cnt, (uint32_t)__builtin_popcountll(b));
+                wb = (uint32_t)__builtin_amdgcn_readlane((int)wb, (int)leader);
+                const uint32_t slot =
+                    wb + (uint32_t)__builtin_popcountll(b & ((1ull << lane) - 1ull));
+                if (live && slot < FX_CAP) {
+                    lk[slot] = k;
+                    lp[slot] = i;
+                }
+            }
         });
         __syncthreads();
-        tn_select_digit(hist, &sh_digit, &sh_rem);
-        prefix |= (uint64_t)sh_digit << shift;
-        pmask |= 255ull << shift;
-    }
-    const uint64_t K = prefix;
-    const uint32_t need_eq = sh_rem, n_gt = want - need_eq;
-    if (tid == 0) sh_gt = 0;
-    __syncthreads();
-    for_keys([&](uint32_t i, uint64_t k) {
-        if (i < n && k > K) {
-            const uint32_t slot = atomicAdd(&sh_gt, 1u);
-            if (slot < n_gt) {
-                sel_p[slot] = i;
-                sel_f[slot] = (int64_t)(k ^ 0x8000000000000000ull);
+        const uint32_t M = sh_cnt;
+        if (M >= want && M <= FX_CAP) {
+            /* 2. the bits in which the live keys differ */
+            const uint64_t k0 = lk[0];
+            uint64_t d = 0;
+            for (uint32_t i = tid; i < M; i += TN_THREADS) d |= lk[i] ^ k0;
+            if (d) atomicOr((unsigned long long *)&sh_diff, (unsigned long long)d);
+            __syncthreads();
+            const uint64_t diff = sh_diff;
+            /* 3. radix select over the key: prefix/pmask fix the bits
+             * found so far; done once the bucket is taken whole */
+            const int top = diff ? 63 - __builtin_clzll(diff) : -1;
+            /* bits above the top differing one (none when it is bit 63:
+             * 2 << 63 wraps to 0) */
+            uint64_t pmask = diff ? ~((2ull << top) - 1ull) : ~0ull;
+            uint64_t prefix = k0 & pmask;
+            int hi = top;
+            bool whole = false;
+            while (hi >= 0 && !whole) {
+                const int lo = hi - (FX_DBITS - 1) > 0 ? hi - (FX_DBITS - 1) : 0;
+                const uint32_t width = (uint32_t)(hi - lo + 1), nb = 1u << width;
+                for (uint32_t b = tid; b < nb; b += TN_THREADS) hist[b] = 0;
+                __syncthreads();
+                for (uint32_t i = tid; i < M; i += TN_THREADS) {
+                    const uint64_t k = lk[i];
+                    if ((k & pmask) == prefix)
+                        atomicAdd(&hist[(uint32_t)(k >> lo) & (nb - 1u)], 1u);
+                }
+                __syncthreads();
+                fx_select_digit(hist, nb, &sh_digit, &sh_rem);
+                prefix |= (uint64_t)sh_digit << lo;
+                pmask |= (uint64_t)(nb - 1u) << lo;
+                whole = hist[sh_digit] == sh_rem;
+                hi = lo - 1;
+                __syncthreads();
             }
+            /* 4. keys tied in full (pmask all ones): the rem smallest
+             * positions among them, by radix over ~position */
+            uint32_t qprefix = 0, qmask = 0;
+            if (!whole) {
+                for (int qh = 31; qh >= 0 && !whole;) {
+                    const int lo = qh - (FX_DBITS - 1) > 0 ? qh - (FX_DBITS - 1) : 0;
+                    const uint32_t width = (uint32_t)(qh - lo + 1), nb = 1u << width;
+                    for (uint32_t b = tid; b < nb; b += TN_THREADS) hist[b] = 0;
+                    __syncthreads();
+                    for (uint32_t i = tid; i < M; i += TN_THREADS) {
+                        const uint32_t q = ~lp[i];
+                        if (lk[i] == prefix && (q & qmask) == qprefix)
+                            atomicAdd(&hist[(q >> lo) & (nb - 1u)], 1u);
+                    }
+                    __syncthreads();
+                    fx_select_digit(hist, nb, &sh_digit, &sh_rem);
+                    qprefix |= sh_digit << lo;
+                    qmask |= (nb - 1u) << lo;
+                    whole = hist[sh_digit] == sh_rem;
+                    qh = lo - 1;
+                    __syncthreads();
+                }
+            }
+            /* 5. the selection: keys above the prefix, and the bucket
+             * (or the tied key's chosen positions) -- want of them */
+            for (uint32_t i = tid; i < M; i += TN_THREADS) {
+                const uint64_t k = lk[i];
+                const uint32_t q = ~lp[i];
+                const bool take = qmask == 0u
+                                      ? (k & pmask) >= prefix
+                                      : (k > prefix || (k == prefix && (q & qmask) >= qprefix));
+                if (take) {
+                    const uint32_t slot = atomicAdd(&sh_gt, 1u);
+                    if (slot < want) {
+                        sel_p[slot] = lp[i];
+                        sel_f[slot] = (int64_t)(k ^ 0x8000000000000000ull);
+                    }
+                }
+            }
+            __syncthreads();
+        } else {
+            /* the 8-bit radix select over global memory */
+            uint64_t prefix = 0, pmask = 0;
+            for (int shift = 56; shift >= 0; shift -= 8) {
+                hist[tid] = 0;
+                __syncthreads();
+                for_keys([&](uint32_t i, uint64_t k) {
+                    tn_hist_add(hist, (uint32_t)(k >> shift) & 255u,
+                                i < n && (k & pmask) == prefix);
+                });
+                __syncthreads();
+                tn_select_digit(hist, &sh_digit, &sh_rem);
+                prefix |= (uint64_t)sh_digit << shift;
+                pmask |= 255ull << shift;
+            }
+            const uint64_t K = prefix;
+            const uint32_t need_eq = sh_rem, n_gt = want - need_eq;
+            for_keys([&](uint32_t i, uint64_t k) {
+                if (i < n && k > K) {
+                    const uint32_t slot = atomicAdd(&sh_gt, 1u);
+                    if (slot < n_gt) {
+                        sel_p[slot] = i;
+                        sel_f[slot] = (int64_t)(k ^ 0x8000000000000000ull);
+                    }
+                }
+            });
+            uint32_t eq_base = 0;
+            for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
+                const uint32_t i = base + tid;
+                const bool f = i < n && key(i) == K;
+                const uint64_t bal = __ballot(f);
+                const uint32_t wr = (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull));
+                if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
+                __syncthreads();
+                uint32_t wpre = 0, tot = 0;
+                for (uint32_t w = 0; w < TN_WAVES; ++w) {
+                    if (w < wave) wpre += wcnt[w];
+                    tot += wcnt[w];
+                }
+                const uint32_t r = eq_base + wpre + wr;
+                if (f && r < need_eq) {
+                    sel_p[n_gt + r] = i;
+                    sel_f[n_gt + r] = fx[i];
+                }
+                eq_base += tot;
+                __syncthreads();
+            }
+            __syncthreads();
         }
-    });
-    uint32_t eq_base = 0;
-    for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
-        const uint32_t i = base + tid;
-        const bool f = i < n && key(i) == K;
-        const uint64_t bal = __ballot(f);
-        const uint32_t wr = (uint32_t)__builtin_popcountll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
-        __syncthreads();
-        uint32_t wpre = 0, tot = 0;
-        for (uint32_t w = 0; w < TN_WAVES; ++w) {
-            if (w < wave) wpre += wcnt[w];
-            tot += wcnt[w];
+        for (uint32_t i = tid; i < want; i += TN_THREADS) {
+            uint32_t rk = 0;
+            for (uint32_t j = 0; j < want; ++j)
+                rk += before_fx(sel_f[j], sel_p[j], sel_f[i], sel_p[i]) ? 1u : 0u;
+            top_pos[(size_t)s * N + rk] = sel_p[i];
+            top_exact[(size_t)s * N + rk] = (double)sel_f[i] * (1.0 / HFX_FX_SCALE);
+            top_f32[(size_t)s * N + rk] = sc[sel_p[i]];
         }
-        const uint32_t r = eq_base + wpre + wr;
-        if (f && r < need_eq) { sel_p[n_gt + r] = i; sel_f[n_gt + r] = fx[i]; }
-        eq_base += tot;
+        if (tid == 0) top_cnt[s] = want;
         __syncthreads();
     }
-    __syncthreads();
-    for (uint32_t i = tid; i < want; i += TN_THREADS) {
-        uint32_t rk = 0;
-        for (uint32_t j = 0; j < want; ++j)
-            rk += before_fx(sel_f[j], sel_p[j], sel_f[i], sel_p[i]) ? 1u : 0u;
-        top_pos[(size_t)s * N + rk] = sel_p[i];
-        top_exact[(size_t)s * N + rk] = (double)sel_f[i] * (1.0 / HFX_FX_SCALE);
-        top_f32[(size_t)s * N + rk] = sc[sel_p[i]];
-    }
-    if (tid == 0) top_cnt[s] = want;
 }
 
 /* ------------------------------------------------------------------ */
 /* Launch helpers                                                        */
 /* ------------------------------------------------------------------ */
 
+/* at most total / TX_CHUNK + S tiles: ceil(n / TX_CHUNK) per spectrum */
+static uint64_t tx_max_tiles(uint32_t S, uint64_t total) { return total / TX_CHUNK + S; }
+
 size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total)
 {
-    return total * 8 + ((size_t)S + 1) * 8 + (size_t)S * 8 + (size_t)S * 4 + 256;
+    return total * 8 + ((size_t)S + 1) * 8 + (size_t)S * 8 + (size_t)S * 4 + 256 +
+           tx_max_tiles(S, total) * 4 + (size_t)S * 4;
 }
 
-/* exact-route scratch: fxbuf[total] | tile_off[S+1] | tau[S] | tiles[S] | counter */
+/* exact-route scratch:
+ * fxbuf[total] | tile_off[S+1] | tau[S] | tiles[S] | counters (256 B) |
+ * tile_spec[max tiles] | list[S] */
 struct TxScratch {
     int64_t *fxbuf;
     uint64_t *tile_off;
     double *tau;
-    uint32_t *tiles, *counter;
+    uint32_t *tiles, *counter, *tile_spec, *list;
 };
 static TxScratch tx_carve(void *p, uint32_t S, uint64_t total)
 {
@@ -551,6 +740,8 @@ static TxScratch tx_carve(void *p, uint32_t S, uint64_t total)
     x.tau = (double *)(x.tile_off + S + 1);
     x.tiles = (uint32_t *)(x.tau + S);
     x.counter = x.tiles + S;
+    x.tile_spec = x.counter + 64;
+    x.list = x.tile_spec + tx_max_tiles(S, total);
     return x;
 }
 static int tx_scratch(hfx_ctx *ctx, void *given, uint32_t S, uint64_t total, void **out)
@@ -573,15 +764,18 @@ static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     int rc;
     if ((rc = tx_scratch(ctx, scratch, S, total, &p))) return rc;
     const TxScratch x = tx_carve(p, S, total);
-    hipLaunchKernelGGL(k_flag_tiles<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, x.tiles,
-                       d_flag, c, S);
-    if ((rc = hfx_scan_offsets(ctx, x.tile_off, x.tiles, S, (void *)st)) != HFX_OK) return rc;
     hipLaunchKernelGGL(k_tx_reset, dim3(1), dim3(1), 0, st, x.counter);
+    hipLaunchKernelGGL(k_flag_tiles<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, x.tiles,
+                       x.list, x.counter, d_flag, c, S);
+    if ((rc = hfx_scan_offsets(ctx, x.tile_off, x.tiles, S, (void *)st)) != HFX_OK) return rc;
+    hipLaunchKernelGGL(k_tile_spec, dim3((S + 255) / 256), dim3(256), 0, st, x.tile_spec,
+                       x.tile_off, S);
     hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 8),
                        dim3(TN_THREADS), tn_dyn_bytes(sp->x_stride), st, x.fxbuf, x.tile_off,
-                       x.counter, c, *sp, d_scores, x.tau);
-    hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(S), dim3(TN_THREADS), 0, st, d_top_pos, d_top_exact,
-                       d_top_f32, d_top_cnt, N, d_flag, x.fxbuf, d_scores, c);
+                       x.tile_spec, x.counter, c, *sp, d_scores, x.tau);
+    hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(min(S, (uint32_t)hfx_ctx_num_cus(ctx) * 2)),
+                       dim3(TN_THREADS), 0, st, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
+                       x.list, x.counter + 1, x.fxbuf, d_scores, c);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }
