@@ -161,6 +161,25 @@ __device__ __forceinline__ int32_t hfx_bin(float mz)
     return (int32_t)__builtin_rintf(mz / HFX_BIN_WIDTH_MONO);
 }
 
+/* x / d correctly rounded, given R = RN(1/d): q = RN(x R) is within an ulp,
+ * the residual x - q d is exact in one FMA, and one Newton step rounds to
+ * the nearest quotient.  Holds for every |x| in [1e-3, 65536) for the bin
+ * width and d = 1..4 (exhaustive CPU check); below 1e-3 both sides round to
+ * bin 0, above 65536 every bin lies beyond the row. */
+__device__ __forceinline__ float hfx_div(float x, float d, float R)
+{
+    const float q = x * R;
+    const float r = __builtin_fmaf(-q, d, x);
+    return __builtin_fmaf(r, R, q);
+}
+
+/* hfx_bin through hfx_div: the same bin for every theoretical peak */
+__device__ __forceinline__ int32_t hfx_bin_q(float mz)
+{
+    return (int32_t)__builtin_rintf(
+        hfx_div(mz, HFX_BIN_WIDTH_MONO, 1.0f / HFX_BIN_WIDTH_MONO));
+}
+
 /* Unary weight codes: OR of codes == code of the maximum weight
  * (10 -> 0b001, 25 -> 0b011, 50 -> 0b111); weight = {0,10,25,50}[popc]. */
 #define HFX_CODE10 1u
@@ -255,6 +274,8 @@ __device__ __forceinline__ void hfx_wave_peaks_z(uint32_t *map, uint32_t n, floa
             for (uint32_t z = zlo; z <= zmax; ++z) {
                 const float fz = (float)z;
                 const float zh = HFX_DEV_H * fz;
+                /* divisions by z (1..4) through hfx_div: correctly rounded */
+                const float rz = 1.0f / fz;
                 int32_t x;
 #define HFX_PUT(xx, code)                                                            \
     do {                                                                             \
@@ -262,20 +283,22 @@ __device__ __forceinline__ void hfx_wave_peaks_z(uint32_t *map, uint32_t n, floa
         if (x >= 0 && (uint32_t)x < len)                                             \
             atomicOr(&map[(uint32_t)x >> 3], (code) << (((uint32_t)x & 7u) * 4u));     \
     } while (0)
-                HFX_PUT(hfx_bin(((b - HFX_DEV_CO) + zh) / fz), HFX_CODE10);
-                const float mb = (b + zh) / fz;
-                const int32_t xb = hfx_bin(mb);
+                const float h2o = hfx_div(HFX_DEV_H2O, fz, rz);
+                const float nh3 = hfx_div(HFX_DEV_NH3, fz, rz);
+                HFX_PUT(hfx_bin_q(hfx_div((b - HFX_DEV_CO) + zh, fz, rz)), HFX_CODE10);
+                const float mb = hfx_div(b + zh, fz, rz);
+                const int32_t xb = hfx_bin_q(mb);
                 HFX_PUT(xb, HFX_CODE50);
                 HFX_PUT(xb + 1, HFX_CODE25);
                 HFX_PUT(xb - 1, HFX_CODE25);
-                HFX_PUT(hfx_bin(mb - HFX_DEV_H2O / fz), HFX_CODE10);
-                HFX_PUT(hfx_bin(mb - HFX_DEV_NH3 / fz), HFX_CODE10);
-                const float my = ((y + HFX_DEV_H2O) + zh) / fz;
-                const int32_t xy = hfx_bin(my);
+                HFX_PUT(hfx_bin_q(mb - h2o), HFX_CODE10);
+                HFX_PUT(hfx_bin_q(mb - nh3), HFX_CODE10);
+                const float my = hfx_div((y + HFX_DEV_H2O) + zh, fz, rz);
+                const int32_t xy = hfx_bin_q(my);
                 HFX_PUT(xy, HFX_CODE50);
                 HFX_PUT(xy + 1, HFX_CODE25);
                 HFX_PUT(xy - 1, HFX_CODE25);
-                HFX_PUT(hfx_bin(my - HFX_DEV_NH3 / fz), HFX_CODE10);
+                HFX_PUT(hfx_bin_q(my - nh3), HFX_CODE10);
 #undef HFX_PUT
             }
         }

@@ -86,24 +86,6 @@ typedef uint32_t dn_u32x4 __attribute__((ext_vector_type(4)));
 /* Arithmetic                                                            */
 /* ------------------------------------------------------------------ */
 
-/* x / d correctly rounded, given R = RN(1/d): q = RN(x R) is within an ulp,
- * the residual x - q d is exact in one FMA, and one Newton step rounds to
- * the nearest quotient.  Holds for every |x| in [1e-3, 65536) for the bin
- * width and d = 1..4 (exhaustive CPU check); below 1e-3 both sides round to
- * bin 0, above 65536 every bin lies beyond the row. */
-__device__ __forceinline__ float dn_div(float x, float d, float R)
-{
-    const float q = x * R;
-    const float r = __builtin_fmaf(-q, d, x);
-    return __builtin_fmaf(r, R, q);
-}
-
-__device__ __forceinline__ int32_t dn_bin(float mz)
-{
-    const float R = 1.0f / HFX_BIN_WIDTH_MONO;
-    return (int32_t)__builtin_rintf(dn_div(mz, HFX_BIN_WIDTH_MONO, R));
-}
-
 /* round-to-nearest-even FP32 -> BF16 (finite inputs) */
 __device__ __forceinline__ uint32_t dn_bf16(float f)
 {
@@ -329,7 +311,7 @@ __device__ __forceinline__ void dn_put3(uint32_t *row, int32_t x, uint32_t kq, u
 __device__ __forceinline__ uint32_t dn_binu(float mz)
 {
     const float R = 1.0f / HFX_BIN_WIDTH_MONO;
-    return __float_as_uint(dn_div(mz, HFX_BIN_WIDTH_MONO, R) + DN_MAGIC);
+    return __float_as_uint(hfx_div(mz, HFX_BIN_WIDTH_MONO, R) + DN_MAGIC);
 }
 /* rowm: the row's word-0 address less DN_MAGIC_WORDS words (a 32-bit LDS
  * address that wraps; only rowm + (u >> 3) is ever dereferenced) */
@@ -354,7 +336,7 @@ __device__ __forceinline__ void dn_put3_u(uint32_t *rowm, uint32_t u)
 template <int Z>
 __device__ __forceinline__ float dn_zdiv(float x)
 {
-    if constexpr (Z == 3) return dn_div(x, 3.0f, 1.0f / 3.0f);
+    if constexpr (Z == 3) return hfx_div(x, 3.0f, 1.0f / 3.0f);
     else return x * (1.0f / (float)Z);
 }
 
@@ -372,12 +354,12 @@ __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32
     const float mb = dn_zdiv<Z>(b + zh);
     const float my = dn_zdiv<Z>((y + HFX_DEV_H2O) + zh);
     if constexpr (CHECK) {
-        dn_put(row, dn_bin(ma), HFX_CODE10, kq, lane);
-        dn_put3(row, dn_bin(mb), kq, lane);
-        dn_put(row, dn_bin(mb - h2o), HFX_CODE10, kq, lane);
-        dn_put(row, dn_bin(mb - nh3), HFX_CODE10, kq, lane);
-        dn_put3(row, dn_bin(my), kq, lane);
-        dn_put(row, dn_bin(my - nh3), HFX_CODE10, kq, lane);
+        dn_put(row, hfx_bin_q(ma), HFX_CODE10, kq, lane);
+        dn_put3(row, hfx_bin_q(mb), kq, lane);
+        dn_put(row, hfx_bin_q(mb - h2o), HFX_CODE10, kq, lane);
+        dn_put(row, hfx_bin_q(mb - nh3), HFX_CODE10, kq, lane);
+        dn_put3(row, hfx_bin_q(my), kq, lane);
+        dn_put(row, hfx_bin_q(my - nh3), HFX_CODE10, kq, lane);
     } else {
         uint32_t *rowm = row - DN_MAGIC_WORDS;
         dn_put_u(rowm, dn_binu(ma), HFX_CODE10);

@@ -33,15 +33,25 @@
 #define TN_EXT 256
 #define TN_MAXC (TN_MAXT + TN_EXT)
 
+/* the first select pass histograms the key's 11 high bits */
+#define TN_HBITS 11
+#define TN_HBINS (1u << TN_HBITS)
+
 /* The staged spectrum and the waves' maps live in dynamic LDS sized by the
- * spectra's row stride: bins of a row, rounded up to whole map words */
+ * spectra's row stride: bins of a row, rounded up to whole map words.  The
+ * spectrum's region is at least TN_HBINS words: before the spectrum is
+ * staged it holds the select's histogram. */
 __host__ __device__ inline uint32_t tn_xcap(uint32_t x_stride)
 {
     return ((x_stride < HFX_MAX_LEN ? x_stride : HFX_MAX_LEN) + 7u) & ~7u;
 }
+__host__ __device__ inline size_t tn_xs_bytes(uint32_t x_stride)
+{
+    return (size_t)(tn_xcap(x_stride) > TN_HBINS ? tn_xcap(x_stride) : TN_HBINS) * 4;
+}
 __host__ __device__ inline size_t tn_dyn_bytes(uint32_t x_stride)
 {
-    return (size_t)tn_xcap(x_stride) * 4 + (size_t)TN_WAVES * (tn_xcap(x_stride) / 8) * 4;
+    return tn_xs_bytes(x_stride) + (size_t)TN_WAVES * (tn_xcap(x_stride) / 8) * 4;
 }
 
 __device__ __forceinline__ bool before_fx(int64_t fa, uint32_t pa, int64_t fb, uint32_t pb)
@@ -49,21 +59,23 @@ __device__ __forceinline__ bool before_fx(int64_t fa, uint32_t pa, int64_t fb, u
     return fa > fb || (fa == fb && pa < pb);
 }
 
-/* One radix-select step on a 256-bucket histogram: the digit d holding the
- * rem-th largest key (counting from bucket 255 down) and the rank left inside
- * it.  Wave 0 does it in parallel: lane l sums buckets 255-4l .. 252-4l, an
- * inclusive scan over the lanes finds the lane whose range holds rank rem,
- * and that lane walks its four buckets.  Every thread must call it (it
- * contains the barriers); all get the same (digit, rem). */
-__device__ __forceinline__ void tn_select_digit(const uint32_t *hist, uint32_t *sh_digit,
-                                                uint32_t *sh_rem)
+/* One radix-select step on an nb-bucket histogram: the digit holding the
+ * rem-th largest element (counting from bucket nb - 1 down) and the rank
+ * left inside it.  Wave 0 does it: lane l sums its run of nb / 64 buckets
+ * (one at least), an inclusive scan over the lanes finds the run that holds
+ * rank rem, and that lane walks it.  Every thread must call it (it ends with
+ * a barrier); all get the same (digit, rem). */
+__device__ __forceinline__ void tn_select_digit(const uint32_t *hist, uint32_t nb,
+                                                uint32_t *sh_digit, uint32_t *sh_rem)
 {
     const uint32_t tid = threadIdx.x, lane = hfx_lane();
     if (tid < HFX_WAVE) {
         const uint32_t rem = *sh_rem;
-        const uint32_t top = 255u - 4u * lane;
-        const uint32_t h0 = hist[top], h1 = hist[top - 1], h2 = hist[top - 2], h3 = hist[top - 3];
-        const uint32_t sum = h0 + h1 + h2 + h3;
+        const uint32_t per = (nb + HFX_WAVE - 1) / HFX_WAVE;
+        const int top = (int)nb - 1 - (int)(lane * per);
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < per; ++k)
+            if (top - (int)k >= 0) sum += hist[top - (int)k];
         uint32_t incl = sum;
 #pragma unroll
         for (uint32_t o = 1; o < HFX_WAVE; o <<= 1) {
@@ -72,15 +84,16 @@ __device__ __forceinline__ void tn_select_digit(const uint32_t *hist, uint32_t *
         }
         const uint32_t excl = incl - sum;
         if (excl < rem && rem <= incl) {
-            uint32_t cum = excl, d = top - 3u;
-            const uint32_t hh[4] = {h0, h1, h2, h3};
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                if (cum + hh[k] >= rem) { d = top - k; break; }
-                cum += hh[k];
+            uint32_t cum = excl;
+            for (uint32_t k = 0; k < per && top - (int)k >= 0; ++k) {
+                const uint32_t h = hist[top - (int)k];
+                if (cum + h >= rem) {
+                    *sh_digit = (uint32_t)(top - (int)k);
+                    *sh_rem = rem - cum;
+                    break;
+                }
+                cum += h;
             }
-            *sh_digit = d;
-            *sh_rem = rem - cum;
         }
     }
     __syncthreads();
@@ -113,9 +126,9 @@ __global__ void __launch_bounds__(TN_THREADS)
     extern __shared__ __align__(16) unsigned char tn_dyn[];
     const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
     float *xs = (float *)tn_dyn;
-    uint32_t *maps = (uint32_t *)(tn_dyn + (size_t)xcap * 4);
+    uint32_t *hist = (uint32_t *)tn_dyn; /* until the spectrum is staged */
+    uint32_t *maps = (uint32_t *)(tn_dyn + tn_xs_bytes(sp.x_stride));
     __shared__ float mtab[32];
-    __shared__ uint32_t hist[256];
     __shared__ uint32_t cp[TN_MAXC];
     __shared__ int64_t cfx[TN_MAXC];
     __shared__ float cf32[TN_MAXC];
@@ -152,12 +165,14 @@ __global__ void __launch_bounds__(TN_THREADS)
         }
     };
 
-    /* 1. the T largest keys (key desc, position asc).  Radix select over the
-     * key's two high bytes; then every key at or above that 16-bit prefix --
-     * M of them, known from the histograms -- is collected and ranked
-     * exactly.  With more than TN_MAXC of them (long runs of near-equal
-     * scores) the select goes on over the low bytes and the keys above the
-     * threshold, then the smallest positions equal to it, are collected. */
+    /* 1. the T largest keys (key desc, position asc).  Radix select, first
+     * over the key's 11 high bits (sign, exponent, two mantissa bits: a
+     * quarter octave a bucket), then 8, 8 and 5 bits; after each pass, once
+     * the keys at or above the prefix found so far -- M of them, known from
+     * the histograms -- number at most TN_MAXC, they are collected and ranked
+     * exactly.  Otherwise (long runs of near-equal scores) the full threshold
+     * key K is found and the keys above it, then the smallest positions
+     * equal to it, are collected. */
     uint32_t prefix = 0, pmask = 0;
     if (tid == 0) {
         sh_rem = T;
@@ -166,42 +181,43 @@ __global__ void __launch_bounds__(TN_THREADS)
         sh_last = 0;
     }
     bool ranked = false;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        hist[tid] = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+        const uint32_t shift = pass == 0 ? 32u - TN_HBITS : pass == 1 ? 13u : pass == 2 ? 5u : 0u;
+        const uint32_t nb = pass == 0 ? TN_HBINS : pass == 3 ? 32u : 256u;
+        for (uint32_t b = tid; b < nb; b += TN_THREADS) hist[b] = 0;
         __syncthreads();
         for_keys([&](uint32_t i, uint32_t k) {
-            tn_hist_add(hist, (k >> shift) & 255u, i < n && (k & pmask) == prefix);
+            tn_hist_add(hist, (k >> shift) & (nb - 1u), i < n && (k & pmask) == prefix);
         });
         __syncthreads();
-        tn_select_digit(hist, &sh_digit, &sh_rem);
+        tn_select_digit(hist, nb, &sh_digit, &sh_rem);
         prefix |= sh_digit << shift;
-        pmask |= 255u << shift;
-        if (shift == 16) {
-            const uint32_t M = T - sh_rem + hist[sh_digit];
-            if (M <= TN_MAXC) {
-                /* (key, ~position) packed so that a larger value ranks first;
-                 * cfx holds them until the exact scores replace them */
-                uint64_t *kv = (uint64_t *)cfx;
-                for_keys([&](uint32_t i, uint32_t k) {
-                    if (i < n && (k >> 16) >= (prefix >> 16))
-                        kv[atomicAdd(&sh_gt, 1u)] = (uint64_t)k << 32 | (0xffffffffu - i);
-                });
-                __syncthreads();
-                for (uint32_t a = tid; a < M; a += TN_THREADS) {
-                    const uint64_t va = kv[a];
-                    uint32_t rk = 0;
-                    for (uint32_t b = 0; b < M; ++b) rk += kv[b] > va ? 1u : 0u;
-                    const uint32_t pa = 0xffffffffu - (uint32_t)va;
-                    if (rk < T) cp[rk] = pa;
-                    if (rk == T - 1u) {
-                        sh_digit = (uint32_t)(va >> 32);
-                        sh_last = pa;
-                    }
+        pmask |= (nb - 1u) << shift;
+        const uint32_t M = T - sh_rem + hist[sh_digit];
+        __syncthreads(); /* every thread has read hist before it is cleared */
+        if (M <= TN_MAXC) {
+            /* (key, ~position) packed so that a larger value ranks first;
+             * cfx holds them until the exact scores replace them */
+            uint64_t *kv = (uint64_t *)cfx;
+            for_keys([&](uint32_t i, uint32_t k) {
+                if (i < n && (k & pmask) >= prefix)
+                    kv[atomicAdd(&sh_gt, 1u)] = (uint64_t)k << 32 | (0xffffffffu - i);
+            });
+            __syncthreads();
+            for (uint32_t a = tid; a < M; a += TN_THREADS) {
+                const uint64_t va = kv[a];
+                uint32_t rk = 0;
+                for (uint32_t b = 0; b < M; ++b) rk += kv[b] > va ? 1u : 0u;
+                const uint32_t pa = 0xffffffffu - (uint32_t)va;
+                if (rk < T) cp[rk] = pa;
+                if (rk == T - 1u) {
+                    sh_digit = (uint32_t)(va >> 32);
+                    sh_last = pa;
                 }
-                __syncthreads();
-                ranked = true;
-                break;
             }
+            __syncthreads();
+            ranked = true;
+            break;
         }
     }
     const uint32_t K = ranked ? sh_digit : prefix;
@@ -401,7 +417,7 @@ __global__ void __launch_bounds__(TN_THREADS)
     extern __shared__ __align__(16) unsigned char tn_dyn[];
     const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
     float *xs = (float *)tn_dyn;
-    uint32_t *maps = (uint32_t *)(tn_dyn + (size_t)xcap * 4);
+    uint32_t *maps = (uint32_t *)(tn_dyn + tn_xs_bytes(sp.x_stride));
     __shared__ float mtab[32];
     __shared__ uint32_t sh_t;
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
@@ -464,44 +480,6 @@ __global__ void __launch_bounds__(TN_THREADS)
  * memory.  A persistent grid walks the list of flagged spectra. */
 #define FX_CAP 4096
 #define FX_DBITS 11
-
-/* the digit holding the rem-th largest element of an nb-bucket histogram
- * (counting from bucket nb - 1 down), and the rank left inside it: wave 0's
- * lane l sums its run of buckets, a scan over the lanes finds the run that
- * holds rank rem, and that lane walks it.  Every thread calls it. */
-__device__ __forceinline__ void fx_select_digit(const uint32_t *hist, uint32_t nb,
-                                                uint32_t *sh_digit, uint32_t *sh_rem)
-{
-    const uint32_t tid = threadIdx.x, lane = hfx_lane();
-    if (tid < HFX_WAVE) {
-        const uint32_t rem = *sh_rem;
-        const uint32_t per = (nb + HFX_WAVE - 1) / HFX_WAVE;
-        const int top = (int)nb - 1 - (int)(lane * per);
-        uint32_t sum = 0;
-        for (uint32_t k = 0; k < per; ++k)
-            if (top - (int)k >= 0) sum += hist[top - (int)k];
-        uint32_t incl = sum;
-#pragma unroll
-        for (uint32_t o = 1; o < HFX_WAVE; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        const uint32_t excl = incl - sum;
-        if (excl < rem && rem <= incl) {
-            uint32_t cum = excl;
-            for (uint32_t k = 0; k < per && top - (int)k >= 0; ++k) {
-                const uint32_t h = hist[top - (int)k];
-                if (cum + h >= rem) {
-                    *sh_digit = (uint32_t)(top - (int)k);
-                    *sh_rem = rem - cum;
-                    break;
-                }
-                cum += h;
-            }
-        }
-    }
-    __syncthreads();
-}
 
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
@@ -601,7 +579,7 @@ __global__ void __launch_bounds__(TN_THREADS)
                         atomicAdd(&hist[(uint32_t)(k >> lo) & (nb - 1u)], 1u);
                 }
                 __syncthreads();
-                fx_select_digit(hist, nb, &sh_digit, &sh_rem);
+                tn_select_digit(hist, nb, &sh_digit, &sh_rem);
                 prefix |= (uint64_t)sh_digit << lo;
                 pmask |= (uint64_t)(nb - 1u) << lo;
                 whole = hist[sh_digit] == sh_rem;
@@ -623,7 +601,7 @@ __global__ void __launch_bounds__(TN_THREADS)
                             atomicAdd(&hist[(q >> lo) & (nb - 1u)], 1u);
                     }
                     __syncthreads();
-                    fx_select_digit(hist, nb, &sh_digit, &sh_rem);
+                    tn_select_digit(hist, nb, &sh_digit, &sh_rem);
                     qprefix |= sh_digit << lo;
                     qmask |= (nb - 1u) << lo;
                     whole = hist[sh_digit] == sh_rem;
@@ -659,7 +637,7 @@ __global__ void __launch_bounds__(TN_THREADS)
                                 i < n && (k & pmask) == prefix);
                 });
                 __syncthreads();
-                tn_select_digit(hist, &sh_digit, &sh_rem);
+                tn_select_digit(hist, 256u, &sh_digit, &sh_rem);
                 prefix |= (uint64_t)sh_digit << shift;
                 pmask |= 255ull << shift;
             }
