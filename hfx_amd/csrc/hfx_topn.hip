@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(TN_THREADS)
     __shared__ uint32_t wcnt[TN_WAVES];
     __shared__ float sh_x1[TN_WAVES];
     __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_last, sh_ext;
-    __shared__ int64_t sh_exN;
+    __shared__ float sh_fN;
 
     const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
     const uint32_t n = c.count(s);
@@ -254,82 +254,94 @@ __global__ void __launch_bounds__(TN_THREADS)
         }
     }
 
-    /* 2. exact re-score of the selection */
+    /* 2. the spectrum, the bound B on |FP32 - exact| (from sum |x|), and the
+     * N-th best FP32 score fN of the selection */
     const uint32_t len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
     uint32_t *map = maps + wave * mwords;
     for (uint32_t w = lane; w < mwords; w += HFX_WAVE) map[w] = 0;
+    for (uint32_t i = tid; i < T; i += TN_THREADS) cf32[i] = sc[cp[i]];
     __syncthreads();
-    const uint32_t zeff = c.zeff(sp.d_zmax[s]);
-    /* the wave's candidates i = wave + 4 j: lane j fetches candidate i's
-     * metadata, all of them in one round */
-    typename Cands::Prep mine{};
-    if (wave + TN_WAVES * lane < T) mine = c.prep(s, cp[wave + TN_WAVES * lane]);
-    for (uint32_t i = wave, j = 0; i < T; i += TN_WAVES, ++j) {
-        c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
-        hfx_wave_sync();
-        const int64_t fx = hfx_wave_dot_fx(map, xs, len);
-        hfx_wave_sync();
-        if (lane == 0) {
-            cfx[i] = fx;
-            cf32[i] = sc[cp[i]];
-        }
-    }
     float x1 = 0.0f;
     for (uint32_t i = tid; i < len; i += TN_THREADS) x1 += fabsf(xs[i]);
     x1 = hfx_wave_sum_f32(x1);
     if (lane == 0) sh_x1[wave] = x1;
-    __syncthreads();
-
-    /* 3. every other candidate that could still reach the N-th exact score */
-    uint32_t M = T;
-    uint32_t fl = 0;
-    if (n > T) {
+    if (n > T)
         for (uint32_t i = tid; i < T; i += TN_THREADS) {
+            const uint32_t ki = hfx_f2key(cf32[i]), pi = cp[i];
             uint32_t rk = 0;
-            for (uint32_t j = 0; j < T; ++j) rk += before_fx(cfx[j], cp[j], cfx[i], cp[i]) ? 1u : 0u;
-            if (rk == N - 1) sh_exN = cfx[i];
+            for (uint32_t j = 0; j < T; ++j) {
+                const uint32_t kj = hfx_f2key(cf32[j]);
+                rk += (kj > ki || (kj == ki && cp[j] < pi)) ? 1u : 0u;
+            }
+            if (rk == N - 1) sh_fN = cf32[i];
         }
-        __syncthreads();
-        float xa = 0.0f;
-        for (uint32_t w = 0; w < TN_WAVES; ++w) xa += sh_x1[w];
-        const double rel = (double)rel0 + (double)rel1 * (double)zeff;
-        const double bound = rel * 50.0 * (double)xa * 1.001 + 1e-6;
-        const double tau = (double)sh_exN * (1.0 / HFX_FX_SCALE) - bound;
+    __syncthreads();
+    const uint32_t zeff = c.zeff(sp.d_zmax[s]);
+    float xa = 0.0f;
+    for (uint32_t w = 0; w < TN_WAVES; ++w) xa += sh_x1[w];
+    const double rel = (double)rel0 + (double)rel1 * (double)zeff;
+    const double bound = rel * 50.0 * (double)xa * 1.001 + 1e-6;
+
+    /* 3. the extension: every other candidate that could reach the N-th
+     * exact score exN.  The N best FP32 scores are selected and each has an
+     * exact score >= fN - B, so exN >= fN - B: a candidate with FP32 score
+     * below fN - 2B has an exact score below exN - B + B = exN, under N
+     * others.  (A relative 1e-12 margin covers the double roundings.)  All
+     * candidates at or above fN - 2B are re-scored with the selection, in one
+     * round.  More than TN_EXT of them: the spectrum is flagged, the
+     * selection alone is re-scored, and the exact route takes every
+     * candidate at or above exN - B. */
+    uint32_t ne = 0, fl = 0;
+    if (n > T) {
+        const double fN = (double)sh_fN;
+        const double tau2 = fN - 2.0 * bound - 1e-12 * (fabs(fN) + 2.0 * bound);
         const uint32_t last = sh_last;
         for_keys([&](uint32_t i, uint32_t k) {
             const bool selected = k > K || (k == K && i <= last);
-            if (i < n && !selected && (double)hfx_key2f(k) >= tau) {
+            if (i < n && !selected && (double)hfx_key2f(k) >= tau2) {
                 const uint32_t slot = atomicAdd(&sh_ext, 1u);
                 if (slot < TN_EXT) cp[T + slot] = i;
             }
         });
         __syncthreads();
-        const uint32_t ne = sh_ext;
+        ne = sh_ext;
         if (ne > TN_EXT) {
-            /* the exact route re-scores every candidate at or above tau */
             fl = 1;
-            if (tid == 0) tau_out[s] = tau;
-        } else {
-            /* at most TN_EXT / 4 = 64 per wave: one lane each */
-            typename Cands::Prep ext{};
-            if (T + wave + TN_WAVES * lane < T + ne)
-                ext = c.prep(s, cp[T + wave + TN_WAVES * lane]);
-            for (uint32_t i = T + wave, j = 0; i < T + ne; i += TN_WAVES, ++j) {
-                c.peaks_prep(map, mtab, Cands::take(ext, j), len, zeff);
-                hfx_wave_sync();
-                const int64_t fx = hfx_wave_dot_fx(map, xs, len);
-                hfx_wave_sync();
-                if (lane == 0) {
-                    cfx[i] = fx;
-                    cf32[i] = sc[cp[i]];
-                }
-            }
-            M = T + ne;
+            ne = 0;
         }
-        __syncthreads();
     }
 
-    /* 4. order by (exact desc, position asc), emit N */
+    /* 4. exact re-score of the selection and the extension: the wave's
+     * candidates i = wave + 4 j, 64 at a time, lane j fetching candidate i's
+     * metadata */
+    const uint32_t M = T + ne;
+    for (uint32_t c0 = 0; c0 < M; c0 += TN_WAVES * HFX_WAVE) {
+        typename Cands::Prep mine{};
+        if (c0 + wave + TN_WAVES * lane < M) mine = c.prep(s, cp[c0 + wave + TN_WAVES * lane]);
+        for (uint32_t i = c0 + wave, j = 0; i < M && j < HFX_WAVE; i += TN_WAVES, ++j) {
+            c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
+            hfx_wave_sync();
+            const int64_t fx = hfx_wave_dot_fx(map, xs, len);
+            hfx_wave_sync();
+            if (lane == 0) {
+                cfx[i] = fx;
+                if (i >= T) cf32[i] = sc[cp[i]];
+            }
+        }
+    }
+    __syncthreads();
+    if (fl) {
+        /* exN, the N-th best exact score of the selection, for the exact
+         * route's threshold */
+        for (uint32_t i = tid; i < T; i += TN_THREADS) {
+            uint32_t rk = 0;
+            for (uint32_t j = 0; j < T; ++j)
+                rk += before_fx(cfx[j], cp[j], cfx[i], cp[i]) ? 1u : 0u;
+            if (rk == N - 1) tau_out[s] = (double)cfx[i] * (1.0 / HFX_FX_SCALE) - bound;
+        }
+    }
+
+    /* 5. order by (exact desc, position asc), emit N */
     const uint32_t outn = min(N, n);
     for (uint32_t i = tid; i < M; i += TN_THREADS) {
         const int64_t fi = cfx[i];
