@@ -41,6 +41,24 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA, no sparsity
 
 
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload`, as measured by the
+    counter runs summarised in profiles/<kernel>_counters_<workload>.md (the
+    line 'HBM traffic per launch (bytes): N': rocprofv3 FETCH_SIZE plus
+    WRITE_SIZE, each scaled by a copy kernel of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_counters_{workload}.md")
+    try:
+        with open(path) as f:
+            m = re.search(r"HBM traffic per launch \(bytes\): ([0-9,]+)", f.read())
+    except OSError:
+        return None
+    if not m:
+        return None
+    return {"bytes": int(m.group(1).replace(",", "")),
+            "source": os.path.relpath(path, ROOT)}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -316,11 +334,15 @@ def main():
     p = paths[main_key]
     if main_key == "batched_bf16x2_dense":
         achieved = p["mfma_flops_executed"] / (p["kernel_ms"] * 1e-3) / 1e12
+        tr = measured_traffic("k_dense", args.workload)
         roofline = {"kernel": p["kernel"], "bound": "mfma", "achieved": achieved,
                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                    "frac": achieved / BF16_PEAK_TFLOPS,
+                    "traffic": tr["bytes"] if tr else None,
                     "kernel_ms": p["kernel_ms"], "flops_executed": p["mfma_flops_executed"],
                     "flops_useful": p["mfma_flops_useful"]}
+        if tr:
+            roofline["traffic_source"] = tr["source"]
         dtype = "bf16x2"
     else:
         ab = algorithmic_bytes(env, res)
