@@ -228,3 +228,42 @@ def test_dense_light_first_residue(orc):
     res2 = bs.results(keep_scores=True)
     check_scores(res2, ref, dsp, orc)
     check_topn(res2, ref, orc)
+
+
+def _expected_topn(ref, orc, s, N):
+    """the oracle's top N of spectrum s for any N: every exact score of the
+    window ordered by (exact desc, position asc)"""
+    a, b = int(ref["off"][s]), int(ref["off"][s + 1])
+    fx = ref["all_fx"][a:b]
+    order = np.lexsort((np.arange(b - a), -fx))[:N]
+    return order.tolist(), orc.fx_to_double(fx[order])
+
+
+@pytest.mark.parametrize("N,margin", [(1, 0), (3, 5), (64, 0), (64, 64)])
+def test_batch_dense_topn_depths(denv, orc, N, margin):
+    """Selections of T = N + margin from 1 to 128 keys, and the band of
+    candidates within 2B of the N-th score re-scored with them: the top N
+    equal the oracle's at every depth."""
+    pl, ref = denv["pl"], denv["ref"]
+    bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=N, margin=margin)
+    bs.set_precision(bs.BF16X2_DENSE)
+    bs.run()
+    res = bs.results()
+    for s in range(len(res.top_cnt)):
+        pos, ex = _expected_topn(ref, orc, s, N)
+        c = int(res.top_cnt[s])
+        assert c == len(pos), s
+        assert res.top_pos[s, :c].tolist() == pos, (s, N, margin)
+        assert (res.top_exact[s, :c] == ex).all(), (s, N, margin)
+
+
+def test_plain_sparse_topn_deep_margin(denv, orc):
+    """The plain form's top-N at the largest selection (N 64, margin 64)."""
+    pl, ref = denv["pl"], denv["ref"]
+    srch = pl.PlainSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=64, margin=64)
+    res = srch.run()
+    for s in range(len(res.top_cnt)):
+        pos, ex = _expected_topn(ref, orc, s, 64)
+        c = int(res.top_cnt[s])
+        assert c == len(pos) and res.top_pos[s, :c].tolist() == pos, s
+        assert (res.top_exact[s, :c] == ex).all(), s
