@@ -64,6 +64,8 @@
  * 2 k-groups that a half-wave reads or clears fall in 32 distinct banks */
 #define DN_MSTR 290
 static_assert(DN_MSTR >= DN_MAXK / 8 + 2 && DN_MSTR % 32 == 2, "map row stride");
+/* the clamped peaks' words past kq / 8 (kq <= DN_MAXK) stay inside the row */
+static_assert(1 + DN_MAXK / 8 + 16 < DN_MSTR, "clamped peak words within a map row");
 #define DN_CSTR (DN_MAXK + 8)       /* BF16 per B column in LDS (padded) */
 #define DN_PRE 2                    /* ladder positions per lane loaded a tile ahead */
 #define DN_NONE 0xffffffffu
@@ -274,32 +276,6 @@ struct DnArgs {
  * the contraction never reads, so a bin in [-8, 0) needs no check).  `row`
  * below points at word 1. */
 
-/* checked form: a bin outside [0, kq) ORs 0 into a word private to the lane
- * (no branch, no shared address) */
-__device__ __forceinline__ void dn_put(uint32_t *row, int32_t x, uint32_t code, uint32_t kq,
-                                       uint32_t lane)
-{
-    const uint32_t v = (uint32_t)x < kq ? code << (((uint32_t)x & 7u) << 2) : 0u;
-    atomicOr(&row[v ? (uint32_t)x >> 3 : lane], v);
-}
-
-/* checked 25 at x-1, 50 at x, 25 at x+1 (each kept only below kq): the
- * nibbles 3, 7, 3 span at most two words; the word of x-1 gets them shifted
- * into place, the word of x+1 whatever spilled over (the same value again
- * when all three share a word) */
-__device__ __forceinline__ void dn_put3(uint32_t *row, int32_t x, uint32_t kq, uint32_t lane)
-{
-    const int32_t xm = x - 1;
-    const uint32_t sh = ((uint32_t)xm & 7u) << 2;
-    const uint32_t t = ((uint32_t)xm < kq ? HFX_CODE25 : 0u) |
-                       ((uint32_t)x < kq ? HFX_CODE50 << 4 : 0u) |
-                       ((uint32_t)(x + 1) < kq ? HFX_CODE25 << 8 : 0u);
-    const uint32_t v1 = t << sh;
-    const uint32_t v2 = sh >= 24u ? t >> (32u - sh) : v1;
-    atomicOr(&row[v1 ? (uint32_t)xm >> 3 : lane], v1);
-    atomicOr(&row[v2 ? (uint32_t)(x + 1) >> 3 : lane], v2);
-}
-
 /* Unchecked forms, for tiles whose every bin lies in [1, kq): a bin comes as
  * the bits u of x + 1.5 * 2^23 -- the float addition rounds x to the nearest
  * integer, ties to even (as rint), and leaves it in the low mantissa bits, so
@@ -332,6 +308,22 @@ __device__ __forceinline__ void dn_put3_u(uint32_t *rowm, uint32_t u)
     atomicOr(w + 1, (uint32_t)(t >> 32));
 }
 
+/* Clamped form, for tiles whose peaks may fall at or past kq (or, with a
+ * light first residue, below 0): the bits u of a bin are clamped into
+ * [bin -8, bin kq + 8 (lane % 16)] (one v_med3).  Bins below 0 land in the
+ * row's margin word 0, which the contraction never reads; bins at or past kq
+ * in words kq / 8 .. kq / 8 + 16 (kq is a multiple of 128; a triple that
+ * starts in the last of a lane's words spills into the next), which this
+ * tile's contraction does not read and which are cleared before the next
+ * tile's peaks.  The lane-dependent ceiling spreads the clamped writes over
+ * sixteen words, so that they do not all queue on one address. */
+#define DN_ULO (0x4B400000u - 8u)
+#define DN_TRASH 17u /* words past kq / 8 that clamped peaks may touch */
+__device__ __forceinline__ uint32_t dn_clamp(uint32_t u, uint32_t uhi)
+{
+    return min(max(u, DN_ULO), uhi);
+}
+
 /* division by the fragment charge Z: exact as a product for Z = 1, 2, 4 */
 template <int Z>
 __device__ __forceinline__ float dn_zdiv(float x)
@@ -343,7 +335,8 @@ __device__ __forceinline__ float dn_zdiv(float x)
 /* peaks of one ladder position (b, y) at charge Z into a map row: the ion
  * series of ion_series.cu:111-156 with the FP32 operation order of the
  * sparse path (zh = H * Z, H2O / Z and NH3 / Z fold to the same FP32 values);
- * CHECK keeps only bins in [0, kq) */
+ * CHECK keeps only bins in [0, kq) (the others go to words this tile's
+ * contraction does not read) */
 template <int Z, bool CHECK>
 __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32_t kq,
                                          uint32_t lane)
@@ -354,12 +347,14 @@ __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32
     const float mb = dn_zdiv<Z>(b + zh);
     const float my = dn_zdiv<Z>((y + HFX_DEV_H2O) + zh);
     if constexpr (CHECK) {
-        dn_put(row, hfx_bin_q(ma), HFX_CODE10, kq, lane);
-        dn_put3(row, hfx_bin_q(mb), kq, lane);
-        dn_put(row, hfx_bin_q(mb - h2o), HFX_CODE10, kq, lane);
-        dn_put(row, hfx_bin_q(mb - nh3), HFX_CODE10, kq, lane);
-        dn_put3(row, hfx_bin_q(my), kq, lane);
-        dn_put(row, hfx_bin_q(my - nh3), HFX_CODE10, kq, lane);
+        uint32_t *rowm = row - DN_MAGIC_WORDS;
+        const uint32_t uhi = 0x4B400000u + kq + 8u * (lane & 15u);
+        dn_put_u(rowm, dn_clamp(dn_binu(ma), uhi), HFX_CODE10);
+        dn_put3_u(rowm, dn_clamp(dn_binu(mb) - 1u, uhi) + 1u);
+        dn_put_u(rowm, dn_clamp(dn_binu(mb - h2o), uhi), HFX_CODE10);
+        dn_put_u(rowm, dn_clamp(dn_binu(mb - nh3), uhi), HFX_CODE10);
+        dn_put3_u(rowm, dn_clamp(dn_binu(my) - 1u, uhi) + 1u);
+        dn_put_u(rowm, dn_clamp(dn_binu(my - nh3), uhi), HFX_CODE10);
     } else {
         uint32_t *rowm = row - DN_MAGIC_WORDS;
         dn_put_u(rowm, dn_binu(ma), HFX_CODE10);
@@ -568,6 +563,13 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
             if (kneed <= kq) peaks(std::false_type{});
             else peaks(std::true_type{});
             __syncthreads(); /* (b) the tile's map rows are complete */
+            /* the clamped peaks' words past the tile's bins, cleared before
+             * the next tile's peaks (which follow barrier (c)) */
+            if (kneed > kq && h == DN_SPLIT - 1)
+                for (uint32_t k = lane; k < DN_ROWS * DN_TRASH; k += HFX_WAVE) {
+                    const uint32_t r = k / DN_TRASH;
+                    map[r * DN_MSTR + 1 + (kq >> 3) + (k - r * DN_TRASH)] = 0u;
+                }
 
             /* 16 x 16 x kq contraction over groups of four k-steps (128
              * bins): wave h of the map takes groups h, h + DN_SPLIT, ... and
