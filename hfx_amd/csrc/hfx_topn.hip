@@ -117,11 +117,39 @@ __device__ __forceinline__ void tn_hist_add(uint32_t *hist, uint32_t d, bool act
     if ((m >> lane) & 1ull) atomicAdd(&hist[d], 1u);
 }
 
+/* Launch order of k_topn: spectra by window size, largest first, in classes
+ * of 256 candidates (any order inside a class).  A workgroup's time grows
+ * with its window; started first, the long ones no longer finish last. */
+#define TN_OCLS 64
+#define TN_OTHREADS 1024
+template <class Cands>
+__global__ void __launch_bounds__(TN_OTHREADS) k_topn_order(uint32_t *order, Cands c, uint32_t S)
+{
+    __shared__ uint32_t cnt[TN_OCLS];
+    const uint32_t tid = threadIdx.x;
+    auto cls = [&](uint32_t s) -> uint32_t {
+        return TN_OCLS - 1u - min(c.count(s) >> 8, TN_OCLS - 1u);
+    };
+    if (tid < TN_OCLS) cnt[tid] = 0;
+    static_assert(TN_OCLS <= TN_OTHREADS, "one thread clears each class counter");
+    __syncthreads();
+    for (uint32_t s = tid; s < S; s += TN_OTHREADS) atomicAdd(&cnt[cls(s)], 1u);
+    __syncthreads();
+    if (tid == 0)
+        for (uint32_t k = 0, a = 0; k < TN_OCLS; ++k) {
+            const uint32_t v = cnt[k];
+            cnt[k] = a;
+            a += v;
+        }
+    __syncthreads();
+    for (uint32_t s = tid; s < S; s += TN_OTHREADS) order[atomicAdd(&cnt[cls(s)], 1u)] = s;
+}
+
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
            uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
-           hfx_spectra sp, float rel0, float rel1, double *tau_out)
+           hfx_spectra sp, float rel0, float rel1, double *tau_out, const uint32_t *order)
 {
     extern __shared__ __align__(16) unsigned char tn_dyn[];
     const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
@@ -137,7 +165,8 @@ __global__ void __launch_bounds__(TN_THREADS)
     __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_last, sh_ext;
     __shared__ float sh_fN;
 
-    const uint32_t s = blockIdx.x, tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
+    const uint32_t s = order[blockIdx.x], tid = threadIdx.x, wave = tid / HFX_WAVE;
+    const uint32_t lane = hfx_lane();
     const uint32_t n = c.count(s);
     const float *sc = scores + c.base(s);
     if (n == 0) {
@@ -788,9 +817,13 @@ static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     int rc;
     if ((rc = tx_scratch(ctx, scratch, sp->num_spec, total, &p))) return rc;
     const TxScratch x = tx_carve(p, sp->num_spec, total);
+    /* the launch order lives in the flagged-spectrum list, which the exact
+     * route fills only after k_topn */
+    hipLaunchKernelGGL(k_topn_order<Cands>, dim3(1), dim3(TN_OTHREADS), 0, st, x.list, c,
+                       sp->num_spec);
     hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS),
                        tn_dyn_bytes(sp->x_stride), st, d_top_pos, d_top_exact, d_top_f32,
-                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, x.tau);
+                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, x.tau, x.list);
     HFX_CHECK_LAUNCH();
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
                        total, sp, p, st);
