@@ -403,8 +403,12 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
     for (uint32_t slot = 0;; slot ^= 1u) {
         __syncthreads(); /* every wave is done with the previous item */
         /* wave-uniform from here: block metadata comes through scalar loads */
-        const uint32_t t = __builtin_amdgcn_readfirstlane(sh_t[slot]);
-        if (t >= ntiles) break;
+        const uint32_t t0 = __builtin_amdgcn_readfirstlane(sh_t[slot]);
+        if (t0 >= ntiles) break;
+        /* items from the last to the first: the plan orders blocks by charge
+         * level, then precursor mass, so the heaviest come first and the
+         * lightest finish the launch */
+        const uint32_t t = (uint32_t)ntiles - 1u - t0;
         if (tid == 0) next = atomicAdd(a.counter, 1u);
         uint32_t lo = 0, hi = a.B;
         while (hi - lo > 1) {
