@@ -228,8 +228,12 @@ __global__ void __launch_bounds__(512) k_batch(BtArgs a)
     for (;;) {
         if (tid == 0) sh_t = atomicAdd(a.counter, 1u);
         __syncthreads();
-        const uint64_t t = sh_t;
-        if (t >= ntiles) break;
+        const uint64_t t0 = sh_t;
+        if (t0 >= ntiles) break;
+        /* tiles from the last to the first: blocks follow the plan's order by
+         * charge level, then precursor mass, so the heaviest come first and
+         * the lightest finish the launch */
+        const uint64_t t = ntiles - 1u - t0;
         /* block of the tile: largest b with tile_off[b] <= t */
         uint32_t lo = 0, hi = a.B;
         while (hi - lo > 1) {
