@@ -699,7 +699,7 @@ __global__ void k_dn_lad_fill(hfx_dn_pos *pos, float *top, const uint64_t *off, 
     }
     /* with a first residue of 32 Da or more every peak bin is >= 1 (the
      * lowest, y - NH3 at charge 4, is above 1.2); a lighter one (a zero-mass
-     * residue code) marks the peptide's tiles for the checked writes */
+     * residue code) marks the peptide's tiles for the clamped writes */
     top[i] = n && b0 < 32.0f ? 1e9f : fmaxf(r, b);
 }
 
