@@ -83,6 +83,17 @@ struct HfxPlainCands {
     {
         hfx_wave_peaks(map, p.n, p.r, zeff, len, HfxPlainMass{db.d_ions + p.tc, mtab});
     }
+    /* lane l's residue code of the candidate's first 64 (loaded a candidate
+     * ahead), and the peaks from it */
+    __device__ __forceinline__ uint32_t first_res(const Prep &p) const
+    {
+        return hfx_lane() < p.n ? (uint32_t)db.d_ions[p.tc + hfx_lane()] : 0u;
+    }
+    __device__ __forceinline__ void peaks_first(uint32_t *map, const float *mtab, const Prep &p,
+                                                uint32_t first, uint32_t len, uint32_t zeff) const
+    {
+        hfx_wave_peaks(map, p.n, p.r, zeff, len, HfxPlainMassFirst{db.d_ions + p.tc, mtab, first});
+    }
     /* peaks of candidate k of spectrum s into the wave's map */
     __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
                                           uint32_t len, uint32_t zeff) const
@@ -128,6 +139,14 @@ struct HfxModCands {
         HfxModMass mm{db.d_ions + p.tc, mtab, ms.d_mod_ma_count + (size_t)p.mod * ms.num_ma,
                       ms.d_unrank + p.uoff, ms.d_ma_mass, na, {0, 0, 0, 0, 0, 0, 0}};
         hfx_wave_peaks(map, p.n, p.r, zeff, len, mm);
+    }
+    /* no read ahead for modified candidates (their masses need the unrank
+     * slice as well) */
+    __device__ __forceinline__ uint32_t first_res(const Prep &) const { return 0u; }
+    __device__ __forceinline__ void peaks_first(uint32_t *map, const float *mtab, const Prep &p,
+                                                uint32_t, uint32_t len, uint32_t zeff) const
+    {
+        peaks_prep(map, mtab, p, len, zeff);
     }
     __device__ __forceinline__ void peaks(uint32_t *map, const float *mtab, uint32_t s, uint32_t k,
                                           uint32_t len, uint32_t zeff) const

@@ -386,6 +386,20 @@ struct HfxPlainMass {
     }
 };
 
+/* HfxPlainMass with the candidate's first 64 residue codes already in hand
+ * (lane j's in first), so that their load can be issued a candidate ahead */
+struct HfxPlainMassFirst {
+    const uint8_t *ions;
+    const float *mtab;
+    uint32_t first;
+    __device__ __forceinline__ float operator()(uint32_t c0, uint32_t cn) const
+    {
+        const uint32_t lane = hfx_lane();
+        if (lane >= cn) return 0.0f;
+        return mtab[HFX_ACID_IDX(c0 == 0 ? first : (uint32_t)ions[c0 + lane])];
+    }
+};
+
 /* Residue masses of a modified candidate: a residue packed with modable-acid
  * index a+1 (prepare_ions) that is the i-th occurrence of acid a in the
  * ladder, with i listed in this peptide's unrank slice for a, weighs

@@ -347,8 +347,13 @@ __global__ void __launch_bounds__(TN_THREADS)
     for (uint32_t c0 = 0; c0 < M; c0 += TN_WAVES * HFX_WAVE) {
         typename Cands::Prep mine{};
         if (c0 + wave + TN_WAVES * lane < M) mine = c.prep(s, cp[c0 + wave + TN_WAVES * lane]);
+        /* the next candidate's first residues are loaded while this one is
+         * scored */
+        uint32_t res = c0 + wave < M ? c.first_res(Cands::take(mine, 0)) : 0u;
         for (uint32_t i = c0 + wave, j = 0; i < M && j < HFX_WAVE; i += TN_WAVES, ++j) {
-            c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
+            const uint32_t cur_res = res;
+            if (i + TN_WAVES < M && j + 1 < HFX_WAVE) res = c.first_res(Cands::take(mine, j + 1));
+            c.peaks_first(map, mtab, Cands::take(mine, j), cur_res, len, zeff);
             hfx_wave_sync();
             const int64_t fx = hfx_wave_dot_fx(map, xs, len);
             hfx_wave_sync();
@@ -498,9 +503,16 @@ __global__ void __launch_bounds__(TN_THREADS)
         }
         typename Cands::Prep mine{};
         if (live) mine = c.prep(s, mi);
-        for (uint64_t m = __ballot(live); m; m &= m - 1ull) {
+        const uint64_t lm = __ballot(live);
+        /* the next live candidate's first residues are loaded while this one
+         * is scored */
+        uint32_t res = lm ? c.first_res(Cands::take(mine, (uint32_t)__builtin_ctzll(lm))) : 0u;
+        for (uint64_t m = lm; m; m &= m - 1ull) {
             const uint32_t j = (uint32_t)__builtin_ctzll(m);
-            c.peaks_prep(map, mtab, Cands::take(mine, j), len, zeff);
+            const uint32_t cur_res = res;
+            const uint64_t rest = m & (m - 1ull);
+            if (rest) res = c.first_res(Cands::take(mine, (uint32_t)__builtin_ctzll(rest)));
+            c.peaks_first(map, mtab, Cands::take(mine, j), cur_res, len, zeff);
             hfx_wave_sync();
             const int64_t fx = hfx_wave_dot_fx(map, xs, len);
             hfx_wave_sync();
