@@ -270,9 +270,10 @@ int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
  * (int64 fixed point, 2^-36 resolution, reported as double), and writes the
  * best N ordered by (exact score desc, candidate position asc):
  * d_top_pos[s*N + k] (position within the spectrum's window), d_top_exact,
- * d_top_f32, d_top_cnt[s].  Every candidate whose FP32 score lies within the
- * FP32 error bound of the N-th exact score is re-scored too; a spectrum with
- * more such candidates than the re-score set holds (exact ties) takes the
+ * d_top_f32, d_top_cnt[s].  Every candidate whose FP32 score lies within twice
+ * the FP32 error bound below the N-th best FP32 score is re-scored too (every
+ * candidate that could reach the N-th exact score is among them); a spectrum
+ * with more such candidates than the re-score set holds (exact ties) takes the
  * exact route on the device: all its candidates scored exactly and the top N
  * selected on the exact scores; d_flag[s] = 1 marks those spectra.  `total`
  * = sum_s d_num[s].  N <= 64, N + margin <= 128. */
