@@ -532,10 +532,14 @@ __global__ void __launch_bounds__(TN_THREADS)
  * candidates (or fewer than N) take the 8-bit radix select over global
  * memory.  A persistent grid walks the list of flagged spectra. */
 #define FX_CAP 4096
+/* a workgroup per flagged spectrum, of few: sixteen waves each, so that a
+ * sweep over the window has 8192 loads in flight */
+#define FX_THREADS 1024
+#define FX_WAVES (FX_THREADS / HFX_WAVE)
 #define FX_DBITS 11
 
 template <class Cands>
-__global__ void __launch_bounds__(TN_THREADS)
+__global__ void __launch_bounds__(FX_THREADS)
     k_topn_fx(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt, uint32_t N,
               const uint32_t *list, const uint32_t *list_cnt, const int64_t *fxbuf,
               const float *scores, Cands c)
@@ -545,7 +549,7 @@ __global__ void __launch_bounds__(TN_THREADS)
     __shared__ uint32_t hist[1u << FX_DBITS];
     __shared__ uint32_t sel_p[TX_MAXN];
     __shared__ int64_t sel_f[TX_MAXN];
-    __shared__ uint32_t wcnt[TN_WAVES];
+    __shared__ uint32_t wcnt[FX_WAVES];
     __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_cnt;
     __shared__ uint64_t sh_diff;
     const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
@@ -567,13 +571,13 @@ __global__ void __launch_bounds__(TN_THREADS)
          * still take part */
         auto for_keys = [&](auto &&f) {
             constexpr uint32_t U = 8;
-            for (uint32_t base = 0; base < n; base += U * TN_THREADS) {
+            for (uint32_t base = 0; base < n; base += U * FX_THREADS) {
                 uint64_t kk[U];
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u)
-                    kk[u] = key(min(base + u * TN_THREADS + tid, n - 1u));
+                    kk[u] = key(min(base + u * FX_THREADS + tid, n - 1u));
 #pragma unroll
-                for (uint32_t u = 0; u < U; ++u) f(base + u * TN_THREADS + tid, kk[u]);
+                for (uint32_t u = 0; u < U; ++u) f(base + u * FX_THREADS + tid, kk[u]);
             }
         };
 
@@ -608,7 +612,7 @@ __global__ void __launch_bounds__(TN_THREADS)
             /* 2. the bits in which the live keys differ */
             const uint64_t k0 = lk[0];
             uint64_t d = 0;
-            for (uint32_t i = tid; i < M; i += TN_THREADS) d |= lk[i] ^ k0;
+            for (uint32_t i = tid; i < M; i += FX_THREADS) d |= lk[i] ^ k0;
             if (d) atomicOr((unsigned long long *)&sh_diff, (unsigned long long)d);
             __syncthreads();
             const uint64_t diff = sh_diff;
@@ -624,9 +628,9 @@ __global__ void __launch_bounds__(TN_THREADS)
             while (hi >= 0 && !whole) {
                 const int lo = hi - (FX_DBITS - 1) > 0 ? hi - (FX_DBITS - 1) : 0;
                 const uint32_t width = (uint32_t)(hi - lo + 1), nb = 1u << width;
-                for (uint32_t b = tid; b < nb; b += TN_THREADS) hist[b] = 0;
+                for (uint32_t b = tid; b < nb; b += FX_THREADS) hist[b] = 0;
                 __syncthreads();
-                for (uint32_t i = tid; i < M; i += TN_THREADS) {
+                for (uint32_t i = tid; i < M; i += FX_THREADS) {
                     const uint64_t k = lk[i];
                     if ((k & pmask) == prefix)
                         atomicAdd(&hist[(uint32_t)(k >> lo) & (nb - 1u)], 1u);
@@ -646,9 +650,9 @@ __global__ void __launch_bounds__(TN_THREADS)
                 for (int qh = 31; qh >= 0 && !whole;) {
                     const int lo = qh - (FX_DBITS - 1) > 0 ? qh - (FX_DBITS - 1) : 0;
                     const uint32_t width = (uint32_t)(qh - lo + 1), nb = 1u << width;
-                    for (uint32_t b = tid; b < nb; b += TN_THREADS) hist[b] = 0;
+                    for (uint32_t b = tid; b < nb; b += FX_THREADS) hist[b] = 0;
                     __syncthreads();
-                    for (uint32_t i = tid; i < M; i += TN_THREADS) {
+                    for (uint32_t i = tid; i < M; i += FX_THREADS) {
                         const uint32_t q = ~lp[i];
                         if (lk[i] == prefix && (q & qmask) == qprefix)
                             atomicAdd(&hist[(q >> lo) & (nb - 1u)], 1u);
@@ -664,7 +668,7 @@ __global__ void __launch_bounds__(TN_THREADS)
             }
             /* 5. the selection: keys above the prefix, and the bucket
              * (or the tied key's chosen positions) -- want of them */
-            for (uint32_t i = tid; i < M; i += TN_THREADS) {
+            for (uint32_t i = tid; i < M; i += FX_THREADS) {
                 const uint64_t k = lk[i];
                 const uint32_t q = ~lp[i];
                 const bool take = qmask == 0u
@@ -683,7 +687,7 @@ __global__ void __launch_bounds__(TN_THREADS)
             /* the 8-bit radix select over global memory */
             uint64_t prefix = 0, pmask = 0;
             for (int shift = 56; shift >= 0; shift -= 8) {
-                hist[tid] = 0;
+                if (tid < 256u) hist[tid] = 0;
                 __syncthreads();
                 for_keys([&](uint32_t i, uint64_t k) {
                     tn_hist_add(hist, (uint32_t)(k >> shift) & 255u,
@@ -706,7 +710,7 @@ __global__ void __launch_bounds__(TN_THREADS)
                 }
             });
             uint32_t eq_base = 0;
-            for (uint32_t base = 0; base < n && eq_base < need_eq; base += TN_THREADS) {
+            for (uint32_t base = 0; base < n && eq_base < need_eq; base += FX_THREADS) {
                 const uint32_t i = base + tid;
                 const bool f = i < n && key(i) == K;
                 const uint64_t bal = __ballot(f);
@@ -714,7 +718,7 @@ __global__ void __launch_bounds__(TN_THREADS)
                 if (lane == 0) wcnt[wave] = (uint32_t)__builtin_popcountll(bal);
                 __syncthreads();
                 uint32_t wpre = 0, tot = 0;
-                for (uint32_t w = 0; w < TN_WAVES; ++w) {
+                for (uint32_t w = 0; w < FX_WAVES; ++w) {
                     if (w < wave) wpre += wcnt[w];
                     tot += wcnt[w];
                 }
@@ -728,7 +732,7 @@ __global__ void __launch_bounds__(TN_THREADS)
             }
             __syncthreads();
         }
-        for (uint32_t i = tid; i < want; i += TN_THREADS) {
+        for (uint32_t i = tid; i < want; i += FX_THREADS) {
             uint32_t rk = 0;
             for (uint32_t j = 0; j < want; ++j)
                 rk += before_fx(sel_f[j], sel_p[j], sel_f[i], sel_p[i]) ? 1u : 0u;
@@ -804,8 +808,8 @@ static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 8),
                        dim3(TN_THREADS), tn_dyn_bytes(sp->x_stride), st, x.fxbuf, x.tile_off,
                        x.tile_spec, x.counter, c, *sp, d_scores, x.tau);
-    hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(min(S, (uint32_t)hfx_ctx_num_cus(ctx) * 2)),
-                       dim3(TN_THREADS), 0, st, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
+    hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(min(S, (uint32_t)hfx_ctx_num_cus(ctx))),
+                       dim3(FX_THREADS), 0, st, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
                        x.list, x.counter + 1, x.fxbuf, d_scores, c);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
