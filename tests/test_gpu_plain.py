@@ -320,3 +320,56 @@ def test_tie_heavy_window_takes_exact_route(orc):
             c = int(res.top_cnt[s])
             assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), (dense, s)
             assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
+
+
+def test_exact_route_past_the_lds_selection(orc):
+    """5000 proteins carry the same tryptic peptide: spectrum 0's window holds
+    5000 candidates tied exactly, more than the exact route's LDS selection
+    takes (4096 live keys), so its top N come from the radix select over
+    global memory.  Top N equal the oracle's for both scoring forms."""
+    from hfx_amd import pipeline, synth
+
+    copies = 5000
+    base = synth.make_proteins(300, seed=71)
+    motif = np.frombuffer(b"GASGTPAGEGSK", np.uint8)
+    tails = synth.make_proteins(copies, seed=72, min_len=20, max_len=60)
+    parts, seg = [base.seq], list(base.seg)
+    for k in range(copies):
+        # "MKR" + motif + "G" + tail: the motif is cleaved on both sides
+        tail = tails.seq[int(tails.seg[k]):int(tails.seg[k + 1])]
+        p = np.concatenate([np.frombuffer(b"MKR", np.uint8), motif,
+                            np.frombuffer(b"G", np.uint8), tail])
+        parts.append(p)
+        seg.append(seg[-1] + len(p))
+    prot = synth.Proteins(seq=np.concatenate(parts).astype(np.uint8),
+                          seg=np.asarray(seg, np.uint32))
+    mass = orc.mass_table()
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=1)
+    sp = synth.make_spectra(prot, r, tc, tn, 16, seed=73, charges=(2,))
+    hit = [i for i in range(len(r))
+           if synth.peptide_residues(prot, int(tc[i]), int(tn[i])) == motif.tobytes()]
+    assert len(hit) == copies
+    one = synth.make_spectra(prot, r[hit[0]:hit[0] + 1], tc[hit[0]:hit[0] + 1],
+                             tn[hit[0]:hit[0] + 1], 1, seed=74, charges=(2,))
+    n0 = int(one.off[1])
+    mz = np.concatenate([one.mz[:n0], sp.mz[sp.off[1]:]])
+    inten = np.concatenate([one.inten[:n0], sp.inten[sp.off[1]:]])
+    off = np.concatenate([[0], n0 + (sp.off[1:] - sp.off[1])]).astype(np.uint32)
+    pcr = np.concatenate([one.pcr[:1], sp.pcr[1:]]).astype(np.float32)
+    ch = np.concatenate([one.charge[:1], sp.charge[1:]]).astype(np.float32)
+
+    ctx = pipeline.Ctx(0)
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
+    dsp = pipeline.DeviceSpectra(ctx, pcr, ch, mz, inten, off, 2112)
+    idx, _ = orc.sort_idx(r)
+    ref = orc.search_plain(orc.prepare_ions(prot.seq), r, tc, tn, mass, idx, dsp.x(),
+                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, 10)
+    assert int(ref["num"][0]) > copies
+    for dense in (False, True):
+        res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=10).run(dense=dense)
+        assert 0 in res.flagged.tolist(), dense
+        assert (res.top_cnt == ref["top_cnt"]).all()
+        for s in range(dsp.num_spec):
+            c = int(res.top_cnt[s])
+            assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), (dense, s)
+            assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
