@@ -149,7 +149,8 @@ template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
            uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
-           hfx_spectra sp, float rel0, float rel1, double *tau_out, const uint32_t *order)
+           hfx_spectra sp, float rel0, float rel1, double *tau_out, uint32_t *band_out,
+           const uint32_t *order)
 {
     extern __shared__ __align__(16) unsigned char tn_dyn[];
     const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
@@ -336,6 +337,7 @@ __global__ void __launch_bounds__(TN_THREADS)
         ne = sh_ext;
         if (ne > TN_EXT) {
             fl = 1;
+            if (tid == 0) band_out[s] = ne; /* sizes the exact route's tiles */
             ne = 0;
         }
     }
@@ -402,6 +404,9 @@ __global__ void __launch_bounds__(TN_THREADS)
 
 #define TX_MAXN 64
 #define TX_CHUNK 256
+/* a flagged spectrum's tiles also number at least its band count / TX_LIVE,
+ * so that the candidates the exact route re-scores spread over the tiles */
+#define TX_LIVE 32
 /* a wave fetches the metadata of its share of a candidate list one lane
  * per candidate */
 static_assert(TN_MAXT / TN_WAVES <= HFX_WAVE && TN_EXT / TN_WAVES <= HFX_WAVE &&
@@ -412,12 +417,17 @@ static_assert(TN_MAXT / TN_WAVES <= HFX_WAVE && TN_EXT / TN_WAVES <= HFX_WAVE &&
  * order) for the final select; counter[1] counts them */
 template <class Cands>
 __global__ void k_flag_tiles(uint32_t *tiles, uint32_t *list, uint32_t *counter,
-                             const uint32_t *flag, Cands c, uint32_t S)
+                             const uint32_t *flag, const uint32_t *band, Cands c, uint32_t S)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const bool f = flag[s] != 0u;
-    tiles[s] = f ? (c.count(s) + TX_CHUNK - 1) / TX_CHUNK : 0u;
+    uint32_t nt = 0;
+    if (f) {
+        const uint32_t n = c.count(s), b = min(band[s], n);
+        nt = max((n + TX_CHUNK - 1) / TX_CHUNK, (b + TX_LIVE - 1) / TX_LIVE);
+    }
+    tiles[s] = nt;
     if (f) list[atomicAdd(&counter[1], 1u)] = s;
 }
 
@@ -440,43 +450,39 @@ __global__ void k_zero_flags(uint32_t *flag, uint32_t S)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S) flag[i] = 0;
 }
-__global__ void k_mark_list(uint32_t *flag, double *tau, const uint32_t *list, uint32_t num_list,
-                            uint32_t S)
+__global__ void k_mark_list(uint32_t *flag, double *tau, uint32_t *band, const uint32_t *list,
+                            uint32_t num_list, uint32_t S)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < num_list && list[i] < S) {
         flag[list[i]] = 1;
         tau[list[i]] = -INFINITY; /* every candidate scored exactly */
+        band[list[i]] = 0xffffffffu;
     }
 }
 
-/* persistent: tiles of flagged spectra from a counter.  A spectrum's nt
+/* persistent: workgroup g takes tiles g, g + grid, ... of the flagged
+ * spectra (about one each: no shared counter to queue on).  A spectrum's nt
  * tiles interleave its candidates (tile k takes k, k + nt, k + 2 nt, ...):
  * candidates near tau sit together in the window's order, and so are
  * spread over the tiles. */
 template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
-    k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, const uint32_t *tile_spec,
-                  uint32_t *counter, Cands c, hfx_spectra sp, const float *scores,
-                  const double *tau)
+    k_exact_tiles(int64_t *fxbuf, const uint64_t *tile_off, const uint32_t *tile_spec, Cands c,
+                  hfx_spectra sp, const float *scores, const double *tau)
 {
     extern __shared__ __align__(16) unsigned char tn_dyn[];
     const uint32_t xcap = tn_xcap(sp.x_stride), mwords = xcap / 8;
     float *xs = (float *)tn_dyn;
     uint32_t *maps = (uint32_t *)(tn_dyn + tn_xs_bytes(sp.x_stride));
     __shared__ float mtab[32];
-    __shared__ uint32_t sh_t;
-    const uint32_t tid = threadIdx.x, wave = tid / HFX_WAVE, lane = hfx_lane();
+    const uint32_t wave = threadIdx.x / HFX_WAVE, lane = hfx_lane();
     const uint32_t S = sp.num_spec;
     const uint64_t ntiles = tile_off[S];
     uint32_t *map = maps + wave * mwords;
     for (uint32_t w = lane; w < mwords; w += HFX_WAVE) map[w] = 0;
     uint32_t cur = 0xffffffffu, len = 0;
-    for (;;) {
-        if (tid == 0) sh_t = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint64_t t = sh_t;
-        if (t >= ntiles) break;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t s = tile_spec[t];
         if (s != cur) {
             len = hfx_stage_spectrum(xs, mtab, c.db, sp, s);
@@ -495,9 +501,10 @@ __global__ void __launch_bounds__(TN_THREADS)
          * others' metadata is fetched by their lanes, and the wave re-scores
          * them one by one. */
         const uint64_t tb = c.base(s);
-        const uint32_t mi = k + nt * (wave + TN_WAVES * lane);
+        const uint64_t mi64 = k + (uint64_t)nt * (wave + TN_WAVES * lane);
+        const uint32_t mi = (uint32_t)mi64;
         bool live = false;
-        if (wave + TN_WAVES * lane < TX_CHUNK && mi < n) {
+        if (wave + TN_WAVES * lane < TX_CHUNK && mi64 < n) {
             live = !((double)scores[tb + mi] < ts);
             if (!live) fxbuf[tb + mi] = INT64_MIN;
         }
@@ -749,23 +756,26 @@ __global__ void __launch_bounds__(FX_THREADS)
 /* Launch helpers                                                        */
 /* ------------------------------------------------------------------ */
 
-/* at most total / TX_CHUNK + S tiles: ceil(n / TX_CHUNK) per spectrum */
-static uint64_t tx_max_tiles(uint32_t S, uint64_t total) { return total / TX_CHUNK + S; }
+/* tiles: max(ceil(n / TX_CHUNK), ceil(band / TX_LIVE)) per spectrum, band <= n */
+static uint64_t tx_max_tiles(uint32_t S, uint64_t total)
+{
+    return total / TX_CHUNK + total / TX_LIVE + 2ull * S;
+}
 
 size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total)
 {
     return total * 8 + ((size_t)S + 1) * 8 + (size_t)S * 8 + (size_t)S * 4 + 256 +
-           tx_max_tiles(S, total) * 4 + (size_t)S * 4;
+           tx_max_tiles(S, total) * 4 + (size_t)S * 8;
 }
 
 /* exact-route scratch:
  * fxbuf[total] | tile_off[S+1] | tau[S] | tiles[S] | counters (256 B) |
- * tile_spec[max tiles] | list[S] */
+ * tile_spec[max tiles] | list[S] | band[S] */
 struct TxScratch {
     int64_t *fxbuf;
     uint64_t *tile_off;
     double *tau;
-    uint32_t *tiles, *counter, *tile_spec, *list;
+    uint32_t *tiles, *counter, *tile_spec, *list, *band;
 };
 static TxScratch tx_carve(void *p, uint32_t S, uint64_t total)
 {
@@ -777,6 +787,7 @@ static TxScratch tx_carve(void *p, uint32_t S, uint64_t total)
     x.counter = x.tiles + S;
     x.tile_spec = x.counter + 64;
     x.list = x.tile_spec + tx_max_tiles(S, total);
+    x.band = x.list + S;
     return x;
 }
 static int tx_scratch(hfx_ctx *ctx, void *given, uint32_t S, uint64_t total, void **out)
@@ -801,13 +812,13 @@ static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
     const TxScratch x = tx_carve(p, S, total);
     hipLaunchKernelGGL(k_tx_reset, dim3(1), dim3(1), 0, st, x.counter);
     hipLaunchKernelGGL(k_flag_tiles<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, x.tiles,
-                       x.list, x.counter, d_flag, c, S);
+                       x.list, x.counter, d_flag, x.band, c, S);
     if ((rc = hfx_scan_offsets(ctx, x.tile_off, x.tiles, S, (void *)st)) != HFX_OK) return rc;
     hipLaunchKernelGGL(k_tile_spec, dim3((S + 255) / 256), dim3(256), 0, st, x.tile_spec,
                        x.tile_off, S);
     hipLaunchKernelGGL(k_exact_tiles<Cands>, dim3((uint32_t)hfx_ctx_num_cus(ctx) * 8),
                        dim3(TN_THREADS), tn_dyn_bytes(sp->x_stride), st, x.fxbuf, x.tile_off,
-                       x.tile_spec, x.counter, c, *sp, d_scores, x.tau);
+                       x.tile_spec, c, *sp, d_scores, x.tau);
     hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(min(S, (uint32_t)hfx_ctx_num_cus(ctx))),
                        dim3(FX_THREADS), 0, st, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
                        x.list, x.counter + 1, x.fxbuf, d_scores, c);
@@ -839,7 +850,8 @@ static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                        sp->num_spec);
     hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS),
                        tn_dyn_bytes(sp->x_stride), st, d_top_pos, d_top_exact, d_top_f32,
-                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, x.tau, x.list);
+                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, x.tau, x.band,
+                       x.list);
     HFX_CHECK_LAUNCH();
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
                        total, sp, p, st);
@@ -864,7 +876,7 @@ static int launch_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_ex
     const TxScratch x = tx_carve(q, S, total);
     hipLaunchKernelGGL(k_zero_flags, dim3((S + 255) / 256), dim3(256), 0, st, flag, S);
     hipLaunchKernelGGL(k_mark_list, dim3((num_list + 255) / 256), dim3(256), 0, st, flag, x.tau,
-                       d_list, num_list, S);
+                       x.band, d_list, num_list, S);
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, flag, d_scores, c,
                        total, sp, q, st);
 }
