@@ -9,15 +9,17 @@
  * Here one workgroup per spectrum:
  *   1. radix-selects the T = min(N + margin, n) best FP32 scores (key desc,
  *      position asc);
- *   2. re-scores them exactly: int64 fixed point, sum of round(w*x*2^36),
- *      independent of summation order, so the GPU and the CPU oracle agree
- *      bit for bit;
- *   3. takes the N-th best exact score exN and re-scores every other candidate
- *      whose FP32 score is >= exN - B (B bounds |FP32 - exact|), so that no
- *      candidate left out can reach the top N;
+ *   2. takes the N-th best FP32 score fN and collects every other candidate
+ *      whose FP32 score is >= fN - 2B (B bounds |FP32 - exact|): the N-th
+ *      best exact score exN is >= fN - B, so no candidate left out can reach
+ *      the top N;
+ *   3. re-scores the selection and that band exactly, in one round: int64
+ *      fixed point, sum of round(w*x*2^36), independent of summation order,
+ *      so the GPU and the CPU oracle agree bit for bit;
  *   4. orders the re-scored set by (exact desc, position asc), writes the best
- *      N.  If more than TN_EXT candidates fall within B, the spectrum is
- *      flagged and hfx_topn_exact scores all of its candidates exactly.
+ *      N.  If the band holds more than TN_EXT candidates, the spectrum is
+ *      flagged and the exact route scores every candidate at or above
+ *      exN - B exactly.
  * Bound: |score - exact| <= (rel0 + rel1 * z) * 50 * sum_b |x_b| + 1e-6, with
  * z the spectrum's fragment charge count and sum|w*x| <= 50 * sum_b |x_b|
  * (each bin counts once, weights <= 50).  The sparse FP32 scores use
