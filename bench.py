@@ -1,9 +1,10 @@
 """Benchmark of the XCorr search path on one MI355X.
 
-Workload (default): tryptic-20k-10k -- 20,000 seeded random proteins, tryptic
-digest with up to 2 missed cleavages and fixed carbamidomethyl Cys, 10,000
-synthetic spectra at 2+/3+ (60/40; 3+ spectra also carry doubly charged
-fragments), +-3 Da precursor window, top-10 per spectrum.
+Workload (default): tryptic-20k-20k, the dense-block workload -- 20,000
+seeded random proteins, tryptic digest with up to 2 missed cleavages and
+fixed carbamidomethyl Cys, 20,000 synthetic spectra at 2+/3+ (50/50; 3+
+spectra also carry doubly charged fragments), no variable modifications,
++-3 Da precursor window, top-5 per spectrum.
 
 A step is one pass of the scoring stage over the whole batch with the
 database and the preprocessed spectra resident in HBM: precursor windows ->
@@ -11,13 +12,19 @@ theoretical peaks -> XCorr of every (spectrum, candidate) pair -> per-spectrum
 top-N with exact re-score.  Three forms are timed:
   plain          hfx_window + hfx_xcorr_sparse + hfx_topn (the drop-in, one
                  theoretical spectrum per (spectrum, candidate) pair, FP32);
-  batched fp32   hfx_batch_* in its FP32 sparse form (per-batch peptide ion
-                 index + persistent work-stealing multi-spectrum XCorr);
-  batched bf16x2 hfx_batch_* in its dense form (blocks of eight spectra
+  batched fp32   hfx_batch_* in its FP32 sparse form (persistent
+                 work-stealing multi-spectrum XCorr over the per-database
+                 peptide ion lists);
+  batched bf16x2 hfx_batch_* in its dense form (blocks of up to eight spectra
                  contracted with their candidates' theoretical rows on the
                  MFMA units, BF16 weights x hi+lo BF16 spectra, FP32
-                 accumulation) + top-N with the exact re-score;
-both batched forms autotuned, captured in a hipGraph and replayed.
+                 accumulation) + top-N with the exact re-score.
+Each batched step runs every per-batch step on the device -- windows,
+spectrum order, block cut and hi/lo rows (dense) or tiles (sparse), scoring,
+top-N -- captured once in a hipGraph and replayed.  The per-database peptide
+index (b ladders, ion lists) is built once before the loop and its build
+time is reported beside the step (`db_setup_ms`), as is the one-off plan
+(allocation) time.
 value = scored pairs per second of the batched BF16-split form (whole job,
 one GPU); every form's top-N must equal the plain form's.
 
@@ -112,10 +119,30 @@ def same_topn(a, b, n=None) -> bool:
                for s in range(S))
 
 
-def cpu_baseline(env, res, topn, target_s=12.0):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(env, res, topn, target_s=1.0, reps=5):
+    """The oracle (C++ restatement of the reference's path: window ->
+    per-candidate dense row with max per bin -> FP32 dot, plus the exact
+    fixed-point score -> top-N; -O3 -march=x86-64-v3 -ffp-contract=off,
+    OpenMP over spectra) on a bounded sample of the same spectra: best of
+    `reps` timed runs after one warm-up, with all the threads this process
+    may use (OMP_NUM_THREADS, else the cores the OS reports, at most 16: the
+    host share of one GPU) and with one thread.  Every sampled spectrum's
+    top-N is compared with the GPU's."""
     from oracle import oracle as orc
 
-    threads = min(os.cpu_count() or 1, 16)
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(nproc, 16)
     dsp = env["dsp"]
     X = dsp.x()
     lens = dsp.lengths()
@@ -123,26 +150,46 @@ def cpu_baseline(env, res, topn, target_s=12.0):
     ions = env["db"].packed_ions()
     S = dsp.num_spec
     args = (ions, env["r"], env["tc"], env["tn"], env["mass"], idx)
-    n = threads * 4
-    t0 = time.time()
-    orc.search_plain(*args, X[:n], lens[:n], dsp.zmax[:n], dsp.mass[:n], 3.0, topn,
-                     nthreads=threads)
-    dt = max(time.time() - t0, 1e-3)
-    n = int(min(S, max(threads * 4, n * target_s / dt)))
-    t0 = time.time()
-    ref = orc.search_plain(*args, X[:n], lens[:n], dsp.zmax[:n], dsp.mass[:n], 3.0, topn,
-                           nthreads=threads)
-    dt = time.time() - t0
+    # spread the sample over the whole order of precursor masses
+    perm = np.random.default_rng(7).permutation(S)
+
+    def run(n, nt):
+        sel = np.sort(perm[:n])
+        t0 = time.perf_counter()
+        ref = orc.search_plain(*args, X[sel], lens[sel], dsp.zmax[sel], dsp.mass[sel], 3.0, topn,
+                               nthreads=nt)
+        return time.perf_counter() - t0, sel, ref
+
+    def timed(nt):
+        n = 4 * nt
+        dt, _, _ = run(n, nt)  # warm-up and size probe
+        n = int(min(S, max(4 * nt, n * target_s / max(dt, 1e-3))))
+        best, sel, ref = None, None, None
+        for _ in range(reps):
+            dt, sel, ref = run(n, nt)
+            best = dt if best is None else min(best, dt)
+        return best, sel, ref
+
+    dt_all, sel, ref = timed(threads)
     scores = int(ref["num"].sum())
-    same = bool((res.top_cnt[:n] == ref["top_cnt"]).all()) and all(
-        res.top_pos[s, : res.top_cnt[s]].tolist() == ref["top_pos"][s, : ref["top_cnt"][s]].tolist()
+    same = all(
+        int(res.top_cnt[s]) == int(ref["top_cnt"][i])
+        and res.top_pos[s, : res.top_cnt[s]].tolist() == ref["top_pos"][i, : ref["top_cnt"][i]].tolist()
         and (res.top_exact[s, : res.top_cnt[s]] ==
-             orc.fx_to_double(ref["top_fx"][s, : ref["top_cnt"][s]])).all()
-        for s in range(n))
-    return dict(value=scores / dt, unit="scores/s", cores=threads, kind="port",
-                sample=f"first {n} of {S} spectra ({scores} scored pairs): oracle dense-row "
-                       f"scoring with exact top-{topn}, OpenMP over spectra, {dt:.1f}s",
-                topn_identical_to_gpu=same)
+             orc.fx_to_double(ref["top_fx"][i, : ref["top_cnt"][i]])).all()
+        for i, s in enumerate(sel))
+    dt_one, sel1, ref1 = timed(1)
+    scores1 = int(ref1["num"].sum())
+    return dict(value=scores / dt_all, unit="scores/s", cores=threads, kind="port",
+                sample=f"{len(sel)} of {S} spectra drawn across the mass range ({scores} scored "
+                       f"pairs): oracle dense-row scoring with exact top-{topn}, OpenMP over "
+                       f"spectra, best of {reps} after a warm-up, {dt_all:.2f}s",
+                nproc=nproc, cpu_model=_cpu_model(),
+                one_thread=dict(value=scores1 / dt_one, unit="scores/s", cores=1,
+                                sample=f"{len(sel1)} spectra ({scores1} scored pairs), best of "
+                                       f"{reps}, {dt_one:.2f}s"),
+                build="g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
+                topn_identical_to_gpu=bool(same))
 
 
 def calib_read(mib: int):
@@ -195,49 +242,37 @@ def bench_plain(env, topn, steps, warmup, margin=0):
     return res, elapsed, kern_ms, flagged
 
 
-def dense_flops(env, res, bstart) -> tuple[float, float]:
-    """(executed, useful) MFMA FLOPs of one dense pass.  The plan's blocks
-    (bstart: block starts in (fragment charge, precursor mass) order, at most
-    eight spectra each); per block and charge level present, the union
-    window in tiles of 16 candidates; a tile is
-    contracted over kq bins, kq = min(K, heaviest candidate's residual + 28
-    rounded up to 128), K = the block's longest spectrum rounded up to 128:
-    16 rows x 16 columns x kq x 2.  Useful = 4 len_s per scored pair (hi and
-    lo columns over the spectrum's own length)."""
+DENSE_K = 2176  # padded bins of a dense row (the kernel's row length)
+
+
+def dense_flops(env, res) -> tuple[float, float]:
+    """(executed, useful) MFMA FLOPs of one dense pass by the per-pair figures
+    of the precision form: executed = 4 K per scored pair (hi and lo BF16
+    columns over the padded row, K = 2176 bins), useful = 2 len_s per scored
+    pair (one FP32 product per bin of the spectrum)."""
     dsp = env["dsp"]
     z = dsp.zmax.astype(np.int64)
-    z = np.where(z <= 4, z, 0)
-    lens = np.minimum(dsp.lengths().astype(np.int64), 2176)
-    order = np.lexsort((dsp.mass, z))
-    b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
-    r_sorted = env["r"][env["db"].idx_sorted()].astype(np.float32)
-    executed = 0.0
-    for i0, i1 in zip(bstart[:-1], bstart[1:]):
-        blk = order[int(i0):int(i1)]
-        blk = blk[(z[blk] > 0) & (n[blk] > 0)]
-        if len(blk) == 0:
-            continue
-        u0, u1 = int(b[blk].min()), int((b[blk] + n[blk]).max())
-        kb = min(2176, (int(lens[blk].max()) + 127) // 128 * 128)
-        top = np.maximum.reduceat(r_sorted[u0:u1], np.arange(0, u1 - u0, 16))
-        kneed = (top + np.float32(24.0)).astype(np.int64) + 4
-        kq = np.minimum(kb, (kneed + 127) // 128 * 128)
-        executed += len(set(z[blk].tolist())) * float(kq.sum()) * 16 * 16 * 2.0
-    useful = float(np.sum(4.0 * lens * n * (z > 0)))
+    scored = (z <= 4) & (z > 0)
+    n = res.num.astype(np.int64)
+    lens = np.minimum(dsp.lengths().astype(np.int64), DENSE_K)
+    executed = float(np.sum(4.0 * DENSE_K * n * scored))
+    useful = float(np.sum(2.0 * lens * n * scored))
     return executed, useful
 
 
-def bench_batched(env, topn, steps, warmup, precision=0, margin=0):
+def bench_batched(env, topn, steps, warmup, index, precision=0, margin=0):
     from hfx_amd import pipeline
 
+    t0 = time.perf_counter()
     bs = pipeline.BatchSearch(env["ctx"], env["db"], env["dsp"], eps=3.0, topn=topn,
-                              margin=margin)
+                              margin=margin, index=index)
     bs.set_precision(precision)
+    bs.stream.sync()
+    plan_ms = (time.perf_counter() - t0) * 1e3
     t0 = time.time()
     cfg = bs.autotune()
     log(f"autotune ({'dense' if precision else 'sparse'}): G={cfg[0]} C={cfg[1]} "
         f"threads={cfg[2]} ({time.time() - t0:.1f}s)")
-    phases = [bs.time_phases() for _ in range(3)][-1]
     bs.capture()
     for _ in range(warmup):
         bs.replay()
@@ -248,18 +283,22 @@ def bench_batched(env, topn, steps, warmup, precision=0, margin=0):
     bs.stream.sync()
     elapsed = time.perf_counter() - t0
     res = bs.results()
+    # the same per-batch pipeline run phase by phase, with events on the
+    # batch's stream around each phase and around the scoring kernel itself
+    ph = [bs.time_phases() for _ in range(max(steps, 3))]
+    phases = {k: float(np.mean([p[k] for p in ph])) for k in ph[0]}
     info = bs.info()
     if precision:
-        info["dense_blocks"] = bs.dense_blocks()
-    return res, elapsed, phases, cfg, info
+        info["dense_blocks"] = int(len(bs.dense_blocks()) - 1)
+    return res, elapsed, phases, cfg, info, plan_ms
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="tryptic-20k-10k")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tryptic-20k-20k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--margin", type=int, default=0,
                     help="extra candidates re-scored exactly beyond the top N")
@@ -274,6 +313,8 @@ def main():
     if args.calib_read:
         calib_read(args.calib_read)
         return
+
+    from hfx_amd import pipeline
 
     env = setup(args.workload)
     topn = env["w"]["topn"]
@@ -290,35 +331,51 @@ def main():
         paths["plain_fp32_sparse"] = dict(
             value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
             kernel="k_xcorr<HfxPlainCands>", kernel_ms=kern_ms,
-            kernel_gbs=ab / (kern_ms * 1e-3) / 1e9, flagged_spectra_per_step=flagged)
-        log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s")
+            kernel_gbs=ab / (kern_ms * 1e-3) / 1e9, algorithmic_bytes=ab,
+            flagged_spectra_per_step=flagged)
+        log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s, kernel {kern_ms:.2f} ms")
+    # the per-database peptide index, built once
+    index = None
+    forms = 0
+    if args.path in ("all", "batched"):
+        forms |= pipeline.PepIndex.IONS
+    if args.path in ("all", "dense"):
+        forms |= pipeline.PepIndex.LADDER
+    db_setup = {}
+    if forms:
+        pipeline.sync()
+        t0 = time.perf_counter()
+        index = pipeline.PepIndex(env["ctx"], env["db"], forms=forms, levels=4, max_len=4096)
+        pipeline.sync()
+        db_setup = dict(ms=(time.perf_counter() - t0) * 1e3, **index.info())
+        log(f"peptide index: {db_setup}")
     batch_res = None
     if args.path in ("all", "batched"):
-        batch_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 0,
-                                                         args.margin)
+        batch_res, el, phases, cfg, info, plan_ms = bench_batched(
+            env, topn, args.steps, args.warmup, index, 0, args.margin)
         sc = int(batch_res.off[-1])
         ab = algorithmic_bytes(env, batch_res)
         paths["batched_fp32_sparse"] = dict(
             value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
-            kernel="k_batch", kernel_ms=phases["score"], phases_ms=phases,
-            kernel_gbs=ab / (phases["score"] * 1e-3) / 1e9,
-            config=dict(G=cfg[0], C=cfg[1], threads=cfg[2]), index=info,
-            graph="captured once, replayed per step",
+            kernel="k_batch", kernel_ms=phases["kernel"], phases_ms=phases,
+            kernel_gbs=ab / (phases["kernel"] * 1e-3) / 1e9,
+            config=dict(G=cfg[0], C=cfg[1], threads=cfg[2]), info=info, plan_ms=plan_ms,
+            graph="every per-batch step captured once, replayed per step",
             flagged_spectra=int(len(batch_res.flagged)))
         log(f"batched fp32: {paths['batched_fp32_sparse']['value']:.3e} scores/s {phases}")
     dense_res = None
     if args.path in ("all", "dense"):
-        dense_res, el, phases, cfg, info = bench_batched(env, topn, args.steps, args.warmup, 1,
-                                                         args.margin)
+        dense_res, el, phases, cfg, info, plan_ms = bench_batched(
+            env, topn, args.steps, args.warmup, index, 1, args.margin)
         sc = int(dense_res.off[-1])
-        ex, useful = dense_flops(env, dense_res, info["dense_blocks"])
+        ex, useful = dense_flops(env, dense_res)
         paths["batched_bf16x2_dense"] = dict(
             value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
-            kernel="k_dense", kernel_ms=phases["score"], phases_ms=phases,
+            kernel="k_dense", kernel_ms=phases["kernel"], phases_ms=phases,
             mfma_flops_executed=ex, mfma_flops_useful=useful,
             config=dict(chunk=cfg[1], threads=cfg[2], spectra_per_block_max=8,
-                        blocks=int(len(info["dense_blocks"]) - 1)),
-            graph="captured once, replayed per step",
+                        blocks=info["dense_blocks"]), plan_ms=plan_ms,
+            graph="every per-batch step captured once, replayed per step",
             flagged_spectra=int(len(dense_res.flagged)))
         log(f"batched bf16x2: {paths['batched_bf16x2_dense']['value']:.3e} scores/s {phases}")
     ref = plain_res
@@ -339,7 +396,9 @@ def main():
                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / BF16_PEAK_TFLOPS,
                     "traffic": tr["bytes"] if tr else None,
-                    "kernel_ms": p["kernel_ms"], "flops_executed": p["mfma_flops_executed"],
+                    "kernel_ms": p["kernel_ms"], "flops_per_launch": p["mfma_flops_executed"],
+                    "flops_rule": f"4 K per scored pair, K = {DENSE_K} padded bins (hi and lo "
+                                  "BF16 columns)",
                     "flops_useful": p["mfma_flops_useful"]}
         if tr:
             roofline["traffic_source"] = tr["source"]
@@ -357,11 +416,11 @@ def main():
                 "peptides": int(len(env["r"])), "spectra": int(env["dsp"].num_spec),
                 "window_da": 3.0, "bin_width": 1.0005079,
                 "max_len": int(env["dsp"].lengths().max()), "topn": topn, "margin": args.margin,
-                "path": main_key,
-                "scores_per_step": p["scores_per_step"]},
-        roofline=roofline, paths=paths)
-    if "plain_fp32_sparse" in paths:
+                "path": main_key, "scores_per_step": p["scores_per_step"]},
+        roofline=roofline, db_setup_ms=db_setup.get("ms"), db_setup=db_setup, paths=paths)
+    if "plain_fp32_sparse" in paths and main_key != "plain_fp32_sparse":
         out["batched_over_plain"] = p["value"] / paths["plain_fp32_sparse"]["value"]
+        out["target_batched_over_plain"] = 10.0
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(env, res, topn)

@@ -174,9 +174,14 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_pepix_build": (_I, [_P, C.POINTER(HfxDb), _U32, _U32, _U32, _P, C.POINTER(_P)]),
+    "hfx_pepix_destroy": (_I, [_P]),
+    "hfx_pepix_info": (_I, [_P, _P]),
     "hfx_batch_create": (_I, [_P, C.POINTER(_P)]),
     "hfx_batch_destroy": (_I, [_P]),
     "hfx_batch_plan": (_I, [_P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _F, _U32, _U32, _P]),
+    "hfx_batch_plan_ix": (_I, [_P, _P, C.POINTER(HfxSpectra), _F, _U32, _U32, _P]),
+    "hfx_batch_use_index": (_I, [_P, _P]),
     "hfx_batch_run": (_I, [_P, _P]),
     "hfx_batch_capture": (_I, [_P, _P]),
     "hfx_batch_replay": (_I, [_P, _P]),
@@ -186,6 +191,7 @@ _SIGS = {
     "hfx_batch_time": (_I, [_P, _P, _P]),
     "hfx_batch_info": (_I, [_P, _P]),
     "hfx_batch_dense_blocks": (_I, [_P, _P, _U32, _P]),
+    "hfx_batch_order": (_I, [_P, _P]),
     "hfx_batch_outputs": (_I, [_P, _P]),
 }
 

@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(512) k_batch(BtArgs a)
     extern __shared__ float bx[]; /* G spectra, stride lstr */
     __shared__ uint32_t m_s[BT_GMAX], m_beg[BT_GMAX], m_end[BT_GMAX], m_z[BT_GMAX], m_len[BT_GMAX];
     __shared__ uint64_t m_off[BT_GMAX];
-    __shared__ uint32_t sh_t, sh_blk, sh_lblk;
+    __shared__ uint32_t sh_t, sh_lblk;
     const uint32_t tid = threadIdx.x, lane = hfx_lane();
     const uint32_t nwaves = blockDim.x / HFX_WAVE, wave = tid / HFX_WAVE;
     const uint64_t ntiles = a.tile_off[a.B];
@@ -314,6 +314,281 @@ __global__ void __launch_bounds__(512) k_batch(BtArgs a)
 }
 
 /* ------------------------------------------------------------------ */
+/* Per-database peptide index                                            */
+/* ------------------------------------------------------------------ */
+
+/* Everything the batched forms precompute per peptide: it depends on the
+ * database only, so it is built once and shared by every batch. */
+struct hfx_pepix {
+    hfx_ctx *ctx;
+    hfx_db db;
+    uint32_t forms, L, lcap;
+    /* HFX_PEPIX_LADDER: b ladders of every peptide, residual-sorted order */
+    void *lad_mem[2];
+    hfx_dn_ladder lad;
+    /* HFX_PEPIX_IONS: peptide q (residual-sorted position), level l <= L:
+     * cnt[q L + l - 1] entries at ent + ent_off[q L + l - 1] */
+    uint32_t *caps, *cnt;
+    uint64_t *ent_off;
+    uint16_t *ent;
+    uint64_t n_ent, n_pos, bytes;
+};
+
+template <class T>
+static int dalloc(T **p, size_t n)
+{
+    hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        hfx_set_error("device allocation of %zu bytes failed: %s", n * sizeof(T),
+                      hipGetErrorString(e));
+        return HFX_ERR_NOMEM;
+    }
+    return HFX_OK;
+}
+
+extern "C" int hfx_pepix_destroy(hfx_pepix *ix)
+{
+    if (!ix) return HFX_OK;
+    (void)hipDeviceSynchronize();
+    void *ps[] = {ix->lad_mem[0], ix->lad_mem[1], ix->caps, ix->cnt, ix->ent_off, ix->ent};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    free(ix);
+    return HFX_OK;
+}
+
+extern "C" int hfx_pepix_build(hfx_ctx *ctx, const hfx_db *db, uint32_t forms, uint32_t levels,
+                               uint32_t max_len, void *stream, hfx_pepix **out)
+{
+    HFX_REQUIRE(ctx && db && out, HFX_ERR_ARG, "hfx_pepix_build: null argument");
+    HFX_REQUIRE(forms && !(forms & ~(HFX_PEPIX_LADDER | HFX_PEPIX_IONS)), HFX_ERR_ARG,
+                "hfx_pepix_build: forms 0x%x", forms);
+    HFX_REQUIRE(!(forms & HFX_PEPIX_IONS) || (levels >= 1 && levels <= HFX_MAX_CHARGE &&
+                                                max_len >= 8 && max_len <= HFX_MAX_LEN),
+                HFX_ERR_ARG, "hfx_pepix_build: levels 1..%d and max_len 8..%d", HFX_MAX_CHARGE,
+                HFX_MAX_LEN);
+    *out = nullptr;
+    hfx_pepix *ix = (hfx_pepix *)calloc(1, sizeof(hfx_pepix));
+    HFX_REQUIRE(ix, HFX_ERR_NOMEM, "hfx_pepix_build: out of host memory");
+    ix->ctx = ctx;
+    ix->db = *db;
+    ix->forms = forms;
+    hipStream_t st = hfx_stream(stream);
+    const uint32_t P = db->num_pep;
+    int rc = HFX_OK;
+    if (forms & HFX_PEPIX_LADDER) {
+        rc = hfx_dense_ladder_build(ctx, db, 0, P, 1, &ix->lad, ix->lad_mem, stream);
+        if (!rc) {
+            ix->n_pos = 0;
+            rc = hipMemcpyAsync(&ix->n_pos, ix->lad.off + P, 8, hipMemcpyDeviceToHost, st) ||
+                         hipStreamSynchronize(st)
+                     ? HFX_ERR_HIP
+                     : HFX_OK;
+            ix->bytes += ((uint64_t)P + 1) * 8 + (uint64_t)P * 8 +
+                         std::max<uint64_t>(ix->n_pos, 1) * sizeof(hfx_dn_pos);
+        }
+    }
+    if (!rc && (forms & HFX_PEPIX_IONS)) {
+        ix->L = levels;
+        ix->lcap = max_len;
+        const uint64_t PL = (uint64_t)P * levels;
+        if (PL >= 0xffffffffull) {
+            hfx_set_error("hfx_pepix_build: %u peptides x %u levels", P, levels);
+            rc = HFX_ERR_ARG;
+        }
+        if (!rc) rc = dalloc(&ix->caps, PL);
+        if (!rc) rc = dalloc(&ix->ent_off, PL + 1);
+        if (!rc) rc = dalloc(&ix->cnt, PL);
+        if (!rc && P) {
+            hipLaunchKernelGGL(k_ix_caps, dim3((P + 255) / 256), dim3(256), 0, st, ix->caps, *db,
+                               0u, P, levels, max_len);
+            rc = hfx_scan_offsets(ctx, ix->ent_off, ix->caps, (uint32_t)PL, stream);
+            if (!rc && (hipMemcpyAsync(&ix->n_ent, ix->ent_off + PL, 8, hipMemcpyDeviceToHost,
+                                       st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess)) {
+                hfx_set_error("hfx_pepix_build: reading the index size failed");
+                rc = HFX_ERR_HIP;
+            }
+        }
+        if (!rc) rc = dalloc(&ix->ent, ix->n_ent);
+        if (!rc && P) {
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(
+                (P + IX_WAVES - 1) / IX_WAVES, (uint64_t)hfx_ctx_num_cus(ctx) * 16);
+            hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, ix->ent, ix->cnt,
+                               ix->ent_off, *db, 0u, P, levels, max_len);
+        }
+        ix->bytes += PL * 4 * 2 + (PL + 1) * 8 + ix->n_ent * 2;
+    }
+    if (!rc && hipGetLastError() != hipSuccess) {
+        hfx_set_error("hfx_pepix_build: kernel launch failed");
+        rc = HFX_ERR_HIP;
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) {
+        hfx_set_error("hfx_pepix_build: synchronize failed");
+        rc = HFX_ERR_HIP;
+    }
+    if (rc) {
+        hfx_pepix_destroy(ix);
+        return rc;
+    }
+    *out = ix;
+    return HFX_OK;
+}
+
+extern "C" int hfx_pepix_info(const hfx_pepix *ix, uint64_t *info /*[6]*/)
+{
+    HFX_REQUIRE(ix && info, HFX_ERR_ARG, "hfx_pepix_info: null argument");
+    info[0] = ix->forms; info[1] = ix->L; info[2] = ix->lcap; info[3] = ix->n_pos;
+    info[4] = ix->n_ent; info[5] = ix->bytes;
+    return HFX_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Spectrum order and dense blocks, on the device                        */
+/* ------------------------------------------------------------------ */
+
+/* Spectra are ordered by (fragment charge level, precursor mass, index):
+ * bucket = level * nbz + (window begin >> sh) -- the window's first
+ * residual-sorted position is a monotone function of the precursor mass --
+ * counted, scattered, then ranked inside each bucket by (mass, index). */
+__global__ void k_ord_fill(uint32_t *a, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+
+__device__ __forceinline__ uint32_t ord_bucket(uint32_t s, const uint32_t *begin,
+                                               const uint32_t *zmax, uint32_t sh, uint32_t nbz)
+{
+    return hfx_zeff(zmax[s]) * nbz + (begin[s] >> sh);
+}
+
+__global__ void k_ord_count(uint32_t *cnt, const uint32_t *begin, const uint32_t *zmax, uint32_t S,
+                            uint32_t sh, uint32_t nbz)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) atomicAdd(&cnt[ord_bucket(s, begin, zmax, sh, nbz)], 1u);
+}
+
+/* counts down to zero, leaving cnt cleared */
+__global__ void k_ord_scatter(uint32_t *tmp, uint32_t *cnt, const uint64_t *boff,
+                              const uint32_t *begin, const uint32_t *zmax, uint32_t S, uint32_t sh,
+                              uint32_t nbz)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint32_t k = ord_bucket(s, begin, zmax, sh, nbz);
+    tmp[boff[k] + atomicSub(&cnt[k], 1u) - 1u] = s;
+}
+
+__global__ void k_ord_rank(uint32_t *order, const uint32_t *tmp, const uint64_t *boff,
+                           const uint32_t *begin, const uint32_t *zmax, const float *mass,
+                           uint32_t S, uint32_t sh, uint32_t nbz)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S) return;
+    const uint32_t s = tmp[i];
+    const uint32_t k = ord_bucket(s, begin, zmax, sh, nbz);
+    const uint32_t a = (uint32_t)boff[k], e = (uint32_t)boff[k + 1];
+    const float ms = mass[s];
+    uint32_t rank = 0;
+    for (uint32_t j = a; j < e; ++j) {
+        const uint32_t t = tmp[j];
+        const float mt = mass[t];
+        rank += (mt < ms || (mt == ms && t < s)) ? 1u : 0u;
+    }
+    order[a + rank] = s;
+}
+
+/* Dense blocks: the order cut into runs of at most DN_NS spectra that
+ * minimise the sum over blocks of (charge levels x rows of the hull of the
+ * windows) -- a block spanning a gap in precursor mass would score every
+ * candidate in the gap.  A dynamic programme per segment of CUT_SEG order
+ * positions (one thread each; segments are cut independently, which costs
+ * about 1% more rows than one programme over the whole order). */
+#define CUT_SEG 32
+#define CUT_WG 64
+#define CUT_NS 8
+__global__ void __launch_bounds__(CUT_WG) k_cut_seg(uint32_t *seg_cuts, uint32_t *seg_n,
+                                                   const uint32_t *order, const uint32_t *begin,
+                                                   const uint32_t *num, const uint32_t *zmax,
+                                                   uint32_t S)
+{
+    __shared__ uint32_t lo_s[CUT_WG * CUT_SEG], hi_s[CUT_WG * CUT_SEG], lv_s[CUT_WG * CUT_SEG];
+    __shared__ uint8_t from_s[CUT_WG][CUT_SEG + 1];
+    const uint32_t base = blockIdx.x * CUT_WG * CUT_SEG;
+    for (uint32_t i = threadIdx.x; i < CUT_WG * CUT_SEG; i += CUT_WG) {
+        const uint32_t p = base + i;
+        uint32_t lo = 0xffffffffu, hi = 0, lv = 0;
+        if (p < S) {
+            const uint32_t s = order[p], z = hfx_zeff(zmax[s]), n = num[s];
+            if (z && n) {
+                lo = begin[s];
+                hi = lo + n;
+                lv = 1u << z;
+            }
+        }
+        lo_s[i] = lo;
+        hi_s[i] = hi;
+        lv_s[i] = lv;
+    }
+    __syncthreads();
+    const uint32_t seg = blockIdx.x * CUT_WG + threadIdx.x;
+    const uint32_t p0 = seg * CUT_SEG;
+    if (p0 >= S) return;
+    const uint32_t len = min((uint32_t)CUT_SEG, S - p0);
+    const uint32_t *lo = lo_s + threadIdx.x * CUT_SEG, *hi = hi_s + threadIdx.x * CUT_SEG,
+                   *lv = lv_s + threadIdx.x * CUT_SEG;
+    uint8_t *from = from_s[threadIdx.x];
+    /* bst[k - 1] = best cost of the first j - k positions */
+    uint64_t bst[CUT_NS];
+#pragma unroll
+    for (int k = 0; k < CUT_NS; ++k) bst[k] = k == 0 ? 0ull : ~0ull;
+    for (uint32_t j = 1; j <= len; ++j) {
+        uint32_t l = 0xffffffffu, h = 0, v = 0, fj = j - 1;
+        uint64_t bj = ~0ull;
+#pragma unroll
+        for (uint32_t k = 1; k <= CUT_NS; ++k) {
+            if (k <= j) {
+                const uint32_t x = j - k;
+                l = min(l, lo[x]);
+                h = max(h, hi[x]);
+                v |= lv[x];
+                const uint64_t rows = h > l ? (uint64_t)(h - l) : 0ull;
+                const uint64_t c = bst[k - 1] + rows * (uint64_t)__builtin_popcount(v);
+                if (c < bj) {
+                    bj = c;
+                    fj = x;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = CUT_NS - 1; k > 0; --k) bst[k] = bst[k - 1];
+        bst[0] = bj;
+        from[j] = (uint8_t)fj;
+    }
+    /* block starts, back from the segment's end, then written in order */
+    uint32_t cnt = 0;
+    for (uint32_t j = len; j > 0; j = from[j]) ++cnt;
+    uint32_t w = cnt;
+    for (uint32_t j = len; j > 0; j = from[j]) seg_cuts[(size_t)seg * CUT_SEG + --w] = p0 + from[j];
+    seg_n[seg] = cnt;
+}
+
+/* bstart[0 .. nb) = the segments' block starts in order, bstart[nb .. cap]
+ * = S (empty blocks), nb = seg_off[nseg] */
+__global__ void k_cut_place(uint32_t *bstart, const uint32_t *seg_cuts, const uint32_t *seg_n,
+                            const uint64_t *seg_off, uint32_t nseg, uint32_t cap, uint32_t S)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t seg = t / CUT_SEG, i = t % CUT_SEG;
+    if (seg < nseg && i < seg_n[seg]) bstart[seg_off[seg] + i] = seg_cuts[t];
+    const uint32_t nb = (uint32_t)seg_off[nseg];
+    if (t <= cap && t >= nb) bstart[t] = S;
+}
+
+/* ------------------------------------------------------------------ */
 /* The batch object                                                       */
 /* ------------------------------------------------------------------ */
 
@@ -323,43 +598,52 @@ struct hfx_batch {
     hfx_spectra sp;
     float eps;
     uint32_t N, margin;
-    uint32_t S, G, C, threads, B, L, lstr, lcap, p0, P;
-    uint64_t total, n_ent, tiles;
+    uint32_t S, G, C, threads, B, L, lstr;
+    uint64_t total;
     bool planned;
+    const hfx_pepix *ix; /* the caller's index, or own */
+    hfx_pepix *own;      /* built on first need when the caller gave none */
     /* device buffers (owned) */
-    uint32_t *begin, *num, *order, *ubeg, *uend, *ntiles, *caps, *cnt, *counter;
-    uint64_t *score_off, *tile_off, *ent_off;
-    uint16_t *ent;
+    uint32_t *begin, *num, *order, *ubeg, *uend, *ntiles, *counter;
+    uint64_t *score_off, *tile_off;
     float *scores;
     uint32_t *top_pos, *top_cnt, *flag;
     double *top_exact;
     float *top_f32;
     uint8_t *tx_scratch; /* top-N exact route */
-    uint32_t precision;  /* HFX_BATCH_FP32_SPARSE or HFX_BATCH_BF16X2_DENSE */
-    uint32_t CH;         /* dense: candidates per work item */
-    uint8_t *dn_scratch; /* dense: hi/lo spectra, blocks (null if unsupported) */
-    void *dn_mem[2];     /* dense: the ladders of the plan's windows */
-    hfx_dn_ladder dn_lad;
-    uint32_t *dn_bstart; /* dense: [DB + 1] block starts in order */
-    uint32_t DB;
+    /* spectrum order: buckets */
+    uint32_t ord_sh, ord_nbz, ord_nb;
+    uint32_t *ord_cnt, *ord_tmp;
+    uint64_t *ord_off;
+    uint32_t precision; /* HFX_BATCH_FP32_SPARSE or HFX_BATCH_BF16X2_DENSE */
+    uint32_t CH;        /* dense: candidates per work item */
+    uint8_t *dn_scratch; /* dense: hi/lo spectra, blocks (allocated with the dense form) */
+    uint32_t *dn_bstart; /* dense: [S + 1] block starts in order (S past the last block) */
+    uint32_t *cut_seg, *cut_n;
+    uint64_t *cut_off;
+    uint32_t nseg;
     hipGraph_t graph;
     hipGraphExec_t exec;
 };
 
 static void bt_free(hfx_batch *b)
 {
-    void *ps[] = {b->begin,    b->num,     b->order,  b->ubeg,      b->uend,    b->ntiles,
-                  b->caps,     b->cnt,     b->counter, b->score_off, b->tile_off, b->ent_off,
-                  b->ent,      b->scores,  b->top_pos, b->top_cnt,   b->flag,     b->top_exact,
-                  b->top_f32,  b->tx_scratch, b->dn_scratch, b->dn_mem[0], b->dn_mem[1],
-                  b->dn_bstart};
+    void *ps[] = {b->begin,    b->num,       b->order,     b->ubeg,      b->uend,
+                  b->ntiles,   b->counter,   b->score_off, b->tile_off,  b->scores,
+                  b->top_pos,  b->top_cnt,   b->flag,      b->top_exact, b->top_f32,
+                  b->tx_scratch, b->ord_cnt, b->ord_tmp,   b->ord_off,   b->dn_scratch,
+                  b->dn_bstart, b->cut_seg,  b->cut_n,     b->cut_off};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     hfx_ctx *ctx = b->ctx;
+    const hfx_pepix *ix = b->ix;
+    hfx_pepix *own = b->own;
     memset(b, 0, sizeof *b);
     b->ctx = ctx;
+    b->own = own;
+    b->ix = ix == own ? nullptr : ix; /* the caller's index stays attached */
 }
 
 extern "C" int hfx_batch_create(hfx_ctx *ctx, hfx_batch **out)
@@ -377,18 +661,23 @@ extern "C" int hfx_batch_destroy(hfx_batch *b)
     if (!b) return HFX_OK;
     (void)hipDeviceSynchronize();
     bt_free(b);
+    if (b->own) hfx_pepix_destroy(b->own);
     free(b);
     return HFX_OK;
 }
 
-template <class T>
-static int dalloc(T **p, size_t n)
+extern "C" int hfx_batch_use_index(hfx_batch *b, const hfx_pepix *ix)
 {
-    hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
-    if (e != hipSuccess) {
-        *p = nullptr;
-        hfx_set_error("batch allocation of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
-        return HFX_ERR_NOMEM;
+    HFX_REQUIRE(b, HFX_ERR_ARG, "hfx_batch_use_index: null batch");
+    HFX_REQUIRE(!b->planned || !ix ||
+                    (ix->db.d_idx_sorted == b->db.d_idx_sorted && ix->db.num_pep == b->db.num_pep),
+                HFX_ERR_ARG, "hfx_batch_use_index: the index belongs to another database");
+    b->ix = ix;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
     }
     return HFX_OK;
 }
@@ -402,7 +691,38 @@ static std::map<std::tuple<int, int, uint32_t>, uint32_t> g_tune_dense;
 static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
 {
     auto lg = [](uint64_t v) { int k = 0; while (v > 1) { v >>= 1; ++k; } return k; };
-    return std::make_tuple(lg(b->S), lg(b->P), b->L);
+    return std::make_tuple(lg(b->S), lg(b->total), b->L);
+}
+
+/* the form of the peptide index the current precision reads */
+static uint32_t bt_form(const hfx_batch *b)
+{
+    return b->precision == HFX_BATCH_BF16X2_DENSE ? HFX_PEPIX_LADDER : HFX_PEPIX_IONS;
+}
+
+/* makes sure the index holds the current precision's form: the caller's,
+ * or one the batch builds (never while a stream is being captured) */
+static int bt_need_index(hfx_batch *b, hipStream_t st)
+{
+    const uint32_t f = bt_form(b);
+    if (b->ix && (b->ix->forms & f) && (f != HFX_PEPIX_IONS || b->ix->L >= b->L)) return HFX_OK;
+    HFX_REQUIRE(!b->ix || b->ix == b->own, HFX_ERR_ARG,
+                "batch: the attached peptide index lacks the %s form this precision reads",
+                f == HFX_PEPIX_LADDER ? "ladder" : "ion list");
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HFX_CHECK_HIP(hipStreamIsCapturing(st, &cs));
+    HFX_REQUIRE(cs == hipStreamCaptureStatusNone, HFX_ERR_ARG,
+                "batch: the peptide index is built on first use, not under capture "
+                "(run or set the precision once before capturing)");
+    uint32_t forms = f;
+    if (b->own) forms |= b->own->forms;
+    hfx_pepix *nx = nullptr;
+    int rc = hfx_pepix_build(b->ctx, &b->db, forms, HFX_MAX_CHARGE, HFX_MAX_LEN, (void *)st, &nx);
+    if (rc) return rc;
+    if (b->own) hfx_pepix_destroy(b->own);
+    b->own = nx;
+    b->ix = nx;
+    return HFX_OK;
 }
 
 extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float eps,
@@ -413,6 +733,15 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
                 "hfx_batch_plan: N + margin must be <= 128 and N <= 64");
     hipStream_t st = hfx_stream(stream);
     bt_free(b);
+    if (b->own && (b->own->db.d_idx_sorted != db->d_idx_sorted ||
+                   b->own->db.num_pep != db->num_pep || b->own->db.d_ions != db->d_ions)) {
+        hfx_pepix_destroy(b->own);
+        b->own = nullptr;
+    }
+    if (b->ix && (b->ix->db.d_idx_sorted != db->d_idx_sorted || b->ix->db.num_pep != db->num_pep))
+        HFX_REQUIRE(false, HFX_ERR_ARG, "hfx_batch_plan: the attached index belongs to another "
+                                        "database");
+    if (!b->ix) b->ix = b->own;
     b->db = *db;
     b->sp = *sp;
     b->eps = eps;
@@ -423,80 +752,31 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     if ((rc = dalloc(&b->begin, S)) || (rc = dalloc(&b->num, S)) ||
         (rc = dalloc(&b->score_off, (size_t)S + 1)) || (rc = dalloc(&b->order, S)))
         return rc;
+    /* the windows once, to size the score arrays (the run computes them again) */
     if ((rc = hfx_window(b->ctx, b->begin, b->num, db, sp->d_mass, S, eps, stream))) return rc;
     if ((rc = hfx_scan_offsets(b->ctx, b->score_off, b->num, S, stream))) return rc;
-    /* host view of the windows, masses and charges (plan only) */
-    std::vector<uint32_t> hb(S), hn(S), hz(S), hl(S);
-    std::vector<float> hm(S);
-    HFX_CHECK_HIP(hipMemcpyAsync(hb.data(), b->begin, 4ull * S, hipMemcpyDeviceToHost, st));
-    HFX_CHECK_HIP(hipMemcpyAsync(hn.data(), b->num, 4ull * S, hipMemcpyDeviceToHost, st));
-    HFX_CHECK_HIP(hipMemcpyAsync(hz.data(), sp->d_zmax, 4ull * S, hipMemcpyDeviceToHost, st));
-    HFX_CHECK_HIP(hipMemcpyAsync(hl.data(), sp->d_len, 4ull * S, hipMemcpyDeviceToHost, st));
-    HFX_CHECK_HIP(hipMemcpyAsync(hm.data(), sp->d_mass, 4ull * S, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> hz(S), hl(S);
+    if (S) {
+        HFX_CHECK_HIP(hipMemcpyAsync(hz.data(), sp->d_zmax, 4ull * S, hipMemcpyDeviceToHost, st));
+        HFX_CHECK_HIP(hipMemcpyAsync(hl.data(), sp->d_len, 4ull * S, hipMemcpyDeviceToHost, st));
+    }
     HFX_CHECK_HIP(hipMemcpyAsync(&b->total, b->score_off + S, 8, hipMemcpyDeviceToHost, st));
     HFX_CHECK_HIP(hipStreamSynchronize(st));
-    std::vector<uint32_t> ord(S);
-    std::iota(ord.begin(), ord.end(), 0u);
-    /* by fragment charge level, then precursor mass: blocks of neighbouring
-     * spectra share one ladder level and most of their windows */
-    auto zlev = [&](uint32_t s) { return hz[s] <= HFX_MAX_CHARGE ? hz[s] : 0u; };
-    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
-        const uint32_t zx = zlev(x), zy = zlev(y);
-        return zx != zy ? zx < zy : hm[x] < hm[y];
-    });
-    HFX_CHECK_HIP(hipMemcpyAsync(b->order, ord.data(), 4ull * S, hipMemcpyHostToDevice, st));
-    /* dense blocks: the order cut into runs of at most eight spectra that
-     * minimise the sum over blocks of (charge levels x rows of the hull of
-     * the windows) -- a block spanning a gap in precursor mass would score
-     * every candidate in the gap.  A per-block constant (B staging, tile
-     * rounding) measured no better than none at tryptic-20k-10k. */
-    std::vector<uint32_t> bcut;
-    {
-        constexpr uint64_t FIXED = 0;
-        std::vector<uint64_t> best(S + 1, UINT64_MAX);
-        std::vector<uint32_t> from(S + 1, 0);
-        best[0] = 0;
-        for (uint32_t j = 1; j <= S; ++j) {
-            uint32_t lo = 0xffffffffu, hi = 0, lv = 0;
-            for (uint32_t k = 1; k <= 8 && k <= j; ++k) {
-                const uint32_t x = ord[j - k], z = zlev(x);
-                if (z && hn[x]) {
-                    lo = std::min(lo, hb[x]);
-                    hi = std::max(hi, hb[x] + hn[x]);
-                    lv |= 1u << z;
-                }
-                const uint64_t rows = hi > lo ? hi - lo : 0;
-                const uint64_t c = best[j - k] + rows * (uint64_t)__builtin_popcount(lv) + FIXED;
-                if (c < best[j]) {
-                    best[j] = c;
-                    from[j] = j - k;
-                }
-            }
-        }
-        for (uint32_t j = S; j > 0; j = from[j]) bcut.push_back(from[j]);
-        std::reverse(bcut.begin(), bcut.end());
-        bcut.push_back(S);
-    }
-    b->DB = (uint32_t)bcut.size() - 1;
-    if ((rc = dalloc(&b->dn_bstart, bcut.size()))) return rc;
-    HFX_CHECK_HIP(hipMemcpyAsync(b->dn_bstart, bcut.data(), 4ull * bcut.size(),
-                                 hipMemcpyHostToDevice, st));
-    HFX_CHECK_HIP(hipStreamSynchronize(st));
-    uint32_t p0 = 0xffffffffu, p1 = 0, L = 0, lmax = 0;
+    uint32_t L = 0, lmax = 0;
     for (uint32_t s = 0; s < S; ++s) {
         lmax = std::max(lmax, std::min(hl[s], std::min(sp->x_stride, (uint32_t)HFX_MAX_LEN)));
-        if (hn[s] == 0) continue;
-        p0 = std::min(p0, hb[s]);
-        p1 = std::max(p1, hb[s] + hn[s]);
-        uint32_t z = hz[s] <= HFX_MAX_CHARGE ? hz[s] : 0u;
-        L = std::max(L, z);
+        L = std::max(L, hz[s] <= HFX_MAX_CHARGE ? hz[s] : 0u);
     }
-    if (p1 <= p0) { p0 = 0; p1 = 0; }
-    b->p0 = p0;
-    b->P = p1 - p0;
     b->L = std::max(L, 1u);
-    b->lcap = std::max(lmax, 8u);
-    b->lstr = (b->lcap + 15) & ~15u;
+    b->lstr = (std::max(lmax, 8u) + 15) & ~15u;
+    /* order buckets: about two spectra a bucket on average */
+    b->ord_sh = 0;
+    while (((uint64_t)db->num_pep >> b->ord_sh) > std::max<uint64_t>(S / 2, 64)) ++b->ord_sh;
+    b->ord_nbz = (db->num_pep >> b->ord_sh) + 1;
+    b->ord_nb = (HFX_MAX_CHARGE + 1) * b->ord_nbz;
+    if ((rc = dalloc(&b->ord_cnt, b->ord_nb)) || (rc = dalloc(&b->ord_off, (size_t)b->ord_nb + 1)) ||
+        (rc = dalloc(&b->ord_tmp, S)))
+        return rc;
     /* defaults, replaced by the autotuner's choice for this shape */
     b->G = 8;
     b->C = 64;
@@ -510,21 +790,6 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
         auto jt = g_tune_dense.find(tune_key(b));
         if (jt != g_tune_dense.end()) b->CH = jt->second;
     }
-    const uint32_t P = b->P, Lv = b->L;
-    /* index capacities and offsets */
-    if ((rc = dalloc(&b->caps, (size_t)P * Lv)) || (rc = dalloc(&b->ent_off, (size_t)P * Lv + 1)) ||
-        (rc = dalloc(&b->cnt, (size_t)P * Lv)))
-        return rc;
-    if (P) {
-        hipLaunchKernelGGL(k_ix_caps, dim3((P + 255) / 256), dim3(256), 0, st, b->caps, *db, p0, P,
-                           Lv, b->lcap);
-        HFX_REQUIRE((uint64_t)P * Lv < 0xffffffffull, HFX_ERR_ARG, "index too large");
-        if ((rc = hfx_scan_offsets(b->ctx, b->ent_off, b->caps, P * Lv, stream))) return rc;
-        HFX_CHECK_HIP(hipMemcpyAsync(&b->n_ent, b->ent_off + (size_t)P * Lv, 8,
-                                     hipMemcpyDeviceToHost, st));
-        HFX_CHECK_HIP(hipStreamSynchronize(st));
-    }
-    if ((rc = dalloc(&b->ent, b->n_ent))) return rc;
     if ((rc = dalloc(&b->scores, b->total)) || (rc = dalloc(&b->counter, 1)) ||
         (rc = dalloc(&b->top_pos, (size_t)S * N)) || (rc = dalloc(&b->top_exact, (size_t)S * N)) ||
         (rc = dalloc(&b->top_f32, (size_t)S * N)) || (rc = dalloc(&b->top_cnt, S)) ||
@@ -533,15 +798,6 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     /* the top-N exact route's scratch is owned by the batch, so that a
      * captured graph never allocates and never sees it move */
     if ((rc = dalloc(&b->tx_scratch, hfx_topn_scratch_bytes(S, b->total)))) return rc;
-    if (sp->x_stride <= hfx_dense_max_len()) {
-        if ((rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(S)))) return rc;
-        /* the spectra's hi/lo BF16 rows and the b ladders of every candidate
-         * of the plan, built once */
-        if ((rc = hfx_dense_split(sp, b->dn_scratch, stream))) return rc;
-        if ((rc = hfx_dense_ladder_build(b->ctx, db, b->p0, b->P, 1, &b->dn_lad, b->dn_mem,
-                                         stream)))
-            return rc;
-    }
     /* blocks are sized for the smallest G the tuner may pick (4) */
     const uint32_t bmax = (S + 3) / 4;
     if ((rc = dalloc(&b->ubeg, bmax)) || (rc = dalloc(&b->uend, bmax)) ||
@@ -552,7 +808,74 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     return HFX_OK;
 }
 
-/* blocks + tiles for the current (G, C) */
+extern "C" int hfx_batch_plan_ix(hfx_batch *b, const hfx_pepix *ix, const hfx_spectra *sp,
+                                 float eps, uint32_t N, uint32_t margin, void *stream)
+{
+    HFX_REQUIRE(b && ix, HFX_ERR_ARG, "hfx_batch_plan_ix: null argument");
+    b->ix = ix;
+    return hfx_batch_plan(b, &ix->db, sp, eps, N, margin, stream);
+}
+
+/* the dense form's per-batch buffers (hi/lo rows of every spectrum, block
+ * bookkeeping), allocated when the form is first chosen */
+static int bt_dense_alloc(hfx_batch *b)
+{
+    if (b->dn_scratch) return HFX_OK;
+    int rc;
+    b->nseg = (b->S + CUT_SEG - 1) / CUT_SEG;
+    if ((rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(b->S))) ||
+        (rc = dalloc(&b->dn_bstart, (size_t)b->S + 1)) ||
+        (rc = dalloc(&b->cut_seg, (size_t)b->nseg * CUT_SEG)) || (rc = dalloc(&b->cut_n, b->nseg)) ||
+        (rc = dalloc(&b->cut_off, (size_t)b->nseg + 1)))
+        return rc;
+    return HFX_OK;
+}
+
+/* ---- the phases of one batched search ---- */
+
+/* windows and score offsets */
+static int bt_windows(hfx_batch *b, hipStream_t st)
+{
+    int rc;
+    if ((rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, (void *)st)))
+        return rc;
+    return hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, (void *)st);
+}
+
+/* spectrum order (fragment charge level, precursor mass, index) */
+static int bt_order(hfx_batch *b, hipStream_t st)
+{
+    const uint32_t S = b->S, g = (S + 255) / 256;
+    hipLaunchKernelGGL(k_ord_fill, dim3((b->ord_nb + 255) / 256), dim3(256), 0, st, b->ord_cnt,
+                       b->ord_nb);
+    hipLaunchKernelGGL(k_ord_count, dim3(g), dim3(256), 0, st, b->ord_cnt, b->begin, b->sp.d_zmax,
+                       S, b->ord_sh, b->ord_nbz);
+    int rc;
+    if ((rc = hfx_scan_offsets(b->ctx, b->ord_off, b->ord_cnt, b->ord_nb, (void *)st))) return rc;
+    hipLaunchKernelGGL(k_ord_scatter, dim3(g), dim3(256), 0, st, b->ord_tmp, b->ord_cnt,
+                       b->ord_off, b->begin, b->sp.d_zmax, S, b->ord_sh, b->ord_nbz);
+    hipLaunchKernelGGL(k_ord_rank, dim3(g), dim3(256), 0, st, b->order, b->ord_tmp, b->ord_off,
+                       b->begin, b->sp.d_zmax, b->sp.d_mass, S, b->ord_sh, b->ord_nbz);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+/* dense blocks (segment DP) and the hi/lo rows of the spectra */
+static int bt_dense_prep(hfx_batch *b, hipStream_t st)
+{
+    const uint32_t S = b->S;
+    hipLaunchKernelGGL(k_cut_seg, dim3((b->nseg + CUT_WG - 1) / CUT_WG), dim3(CUT_WG), 0, st,
+                       b->cut_seg, b->cut_n, b->order, b->begin, b->num, b->sp.d_zmax, S);
+    int rc;
+    if ((rc = hfx_scan_offsets(b->ctx, b->cut_off, b->cut_n, b->nseg, (void *)st))) return rc;
+    const uint32_t nt = std::max(b->nseg * CUT_SEG, S + 1);
+    hipLaunchKernelGGL(k_cut_place, dim3((nt + 255) / 256), dim3(256), 0, st, b->dn_bstart,
+                       b->cut_seg, b->cut_n, b->cut_off, b->nseg, S, S);
+    HFX_CHECK_LAUNCH();
+    return hfx_dense_split(&b->sp, b->dn_scratch, (void *)st);
+}
+
+/* blocks of G spectra in order + tiles for the current (G, C) */
 static int bt_tiles(hfx_batch *b, hipStream_t st)
 {
     b->B = (b->S + b->G - 1) / b->G;
@@ -562,16 +885,24 @@ static int bt_tiles(hfx_batch *b, hipStream_t st)
     return hfx_scan_offsets(b->ctx, b->tile_off, b->ntiles, b->B, (void *)st);
 }
 
-static int bt_score(hfx_batch *b, hipStream_t st)
+static int bt_prep(hfx_batch *b, hipStream_t st)
+{
+    int rc;
+    if ((rc = bt_order(b, st))) return rc;
+    if (b->precision == HFX_BATCH_BF16X2_DENSE) return bt_dense_prep(b, st);
+    return bt_tiles(b, st);
+}
+
+static int bt_score(hfx_batch *b, hipStream_t st, void **ev)
 {
     if (b->B == 0 || b->total == 0) return HFX_OK;
+    const hfx_pepix *ix = b->ix;
     hipLaunchKernelGGL(k_bt_reset, dim3(1), dim3(1), 0, st, b->counter);
-    BtArgs a{b->scores, b->score_off, b->begin, b->num,  b->order,   b->S,     b->G,
-             b->B,      b->C,         b->ubeg,  b->uend, b->tile_off, b->ent,  b->cnt,
-             b->ent_off, b->p0,       b->P,     b->L,    b->lstr,    b->counter, b->sp};
+    BtArgs a{b->scores, b->score_off, b->begin,   b->num,     b->order,   b->S,
+             b->G,      b->B,         b->C,       b->ubeg,    b->uend,    b->tile_off,
+             ix->ent,   ix->cnt,      ix->ent_off, 0u,        b->db.num_pep, ix->L,
+             b->lstr,   b->counter,   b->sp};
     const size_t smem = (size_t)b->G * b->lstr * 4;
-    const int waves_per_wg = (int)(b->threads / HFX_WAVE);
-    (void)waves_per_wg;
     const uint32_t grid = (uint32_t)hfx_ctx_num_cus(b->ctx) * (b->G == 8 ? 2u : 4u) *
                           (b->threads == 256 ? 2u : 1u);
     /* gfx950 gives a workgroup up to 160 KiB of LDS; ask for what this
@@ -579,49 +910,31 @@ static int bt_score(hfx_batch *b, hipStream_t st)
     if (b->G == 8) {
         HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_batch<8>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[0], st));
         hipLaunchKernelGGL(k_batch<8>, dim3(grid), dim3(b->threads), smem, st, a);
     } else {
         HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_batch<4>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[0], st));
         hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(b->threads), smem, st, a);
     }
+    if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[1], st));
     hipLaunchKernelGGL(k_bt_zero_unscored, dim3(b->S), dim3(256), 0, st, b->scores, b->score_off,
                        b->sp.d_zmax, b->S);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }
 
-/* the phases of one batched search */
-static int bt_windows(hfx_batch *b, hipStream_t st)
+/* ev (optional): events recorded around the scoring kernel itself */
+static int bt_scores(hfx_batch *b, hipStream_t st, void **ev = nullptr)
 {
-    int rc;
-    if ((rc = hfx_window(b->ctx, b->begin, b->num, &b->db, b->sp.d_mass, b->S, b->eps, (void *)st)))
-        return rc;
-    return hfx_scan_offsets(b->ctx, b->score_off, b->num, b->S, (void *)st);
-}
-
-static int bt_index(hfx_batch *b, hipStream_t st)
-{
-    if (b->precision != HFX_BATCH_FP32_SPARSE || b->P == 0) return HFX_OK;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((b->P + IX_WAVES - 1) / IX_WAVES,
-                                                       (uint64_t)hfx_ctx_num_cus(b->ctx) * 16);
-    hipLaunchKernelGGL(k_ix_build, dim3(grid), dim3(IX_THREADS), 0, st, b->ent, b->cnt, b->ent_off,
-                       b->db, b->p0, b->P, b->L, b->lcap);
-    HFX_CHECK_LAUNCH();
-    return HFX_OK;
-}
-
-static int bt_scores(hfx_batch *b, hipStream_t st)
-{
-    int rc;
     if (b->precision == HFX_BATCH_BF16X2_DENSE) {
         if (b->total == 0) return HFX_OK;
         return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order,
-                                b->dn_bstart, b->DB, &b->dn_lad, &b->sp, b->CH, b->dn_scratch, 1,
-                                (void *)st);
+                                b->dn_bstart, b->S, &b->ix->lad, &b->sp, b->CH, b->dn_scratch, 1,
+                                (void *)st, ev);
     }
-    if ((rc = bt_tiles(b, st))) return rc;
-    return bt_score(b, st);
+    return bt_score(b, st, ev);
 }
 
 /* |dense - exact| <= (2^-16 + 210 * 2^-24) * 50 * sum|x|: the hi/lo split
@@ -644,7 +957,8 @@ extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
     hipStream_t st = hfx_stream(stream);
     int rc;
     if (b->S == 0) return HFX_OK;
-    if ((rc = bt_windows(b, st)) || (rc = bt_index(b, st)) || (rc = bt_scores(b, st))) return rc;
+    if ((rc = bt_need_index(b, st))) return rc;
+    if ((rc = bt_windows(b, st)) || (rc = bt_prep(b, st)) || (rc = bt_scores(b, st))) return rc;
     return bt_topn(b, st);
 }
 
@@ -653,9 +967,12 @@ extern "C" int hfx_batch_set_precision(hfx_batch *b, uint32_t precision)
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_precision: batch not planned");
     HFX_REQUIRE(precision == HFX_BATCH_FP32_SPARSE || precision == HFX_BATCH_BF16X2_DENSE,
                 HFX_ERR_ARG, "hfx_batch_set_precision: unknown precision %u", precision);
-    HFX_REQUIRE(precision != HFX_BATCH_BF16X2_DENSE || b->dn_scratch, HFX_ERR_ARG,
+    HFX_REQUIRE(precision != HFX_BATCH_BF16X2_DENSE || b->sp.x_stride <= hfx_dense_max_len(),
+                HFX_ERR_ARG,
                 "hfx_batch_set_precision: the dense form takes spectra of up to %u bins "
                 "(x_stride %u)", hfx_dense_max_len(), b->sp.x_stride);
+    int rc;
+    if (precision == HFX_BATCH_BF16X2_DENSE && (rc = bt_dense_alloc(b))) return rc;
     if (precision != b->precision && b->exec) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
@@ -671,10 +988,13 @@ extern "C" int hfx_batch_capture(hfx_batch *b, void *stream)
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_capture: batch not planned");
     HFX_REQUIRE(stream != nullptr, HFX_ERR_ARG, "hfx_batch_capture: needs a non-null stream");
     hipStream_t st = hfx_stream(stream);
+    int rc;
+    if ((rc = bt_need_index(b, st))) return rc;
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    /* kernel attributes are set outside the capture */
     HFX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = hfx_batch_run(b, stream);
+    rc = hfx_batch_run(b, stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(st, &g);
     HFX_REQUIRE(rc == HFX_OK, rc, "hfx_batch_capture: run failed during capture: %s",
@@ -692,15 +1012,16 @@ extern "C" int hfx_batch_replay(hfx_batch *b, void *stream)
     return HFX_OK;
 }
 
-/* Times the scoring kernel of the planned batch for each (G, C, threads) in
- * a small grid and keeps the fastest; the choice is remembered for batches of
- * the same shape bucket in this process. */
+/* Times the scoring kernel of the planned batch for each configuration in a
+ * small grid and keeps the fastest; the choice is remembered for batches of
+ * the same shape bucket in this process.  Sparse form: (G, C, threads);
+ * dense form: the chunk of candidates per work item. */
 extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_autotune: batch not planned");
     hipStream_t st = hfx_stream(stream);
     int rc;
-    /* index and windows once */
+    /* index, windows and order once */
     if ((rc = hfx_batch_run(b, stream))) return rc;
     hipEvent_t e0, e1;
     HFX_CHECK_HIP(hipEventCreate(&e0));
@@ -709,7 +1030,7 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
      * least disturbed by clocks and other work) */
     auto time2 = [&](float *ms) -> int {
         int r;
-        if ((r = bt_scores(b, st))) return r; /* warm */
+        if ((r = bt_prep(b, st)) || (r = bt_scores(b, st))) return r; /* warm */
         *ms = 1e30f;
         for (int k = 0; k < 3; ++k) {
             float t = 0.0f;
@@ -763,34 +1084,43 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
         (void)hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
+    /* leave the batch's outputs those of the chosen configuration */
+    if ((rc = hfx_batch_run(b, stream))) return rc;
     if (out_cfg) {
         out_cfg[0] = bG; out_cfg[1] = bC; out_cfg[2] = bT;
-        if (b->precision == HFX_BATCH_BF16X2_DENSE) { out_cfg[0] = 0; out_cfg[1] = bCH; out_cfg[2] = 256; }
+        if (b->precision == HFX_BATCH_BF16X2_DENSE) { out_cfg[0] = 0; out_cfg[1] = bCH; out_cfg[2] = 1024; }
     }
     return HFX_OK;
 }
 
 /* Runs the planned batch phase by phase with events between the phases:
- * ms[0] windows + offsets, ms[1] ion index (sparse form only), ms[2] blocks,
- * tiles + scoring kernel, ms[3] top-N (+ exact route). */
+ * ms[0] windows + offsets, ms[1] per-batch preparation (spectrum order;
+ * dense form: block cut and hi/lo rows; sparse form: tiles), ms[2] scoring,
+ * ms[3] top-N (+ exact route). */
 extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
 {
     HFX_REQUIRE(b && b->planned && ms, HFX_ERR_ARG, "hfx_batch_time: batch not planned");
     hipStream_t st = hfx_stream(stream);
-    hipEvent_t ev[5];
-    for (auto &e : ev) HFX_CHECK_HIP(hipEventCreate(&e));
     int rc = HFX_OK;
+    for (int i = 0; i < 5; ++i) ms[i] = 0.0f;
+    if (b->S == 0) return HFX_OK;
+    if ((rc = bt_need_index(b, st))) return rc;
+    hipEvent_t ev[7];
+    for (auto &e : ev) HFX_CHECK_HIP(hipEventCreate(&e));
+    void *kev[2] = {ev[5], ev[6]};
     HFX_CHECK_HIP(hipEventRecord(ev[0], st));
     if (!rc) rc = bt_windows(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[1], st));
-    if (!rc) rc = bt_index(b, st);
+    if (!rc) rc = bt_prep(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[2], st));
-    if (!rc) rc = bt_scores(b, st);
+    const bool scored = b->total != 0 && (b->precision == HFX_BATCH_BF16X2_DENSE || b->B != 0);
+    if (!rc) rc = bt_scores(b, st, kev);
     HFX_CHECK_HIP(hipEventRecord(ev[3], st));
     if (!rc) rc = bt_topn(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[4], st));
     HFX_CHECK_HIP(hipEventSynchronize(ev[4]));
     for (int i = 0; i < 4; ++i) HFX_CHECK_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    if (scored && !rc) HFX_CHECK_HIP(hipEventElapsedTime(&ms[4], ev[5], ev[6]));
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
 }
@@ -817,21 +1147,34 @@ extern "C" int hfx_batch_dense_blocks(const hfx_batch *b, uint32_t *h_bstart, ui
                                       uint32_t *num_blocks)
 {
     HFX_REQUIRE(b && num_blocks, HFX_ERR_ARG, "hfx_batch_dense_blocks: null argument");
-    HFX_REQUIRE(b->planned, HFX_ERR_ARG, "hfx_batch_dense_blocks: batch not planned");
-    *num_blocks = b->DB;
+    HFX_REQUIRE(b->planned && b->dn_scratch, HFX_ERR_ARG,
+                "hfx_batch_dense_blocks: no dense run of this batch");
+    uint64_t nb = 0;
+    HFX_CHECK_HIP(hipDeviceSynchronize());
+    HFX_CHECK_HIP(hipMemcpy(&nb, b->cut_off + b->nseg, 8, hipMemcpyDeviceToHost));
+    *num_blocks = (uint32_t)nb;
     if (h_bstart && cap) {
-        const size_t n = std::min<size_t>(cap, (size_t)b->DB + 1);
+        const size_t n = std::min<size_t>(cap, (size_t)nb + 1);
         HFX_CHECK_HIP(hipMemcpy(h_bstart, b->dn_bstart, 4 * n, hipMemcpyDeviceToHost));
     }
+    return HFX_OK;
+}
+
+extern "C" int hfx_batch_order(const hfx_batch *b, uint32_t *h_order)
+{
+    HFX_REQUIRE(b && b->planned && h_order, HFX_ERR_ARG, "hfx_batch_order: batch not planned");
+    HFX_CHECK_HIP(hipDeviceSynchronize());
+    if (b->S) HFX_CHECK_HIP(hipMemcpy(h_order, b->order, 4ull * b->S, hipMemcpyDeviceToHost));
     return HFX_OK;
 }
 
 extern "C" int hfx_batch_info(const hfx_batch *b, uint64_t *info /*[12]*/)
 {
     HFX_REQUIRE(b && info, HFX_ERR_ARG, "hfx_batch_info: null argument");
-    info[0] = b->S; info[1] = b->total; info[2] = b->P; info[3] = b->L; info[4] = b->n_ent;
-    info[5] = b->G; info[6] = b->C; info[7] = b->threads; info[8] = b->lstr; info[9] = b->p0;
-    info[10] = b->precision; info[11] = b->CH;
+    const hfx_pepix *ix = b->ix;
+    info[0] = b->S; info[1] = b->total; info[2] = b->db.num_pep; info[3] = b->L;
+    info[4] = ix ? ix->n_ent : 0; info[5] = b->G; info[6] = b->C; info[7] = b->threads;
+    info[8] = b->lstr; info[9] = 0; info[10] = b->precision; info[11] = b->CH;
     return HFX_OK;
 }
 
@@ -840,6 +1183,6 @@ extern "C" int hfx_batch_outputs(const hfx_batch *b, void **ptrs /*[10]*/)
     HFX_REQUIRE(b && ptrs, HFX_ERR_ARG, "hfx_batch_outputs: null argument");
     ptrs[0] = b->top_pos; ptrs[1] = b->top_exact; ptrs[2] = b->top_f32; ptrs[3] = b->top_cnt;
     ptrs[4] = b->flag; ptrs[5] = b->scores; ptrs[6] = b->score_off; ptrs[7] = b->begin;
-    ptrs[8] = b->num; ptrs[9] = b->ent;
+    ptrs[8] = b->num; ptrs[9] = b->ix ? (void *)b->ix->ent : nullptr;
     return HFX_OK;
 }

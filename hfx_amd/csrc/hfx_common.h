@@ -140,10 +140,12 @@ int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t
 size_t hfx_dense_scratch_bytes(uint32_t S);
 uint32_t hfx_dense_max_len(void);
 int hfx_dense_split(const hfx_spectra *sp, void *scratch, void *stream);
+/* ev (optional): two hipEvent_t recorded right before and after the
+ * contraction kernel itself */
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
                      const uint32_t *d_num, const uint32_t *d_order, const uint32_t *d_bstart,
                      uint32_t nblocks, const hfx_dn_ladder *lad, const hfx_spectra *sp,
-                     uint32_t CH, void *scratch, int presplit, void *stream);
+                     uint32_t CH, void *scratch, int presplit, void *stream, void **ev);
 
 /* ------------------------------------------------------------------ */
 /* Device helpers                                                        */

@@ -409,12 +409,130 @@ __global__ void __launch_bounds__(1024) k_scan_offsets(uint64_t *off, const uint
     if (threadIdx.x == 0) off[n] = carry_s;
 }
 
+/* Multi-workgroup form (reduce, then scan), with no scratch: the offsets
+ * array itself carries the tile totals between the passes.
+ *   k_scan_tile_sums   tile b (SC_TILE counts) writes its sum to
+ *                      off[min((b + 1) SC_TILE, n)];
+ *   k_scan_tile_bases  one workgroup turns those entries, and off[0] = 0,
+ *                      into the exclusive prefix of each tile (inclusive
+ *                      prefix of the sums: the entry at (b + 1) SC_TILE is
+ *                      tile b + 1's base, the one at n the total);
+ *   k_scan_tiles       tile b reads its base from off[b SC_TILE] and writes
+ *                      off[b SC_TILE + i] for its counts (the first one
+ *                      being the base again); no other tile touches them. */
+#define SC_THREADS 1024
+#define SC_ITEMS 8
+#define SC_TILE (SC_THREADS * SC_ITEMS)
+/* below this count one workgroup's pass is quicker than three launches */
+#define SC_SMALL 32768u
+
+__device__ __forceinline__ uint64_t sc_block_sum(uint64_t v, uint64_t *part)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < SC_THREADS / 64; ++w) t += part[w];
+    return t;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_tile_sums(uint64_t *off, const uint32_t *cnt,
+                                                               uint32_t n)
+{
+    __shared__ uint64_t part[SC_THREADS / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SC_ITEMS; ++k) {
+        const uint64_t i = base + k * SC_THREADS + threadIdx.x;
+        if (i < n) s += cnt[i];
+    }
+    const uint64_t t = sc_block_sum(s, part);
+    if (threadIdx.x == 0) off[min(base + SC_TILE, (uint64_t)n)] = t;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_tile_bases(uint64_t *off, uint32_t n,
+                                                                uint32_t ntiles)
+{
+    __shared__ uint64_t warp_tot[SC_THREADS / 64];
+    __shared__ uint64_t carry_s;
+    if (threadIdx.x == 0) {
+        carry_s = 0;
+        off[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < ntiles; b0 += SC_THREADS) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint64_t at = min((uint64_t)(b + 1) * SC_TILE, (uint64_t)n);
+        const uint64_t v = b < ntiles ? off[at] : 0;
+        uint64_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == 63) warp_tot[wid] = x;
+        __syncthreads();
+        uint64_t wpre = 0;
+        for (uint32_t w = 0; w < wid; ++w) wpre += warp_tot[w];
+        const uint64_t carry = carry_s;
+        if (b < ntiles) off[at] = carry + wpre + x;
+        __syncthreads();
+        if (threadIdx.x == SC_THREADS - 1) carry_s = carry + wpre + x;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(uint64_t *off, const uint32_t *cnt,
+                                                           uint32_t n)
+{
+    __shared__ uint64_t warp_tot[SC_THREADS / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    const uint64_t tile_base = off[base];
+    /* thread t takes counts t * SC_ITEMS .. + SC_ITEMS - 1 of the tile */
+    const uint64_t first = base + (uint64_t)threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SC_ITEMS; ++k) {
+        v[k] = first + k < n ? cnt[first + k] : 0u;
+        s += v[k];
+    }
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint64_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) warp_tot[wid] = x;
+    __syncthreads();
+    uint64_t pre = tile_base + x - s;
+    for (uint32_t w = 0; w < wid; ++w) pre += warp_tot[w];
+#pragma unroll
+    for (uint32_t k = 0; k < SC_ITEMS; ++k) {
+        if (first + k < n) off[first + k] = pre;
+        pre += v[k];
+    }
+}
+
 extern "C" int hfx_scan_offsets(hfx_ctx *ctx, uint64_t *d_off, const uint32_t *d_count, uint32_t n,
                                 void *stream)
 {
     (void)ctx;
     HFX_REQUIRE(d_off != nullptr, HFX_ERR_ARG, "hfx_scan_offsets: null output");
-    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, hfx_stream(stream), d_off, d_count, n);
+    hipStream_t st = hfx_stream(stream);
+    if (n < SC_SMALL) {
+        hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, d_off, d_count, n);
+    } else {
+        const uint32_t nt = (uint32_t)(((uint64_t)n + SC_TILE - 1) / SC_TILE);
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(SC_THREADS), 0, st, d_off, d_count, n);
+        hipLaunchKernelGGL(k_scan_tile_bases, dim3(1), dim3(SC_THREADS), 0, st, d_off, n, nt);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, st, d_off, d_count, n);
+    }
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }

@@ -774,7 +774,7 @@ int hfx_dense_split(const hfx_spectra *sp, void *scratch, void *stream)
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
                      const uint32_t *d_num, const uint32_t *d_order, const uint32_t *d_bstart,
                      uint32_t nblocks, const hfx_dn_ladder *lad, const hfx_spectra *sp,
-                     uint32_t CH, void *scratch, int presplit, void *stream)
+                     uint32_t CH, void *scratch, int presplit, void *stream, void **ev)
 {
     HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && lad && sp, HFX_ERR_ARG,
                 "dense XCorr: null argument");
@@ -808,8 +808,10 @@ int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const
                 *sp};
     HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_dense,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, DN_LDS_BYTES));
+    if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[0], st));
     hipLaunchKernelGGL(k_dense, dim3((uint32_t)hfx_ctx_num_cus(ctx)), dim3(DN_THREADS),
                        DN_LDS_BYTES, st, args);
+    if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[1], st));
     hipLaunchKernelGGL(k_dn_zero_unscored, dim3(S), dim3(256), 0, st, d_scores, d_off, sp->d_zmax,
                        S);
     HFX_CHECK_LAUNCH();
@@ -830,5 +832,5 @@ extern "C" int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint6
     if ((rc = hfx_dense_ladder_build(ctx, db, 0, db->num_pep, 0, &lad, nullptr, stream)))
         return rc;
     return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, nullptr, 0, &lad, sp,
-                            256, nullptr, 0, stream);
+                            256, nullptr, 0, stream, nullptr);
 }

@@ -414,26 +414,69 @@ class Stream:
             pass
 
 
+class PepIndex:
+    """The per-database peptide index (hfx_pepix): b ladders of every
+    peptide (dense form) and/or their sparse ion lists for fragment charge
+    levels 1..levels (FP32 batched form).  Built once per database."""
+
+    LADDER = 1
+    IONS = 2
+    INFO = ("forms", "levels", "max_len", "ladder_positions", "ion_entries", "bytes")
+
+    def __init__(self, ctx: Ctx, db: DeviceDB, forms: int = LADDER | IONS, levels: int = 4,
+                 max_len: int = 4096, stream: "Stream | None" = None):
+        self.ctx, self.db = ctx, db
+        h = C.c_void_p()
+        check(lib().hfx_pepix_build(ctx.h, C.byref(db.desc), int(forms), int(levels),
+                                    int(max_len), stream.p if stream is not None else None,
+                                    C.byref(h)), "hfx_pepix_build")
+        self.h = h.value
+
+    def info(self) -> dict:
+        a = np.zeros(6, np.uint64)
+        check(lib().hfx_pepix_info(self.h, a.ctypes.data), "hfx_pepix_info")
+        return {k: int(v) for k, v in zip(self.INFO, a)}
+
+    def __del__(self):
+        try:
+            lib().hfx_pepix_destroy(self.h)
+        except Exception:
+            pass
+
+
 class BatchSearch:
     """The batched path (hfx_batch_*): plan once, then run or replay a
-    captured graph; results in PlainResult form."""
+    captured graph; results in PlainResult form.  With `index` the batch
+    reads that per-database index; without, it builds its own on first use."""
 
-    INFO = ("spectra", "scores", "indexed", "levels", "entries", "G", "C", "threads", "lstr", "p0",
-            "precision", "chunk")
+    INFO = ("spectra", "scores", "peptides", "levels", "entries", "G", "C", "threads", "lstr",
+            "zero", "precision", "chunk")
     FP32_SPARSE = 0
     BF16X2_DENSE = 1
 
     def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, eps: float, topn: int,
-                 margin: int = 0, stream: Stream | None = None):
+                 margin: int = 0, stream: Stream | None = None, index: PepIndex | None = None):
         L = lib()
         self.ctx, self.db, self.sp, self.topn = ctx, db, sp, int(topn)
+        self.index = index
         self.stream = stream or Stream()
         h = C.c_void_p()
         check(L.hfx_batch_create(ctx.h, C.byref(h)), "hfx_batch_create")
         self.h = h.value
-        check(L.hfx_batch_plan(self.h, C.byref(db.desc), C.byref(sp.desc), float(eps), self.topn,
-                               int(margin), self.stream.p), "hfx_batch_plan")
+        if index is not None:
+            check(L.hfx_batch_plan_ix(self.h, index.h, C.byref(sp.desc), float(eps), self.topn,
+                                      int(margin), self.stream.p), "hfx_batch_plan_ix")
+        else:
+            check(L.hfx_batch_plan(self.h, C.byref(db.desc), C.byref(sp.desc), float(eps),
+                                   self.topn, int(margin), self.stream.p), "hfx_batch_plan")
         self.captured = False
+
+    def order(self) -> np.ndarray:
+        """the spectrum order of the last run: (fragment charge level, precursor
+        mass, index)"""
+        out = np.zeros(self.sp.num_spec, np.uint32)
+        check(lib().hfx_batch_order(self.h, out.ctypes.data), "hfx_batch_order")
+        return out
 
     def info(self) -> dict:
         a = np.zeros(12, np.uint64)
@@ -457,9 +500,9 @@ class BatchSearch:
         return tuple(int(v) for v in cfg)
 
     def time_phases(self) -> dict:
-        ms = np.zeros(4, np.float32)
+        ms = np.zeros(5, np.float32)
         check(lib().hfx_batch_time(self.h, self.stream.p, ms.ctypes.data), "hfx_batch_time")
-        return dict(zip(("window", "index", "score", "topn"), (float(v) for v in ms)))
+        return dict(zip(("window", "prep", "score", "topn", "kernel"), (float(v) for v in ms)))
 
     def set_config(self, G: int, Cn: int, threads: int):
         check(lib().hfx_batch_set_config(self.h, G, Cn, threads), "hfx_batch_set_config")

@@ -132,4 +132,10 @@ WORKLOADS = {
         proteins=20000, spectra=10000, seed=2, missed=2, p_rule=0, min_len=0, max_len=0,
         charges=(2, 3), charge_p=(0.6, 0.4), static={"C": 57.02146}, double3=True, topn=10,
     ),
+    # the dense-block workload: the same database, 20,000 spectra at 2+/3+,
+    # no variable modifications, top-5
+    "tryptic-20k-20k": dict(
+        proteins=20000, spectra=20000, seed=4, missed=2, p_rule=0, min_len=0, max_len=0,
+        charges=(2, 3), charge_p=(0.5, 0.5), static={"C": 57.02146}, double3=True, topn=5,
+    ),
 }

@@ -349,14 +349,45 @@ int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
 /* synchronisation or allocation -- directly or as a captured hipGraph.  */
 /* Results use the layouts of hfx_xcorr_sparse / hfx_topn.               */
 /* ------------------------------------------------------------------ */
+/* Per-database peptide index (new): what the batched forms precompute per
+ * peptide, built once per database and shared by every batch --
+ * HFX_PEPIX_LADDER: the b ladder of every peptide in residual-sorted order
+ *   (sequential FP32 prefix of its residue masses, 12 bytes a position; the
+ *   dense form's input);
+ * HFX_PEPIX_IONS: every peptide's theoretical spectrum for fragment charge
+ *   levels 1..levels (level l = charges 1..l, max weight per bin), bins below
+ *   max_len, as sorted uint16 entries bin | code << 14 (the FP32 batched
+ *   form's input).
+ * The reference has no counterpart: it rebuilds every theoretical spectrum
+ * per (spectrum, candidate) pair (Search.hs:177-245).  Synchronises. */
+typedef struct hfx_pepix hfx_pepix;
+#define HFX_PEPIX_LADDER 1u
+#define HFX_PEPIX_IONS 2u
+int hfx_pepix_build(hfx_ctx *ctx, const hfx_db *db, uint32_t forms, uint32_t levels,
+                    uint32_t max_len, void *stream, hfx_pepix **out);
+int hfx_pepix_destroy(hfx_pepix *ix);
+/* info = {forms, levels, max_len, ladder positions, ion entries, device bytes} */
+int hfx_pepix_info(const hfx_pepix *ix, uint64_t *info);
+
 typedef struct hfx_batch hfx_batch;
 int hfx_batch_create(hfx_ctx *ctx, hfx_batch **out);
 int hfx_batch_destroy(hfx_batch *b);
-/* db and sp must stay valid and unchanged while the batch is used */
+/* Plans a batch: allocates every buffer for these spectra (synchronises).
+ * db and sp must stay valid while the batch is used; hfx_batch_run reads the
+ * spectra afresh each time.  Without an attached index (hfx_batch_plan_ix,
+ * hfx_batch_use_index) the batch builds its own over the whole database the
+ * first time a form needs it (outside any capture). */
 int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float eps, uint32_t N,
                    uint32_t margin, void *stream);
-/* windows -> peptide ion index -> persistent multi-spectrum XCorr -> top-N
- * (FP32 sparse form), or windows -> dense BF16 hi+lo XCorr -> top-N */
+int hfx_batch_plan_ix(hfx_batch *b, const hfx_pepix *ix, const hfx_spectra *sp, float eps,
+                      uint32_t N, uint32_t margin, void *stream);
+int hfx_batch_use_index(hfx_batch *b, const hfx_pepix *ix);
+/* One batched search, every per-batch step on the device and without host
+ * synchronisation or allocation: windows -> spectrum order (fragment charge
+ * level, precursor mass) -> FP32 sparse form: blocks of G spectra,
+ * persistent multi-spectrum XCorr over the index's ion lists; dense form:
+ * block cut, hi/lo BF16 rows of the spectra, MFMA XCorr over the index's
+ * ladders -> top-N with exact re-score. */
 int hfx_batch_run(hfx_batch *b, void *stream);
 #define HFX_BATCH_FP32_SPARSE 0u  /* default after hfx_batch_plan */
 #define HFX_BATCH_BF16X2_DENSE 1u /* hfx_xcorr_dense_bf16x2 scores */
@@ -369,20 +400,22 @@ int hfx_batch_replay(hfx_batch *b, void *stream);
  * the fastest; out_cfg = {G, C, threads} ({0, C, 256} for the dense form) */
 int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg);
 int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads);
-/* one run, phase by phase, with event timings in ms: {windows, ion index,
- * scoring, top-N} */
+/* one run, phase by phase, with event timings in ms (5 entries): {windows,
+ * per-batch preparation (order; dense: block cut + hi/lo rows; sparse:
+ * tiles), scoring, top-N, the scoring kernel alone} */
 int hfx_batch_time(hfx_batch *b, void *stream, float *ms);
-/* info = {spectra, scores, indexed peptides, charge levels, index entries,
- *         G, C, threads, LDS stride, first indexed position, precision,
- *         dense chunk} (12 entries) */
+/* info = {spectra, scores, database peptides, charge levels, index entries,
+ *         G, C, threads, LDS stride, 0, precision, dense chunk} (12 entries) */
 int hfx_batch_info(const hfx_batch *b, uint64_t *info);
-/* blocks of the dense form (new): the plan cuts the spectra, in fragment
- * charge then precursor mass order, into runs of at most eight whose
- * precursor windows overlap most.  *num_blocks = their number; h_bstart
- * (host, optional) receives min(cap, num_blocks + 1) block starts as
- * positions in that order, the last one the spectrum count. */
+/* blocks of the dense form's last run (new): the spectra, in fragment charge
+ * then precursor mass order, cut into runs of at most eight whose precursor
+ * windows overlap most.  *num_blocks = their number; h_bstart (host,
+ * optional) receives min(cap, num_blocks + 1) block starts as positions in
+ * that order, the last one the spectrum count.  Synchronises. */
 int hfx_batch_dense_blocks(const hfx_batch *b, uint32_t *h_bstart, uint32_t cap,
                            uint32_t *num_blocks);
+/* the spectrum order of the last run (S entries, host).  Synchronises. */
+int hfx_batch_order(const hfx_batch *b, uint32_t *h_order);
 /* device pointers {top_pos, top_exact, top_f32, top_cnt, flag, scores,
  *                  score offsets, window begin, window size, index entries} */
 int hfx_batch_outputs(const hfx_batch *b, void **ptrs);

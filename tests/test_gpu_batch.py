@@ -50,9 +50,9 @@ def check_against_oracle(res, ref, orc):
 
 def test_batch_run_matches_oracle(benv, orc):
     bs = benv["pl"].BatchSearch(benv["ctx"], benv["db"], benv["dsp"], eps=3.0, topn=10)
-    info = bs.info()
-    assert info["levels"] == 2 and info["indexed"] > 0 and info["entries"] > 0
     bs.run()
+    info = bs.info()
+    assert info["levels"] == 2 and info["peptides"] > 0 and info["entries"] > 0
     res = bs.results(keep_scores=True)
     check_against_oracle(res, benv["ref"], orc)
     for s in (5, 17):

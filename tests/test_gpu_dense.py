@@ -127,25 +127,33 @@ def test_batch_dense_matches_oracle(denv, orc):
     check_topn(res, denv["ref"], orc)
 
 
+SEG = 32  # order positions per segment of the device's block cut
+
+
 def test_batch_dense_blocks_cover_the_order(denv):
-    """The plan's dense blocks are consecutive runs of one to eight spectra of
-    the (fragment charge, precursor mass) order covering every spectrum, and
-    their total cost -- charge levels x rows of the hull of the windows,
-    summed over blocks -- is the minimum over all such cuts (an independent
-    dynamic programme here; ties may cut differently)."""
+    """The run orders the spectra by (fragment charge, precursor mass, index)
+    on the device, and its dense blocks are consecutive runs of one to eight
+    spectra of that order covering every spectrum, cut inside segments of 32
+    order positions; their total cost -- charge levels x rows of the hull of
+    the windows, summed over blocks -- is the minimum over all such cuts (an
+    independent dynamic programme per segment here; ties may cut
+    differently), and no worse than fixed runs of eight."""
     pl, dsp = denv["pl"], denv["dsp"]
     bs = pl.BatchSearch(denv["ctx"], denv["db"], dsp, eps=3.0, topn=10)
     bs.set_precision(bs.BF16X2_DENSE)
     bs.run()
     res = bs.results()
-    st = bs.dense_blocks().astype(np.int64)
     S = dsp.num_spec
+    z = dsp.zmax.astype(np.int64)
+    z = np.where(z <= 4, z, 0)
+    order = np.lexsort((np.arange(S), dsp.mass, z))
+    assert np.array_equal(bs.order(), order)
+    st = bs.dense_blocks().astype(np.int64)
     assert st[0] == 0 and st[-1] == S
     sizes = np.diff(st)
     assert sizes.min() >= 1 and sizes.max() <= 8
-    z = dsp.zmax.astype(np.int64)
-    z = np.where(z <= 4, z, 0)
-    order = np.lexsort((dsp.mass, z))
+    # no block crosses a segment boundary
+    assert all((a // SEG) == ((c - 1) // SEG) for a, c in zip(st[:-1], st[1:]))
     b, n = res.begin.astype(np.int64), res.num.astype(np.int64)
 
     def cost(lo, hi):
@@ -156,11 +164,14 @@ def test_batch_dense_blocks_cover_the_order(denv):
         return len(set(z[blk].tolist())) * int((b[blk] + n[blk]).max() - b[blk].min())
 
     plan = sum(cost(int(a), int(c)) for a, c in zip(st[:-1], st[1:]))
-    best = [0] + [None] * S
-    for j in range(1, S + 1):
-        best[j] = min(best[j - k] + cost(j - k, j) for k in range(1, min(8, j) + 1))
-    assert plan == best[S]
-    # and no worse than fixed runs of eight
+    total = 0
+    for p0 in range(0, S, SEG):
+        p1 = min(S, p0 + SEG)
+        best = [0] + [None] * (p1 - p0)
+        for j in range(1, p1 - p0 + 1):
+            best[j] = min(best[j - k] + cost(p0 + j - k, p0 + j) for k in range(1, min(8, j) + 1))
+        total += best[p1 - p0]
+    assert plan == total
     assert plan <= sum(cost(i, min(S, i + 8)) for i in range(0, S, 8))
 
 
@@ -181,7 +192,7 @@ def test_batch_dense_graph_and_autotune(denv, orc):
     bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
     bs.set_precision(bs.BF16X2_DENSE)
     G, Cn, T = bs.autotune()
-    assert G == 0 and Cn in (256, 512, 1024, 2048, 4096) and T == 256
+    assert G == 0 and Cn in (256, 512, 1024, 2048, 4096) and T == 1024
     bs.run()
     first = bs.results(keep_scores=True)
     check_topn(first, denv["ref"], orc)

@@ -373,3 +373,25 @@ def test_exact_route_past_the_lds_selection(orc):
             c = int(res.top_cnt[s])
             assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), (dense, s)
             assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 32767, 32768, 32769, 8192 * 7, 8192 * 7 + 1, 5_450_001])
+def test_scan_offsets(env, n):
+    """hfx_scan_offsets (one workgroup below 32768 counts, three passes of
+    8192-count tiles above) against numpy's cumulative sum, with counts up to
+    2^32 - 1 so that the 64-bit offsets carry."""
+    pl, L = env["pl"], env["L"]
+    rng = np.random.default_rng(n)
+    cnt = rng.integers(0, 2**32 - 1, size=n, dtype=np.uint64).astype(np.uint32)
+    if n > 2:
+        cnt[: n // 3] = rng.integers(0, 50, size=n // 3).astype(np.uint32)
+    t_cnt = pl.dev(cnt) if n else pl.empty(4)
+    t_off = pl.empty(8 * (n + 1))
+    from hfx_amd import check
+
+    check(L.hfx_scan_offsets(env["ctx"].h, t_off.p, t_cnt.p, n, None), "hfx_scan_offsets")
+    pl.sync()
+    got = pl.host(t_off, np.uint64, n + 1)
+    want = np.zeros(n + 1, np.uint64)
+    np.cumsum(cnt.astype(np.uint64), out=want[1:])
+    assert np.array_equal(got, want)
