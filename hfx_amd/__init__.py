@@ -140,6 +140,10 @@ _SIGS = {
         _I,
         [_P, _P, _P, _P, _P, _U64, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_xcorr_sparse_noalloc": (
+        _I,
+        [_P, _P, _P, _P, _U64, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
+    ),
     "hfx_topn": (
         _I,
         [_P, _P, _P, _P, _P, _P, _U32, _U32, _P, _P, _U64, _P, _P, C.POINTER(HfxDb),

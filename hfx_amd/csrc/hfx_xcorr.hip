@@ -15,6 +15,8 @@
  */
 #include "hfx_cand.h"
 
+#include <mutex>
+
 #define XC_THREADS 256
 #define XC_WAVES (XC_THREADS / HFX_WAVE)
 #define XC_CHUNK 64
@@ -66,23 +68,23 @@ __global__ void __launch_bounds__(XC_THREADS)
     }
 }
 
+/* device scratch of the sparse XCorr for S spectra: per-spectrum tile counts
+ * and their offsets */
+static size_t xc_scratch_bytes(uint32_t S) { return (size_t)S * 4 + ((size_t)S + 1) * 8 + 64; }
+
 template <class Cands>
-static int launch_xcorr(hfx_ctx *ctx, float *d_scores, const Cands &c, uint64_t total,
+static int launch_xcorr(void *scratch, float *d_scores, const Cands &c, uint64_t total,
                         const hfx_spectra *sp, void *stream)
 {
     const uint32_t S = sp->num_spec;
     if (S == 0 || total == 0) return HFX_OK;
     hipStream_t st = hfx_stream(stream);
-    void *p;
     int rc;
-    if ((rc = hfx_ctx_scratch(ctx, HFX_SLOT_TILES, (size_t)S * 4 + ((size_t)S + 1) * 8 + 64, &p)) !=
-        HFX_OK)
-        return rc;
-    uint32_t *tiles = (uint32_t *)p;
+    uint32_t *tiles = (uint32_t *)scratch;
     uint64_t *tile_off = (uint64_t *)(((uintptr_t)(tiles + S) + 15) & ~(uintptr_t)15);
     hipLaunchKernelGGL(k_tile_count<Cands>, dim3((S + 255) / 256), dim3(256), 0, st, tiles, c, S,
                        (uint32_t)XC_CHUNK);
-    if ((rc = hfx_scan_offsets(ctx, tile_off, tiles, S, stream)) != HFX_OK) return rc;
+    if ((rc = hfx_scan_offsets(nullptr, tile_off, tiles, S, stream)) != HFX_OK) return rc;
     /* sum_s ceil(n_s / chunk) <= total / chunk + S */
     const uint64_t grid = (total + XC_CHUNK - 1) / XC_CHUNK + S;
     HFX_REQUIRE(grid < (1ull << 31), HFX_ERR_ARG, "xcorr: %llu tiles", (unsigned long long)grid);
@@ -99,7 +101,12 @@ extern "C" int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d
     HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && db && sp, HFX_ERR_ARG,
                 "hfx_xcorr_sparse: null argument");
     HfxPlainCands c{*db, d_begin, d_num, d_off};
-    return launch_xcorr(ctx, d_scores, c, total, sp, stream);
+    void *p = nullptr;
+    int rc;
+    if (sp->num_spec &&
+        (rc = hfx_ctx_scratch(ctx, HFX_SLOT_TILES, xc_scratch_bytes(sp->num_spec), &p)) != HFX_OK)
+        return rc;
+    return launch_xcorr(p, d_scores, c, total, sp, stream);
 }
 
 extern "C" int hfx_xcorr_sparse_mod(hfx_ctx *ctx, float *d_scores, const uint64_t *d_mp_off,
@@ -111,7 +118,68 @@ extern "C" int hfx_xcorr_sparse_mod(hfx_ctx *ctx, float *d_scores, const uint64_
     HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_xcorr_sparse_mod: num_ma %u > %d",
                 ms->num_ma, HFX_MAX_MA);
     HfxModCands c{*db, *ms, d_mp_off};
-    return launch_xcorr(ctx, d_scores, c, total, sp, stream);
+    void *p = nullptr;
+    int rc;
+    if (sp->num_spec &&
+        (rc = hfx_ctx_scratch(ctx, HFX_SLOT_TILES, xc_scratch_bytes(sp->num_spec), &p)) != HFX_OK)
+        return rc;
+    return launch_xcorr(p, d_scores, c, total, sp, stream);
+}
+
+/* ------------------------------------------------------------------ */
+/* hfx_xcorr_sparse for callers that cannot allocate                     */
+/* ------------------------------------------------------------------ */
+
+/* The tile table of up to HFX_NOALLOC_MAX_SPECTRA spectra lives in this
+ * file-scope device array (12 MiB, held by every GPU the library runs on
+ * until the process ends).  In host code `scratch` names a host-side stand-in,
+ * never the device memory: the device address comes from
+ * hipGetSymbolAddress, once per device, and only that address is passed on. */
+#define NA_BYTES (((size_t)HFX_NOALLOC_MAX_SPECTRA * 4 + ((size_t)HFX_NOALLOC_MAX_SPECTRA + 1) * 8 + \
+                   64 + 255) & ~(size_t)255)
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+#define NA_MAX_DEVICES 64
+static std::mutex g_na_mu;
+static void *g_na_ptr[NA_MAX_DEVICES];
+
+/* the current device's address of `scratch`, looked up on the first call on
+ * that device */
+static int na_scratch(void **out)
+{
+    int dev = 0;
+    HFX_CHECK_HIP(hipGetDevice(&dev));
+    HFX_REQUIRE(dev >= 0 && dev < NA_MAX_DEVICES, HFX_ERR_ARG, "no-alloc scratch: device %d", dev);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        HFX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HFX_REQUIRE(p != nullptr, HFX_ERR_HIP, "no-alloc scratch: no device address");
+        g_na_ptr[dev] = p;
+    }
+    *out = g_na_ptr[dev];
+    return HFX_OK;
+}
+
+extern "C" int hfx_xcorr_sparse_noalloc(float *d_scores, const uint64_t *d_off,
+                                        const uint32_t *d_begin, const uint32_t *d_num,
+                                        uint64_t total, const hfx_db *db, const hfx_spectra *sp,
+                                        void *stream)
+{
+    HFX_REQUIRE(d_scores && d_off && d_begin && d_num && db && sp, HFX_ERR_ARG,
+                "hfx_xcorr_sparse_noalloc: null argument");
+    HFX_REQUIRE(sp->num_spec <= HFX_NOALLOC_MAX_SPECTRA, HFX_ERR_ARG,
+                "hfx_xcorr_sparse_noalloc: %u spectra, at most %u in one call", sp->num_spec,
+                (unsigned)HFX_NOALLOC_MAX_SPECTRA);
+    static_assert(NA_BYTES >= ((size_t)HFX_NOALLOC_MAX_SPECTRA * 4 +
+                               ((size_t)HFX_NOALLOC_MAX_SPECTRA + 1) * 8 + 64),
+                  "no-alloc scratch holds the tile table of the largest call");
+    if (sp->num_spec == 0 || total == 0) return HFX_OK;
+    void *p = nullptr;
+    int rc;
+    if ((rc = na_scratch(&p))) return rc;
+    HfxPlainCands c{*db, d_begin, d_num, d_off};
+    return launch_xcorr(p, d_scores, c, total, sp, stream);
 }
 
 /* ------------------------------------------------------------------ */

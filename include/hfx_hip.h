@@ -265,6 +265,21 @@ int hfx_xcorr_sparse(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
                      const uint32_t *d_begin, const uint32_t *d_num, uint64_t total,
                      const hfx_db *db, const hfx_spectra *sp, void *stream);
 
+/* hfx_xcorr_sparse for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops): no counterpart in the
+ * reference.  Same arguments and results as hfx_xcorr_sparse, without the
+ * context: the tile table (a few bytes per spectrum) lives in one static
+ * device array of the library, so at most HFX_NOALLOC_MAX_SPECTRA spectra per
+ * call (larger calls return HFX_ERR_ARG), and no hipMalloc or other
+ * allocating routine is called.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (different streams, threads) race.
+ * The first call on a device looks the array's address up (not under
+ * capture). */
+#define HFX_NOALLOC_MAX_SPECTRA (1u << 20)
+int hfx_xcorr_sparse_noalloc(float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,
+                             const uint32_t *d_num, uint64_t total, const hfx_db *db,
+                             const hfx_spectra *sp, void *stream);
+
 /* Per-spectrum top-N (replaces sort_b40c_f + full D2H, Search.hs:250-276):
  * selects the top N + margin candidates by FP32 score, re-scores them exactly
  * (int64 fixed point, 2^-36 resolution, reported as double), and writes the
