@@ -541,26 +541,41 @@ __global__ void __launch_bounds__(CUT_WG) k_cut_seg(uint32_t *seg_cuts, uint32_t
     const uint32_t *lo = lo_s + threadIdx.x * CUT_SEG, *hi = hi_s + threadIdx.x * CUT_SEG,
                    *lv = lv_s + threadIdx.x * CUT_SEG;
     uint8_t *from = from_s[threadIdx.x];
-    /* bst[k - 1] = best cost of the first j - k positions */
+    /* registers hold the last CUT_NS positions (index 0 the newest) and the
+     * best costs before them: bst[k - 1] = best cost of the first j - k
+     * positions */
     uint64_t bst[CUT_NS];
+    uint32_t rlo[CUT_NS], rhi[CUT_NS], rlv[CUT_NS];
 #pragma unroll
-    for (int k = 0; k < CUT_NS; ++k) bst[k] = k == 0 ? 0ull : ~0ull;
+    for (int k = 0; k < CUT_NS; ++k) {
+        bst[k] = k == 0 ? 0ull : ~0ull;
+        rlo[k] = 0xffffffffu;
+        rhi[k] = 0;
+        rlv[k] = 0;
+    }
     for (uint32_t j = 1; j <= len; ++j) {
+#pragma unroll
+        for (int k = CUT_NS - 1; k > 0; --k) {
+            rlo[k] = rlo[k - 1];
+            rhi[k] = rhi[k - 1];
+            rlv[k] = rlv[k - 1];
+        }
+        rlo[0] = lo[j - 1];
+        rhi[0] = hi[j - 1];
+        rlv[0] = lv[j - 1];
         uint32_t l = 0xffffffffu, h = 0, v = 0, fj = j - 1;
         uint64_t bj = ~0ull;
 #pragma unroll
         for (uint32_t k = 1; k <= CUT_NS; ++k) {
-            if (k <= j) {
-                const uint32_t x = j - k;
-                l = min(l, lo[x]);
-                h = max(h, hi[x]);
-                v |= lv[x];
-                const uint64_t rows = h > l ? (uint64_t)(h - l) : 0ull;
-                const uint64_t c = bst[k - 1] + rows * (uint64_t)__builtin_popcount(v);
-                if (c < bj) {
-                    bj = c;
-                    fj = x;
-                }
+            l = min(l, rlo[k - 1]);
+            h = max(h, rhi[k - 1]);
+            v |= rlv[k - 1];
+            const uint64_t rows = h > l ? (uint64_t)(h - l) : 0ull;
+            const uint64_t c = bst[k - 1] + rows * (uint64_t)__builtin_popcount(v);
+            /* bst[k - 1] is ~0 for k > j: never chosen */
+            if (k <= j && c < bj) {
+                bj = c;
+                fj = j - k;
             }
         }
 #pragma unroll

@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(1024) k_scan_offsets(uint64_t *off, const uint
 #define SC_ITEMS 8
 #define SC_TILE (SC_THREADS * SC_ITEMS)
 /* below this count one workgroup's pass is quicker than three launches */
-#define SC_SMALL 32768u
+#define SC_SMALL 8192u
 
 __device__ __forceinline__ uint64_t sc_block_sum(uint64_t v, uint64_t *part)
 {

@@ -375,9 +375,9 @@ def test_exact_route_past_the_lds_selection(orc):
             assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
 
 
-@pytest.mark.parametrize("n", [0, 1, 1000, 32767, 32768, 32769, 8192 * 7, 8192 * 7 + 1, 5_450_001])
+@pytest.mark.parametrize("n", [0, 1, 1000, 8191, 8192, 8193, 8192 * 7, 8192 * 7 + 1, 5_450_001])
 def test_scan_offsets(env, n):
-    """hfx_scan_offsets (one workgroup below 32768 counts, three passes of
+    """hfx_scan_offsets (one workgroup below 8192 counts, three passes of
     8192-count tiles above) against numpy's cumulative sum, with counts up to
     2^32 - 1 so that the 64-bit offsets carry."""
     pl, L = env["pl"], env["L"]
@@ -395,3 +395,62 @@ def test_scan_offsets(env, n):
     want = np.zeros(n + 1, np.uint64)
     np.cumsum(cnt.astype(np.uint64), out=want[1:])
     assert np.array_equal(got, want)
+
+
+def test_long_candidates_take_the_general_path(orc):
+    """Proteins without K or R digest into peptides of up to 110 residues
+    (mass-limited at 7200 Da): candidates of more than 64 ladder positions do
+    not fit a wave's lanes and take the general path of the plain kernel,
+    mixed with packed short ones; scores within the FP32 bound of the
+    oracle's exact scores and identical top-N."""
+    from hfx_amd import pipeline, synth
+
+    rng = np.random.default_rng(71)
+    light = np.frombuffer(b"GGGAAS", np.uint8)
+    plain = np.frombuffer(b"ARNDCQEGHILKMFPSTWYV", np.uint8)
+    seqs = []
+    for i in range(300):
+        if i % 2 == 0:  # long K/R-free stretches
+            seqs.append(light[rng.integers(0, len(light), rng.integers(70, 140))])
+        else:
+            seqs.append(plain[rng.integers(0, len(plain), rng.integers(100, 400))])
+    seg = np.zeros(len(seqs) + 1, np.uint32)
+    np.cumsum([len(q) for q in seqs], out=seg[1:])
+    prot = synth.Proteins(seq=np.ascontiguousarray(np.concatenate(seqs)), seg=seg)
+    mass = orc.mass_table({})
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=1)
+    nres = (tn.astype(np.int64) - tc.astype(np.int64))
+    assert (nres > 64).sum() > 60
+    sp = synth.make_spectra(prot, r, tc, tn, 60, seed=72, charges=(2, 3), charge_p=(0.5, 0.5),
+                            double_for_3plus=True)
+    # half the spectra moved to the precursors of long peptides
+    long_idx = np.nonzero(nres > 64)[0]
+    ctx = pipeline.Ctx(0)
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass))
+    pcr = sp.pcr.copy()
+    H = 1.007276
+    for s in range(0, 60, 2):
+        q = int(long_idx[s % len(long_idx)])
+        z = float(sp.charge[s])
+        pcr[s] = np.float32((float(r[q]) + 18.010565 + z * H) / z)
+    dsp = pipeline.DeviceSpectra(ctx, pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE)
+    srch = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=5)
+    res = srch.run(keep_scores=True)
+    idx, _ = orc.sort_idx(r)
+    ref = orc.search_plain(orc.prepare_ions(prot.seq), r, tc, tn, mass, idx, dsp.x(),
+                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, 5, want_all=True)
+    # long candidates are in the windows
+    isort = idx.astype(np.int64)
+    win_long = sum(int((nres[isort[int(res.begin[s]):int(res.begin[s] + res.num[s])]] > 64).sum())
+                   for s in range(60))
+    assert win_long > 20
+    exact = orc.fx_to_double(ref["all_fx"])
+    X, lens = dsp.x(), dsp.lengths()
+    x1 = np.array([np.abs(X[s, : lens[s]].astype(np.float64)).sum() for s in range(60)])
+    bound = np.repeat(80 * 2.0**-24 * 50 * x1 * 1.001 + 1e-6, np.diff(res.off.astype(np.int64)))
+    assert np.all(np.abs(res.scores.astype(np.float64) - exact) <= bound)
+    assert (res.top_cnt == ref["top_cnt"]).all()
+    for s in range(60):
+        c = int(res.top_cnt[s])
+        assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), s
+        assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all(), s
