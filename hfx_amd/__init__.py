@@ -97,10 +97,17 @@ _SIGS = {
     "hfx_event_destroy": (_I, [_P]),
     "hfx_event_record": (_I, [_P, _P]),
     "hfx_event_elapsed_ms": (_I, [C.POINTER(_F), _P, _P]),
+    "hfx_blas_create": (_I, [C.POINTER(_P)]),
+    "hfx_blas_destroy": (_I, [_P]),
+    "hfx_blas_set_stream": (_I, [_P, _P]),
     # reference ABI (algorithms.h)
     "sort_idx_f": (None, [_P, _P, _U32, _U32]),
     "sort_val_f": (None, [_P, _P, _U32]),
     "sort_b40c_f": (None, [_P, _P, _U32]),
+    "sort_rf": (None, [_P, _P, _U32]),
+    "sort_idx_rf": (None, [_P, _P, _U32, _U32]),
+    "findIndicesInRange_f": (_U32, [_P, _P, _U32, _F, _F]),
+    "mvm_ff": (None, [_P, _P, _P, _P, _U32, _U32]),
     "prepare_ions": (None, [_P, _U32, _P, _U32]),
     "findSpecCandsByMass": (_U32, [_P, _P, _P, _P, _P, _U32, _P, _U32, _F]),
     "findBeginEnd_f": (_U32, [_P, _P, _P, _P, _P, _U32, _P, _U32, _F, _F]),
@@ -186,6 +193,8 @@ _SIGS = {
     "hfx_batch_plan": (_I, [_P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _F, _U32, _U32, _P]),
     "hfx_batch_plan_ix": (_I, [_P, _P, C.POINTER(HfxSpectra), _F, _U32, _U32, _P]),
     "hfx_batch_use_index": (_I, [_P, _P]),
+    "hfx_batch_required_bytes": (_I, [_U32, _U64, _U32, _U32, _U32, C.POINTER(_U64)]),
+    "hfx_batch_split": (_I, [_P, _U32, _U32, _U32, _U32, _U64, _P, _U32, C.POINTER(_U32)]),
     "hfx_batch_run": (_I, [_P, _P]),
     "hfx_batch_capture": (_I, [_P, _P]),
     "hfx_batch_replay": (_I, [_P, _P]),

@@ -740,6 +740,77 @@ static int bt_need_index(hfx_batch *b, hipStream_t st)
     return HFX_OK;
 }
 
+/* Device bytes hfx_batch_plan allocates for S spectra with `total` scored
+ * pairs (the precision form's own buffers included, the per-database index
+ * not): the same sums as the allocations below. */
+static uint64_t bt_bytes(uint32_t S, uint64_t total, uint32_t N, uint32_t num_pep,
+                         uint32_t precision)
+{
+    auto al = [](uint64_t n, uint64_t sz) { return (std::max<uint64_t>(n, 1) * sz + 255) & ~255ull; };
+    uint32_t sh = 0;
+    while (((uint64_t)num_pep >> sh) > std::max<uint64_t>(S / 2, 64)) ++sh;
+    const uint64_t nb = (uint64_t)(HFX_MAX_CHARGE + 1) * ((num_pep >> sh) + 1);
+    const uint64_t bmax = (S + 3) / 4;
+    uint64_t b = al(S, 4) * 3 + al((uint64_t)S + 1, 8)                  /* begin, num, order, off */
+                 + al(nb, 4) + al(nb + 1, 8) + al(S, 4)                   /* order buckets */
+                 + al(total, 4) + al(1, 4)                               /* scores, counter */
+                 + al((uint64_t)S * N, 4) + al((uint64_t)S * N, 8) + al((uint64_t)S * N, 4) +
+                 al(S, 4) * 2                                            /* top-N */
+                 + al(hfx_topn_scratch_bytes(S, total), 1)               /* exact route */
+                 + al(bmax, 4) * 3 + al(bmax + 1, 8);                    /* sparse blocks */
+    if (precision == HFX_BATCH_BF16X2_DENSE) {
+        const uint64_t nseg = (S + CUT_SEG - 1) / CUT_SEG;
+        b += al(hfx_dense_scratch_bytes(S), 1) + al((uint64_t)S + 1, 4) + al(nseg * CUT_SEG, 4) +
+             al(nseg, 4) + al(nseg + 1, 8);
+    }
+    return b;
+}
+
+extern "C" int hfx_batch_required_bytes(uint32_t S, uint64_t total, uint32_t N, uint32_t num_pep,
+                                        uint32_t precision, uint64_t *bytes)
+{
+    HFX_REQUIRE(bytes, HFX_ERR_ARG, "hfx_batch_required_bytes: null out");
+    HFX_REQUIRE(precision <= HFX_BATCH_BF16X2_DENSE, HFX_ERR_ARG,
+                "hfx_batch_required_bytes: unknown precision %u", precision);
+    *bytes = bt_bytes(S, total, N, num_pep, precision);
+    return HFX_OK;
+}
+
+/* Search.hs:427-433 / Execution.hs:77-93 (groupScorePlan): consecutive runs
+ * of spectra whose batch fits `budget` device bytes, each as long as it can
+ * be (h_off: the S + 1 score offsets, window sizes summed).  A spectrum that
+ * alone needs more than the budget is an error (the reference splits such a
+ * spectrum's candidates instead; a batch here scores whole windows). */
+extern "C" int hfx_batch_split(const uint64_t *h_off, uint32_t S, uint32_t N, uint32_t num_pep,
+                               uint32_t precision, uint64_t budget, uint32_t *h_first,
+                               uint32_t cap, uint32_t *num_batches)
+{
+    HFX_REQUIRE(h_off && num_batches, HFX_ERR_ARG, "hfx_batch_split: null argument");
+    HFX_REQUIRE(precision <= HFX_BATCH_BF16X2_DENSE, HFX_ERR_ARG,
+                "hfx_batch_split: unknown precision %u", precision);
+    uint32_t nb = 0, a = 0;
+    while (a < S) {
+        /* the longest run [a, e) that fits: the requirement grows with the
+         * run, so a binary search over its end */
+        uint32_t lo = a, hi = S;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo + 1) / 2;
+            if (bt_bytes(mid - a, h_off[mid] - h_off[a], N, num_pep, precision) <= budget) lo = mid;
+            else hi = mid - 1;
+        }
+        HFX_REQUIRE(lo > a, HFX_ERR_NOMEM,
+                    "hfx_batch_split: spectrum %u alone needs %llu bytes, budget %llu", a,
+                    (unsigned long long)bt_bytes(1, h_off[a + 1] - h_off[a], N, num_pep, precision),
+                    (unsigned long long)budget);
+        if (h_first && nb < cap) h_first[nb] = a;
+        ++nb;
+        a = lo;
+    }
+    if (h_first && nb < cap) h_first[nb] = S;
+    *num_batches = nb;
+    return HFX_OK;
+}
+
 extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float eps,
                               uint32_t N, uint32_t margin, void *stream)
 {
@@ -777,6 +848,17 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     }
     HFX_CHECK_HIP(hipMemcpyAsync(&b->total, b->score_off + S, 8, hipMemcpyDeviceToHost, st));
     HFX_CHECK_HIP(hipStreamSynchronize(st));
+    {
+        /* fail with a clear message, before allocating, when the batch
+         * cannot fit (hfx_batch_split cuts a larger set into batches that do) */
+        size_t fr = 0, tot = 0;
+        HFX_CHECK_HIP(hipMemGetInfo(&fr, &tot));
+        const uint64_t need = bt_bytes(S, b->total, N, db->num_pep, HFX_BATCH_FP32_SPARSE);
+        HFX_REQUIRE(need <= fr, HFX_ERR_NOMEM,
+                    "hfx_batch_plan: %u spectra with %llu scores need %llu device bytes, %zu free "
+                    "(split them with hfx_batch_split)",
+                    S, (unsigned long long)b->total, (unsigned long long)need, fr);
+    }
     uint32_t L = 0, lmax = 0;
     for (uint32_t s = 0; s < S; ++s) {
         lmax = std::max(lmax, std::min(hl[s], std::min(sp->x_stride, (uint32_t)HFX_MAX_LEN)));

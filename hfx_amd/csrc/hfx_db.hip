@@ -175,15 +175,18 @@ int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream)
     return HFX_OK;
 }
 
-__global__ void k_float_to_key(const float *f, uint32_t *k, uint32_t n)
+/* keys in radix order: ascending float order, or (desc) its reverse --
+ * the complement of an order-preserving key reverses the order, and a
+ * stable sort of the complements is a stable descending sort */
+__global__ void k_float_to_key(const float *f, uint32_t *k, uint32_t n, uint32_t desc)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) k[i] = hfx_f2key(f[i]);
+    if (i < n) k[i] = desc ? ~hfx_f2key(f[i]) : hfx_f2key(f[i]);
 }
-__global__ void k_key_to_float(const uint32_t *k, float *f, uint32_t n)
+__global__ void k_key_to_float(const uint32_t *k, float *f, uint32_t n, uint32_t desc)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) f[i] = hfx_key2f(k[i]);
+    if (i < n) f[i] = hfx_key2f(desc ? ~k[i] : k[i]);
 }
 __global__ void k_iota(uint32_t *v, uint32_t n, uint32_t init)
 {
@@ -193,7 +196,7 @@ __global__ void k_iota(uint32_t *v, uint32_t n, uint32_t init)
 
 /* Sorts float keys ascending with values, stably, in place. */
 static int radix_sort_f32(hfx_ctx *ctx, float *d_keys, uint32_t *d_vals, uint32_t n, bool iota,
-                          uint32_t init, hipStream_t st)
+                          uint32_t init, hipStream_t st, uint32_t desc = 0)
 {
     if (n == 0) return HFX_OK;
     uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
@@ -203,7 +206,7 @@ static int radix_sort_f32(hfx_ctx *ctx, float *d_keys, uint32_t *d_vals, uint32_
         return rc;
     uint32_t *ka = (uint32_t *)p, *va = ka + n, *kb = va + n, *vb = kb + n, *hist = vb + n;
     uint32_t g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_float_to_key, dim3(g), dim3(256), 0, st, d_keys, ka, n);
+    hipLaunchKernelGGL(k_float_to_key, dim3(g), dim3(256), 0, st, d_keys, ka, n, desc);
     if (iota) hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, st, va, n, init);
     else HFX_CHECK_HIP(hipMemcpyAsync(va, d_vals, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     for (uint32_t shift = 0; shift < 32; shift += 4) {
@@ -214,7 +217,7 @@ static int radix_sort_f32(hfx_ctx *ctx, float *d_keys, uint32_t *d_vals, uint32_
         uint32_t *t = ka; ka = kb; kb = t;
         t = va; va = vb; vb = t;
     }
-    hipLaunchKernelGGL(k_key_to_float, dim3(g), dim3(256), 0, st, ka, d_keys, n);
+    hipLaunchKernelGGL(k_key_to_float, dim3(g), dim3(256), 0, st, ka, d_keys, n, desc);
     HFX_CHECK_HIP(hipMemcpyAsync(d_vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     HFX_CHECK_LAUNCH();
     return HFX_OK;
@@ -233,6 +236,59 @@ extern "C" void sort_val_f(float *d_keys, uint32_t *d_vals, uint32_t N)
     hfx_ctx *ctx = hfx_default_ctx();
     if (!ctx) return;
     if (radix_sort_f32(ctx, d_keys, d_vals, N, false, 0, 0) == HFX_OK) (void)hipStreamSynchronize(0);
+}
+
+extern "C" void sort_rf(float *d_keys, uint32_t *d_vals, uint32_t N)
+{
+    hfx_ctx *ctx = hfx_default_ctx();
+    if (!ctx) return;
+    if (radix_sort_f32(ctx, d_keys, d_vals, N, false, 0, 0, 1) == HFX_OK)
+        (void)hipStreamSynchronize(0);
+}
+
+extern "C" void sort_idx_rf(float *d_keys_raw, uint32_t *d_idx_raw, uint32_t N, uint32_t init)
+{
+    hfx_ctx *ctx = hfx_default_ctx();
+    if (!ctx) return;
+    if (radix_sort_f32(ctx, d_keys_raw, d_idx_raw, N, true, init, 0, 1) == HFX_OK)
+        (void)hipStreamSynchronize(0);
+}
+
+/* findIndicesInRange_f (filter.cu:38-73): the indices i, ascending, with
+ * min <= d_in[i] <= max; stable compaction (flags, offsets, scatter) */
+__global__ void k_range_flags(uint32_t *flag, const float *in, uint32_t n, float lo, float hi)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (lo <= in[i] && in[i] <= hi) ? 1u : 0u;
+}
+__global__ void k_range_scatter(uint32_t *out, const uint32_t *flag, const uint64_t *pos, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) out[pos[i]] = i;
+}
+
+extern "C" uint32_t findIndicesInRange_f(const float *d_in, uint32_t *d_out, const uint32_t N,
+                                         const float min, const float max)
+{
+    if (N == 0) return 0;
+    hfx_ctx *ctx = hfx_default_ctx();
+    if (!ctx) return 0;
+    void *p = nullptr, *h = nullptr;
+    const size_t fb = ((size_t)N * 4 + 255) & ~(size_t)255;
+    if (hfx_ctx_scratch(ctx, HFX_SLOT_SCAN, fb + ((size_t)N + 1) * 8, &p) != HFX_OK) return 0;
+    if (hfx_ctx_pinned(ctx, 16, &h) != HFX_OK) return 0;
+    uint32_t *flag = (uint32_t *)p;
+    uint64_t *pos = (uint64_t *)((uint8_t *)p + fb);
+    const uint32_t g = (N + 255) / 256;
+    hipLaunchKernelGGL(k_range_flags, dim3(g), dim3(256), 0, 0, flag, d_in, N, min, max);
+    if (hfx_scan_offsets(ctx, pos, flag, N, nullptr) != HFX_OK) return 0;
+    hipLaunchKernelGGL(k_range_scatter, dim3(g), dim3(256), 0, 0, d_out, flag, pos, N);
+    if (hipMemcpyAsync(h, pos + N, 8, hipMemcpyDeviceToHost, 0) != hipSuccess ||
+        hipStreamSynchronize(0) != hipSuccess) {
+        hfx_set_error("findIndicesInRange_f: %s", hipGetErrorString(hipGetLastError()));
+        return 0;
+    }
+    return (uint32_t)*(uint64_t *)h;
 }
 
 extern "C" void sort_b40c_f(float *d_keys, uint32_t *d_vals, uint32_t N)

@@ -15,6 +15,8 @@
  */
 #include "hfx_cand.h"
 
+#include <stdlib.h>
+
 #include <mutex>
 #include <type_traits>
 
@@ -705,6 +707,47 @@ __global__ void __launch_bounds__(256) k_mvm_rows(float *y, const float *A, uint
     for (uint32_t j = hfx_lane(); j < n; j += HFX_WAVE) acc = acc + a[j] * x[j];
     acc = hfx_wave_sum_f32(acc);
     if (hfx_lane() == 0) y[row] = acc;
+}
+
+/* mvm_ff (mvm.cu:21-56): y = A x for row-major A of m rows and n columns,
+ * on the handle's stream (cublasSgemv(CUBLAS_OP_T) over the transposed
+ * view, Level2.chs:65-77).  The handle replaces the cuBLAS one of
+ * Execution.hs:49-53: it only carries the stream set by hfx_blas_set_stream
+ * (cublasSetStream_v2). */
+struct hfx_blas {
+    hipStream_t stream;
+};
+
+extern "C" int hfx_blas_create(void **handle)
+{
+    HFX_REQUIRE(handle, HFX_ERR_ARG, "hfx_blas_create: null out");
+    hfx_blas *b = (hfx_blas *)calloc(1, sizeof(hfx_blas));
+    HFX_REQUIRE(b, HFX_ERR_NOMEM, "hfx_blas_create: out of host memory");
+    *handle = b;
+    return HFX_OK;
+}
+
+extern "C" int hfx_blas_destroy(void *handle)
+{
+    free(handle);
+    return HFX_OK;
+}
+
+extern "C" int hfx_blas_set_stream(void *handle, void *stream)
+{
+    HFX_REQUIRE(handle, HFX_ERR_ARG, "hfx_blas_set_stream: null handle");
+    ((hfx_blas *)handle)->stream = hfx_stream(stream);
+    return HFX_OK;
+}
+
+extern "C" void mvm_ff(void *handle, float *d_y, const float *d_A, const float *d_x,
+                       const uint32_t m, const uint32_t n)
+{
+    if (m == 0) return;
+    const hipStream_t st = handle ? ((hfx_blas *)handle)->stream : nullptr;
+    hipLaunchKernelGGL(k_mvm_rows, dim3((m + 3) / 4), dim3(256), 0, st, d_y, d_A, n, d_x, m, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) hfx_set_error("mvm_ff: %s", hipGetErrorString(e));
 }
 
 extern "C" int hfx_mvm_rows(hfx_ctx *ctx, float *d_y, const float *d_A, uint32_t lda,

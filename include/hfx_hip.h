@@ -108,6 +108,11 @@ int hfx_event_create(void **ev);
 int hfx_event_destroy(void *ev);
 int hfx_event_record(void *ev, void *stream);
 int hfx_event_elapsed_ms(float *ms, void *start, void *stop);
+/* Context.chs:38-62 cublasCreate/Destroy/SetStream: the handle mvm_ff takes
+ * (it carries only the stream) */
+int hfx_blas_create(void **handle);
+int hfx_blas_destroy(void *handle);
+int hfx_blas_set_stream(void *handle, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Reference ABI (algorithms.h), same names and meaning.                 */
@@ -123,6 +128,25 @@ void sort_val_f(float *d_keys, uint32_t *d_vals, uint32_t N);
 /* algorithms.h:54, sort_b40c.cu:34-41: stable ascending key-value sort in
  * place (b40c radix sort; the alternate-buffer selector quirk is not kept). */
 void sort_b40c_f(float *d_keys, uint32_t *d_vals, uint32_t N);
+
+/* algorithms.h:37, sort.cu:59-76: stable descending key-value sort in place
+ * (thrust::greater keys; equal keys keep their order). */
+void sort_rf(float *d_keys, uint32_t *d_vals, uint32_t N);
+
+/* algorithms.h:49, sort.cu:37-57: d_idx = init..init+N-1 stably sorted by the
+ * keys, descending; the keys are sorted in place. */
+void sort_idx_rf(float *d_keys_raw, uint32_t *d_idx_raw, uint32_t N, uint32_t init);
+
+/* algorithms.h:89-97, filter.cu:38-73: the indices i (ascending) with
+ * min <= d_in[i] <= max written to d_out; returns their number. */
+uint32_t findIndicesInRange_f(const float *d_in, uint32_t *d_out, const uint32_t N,
+                              const float min, const float max);
+
+/* algorithms.h:293, mvm.cu:21-56: d_y = A x for row-major A (m rows of n)
+ * on the stream of `handle` (an hfx_blas_create handle in place of the
+ * cuBLAS one: Algorithms.hs:370 passes CUBLAS.Handle, here a Ptr ()). */
+void mvm_ff(void *handle, float *d_y, const float *d_A, const float *d_x, const uint32_t m,
+            const uint32_t n);
 
 /* algorithms.h:80-87, prepare_ions.cu:105-132: pack each residue byte as
  * (c - 'A') | mod << 5 (all N bytes, also N < 512). */
@@ -396,6 +420,23 @@ int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra *sp, float 
                    uint32_t margin, void *stream);
 int hfx_batch_plan_ix(hfx_batch *b, const hfx_pepix *ix, const hfx_spectra *sp, float eps,
                       uint32_t N, uint32_t margin, void *stream);
+/* Device bytes a batch plan of S spectra with `total` scored pairs (the sum
+ * of their window sizes, e.g. findSpecCandsByMass's return) allocates in
+ * the given precision form, the per-database index excepted (host
+ * arithmetic; Search.hs:427-433 sizes its work the same way).
+ * hfx_batch_plan fails with HFX_ERR_NOMEM, allocating nothing, when its
+ * batch exceeds the device's free memory. */
+int hfx_batch_required_bytes(uint32_t S, uint64_t total, uint32_t N, uint32_t num_pep,
+                             uint32_t precision, uint64_t *bytes);
+/* Execution.hs:77-93 groupScorePlan: cuts spectra 0..S-1 (score offsets
+ * h_off[0..S]) into consecutive batches, each the longest run whose plan
+ * fits `budget` bytes; h_first (optional, cap entries) receives the first
+ * spectrum of each batch and then S; *num_batches their number.  A spectrum
+ * that alone exceeds the budget returns HFX_ERR_NOMEM (the reference would
+ * split its candidates). */
+int hfx_batch_split(const uint64_t *h_off, uint32_t S, uint32_t N, uint32_t num_pep,
+                    uint32_t precision, uint64_t budget, uint32_t *h_first, uint32_t cap,
+                    uint32_t *num_batches);
 int hfx_batch_use_index(hfx_batch *b, const hfx_pepix *ix);
 /* One batched search, every per-batch step on the device and without host
  * synchronisation or allocation: windows -> spectrum order (fragment charge

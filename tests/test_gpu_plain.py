@@ -454,3 +454,61 @@ def test_long_candidates_take_the_general_path(orc):
         c = int(res.top_cnt[s])
         assert res.top_pos[s, :c].tolist() == ref["top_pos"][s, :c].tolist(), s
         assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all(), s
+
+
+@pytest.mark.parametrize("n", [1, 7, 2048, 100_003])
+def test_descending_sorts_and_range_indices(env, n):
+    """sort_rf / sort_idx_rf (sort.cu:37-76, thrust::greater): stable
+    descending sorts, checked against numpy's stable sort of the negated
+    keys (the keys hold many ties and both signs); findIndicesInRange_f
+    (filter.cu:38-73): the ascending indices of the keys inside [min, max],
+    bounds inclusive."""
+    pl, L = env["pl"], env["L"]
+    rng = np.random.default_rng(n)
+    keys = (rng.integers(-40, 40, size=n) * 0.25).astype(np.float32)
+    vals = rng.integers(0, 2**32 - 1, size=n, dtype=np.uint64).astype(np.uint32)
+    want = np.argsort(-keys.astype(np.float64), kind="stable")
+    tk, tv = pl.dev(keys), pl.dev(vals)
+    L.sort_rf(tk.p, tv.p, n)
+    pl.sync()
+    assert np.array_equal(pl.host(tk, np.float32, n), keys[want])
+    assert np.array_equal(pl.host(tv, np.uint32, n), vals[want])
+    tk = pl.dev(keys)
+    ti = pl.empty(4 * n)
+    L.sort_idx_rf(tk.p, ti.p, n, 5)
+    pl.sync()
+    assert np.array_equal(pl.host(ti, np.uint32, n), want.astype(np.uint32) + 5)
+    assert np.array_equal(pl.host(tk, np.float32, n), keys[want])
+    for lo, hi in ((-1.0, 2.5), (3.0, 3.0), (20.0, 30.0), (-100.0, 100.0)):
+        tk = pl.dev(keys)
+        to = pl.empty(4 * n)
+        cnt = L.findIndicesInRange_f(tk.p, to.p, n, lo, hi)
+        exp = np.nonzero((keys >= np.float32(lo)) & (keys <= np.float32(hi)))[0]
+        assert cnt == len(exp), (lo, hi)
+        assert np.array_equal(pl.host(to, np.uint32, cnt), exp.astype(np.uint32)), (lo, hi)
+
+
+def test_mvm_ff_on_a_handle_stream(env):
+    """mvm_ff (mvm.cu:21-56, cublasSgemv over a row-major matrix): y = A x on
+    the stream its handle carries, within the FP32 bound of a float64 dot
+    (at most 64 + 6 roundings per row sum: n / 64 terms a lane, then a
+    six-level butterfly)."""
+    pl, L = env["pl"], env["L"]
+    from hfx_amd import check
+
+    rng = np.random.default_rng(3)
+    for m, n in ((1, 1), (37, 2075), (1000, 512)):
+        A = rng.standard_normal((m, n)).astype(np.float32)
+        x = rng.standard_normal(n).astype(np.float32)
+        tA, tx, ty = pl.dev(A), pl.dev(x), pl.empty(4 * m)
+        h = C.c_void_p()
+        check(L.hfx_blas_create(C.byref(h)), "hfx_blas_create")
+        st = pl.Stream()
+        check(L.hfx_blas_set_stream(h, st.p), "hfx_blas_set_stream")
+        L.mvm_ff(h, ty.p, tA.p, tx.p, m, n)
+        st.sync()
+        y = pl.host(ty, np.float32, m).astype(np.float64)
+        exact = A.astype(np.float64) @ x.astype(np.float64)
+        bound = (n // 64 + 8) * 2.0**-24 * (np.abs(A).astype(np.float64) @ np.abs(x)) + 1e-30
+        assert np.all(np.abs(y - exact) <= bound), (m, n)
+        check(L.hfx_blas_destroy(h), "hfx_blas_destroy")

@@ -108,3 +108,43 @@ def test_spectra_offsets_checked_before_upload(case):
            "beyond": [0, 10, 20, 31]}[case]
     with pytest.raises(ValueError):
         pipeline.DeviceSpectra(None, pcr, ch, mz, it, np.array(off, np.uint32))
+
+
+def test_batch_sizing_and_split():
+    """hfx_batch_required_bytes grows with the spectra and the scores, the
+    dense form needs more than the sparse one; hfx_batch_split
+    (groupScorePlan, Execution.hs:77-93) covers the spectra with consecutive
+    batches that each fit the budget and could not take their next spectrum,
+    and refuses a spectrum that alone exceeds the budget."""
+    L = hfx_amd.lib()
+
+    def need(S, total, prec=0):
+        b = C.c_uint64()
+        assert L.hfx_batch_required_bytes(S, total, 5, 2_700_000, prec, C.byref(b)) == 0
+        return b.value
+
+    assert need(100, 10**6) < need(200, 10**6) < need(200, 2 * 10**6)
+    assert need(1000, 10**6, 0) < need(1000, 10**6, 1)
+    rng = np.random.default_rng(5)
+    S = 5000
+    num = rng.integers(0, 20000, size=S).astype(np.uint64)
+    off = np.zeros(S + 1, np.uint64)
+    np.cumsum(num, out=off[1:])
+    for prec in (0, 1):
+        budget = need(S, int(off[-1]), prec) // 7
+        first = np.zeros(S + 2, np.uint32)
+        nb = C.c_uint32()
+        assert L.hfx_batch_split(off.ctypes.data, S, 5, 2_700_000, prec, budget, first.ctypes.data,
+                                 len(first), C.byref(nb)) == 0
+        k = nb.value
+        assert 7 <= k <= 9
+        cuts = first[: k + 1].astype(np.int64)
+        assert cuts[0] == 0 and cuts[-1] == S and np.all(np.diff(cuts) > 0)
+        for a, e in zip(cuts[:-1], cuts[1:]):
+            assert need(int(e - a), int(off[e] - off[a]), prec) <= budget
+            if e < S:
+                assert need(int(e + 1 - a), int(off[e + 1] - off[a]), prec) > budget
+    one = need(1, int(num.max()), 0)
+    nb = C.c_uint32()
+    rc = L.hfx_batch_split(off.ctypes.data, S, 5, 2_700_000, 0, one // 2, None, 0, C.byref(nb))
+    assert rc == 3 and "alone needs" in hfx_amd.last_error()
