@@ -505,7 +505,7 @@ uint32_t orc_prepare_gen_mod(uint32_t *out_mpep_pep_idx, uint32_t *out_mpep_pep_
 
 /* gen_mod_pep.cu:127-157 add_ma_ranks: mixed-radix split of the mpep rank over
  * the suffix products. */
-static void add_ma_ranks(uint32_t *ranks, uint32_t rank, const uint32_t *scan_row, uint32_t num_ma)
+void orc_add_ma_ranks(uint32_t *ranks, uint32_t rank, const uint32_t *scan_row, uint32_t num_ma)
 {
     uint32_t x = rank;
     for (uint32_t i = 0; i < num_ma; ++i) {
@@ -531,7 +531,7 @@ void orc_gen_mod_cands(uint32_t *out_unrank, const uint32_t *mod_ma_count,
     for (uint32_t mp = 0; mp < num_mpep; ++mp) {
         uint32_t ith = mpep_ith_cand[mp];
         uint32_t pm = pep_mod_idx[ith];
-        add_ma_ranks(ranks.data(), mpep_rank[mp], pep_ma_num_comb_scan + (size_t)ith * num_ma, num_ma);
+        orc_add_ma_ranks(ranks.data(), mpep_rank[mp], pep_ma_num_comb_scan + (size_t)ith * num_ma, num_ma);
         uint32_t start = mpep_mod_ma_count_sum_scan[mp], pos = 0;
         for (uint32_t i = 0; i < num_ma; ++i) {
             uint32_t k = mod_ma_count[pm * num_ma + i];

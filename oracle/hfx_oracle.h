@@ -17,9 +17,29 @@
  *     order-independent fixed-point accumulation (2^-36 resolution) that both
  *     the oracle and the GPU reproduce bit for bit; reported as a double.
  *
- * Parity pins for each function are listed in tests/test_oracle_pins.py.
- * Functions pinned only by restated semantics (no closed form, no reference
- * golden vector) are marked "parity unpinned" below.
+ * Every function is pinned in tests/test_oracle_pins.py by a method other
+ * than its own formula:
+ *   mass table            the 35 matches printed in the reference's result
+ *                         files (tests/golden/reference_out_peptides.json);
+ *                         average > monoisotopic for every residue
+ *   ms2_mass, dta_precursor   m/z <-> neutral mass identities, exact decimals
+ *   mod_info              hand-derived vectors, itertools.product
+ *   digest, sort_idx, prepare_ions   hand examples, stability, bit packing
+ *   windows (both)        linear scans of the sorted residuals
+ *   prepare_scoring       concatenation of the windows
+ *   choose, unrank_comb   the reference's Pascal table rows, math.comb, the
+ *                         combinatorial number system (and its reflection)
+ *   find_modable_peptides linear scans + bytes.count
+ *   calc_total_mod_cands  enumeration of occurrence subsets
+ *   prepare_gen_mod       explicit expansion, itertools.accumulate
+ *   add_ma_ranks, gen_mod_cands   mixed-radix digits and their bijection
+ *   peaks_list            exact decimal m/z and bins
+ *   peaks_dense           per-bin max of the peak list
+ *   mpeaks_dense          hand-modified peptides, exact decimal bins
+ *   score_fx, score_f32   float64 dot, error bound
+ *   preprocess            closed forms on constructed spectra
+ *   search_plain, search_mods   brute-force searches composed of the above
+ *   check_fast_div        a planted wrong reciprocal it must detect
  */
 #ifndef HFX_ORACLE_H
 #define HFX_ORACLE_H
@@ -86,6 +106,9 @@ uint32_t orc_prepare_gen_mod(uint32_t *out_mpep_pep_idx, uint32_t *out_mpep_pep_
                              const uint32_t *mod_ma_count_sum, const uint32_t *pep_idx,
                              const uint32_t *pep_mod_idx, const uint32_t *pep_num_mpep,
                              uint32_t num_cand, uint32_t num_mpep);
+/* the per-acid ranks of one modified-peptide rank (mixed radix over the
+ * suffix products scan_row[i] = prod_{j>=i} C(D_j, d_j)) */
+void orc_add_ma_ranks(uint32_t *ranks, uint32_t rank, const uint32_t *scan_row, uint32_t num_ma);
 void orc_gen_mod_cands(uint32_t *out_unrank, const uint32_t *mod_ma_count,
                        const uint32_t *mpep_ith_cand, const uint32_t *mpep_rank,
                        const uint32_t *mpep_mod_ma_count_sum_scan,

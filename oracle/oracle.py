@@ -296,6 +296,13 @@ def prepare_gen_mod(mod_sum, pep_idx, pep_mod_idx, pep_num_mpep, num_mpep):
     return (int(lu),) + tuple(o[:m].copy() for o in outs)
 
 
+def add_ma_ranks(rank, scan_row):
+    row = u32(scan_row)
+    out = np.zeros(max(len(row), 1), np.uint32)
+    lib().orc_add_ma_ranks(_p(out), C.c_uint32(rank), _p(row), C.c_uint32(len(row)))
+    return out[: len(row)].copy()
+
+
 def gen_mod_cands(len_unrank, mod_ma_count, mpep_ith, mpep_rank, mpep_scan, pep_mod_idx,
                   pep_ma_count, comb_scan, num_ma):
     out = np.zeros(max(int(len_unrank), 1), np.uint32)

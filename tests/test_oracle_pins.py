@@ -6,7 +6,9 @@ re-typed the oracle's formula would pin nothing; these use a different
 method (exact rationals, enumeration, linear scans, hand-derived values).
 """
 import itertools
+import json
 import math
+import os
 from decimal import Decimal, getcontext
 
 import numpy as np
@@ -368,3 +370,428 @@ def test_fast_division_check_detects_a_wrong_reciprocal(orc):
         assert orc.check_fast_div(np.float32(d), np.float32(1.0), np.float32(2.0), False) > 1000
     # powers of two have exact reciprocals: nothing to correct
     assert orc.check_fast_div(np.float32(2.0), np.float32(1.0), np.float32(2.0), False) == 0
+
+
+# ---------------------------------------------------------------------------
+# mass table and precursor masses
+# ---------------------------------------------------------------------------
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+H_PROTON = Decimal("1.007276")
+H2O = Decimal("18.010565")
+E_MASS = 0.000549  # electron mass: hydrogen atom minus proton
+
+
+def _reference_matches():
+    with open(os.path.join(GOLDEN, "reference_out_peptides.json")) as f:
+        files = json.load(f)
+    return [(m["peptide"], m["mh"]) for e in files for m in e["matches"]]
+
+
+def _f32_residue_sum(mass, pep):
+    r = np.float32(0)
+    for c in pep:
+        r = np.float32(r + mass[ord(c) - 65])
+    return r
+
+
+def test_mass_table_against_reference_outputs(orc):
+    """Mass.hs:147-215 (the monoisotopic table), Config.hs:266-272 (static
+    add) and Location.hs:58 ((M+H)+ = residues + H + H2O) against the 35
+    matches printed in the reference's own result files (tests/spectra/*.out,
+    written with add_C = 57.0 as in hfx.params; all 20 standard residues
+    occur).  The files were written with the hydrogen atom's mass where the
+    current constant is the proton's, so every printed value sits one
+    electron mass (5.49e-4) above the residue sum + 1.007276 + 18.010565;
+    printing to 4 decimals and the FP32 residue sum leave 2e-4 around that.
+    A transposed or mistyped entry moves a sum by far more."""
+    mass = orc.mass_table({"C": 57.0})
+    rows = _reference_matches()
+    assert len(rows) == 35
+    assert set("".join(p for p, _ in rows)) == set("ACDEFGHIKLMNPQRSTVWY")
+    for pep, mh in rows:
+        calc = float(_f32_residue_sum(mass, pep)) + float(H_PROTON + H2O)
+        assert abs(float(mh) - calc - E_MASS) <= 2e-4, (pep, mh, calc)
+    # the static modification adds to its residue only
+    m0, m1 = orc.mass_table(), orc.mass_table({"C": 57.0})
+    assert m1[2] == np.float32(m0[2] + np.float32(57.0))
+    assert (np.delete(m1, 2) == np.delete(m0, 2)).all()
+
+
+def test_average_masses_exceed_monoisotopic(orc):
+    """Mass.hs (average table): every residue's average mass exceeds its
+    monoisotopic mass by the heavy-isotope share, 0.03-0.2 Da for the
+    standard residues; a transposed entry breaks this."""
+    mono, avg = orc.mass_table(mono=True), orc.mass_table(mono=False)
+    for c in "ACDEFGHIKLMNPQRSTVWY":
+        d = float(avg[ord(c) - 65]) - float(mono[ord(c) - 65])
+        assert 0.03 < d < 0.2, (c, d)
+
+
+@pytest.mark.parametrize("z", [1, 2, 3, 4])
+def test_precursor_masses_physical_identities(orc, z):
+    """Spectrum/Data.hs:58-59 ms2Mass and DTA.hs:51.  For a peptide of
+    residue sum R (neutral mass M = R + H2O) the z+ precursor has
+    m/z = (M + z*H+)/z, and ms2Mass of that m/z must give back R; a DTA
+    file's (M+H)+ line converts to the same m/z.  Checked in exact decimals
+    on the reference's 35 printed peptides: a dropped or sign-flipped term
+    is off by at least 1 Da, FP32 rounding by at most 1e-3."""
+    mass = orc.mass_table({"C": 57.0})
+    for pep, _ in _reference_matches():
+        R = Decimal(float(_f32_residue_sum(mass, pep)))
+        M = R + H2O
+        mz = (M + z * H_PROTON) / z
+        got = orc.ms2_mass(np.float32(float(mz)), np.float32(z))
+        assert abs(Decimal(float(got)) - R) <= Decimal("1e-3"), (pep, z, got, R)
+        MH = M + H_PROTON
+        pcr = orc.dta_precursor(np.float32(float(MH)), np.float32(z))
+        assert abs(Decimal(float(pcr)) - mz) <= Decimal("3e-4"), (pep, z, pcr, mz)
+
+
+# ---------------------------------------------------------------------------
+# variable-modification enumeration
+# ---------------------------------------------------------------------------
+
+VAR = {"M": 15.9949, "S": 79.9663, "T": 79.9663}
+MAX_MA = 3  # at most 2 modified residues per peptide
+
+
+@pytest.fixture(scope="module")
+def mdb(orc):
+    """150 proteins, tryptic with one missed cleavage, 5-25 residues; 12
+    spectrum masses, 8 of them a modified peptide's mass."""
+    from hfx_amd import synth
+
+    prot = synth.make_proteins(150, seed=61)
+    mass = orc.mass_table()
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=1, p_rule=1, min_len=5,
+                           max_len=25)
+    ma, mam, cnt, msum, dl = orc.mod_info(VAR, MAX_MA)
+    rng = np.random.default_rng(62)
+    seqs = [bytes(prot.seq[tc[p]:tn[p]]) for p in range(len(r))]
+    modable = [p for p in range(len(r)) if sum(seqs[p].count(a) for a in b"MST") >= 2]
+    masses = []
+    for k in range(8):
+        p = modable[int(rng.integers(len(modable)))]
+        m = int(rng.integers(len(dl)))
+        masses.append(np.float32(r[p] + dl[m]) + np.float32(rng.uniform(-0.5, 0.5)))
+    masses += list(rng.uniform(900, 2500, 4))
+    return dict(seq=prot.seq, mass=mass, r=r, tc=tc, tn=tn, seqs=seqs, ma=ma, mam=mam, cnt=cnt,
+                msum=msum, dl=dl, masses=np.array(masses, np.float32), eps=np.float32(1.5),
+                order=np.argsort(r, kind="stable"))
+
+
+def _brute_rows(d, mass_s):
+    """(peptide, mod, counts) rows of one spectrum by a linear scan of the
+    residual-sorted peptides per modification, counts by bytes.count."""
+    rs = d["r"][d["order"]]
+    rows = []
+    for m, delta in enumerate(d["dl"]):
+        lo = np.float32(np.float32(mass_s - d["eps"]) - delta)
+        hi = np.float32(np.float32(mass_s + d["eps"]) - delta)
+        for p in d["order"][(rs >= lo) & (rs <= hi)]:
+            c = [d["seqs"][p].count(bytes([a])) for a in d["ma"]]
+            if all(c[i] >= d["cnt"][m][i] for i in range(len(c))):
+                rows.append((int(p), m, c))
+    return rows
+
+
+def _oracle_rows(orc, d):
+    idx, _ = orc.sort_idx(d["r"])
+    S, M = len(d["masses"]), len(d["dl"])
+    b = np.zeros(S * M, np.uint32)
+    e = np.zeros(S * M, np.uint32)
+    sc = np.zeros(S * M, np.uint32)
+    cand = np.zeros(S, np.uint32)
+    for s in range(S):
+        bb, ee, ss, tot = orc.find_begin_end(d["r"], idx, d["dl"], d["masses"][s], d["eps"])
+        b[s * M:(s + 1) * M], e[s * M:(s + 1) * M], sc[s * M:(s + 1) * M] = bb, ee, ss
+        cand[s] = tot
+    scan = np.concatenate([[0], np.cumsum(cand)]).astype(np.uint32)
+    ions = orc.prepare_ions(d["seq"], d["ma"])
+    return orc.find_modable_peptides(scan, scan[-1], ions, d["tc"], d["tn"], idx, b, e, sc,
+                                     d["cnt"], d["ma"])
+
+
+def test_find_modable_peptides_brute_force(orc, mdb):
+    """filter_by_mod.cu:150-378 findModablePeptides (with findBeginEnd_f,
+    :35-143): rows spectrum-major, modification-minor, residual order within
+    a window; a row is kept when every modable acid occurs in [tc, tn) at
+    least as often as the modification asks; spec_num_valid_scan counts the
+    kept rows of earlier spectra."""
+    d = mdb
+    nv, pep, pmod, pcnt, vs = _oracle_rows(orc, d)
+    want, scan = [], []
+    for ms in d["masses"]:
+        scan.append(len(want))
+        want += _brute_rows(d, ms)
+    assert nv == len(want) > 100
+    assert pep.tolist() == [w[0] for w in want]
+    assert pmod.tolist() == [w[1] for w in want]
+    assert pcnt.reshape(nv, -1).tolist() == [w[2] for w in want]
+    assert vs.tolist() == scan
+    # the windows also held rows that lack a modable acid and were dropped
+    rs = d["r"][d["order"]]
+    in_windows = sum(int(np.count_nonzero(
+        (rs >= np.float32(np.float32(ms - d["eps"]) - dl)) &
+        (rs <= np.float32(np.float32(ms + d["eps"]) - dl))))
+        for ms in d["masses"] for dl in d["dl"])
+    assert in_windows > nv
+
+
+def test_calc_total_mod_cands_by_enumeration(orc, mdb):
+    """calc_mod_cands.cu:27-119: modified peptides per row = the number of
+    ways to pick d_i of the D_i occurrences of every modable acid, counted
+    here by enumerating the subsets; the row of suffix products."""
+    d = mdb
+    nv, pep, pmod, pcnt, vs = _oracle_rows(orc, d)
+    na = len(d["ma"])
+    tot, num, comb = orc.calc_total_mod_cands(d["cnt"], pep, pmod, pcnt, na)
+    pcnt = pcnt.reshape(nv, na)
+    comb = comb.reshape(nv, na)
+    want_total = 0
+    for i in range(nv):
+        ways = [sum(1 for _ in itertools.combinations(range(int(pcnt[i][a])),
+                                                       int(d["cnt"][pmod[i]][a])))
+                for a in range(na)]
+        assert num[i] == math.prod(ways)
+        assert comb[i].tolist() == [math.prod(ways[a:]) for a in range(na)]
+        want_total += math.prod(ways)
+    assert tot == want_total > nv
+
+
+def test_prepare_gen_mod_expansion(orc, mdb):
+    """prepare_gen_mod.cu:23-129: one record per modified peptide, (peptide,
+    modification, rank 0..n-1, row), and the exclusive scan of the
+    modification's residue count."""
+    d = mdb
+    nv, pep, pmod, pcnt, vs = _oracle_rows(orc, d)
+    tot, num, comb = orc.calc_total_mod_cands(d["cnt"], pep, pmod, pcnt, len(d["ma"]))
+    lu, mp_pep, mp_mod, mp_rank, mp_ith, mp_scan = orc.prepare_gen_mod(d["msum"], pep, pmod, num,
+                                                                       tot)
+    rec = [(int(pep[i]), int(pmod[i]), k, i) for i in range(nv) for k in range(int(num[i]))]
+    assert list(zip(mp_pep.tolist(), mp_mod.tolist(), mp_rank.tolist(), mp_ith.tolist())) == rec
+    sums = [int(d["msum"][m]) for _, m, _, _ in rec]
+    assert mp_scan.tolist() == [0] + list(itertools.accumulate(sums))[:-1]
+    assert lu == sum(sums)
+
+
+def test_rank_split_is_a_mixed_radix_bijection(orc, mdb):
+    """gen_mod_pep.cu:127-157 add_ma_ranks: rank -> per-acid ranks is the
+    mixed-radix decomposition over C(D_i, d_i) (digit i = rank // prod_{j>i}
+    C_j mod C_i), a bijection of [0, prod C) onto the digit boxes; every
+    modified peptide's unrank slice is the per-acid unrankComb of its
+    digits (gen_mod_pep.cu:159-312)."""
+    d = mdb
+    nv, pep, pmod, pcnt, vs = _oracle_rows(orc, d)
+    na = len(d["ma"])
+    tot, num, comb = orc.calc_total_mod_cands(d["cnt"], pep, pmod, pcnt, na)
+    lu, mp_pep, mp_mod, mp_rank, mp_ith, mp_scan = orc.prepare_gen_mod(d["msum"], pep, pmod, num,
+                                                                       tot)
+    unrank = orc.gen_mod_cands(lu, d["cnt"], mp_ith, mp_rank, mp_scan, pmod, pcnt, comb, na)
+    pcnt = pcnt.reshape(nv, na)
+    comb = comb.reshape(nv, na)
+    row_first = np.concatenate([[0], np.cumsum(num)])
+    boxes_seen = 0
+    for i in range(nv):
+        m = int(pmod[i])
+        C_ = [math.comb(int(pcnt[i][a]), int(d["cnt"][m][a])) for a in range(na)]
+        digits = set()
+        for rank in range(int(num[i])):
+            want = [(rank // math.prod(C_[a + 1:])) % C_[a] for a in range(na)]
+            got = orc.add_ma_ranks(rank, comb[i]).tolist()
+            assert got == want, (i, rank)
+            digits.add(tuple(got))
+            mp = int(row_first[i]) + rank
+            sl = unrank[int(mp_scan[mp]): int(mp_scan[mp]) + int(d["msum"][m])].tolist()
+            exp = []
+            for a in range(na):
+                k = int(d["cnt"][m][a])
+                if k:
+                    exp += orc.unrank_comb(int(pcnt[i][a]), k, want[a]).tolist()
+            assert sl == exp, (i, rank)
+        assert digits == set(itertools.product(*[range(c) for c in C_]))
+        boxes_seen += sum(c > 1 for c in C_) >= 2
+    assert boxes_seen > 0  # rows whose rank splits over two acids
+
+
+def test_modified_ladder_hand_built(orc):
+    """check_series.cu:61-99 genMIons + mion_series.cu:226,277: the i-th
+    occurrence of a modable acid (counting that acid only) is modified when i
+    is in its unrank slice; a modified residue weighs table + variable mass;
+    the residual is r + delta.  Positions below are worked out by hand for
+    AMSGMSTPMK (M at 1, 4, 8; S at 2, 5; T at 6); every bin is the rounded
+    exact m/z of the hand-modified residue masses, except the bins an ion
+    whose exact quotient lies within 1e-3 of a rounding boundary could
+    reach, which are left out of the comparison."""
+    pep = "AMSGMSTPMK"
+    ma, mam = b"MST", np.array([15.9949, 79.9663, 79.9663], np.float32)
+    mass = orc.mass_table()
+    ions = orc.prepare_ions(np.frombuffer(pep.encode(), np.uint8), ma)
+    base = np.array([mass[ord(c) - 65] for c in pep], np.float32)
+    r = _f32_residue_sum(mass, pep)
+    cases = [  # (row of modified counts M,S,T; unrank slice; modified positions)
+        ([1, 0, 0], [2], {8: 0}),
+        ([1, 1, 0], [0, 1], {1: 0, 5: 1}),
+        ([0, 1, 1], [0, 0], {2: 1, 6: 2}),
+        ([2, 0, 0], [2, 0], {8: 0, 1: 0}),
+        ([0, 2, 0], [1, 0], {5: 1, 2: 1}),
+    ]
+    L = 2100
+    for row, slice_, pos in cases:
+        delta = np.float32(0)
+        for c, mv in zip(row, mam):
+            delta = np.float32(delta + np.float32(mv * np.float32(c)))
+        masses = base.copy()
+        for k, a in pos.items():
+            masses[k] = np.float32(masses[k] + mam[a])
+        rr = np.float32(r + delta)
+        for zmax in (1, 2, 3):
+            want = np.zeros(L, np.uint8)
+            sure = np.ones(L, bool)
+            wts = [10, 50, 25, 25, 10, 10, 50, 25, 25, 10]
+            nbr = {0: (0,), 1: (-1, 0, 1), 4: (0,), 5: (0,), 6: (-1, 0, 1), 9: (0,)}
+            for q in _exact_peaks(masses, rr, zmax):
+                xb, xy = int(q[1].to_integral_value()), int(q[6].to_integral_value())
+                bins = [int(q[0].to_integral_value()), xb, xb + 1, xb - 1,
+                        int(q[4].to_integral_value()), int(q[5].to_integral_value()),
+                        xy, xy + 1, xy - 1, int(q[9].to_integral_value())]
+                for j, v in enumerate(q):
+                    if v is not None and abs((v - int(v)) - Decimal("0.5")) < Decimal("1e-3"):
+                        for o in nbr[j]:
+                            for k in (int(v) + o, int(v) + 1 + o):
+                                if 0 <= k < L:
+                                    sure[k] = False
+                for x, w in zip(bins, wts):
+                    if 0 <= x < L:
+                        want[x] = max(want[x], w)
+            got = orc.mpeaks_dense(ions, 0, len(pep), r, mass, zmax, L, row, delta, ma, mam,
+                                   slice_)
+            assert np.count_nonzero(want[sure]) > 20 * zmax and (~sure).sum() <= 4
+            assert (got[sure] == want[sure]).all(), (row, slice_, zmax)
+
+
+def _dense_from_list(masses, r, zmax, L, orc):
+    bins, w = orc.peaks_list(masses, r, zmax)
+    row = np.zeros(L, np.uint8)
+    for x, wt in zip(bins, w):
+        if 0 <= x < L:
+            row[x] = max(row[x], wt)
+    return row
+
+
+def _fx(row, x):
+    nz = np.nonzero(row)[0]
+    t = np.ldexp(row[nz].astype(np.float64) * x[nz].astype(np.float64), 36)
+    return int(sum(int(v) for v in np.rint(t)))
+
+
+def test_search_mods_brute_force(orc, mdb):
+    """Search.hs:282-622 searchWithMods, composed here from independent
+    pieces: windows and modable rows by linear scan (as above), ranks by the
+    mixed radix, occurrence subsets by unrankComb (pinned above), modified
+    residue masses placed by str.find, theoretical rows from the pinned peak
+    list, exact scores by float64 rounding, ranking by Python's sort on
+    (score desc, position asc)."""
+    d = mdb
+    S, topn = len(d["masses"]), 5
+    rng = np.random.default_rng(63)
+    stride = 2112
+    X = np.zeros((S, stride), np.float32)
+    lens = rng.integers(1500, stride + 1, S).astype(np.uint32)
+    for s in range(S):
+        X[s, : lens[s]] = rng.normal(0, 3, lens[s]).astype(np.float32)
+    zmax = rng.integers(1, 5, S).astype(np.uint32)
+    idx, _ = orc.sort_idx(d["r"])
+    ions = orc.prepare_ions(d["seq"], d["ma"])
+    ref = orc.search_mods(ions, d["r"], d["tc"], d["tn"], d["mass"], idx, d["ma"], d["mam"],
+                          d["cnt"], d["msum"], d["dl"], X, lens, zmax, d["masses"], d["eps"], topn)
+    na = len(d["ma"])
+    k_mp = 0
+    for s in range(S):
+        scored = []
+        for p, m, c in _brute_rows(d, d["masses"][s]):
+            C_ = [math.comb(c[a], int(d["cnt"][m][a])) for a in range(na)]
+            for rank in range(math.prod(C_)):
+                digits = [(rank // math.prod(C_[a + 1:])) % C_[a] for a in range(na)]
+                masses = np.array([d["mass"][ch - 65] for ch in d["seqs"][p]], np.float32)
+                sl = []
+                for a in range(na):
+                    k = int(d["cnt"][m][a])
+                    if not k:
+                        continue
+                    occ = orc.unrank_comb(c[a], k, digits[a]).tolist()
+                    sl += occ
+                    at = [i for i, ch in enumerate(d["seqs"][p]) if ch == d["ma"][a]]
+                    for o in occ:
+                        masses[at[o]] = np.float32(masses[at[o]] + d["mam"][a])
+                assert ref["mpep_pep"][k_mp] == p and ref["mpep_mod"][k_mp] == m
+                off = int(ref["mpep_unrank_off"][k_mp])
+                assert ref["unrank"][off: off + len(sl)].tolist() == sl
+                k_mp += 1
+                rr = np.float32(d["r"][p] + d["dl"][m])
+                row = _dense_from_list(masses, rr, int(zmax[s]), int(lens[s]), orc)
+                scored.append(_fx(row, X[s]))
+        assert ref["spec_num_mpep"][s] == len(scored)
+        order = sorted(range(len(scored)), key=lambda i: (-scored[i], i))[:topn]
+        c = int(ref["top_cnt"][s])
+        assert c == min(topn, len(scored))
+        assert ref["top_pos"][s, :c].tolist() == order
+        assert ref["top_fx"][s, :c].tolist() == [scored[i] for i in order]
+    assert k_mp == len(ref["mpep_pep"]) > 200
+
+
+def test_search_plain_brute_force(orc, mdb):
+    """Search.hs:98-124 searchWithoutMods (filter.cu:75-127 windows,
+    ion_series.cu ladders, the dense dot, Search.hs:119-123 ranking) composed
+    from the pinned pieces: linear-scan windows, the pinned peak list,
+    float64-rounded exact scores, Python's sort; FP32 scores within
+    1e-5 * sum w|x| of the exact."""
+    d = mdb
+    S, topn = len(d["masses"]), 4
+    rng = np.random.default_rng(64)
+    stride = 2112
+    X = np.zeros((S, stride), np.float32)
+    lens = rng.integers(1000, stride + 1, S).astype(np.uint32)
+    for s in range(S):
+        X[s, : lens[s]] = rng.normal(0, 3, lens[s]).astype(np.float32)
+    zmax = np.array([1, 2, 3, 4, 5, 1, 2, 3, 4, 2, 3, 1], np.uint32)[:S]
+    idx, _ = orc.sort_idx(d["r"])
+    ions = orc.prepare_ions(d["seq"])
+    ref = orc.search_plain(ions, d["r"], d["tc"], d["tn"], d["mass"], idx, X, lens, zmax,
+                           d["masses"], d["eps"], topn, want_all=True)
+    rs = d["r"][d["order"]]
+    for s in range(S):
+        lo = np.float32(d["masses"][s] - d["eps"])
+        hi = np.float32(d["masses"][s] + d["eps"])
+        win = np.nonzero((rs >= lo) & (rs <= hi))[0]
+        assert ref["num"][s] == len(win)
+        if len(win):
+            assert ref["begin"][s] == win[0]
+        fx = []
+        for i in win:
+            p = d["order"][i]
+            masses = np.array([d["mass"][ch - 65] for ch in d["seqs"][p]], np.float32)
+            z = int(zmax[s]) if zmax[s] <= 4 else 0  # no ladder above fragment charge 4
+            row = _dense_from_list(masses, d["r"][p], z, int(lens[s]), orc)
+            fx.append(_fx(row, X[s]))
+            a = int(ref["off"][s]) + len(fx) - 1
+            assert ref["all_fx"][a] == fx[-1]
+            scale = float(np.sum(row * np.abs(X[s, : lens[s]].astype(np.float64))))
+            assert abs(float(ref["all_f32"][a]) - fx[-1] * 2.0**-36) <= 1e-5 * scale + 1e-6
+        order = sorted(range(len(fx)), key=lambda i: (-fx[i], i))[:topn]
+        c = int(ref["top_cnt"][s])
+        assert ref["top_pos"][s, :c].tolist() == order
+        assert ref["top_fx"][s, :c].tolist() == [fx[i] for i in order]
+
+
+def test_prepare_scoring_is_window_concatenation(orc):
+    """prepare_scoring.cu:20-67: spectrum s's candidates are
+    idx_sorted[begin_s : begin_s + num_s], spectra in order."""
+    rng = np.random.default_rng(65)
+    idx = rng.permutation(1000).astype(np.uint32)
+    begin = rng.integers(0, 900, 20).astype(np.uint32)
+    num = rng.integers(0, 100, 20).astype(np.uint32)
+    out = orc.prepare_scoring(idx, begin, num)
+    want = np.concatenate([idx[b:b + n] for b, n in zip(begin, num)])
+    assert out.tolist() == want.tolist()
