@@ -330,7 +330,7 @@ def main():
         ab = algorithmic_bytes(env, plain_res)
         paths["plain_fp32_sparse"] = dict(
             value=sc * args.steps / el, ms_per_step=el * 1e3 / args.steps, scores_per_step=sc,
-            kernel="k_xcorr<HfxPlainCands>", kernel_ms=kern_ms,
+            kernel="k_xcorr_pack", kernel_ms=kern_ms,
             kernel_gbs=ab / (kern_ms * 1e-3) / 1e9, algorithmic_bytes=ab,
             flagged_spectra_per_step=flagged)
         log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s, kernel {kern_ms:.2f} ms")

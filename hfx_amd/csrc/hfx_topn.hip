@@ -549,9 +549,9 @@ __global__ void __launch_bounds__(TN_THREADS)
 
 template <class Cands>
 __global__ void __launch_bounds__(FX_THREADS)
-    k_topn_fx(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt, uint32_t N,
-              const uint32_t *list, const uint32_t *list_cnt, const int64_t *fxbuf,
-              const float *scores, Cands c)
+    k_topn_fx(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
+              uint32_t *flag, uint32_t N, const uint32_t *list, const uint32_t *list_cnt,
+              const int64_t *fxbuf, const float *scores, Cands c)
 {
     __shared__ uint64_t lk[FX_CAP];
     __shared__ uint32_t lp[FX_CAP];
@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(FX_THREADS)
         const uint32_t n = c.count(s);
         const int64_t *fx = fxbuf + c.base(s);
         const float *sc = scores + c.base(s);
-        const uint32_t want = min(N, n);
+        uint32_t want = min(N, n);
         if (want == 0) {
             if (tid == 0) top_cnt[s] = 0;
             continue;
@@ -617,7 +617,23 @@ __global__ void __launch_bounds__(FX_THREADS)
         });
         __syncthreads();
         const uint32_t M = sh_cnt;
-        if (M >= want && M <= FX_CAP) {
+        if (M < want) {
+            /* Fewer live candidates than N: the N-th exact score of the
+             * selection lies above tau + bound, so at least N candidates
+             * have exact scores >= tau + bound, and each of them whose FP32
+             * score fell below tau lies further below its exact score than
+             * the stated bound allows.  Only the live candidates are ranked
+             * and the spectrum is marked for the host. */
+            want = M;
+            if (tid == 0) {
+                flag[s] |= HFX_FLAG_SHORT;
+                sh_rem = want;
+                if (want == 0) top_cnt[s] = 0;
+            }
+            __syncthreads();
+            if (want == 0) continue;
+        }
+        if (M <= FX_CAP) {
             /* 2. the bits in which the live keys differ */
             const uint64_t k0 = lk[0];
             uint64_t d = 0;
@@ -801,7 +817,7 @@ static int tx_scratch(hfx_ctx *ctx, void *given, uint32_t S, uint64_t total, voi
 
 template <class Cands>
 static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
-                       uint32_t *d_top_cnt, uint32_t N, const uint32_t *d_flag,
+                       uint32_t *d_top_cnt, uint32_t N, uint32_t *d_flag,
                        const float *d_scores, const Cands &c, uint64_t total,
                        const hfx_spectra *sp, void *scratch, hipStream_t st)
 {
@@ -822,8 +838,8 @@ static int exact_route(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                        dim3(TN_THREADS), tn_dyn_bytes(sp->x_stride), st, x.fxbuf, x.tile_off,
                        x.tile_spec, c, *sp, d_scores, x.tau);
     hipLaunchKernelGGL(k_topn_fx<Cands>, dim3(min(S, (uint32_t)hfx_ctx_num_cus(ctx))),
-                       dim3(FX_THREADS), 0, st, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N,
-                       x.list, x.counter + 1, x.fxbuf, d_scores, c);
+                       dim3(FX_THREADS), 0, st, d_top_pos, d_top_exact, d_top_f32, d_top_cnt,
+                       d_flag, N, x.list, x.counter + 1, x.fxbuf, d_scores, c);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }

@@ -13,11 +13,15 @@ loaded in the process; every computation is a call into libhfx_hip.so.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import HfxDb, HfxModset, HfxSpectra, check, lib
+
+# per-spectrum flags of the top-N (include/hfx_hip.h)
+FLAG_EXACT = 1
+FLAG_SHORT = 2
 
 _gpu_checked = False
 
@@ -275,6 +279,9 @@ class PlainResult:
     top_cnt: np.ndarray
     flagged: np.ndarray     # spectra completed by the exact fallback
     scores: np.ndarray | None = None
+    # spectra whose FP32 scores broke the stated error bound: their top N
+    # holds only the candidates the exact route could vouch for
+    short: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
 
 
 class PlainSearch:
@@ -365,7 +372,14 @@ class PlainSearch:
 
     def flagged(self) -> np.ndarray:
         """Spectra that took the exact route in the last select()."""
-        return np.nonzero(host(self.t_flag, np.uint32, self.sp.num_spec))[0].astype(np.uint32)
+        f = host(self.t_flag, np.uint32, self.sp.num_spec)
+        return np.nonzero(f & FLAG_EXACT)[0].astype(np.uint32)
+
+    def short(self) -> np.ndarray:
+        """Spectra of the last select() whose FP32 scores broke the stated
+        error bound (fewer than min(N, window) ranked)."""
+        f = host(self.t_flag, np.uint32, self.sp.num_spec)
+        return np.nonzero(f & FLAG_SHORT)[0].astype(np.uint32)
 
     def run(self, keep_scores: bool = False, dense: bool = False) -> PlainResult:
         """window -> XCorr (FP32 sparse, or dense BF16 hi+lo) -> top-N."""
@@ -378,9 +392,9 @@ class PlainSearch:
             else:
                 self.score()
                 self.select()
-            flagged = self.flagged()
+            flagged, short = self.flagged(), self.short()
         else:
-            flagged = np.zeros(0, np.uint32)
+            flagged = short = np.zeros(0, np.uint32)
             self.t_cnt.zero()
         sync()
         return PlainResult(
@@ -393,6 +407,7 @@ class PlainSearch:
             top_cnt=host(self.t_cnt, np.uint32, S).copy(),
             flagged=flagged,
             scores=host(self.t_scores, np.float32, self.total).copy() if keep_scores else None,
+            short=short,
         )
 
 
@@ -536,13 +551,15 @@ class BatchSearch:
                       "d2h")
             return out
 
+        flags = get(4, np.uint32, S)
         r = PlainResult(
             begin=get(7, np.uint32, S), num=get(8, np.uint32, S), off=get(6, np.uint64, S + 1),
             top_pos=get(0, np.uint32, S * N).reshape(S, N),
             top_exact=get(1, np.float64, S * N).reshape(S, N),
             top_f32=get(2, np.float32, S * N).reshape(S, N), top_cnt=get(3, np.uint32, S),
-            flagged=np.nonzero(get(4, np.uint32, S))[0].astype(np.uint32),
-            scores=get(5, np.float32, inf["scores"]) if keep_scores else None)
+            flagged=np.nonzero(flags & FLAG_EXACT)[0].astype(np.uint32),
+            scores=get(5, np.float32, inf["scores"]) if keep_scores else None,
+            short=np.nonzero(flags & FLAG_SHORT)[0].astype(np.uint32))
         self.stream.sync()
         return r
 
@@ -607,6 +624,7 @@ class ModResult:
     flagged: np.ndarray
     stages: dict              # every intermediate array, for parity checks
     scores: np.ndarray | None = None
+    short: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
 
 
 class ModSearch:
@@ -674,7 +692,7 @@ class ModSearch:
         t_scores = empty(4 * max(nmp, 1))
         t_pos, t_ex = empty(4 * S * N), empty(8 * S * N)
         t_f32, t_cnt, t_flag = empty(4 * S * N), empty(4 * S), empty(4 * S)
-        flagged = np.zeros(0, np.uint32)
+        flagged = short = np.zeros(0, np.uint32)
         if nmp:
             check(L.hfx_xcorr_sparse_mod(self.ctx.h, t_scores.p, t_mpoff.p, nmp,
                                          C.byref(self.modset), C.byref(db.desc),
@@ -684,7 +702,8 @@ class ModSearch:
                                  C.byref(db.desc), C.byref(self.sp.desc), None), "hfx_topn_mod")
             # flagged spectra were re-ranked exactly inside hfx_topn_mod
             flags = host(t_flag, np.uint32, S)
-            flagged = np.nonzero(flags)[0].astype(np.uint32)
+            flagged = np.nonzero(flags & FLAG_EXACT)[0].astype(np.uint32)
+            short = np.nonzero(flags & FLAG_SHORT)[0].astype(np.uint32)
         else:
             t_cnt.zero()
         sync()
@@ -697,7 +716,7 @@ class ModSearch:
             top_exact=host(t_ex, np.float64, S * N).reshape(S, N),
             top_f32=host(t_f32, np.float32, S * N).reshape(S, N),
             top_cnt=host(t_cnt, np.uint32, S), flagged=flagged, stages=st,
-            scores=host(t_scores, np.float32, nmp) if keep_scores else None)
+            scores=host(t_scores, np.float32, nmp) if keep_scores else None, short=short)
 
 
     def exact(self, spectra: np.ndarray):

@@ -314,8 +314,14 @@ int hfx_xcorr_sparse_noalloc(float *d_scores, const uint64_t *d_off, const uint3
  * candidate that could reach the N-th exact score is among them); a spectrum
  * with more such candidates than the re-score set holds (exact ties) takes the
  * exact route on the device: all its candidates scored exactly and the top N
- * selected on the exact scores; d_flag[s] = 1 marks those spectra.  `total`
- * = sum_s d_num[s].  N <= 64, N + margin <= 128. */
+ * selected on the exact scores; d_flag[s] has HFX_FLAG_EXACT set for those
+ * spectra.  The exact route keeps every candidate whose FP32 score lies at
+ * most the error bound below the N-th exact score of the re-score set; if
+ * fewer than min(N, d_num[s]) remain, some FP32 score broke the bound: only
+ * those are ranked, d_top_cnt[s] counts them and HFX_FLAG_SHORT is set.
+ * `total` = sum_s d_num[s].  N <= 64, N + margin <= 128. */
+#define HFX_FLAG_EXACT 1u
+#define HFX_FLAG_SHORT 2u
 int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
              uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
              const float *d_scores, const uint64_t *d_off, uint64_t total,

@@ -512,3 +512,41 @@ def test_mvm_ff_on_a_handle_stream(env):
         bound = (n // 64 + 8) * 2.0**-24 * (np.abs(A).astype(np.float64) @ np.abs(x)) + 1e-30
         assert np.all(np.abs(y - exact) <= bound), (m, n)
         check(L.hfx_blas_destroy(h), "hfx_blas_destroy")
+
+
+def test_scores_that_break_the_bound_are_reported_short(env, orc):
+    """hfx_topn_rel given FP32 scores that are not within its stated bound of
+    the exact ones (every score 0, bound factors 0; eps 10 for windows of
+    hundreds of candidates, top 1): the re-score band then holds the whole
+    window, big windows take the exact route, and where the selected
+    candidate's exact score is positive no candidate's FP32 score reaches
+    tau.  Those spectra are flagged short with top_cnt 0; every other
+    spectrum gets the oracle's top 1.  A following select on real scores
+    clears the mark."""
+    pl, w, dsp = env["pl"], env["w"], env["dsp"]
+    S = dsp.num_spec
+    srch = pl.PlainSearch(env["ctx"], env["db"], dsp, eps=10.0, topn=1)
+    total = srch.window()
+    num = pl.host(srch.t_num, np.uint32, S)
+    srch.t_scores.upload(np.zeros(total, np.float32))
+    srch.select_rel(0.0, 0.0)
+    pl.sync()
+    flags = pl.host(srch.t_flag, np.uint32, S)
+    cnt = pl.host(srch.t_cnt, np.uint32, S)
+    pos = pl.host(srch.t_pos, np.uint32, S)
+    ex = pl.host(srch.t_exact, np.float64, S)
+    short = (flags & pl.FLAG_SHORT) != 0
+    print(f"{int((num > 257).sum())} windows above 257 candidates, {int(short.sum())} short")
+    assert short.any() and (flags[short] & pl.FLAG_EXACT).all()
+    assert (cnt[short] == 0).all() and (cnt[~short] == np.minimum(1, num[~short])).all()
+    assert (srch.short() == np.nonzero(short)[0]).all()
+    idx, _ = orc.sort_idx(w["r"])
+    ref = orc.search_plain(env["db"].packed_ions(), w["r"], w["tc"], w["tn"], w["mass"], idx,
+                           dsp.x(), dsp.lengths(), dsp.zmax, dsp.mass, 10.0, 1)
+    for s in np.nonzero(~short)[0]:
+        if cnt[s]:
+            assert pos[s] == ref["top_pos"][s, 0] and ex[s] == orc.fx_to_double(ref["top_fx"][s, 0])
+    srch.score()
+    srch.select()
+    pl.sync()
+    assert len(srch.short()) == 0
