@@ -166,7 +166,16 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _P, _U32, _U32, _F, _F, _P, _P, _U64, _P, _P, C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_topn_rel_q": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, _U32, _U32, _F, _F, _F, _P, _P, _U64, _P, _P,
+         C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
+    ),
     "hfx_xcorr_dense_bf16x2": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
+    ),
+    "hfx_xcorr_dense_i8x3": (
         _I,
         [_P, _P, _P, _P, _P, _P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
     ),

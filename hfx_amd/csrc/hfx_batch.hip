@@ -509,7 +509,7 @@ __global__ void k_ord_rank(uint32_t *order, const uint32_t *tmp, const uint64_t 
  * about 1% more rows than one programme over the whole order). */
 #define CUT_SEG 32
 #define CUT_WG 64
-#define CUT_NS 8
+template <int CUT_NS>
 __global__ void __launch_bounds__(CUT_WG) k_cut_seg(uint32_t *seg_cuts, uint32_t *seg_n,
                                                    const uint32_t *order, const uint32_t *begin,
                                                    const uint32_t *num, const uint32_t *zmax,
@@ -630,7 +630,7 @@ struct hfx_batch {
     uint32_t ord_sh, ord_nbz, ord_nb;
     uint32_t *ord_cnt, *ord_tmp;
     uint64_t *ord_off;
-    uint32_t precision; /* HFX_BATCH_FP32_SPARSE or HFX_BATCH_BF16X2_DENSE */
+    uint32_t precision; /* HFX_BATCH_FP32_SPARSE, _BF16X2_DENSE or _I8X3_DENSE */
     uint32_t CH;        /* dense: candidates per work item */
     uint8_t *dn_scratch; /* dense: hi/lo spectra, blocks (allocated with the dense form) */
     uint32_t *dn_bstart; /* dense: [S + 1] block starts in order (S past the last block) */
@@ -701,7 +701,22 @@ extern "C" int hfx_batch_use_index(hfx_batch *b, const hfx_pepix *ix)
  * -> (G, C, threads). */
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t>> g_tune;
-static std::map<std::tuple<int, int, uint32_t>, uint32_t> g_tune_dense;
+static std::map<std::tuple<int, int, uint32_t>, uint32_t> g_tune_dense, g_tune_dense8;
+
+/* the two dense forms share the ladders, block cut and work items */
+static inline bool bt_is_dense(uint32_t precision)
+{
+    return precision == HFX_BATCH_BF16X2_DENSE || precision == HFX_BATCH_I8X3_DENSE;
+}
+static std::map<std::tuple<int, int, uint32_t>, uint32_t> &bt_tune_dense(uint32_t precision)
+{
+    return precision == HFX_BATCH_I8X3_DENSE ? g_tune_dense8 : g_tune_dense;
+}
+/* the dense forms' per-batch scratch: the larger of the two */
+static size_t bt_dense_scratch(uint32_t S)
+{
+    return std::max(hfx_dense_scratch_bytes(S), hfx_dense8_scratch_bytes(S));
+}
 
 static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
 {
@@ -712,7 +727,7 @@ static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
 /* the form of the peptide index the current precision reads */
 static uint32_t bt_form(const hfx_batch *b)
 {
-    return b->precision == HFX_BATCH_BF16X2_DENSE ? HFX_PEPIX_LADDER : HFX_PEPIX_IONS;
+    return bt_is_dense(b->precision) ? HFX_PEPIX_LADDER : HFX_PEPIX_IONS;
 }
 
 /* makes sure the index holds the current precision's form: the caller's,
@@ -758,9 +773,9 @@ static uint64_t bt_bytes(uint32_t S, uint64_t total, uint32_t N, uint32_t num_pe
                  al(S, 4) * 2                                            /* top-N */
                  + al(hfx_topn_scratch_bytes(S, total), 1)               /* exact route */
                  + al(bmax, 4) * 3 + al(bmax + 1, 8);                    /* sparse blocks */
-    if (precision == HFX_BATCH_BF16X2_DENSE) {
+    if (bt_is_dense(precision)) {
         const uint64_t nseg = (S + CUT_SEG - 1) / CUT_SEG;
-        b += al(hfx_dense_scratch_bytes(S), 1) + al((uint64_t)S + 1, 4) + al(nseg * CUT_SEG, 4) +
+        b += al(bt_dense_scratch(S), 1) + al((uint64_t)S + 1, 4) + al(nseg * CUT_SEG, 4) +
              al(nseg, 4) + al(nseg + 1, 8);
     }
     return b;
@@ -770,7 +785,7 @@ extern "C" int hfx_batch_required_bytes(uint32_t S, uint64_t total, uint32_t N, 
                                         uint32_t precision, uint64_t *bytes)
 {
     HFX_REQUIRE(bytes, HFX_ERR_ARG, "hfx_batch_required_bytes: null out");
-    HFX_REQUIRE(precision <= HFX_BATCH_BF16X2_DENSE, HFX_ERR_ARG,
+    HFX_REQUIRE(precision <= HFX_BATCH_I8X3_DENSE, HFX_ERR_ARG,
                 "hfx_batch_required_bytes: unknown precision %u", precision);
     *bytes = bt_bytes(S, total, N, num_pep, precision);
     return HFX_OK;
@@ -786,7 +801,7 @@ extern "C" int hfx_batch_split(const uint64_t *h_off, uint32_t S, uint32_t N, ui
                                uint32_t cap, uint32_t *num_batches)
 {
     HFX_REQUIRE(h_off && num_batches, HFX_ERR_ARG, "hfx_batch_split: null argument");
-    HFX_REQUIRE(precision <= HFX_BATCH_BF16X2_DENSE, HFX_ERR_ARG,
+    HFX_REQUIRE(precision <= HFX_BATCH_I8X3_DENSE, HFX_ERR_ARG,
                 "hfx_batch_split: unknown precision %u", precision);
     uint32_t nb = 0, a = 0;
     while (a < S) {
@@ -920,7 +935,7 @@ static int bt_dense_alloc(hfx_batch *b)
     if (b->dn_scratch) return HFX_OK;
     int rc;
     b->nseg = (b->S + CUT_SEG - 1) / CUT_SEG;
-    if ((rc = dalloc(&b->dn_scratch, hfx_dense_scratch_bytes(b->S))) ||
+    if ((rc = dalloc(&b->dn_scratch, bt_dense_scratch(b->S))) ||
         (rc = dalloc(&b->dn_bstart, (size_t)b->S + 1)) ||
         (rc = dalloc(&b->cut_seg, (size_t)b->nseg * CUT_SEG)) || (rc = dalloc(&b->cut_n, b->nseg)) ||
         (rc = dalloc(&b->cut_off, (size_t)b->nseg + 1)))
@@ -957,19 +972,26 @@ static int bt_order(hfx_batch *b, hipStream_t st)
     return HFX_OK;
 }
 
-/* dense blocks (segment DP) and the hi/lo rows of the spectra */
+/* dense blocks (segment DP; eight spectra a block in the BF16 form, sixteen
+ * in the integer form) and the spectra's hi/lo rows or digits */
 static int bt_dense_prep(hfx_batch *b, hipStream_t st)
 {
     const uint32_t S = b->S;
-    hipLaunchKernelGGL(k_cut_seg, dim3((b->nseg + CUT_WG - 1) / CUT_WG), dim3(CUT_WG), 0, st,
-                       b->cut_seg, b->cut_n, b->order, b->begin, b->num, b->sp.d_zmax, S);
+    const bool i8 = b->precision == HFX_BATCH_I8X3_DENSE;
+    if (i8)
+        hipLaunchKernelGGL(k_cut_seg<16>, dim3((b->nseg + CUT_WG - 1) / CUT_WG), dim3(CUT_WG), 0,
+                           st, b->cut_seg, b->cut_n, b->order, b->begin, b->num, b->sp.d_zmax, S);
+    else
+        hipLaunchKernelGGL(k_cut_seg<8>, dim3((b->nseg + CUT_WG - 1) / CUT_WG), dim3(CUT_WG), 0,
+                           st, b->cut_seg, b->cut_n, b->order, b->begin, b->num, b->sp.d_zmax, S);
     int rc;
     if ((rc = hfx_scan_offsets(b->ctx, b->cut_off, b->cut_n, b->nseg, (void *)st))) return rc;
     const uint32_t nt = std::max(b->nseg * CUT_SEG, S + 1);
     hipLaunchKernelGGL(k_cut_place, dim3((nt + 255) / 256), dim3(256), 0, st, b->dn_bstart,
                        b->cut_seg, b->cut_n, b->cut_off, b->nseg, S, S);
     HFX_CHECK_LAUNCH();
-    return hfx_dense_split(&b->sp, b->dn_scratch, (void *)st);
+    return i8 ? hfx_dense8_split(&b->sp, b->dn_scratch, (void *)st)
+              : hfx_dense_split(&b->sp, b->dn_scratch, (void *)st);
 }
 
 /* blocks of G spectra in order + tiles for the current (G, C) */
@@ -986,7 +1008,7 @@ static int bt_prep(hfx_batch *b, hipStream_t st)
 {
     int rc;
     if ((rc = bt_order(b, st))) return rc;
-    if (b->precision == HFX_BATCH_BF16X2_DENSE) return bt_dense_prep(b, st);
+    if (bt_is_dense(b->precision)) return bt_dense_prep(b, st);
     return bt_tiles(b, st);
 }
 
@@ -1025,6 +1047,12 @@ static int bt_score(hfx_batch *b, hipStream_t st, void **ev)
 /* ev (optional): events recorded around the scoring kernel itself */
 static int bt_scores(hfx_batch *b, hipStream_t st, void **ev = nullptr)
 {
+    if (b->precision == HFX_BATCH_I8X3_DENSE) {
+        if (b->total == 0) return HFX_OK;
+        return hfx_dense8_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order,
+                                 b->dn_bstart, b->S, &b->ix->lad, &b->sp, b->CH, b->dn_scratch, 1,
+                                 (void *)st, ev);
+    }
     if (b->precision == HFX_BATCH_BF16X2_DENSE) {
         if (b->total == 0) return HFX_OK;
         return hfx_dense_scores(b->ctx, b->scores, b->score_off, b->begin, b->num, b->order,
@@ -1039,13 +1067,20 @@ static int bt_scores(hfx_batch *b, hipStream_t st, void **ev = nullptr)
  * one MFMA, one per k-step after it) and the hi + lo sum, each allowed 2 ulp */
 #define BT_REL_DENSE (1.52587890625e-05f + 210.0f * 5.9604644775390625e-08f)
 #define BT_REL_F32 (80.0f * 5.9604644775390625e-08f)
+/* integer form: |score - exact| <= max|x| / 8355711 * 50 len (the rounding of
+ * x 2^k to integers) + 2^-24 * 50 sum|x| (the one FP32 rounding) */
+#define BT_REL_I8 5.9604644775390625e-08f
+#define BT_RELQ_I8 (1.0f / 8355711.0f)
 
 static int bt_topn(hfx_batch *b, hipStream_t st)
 {
-    const float rel0 = b->precision == HFX_BATCH_BF16X2_DENSE ? BT_REL_DENSE : BT_REL_F32;
+    const bool i8 = b->precision == HFX_BATCH_I8X3_DENSE;
+    const float rel0 = i8 ? BT_REL_I8 : b->precision == HFX_BATCH_BF16X2_DENSE ? BT_REL_DENSE
+                                                                             : BT_REL_F32;
     return hfx_topn_device(b->ctx, b->top_pos, b->top_exact, b->top_f32, b->top_cnt, b->flag,
                            b->N, b->margin, b->scores, b->score_off, b->total, b->begin, b->num,
-                           &b->db, &b->sp, rel0, 0.0f, b->tx_scratch, (void *)st);
+                           &b->db, &b->sp, rel0, 0.0f, i8 ? BT_RELQ_I8 : 0.0f, b->tx_scratch,
+                           (void *)st);
 }
 
 extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
@@ -1062,14 +1097,21 @@ extern "C" int hfx_batch_run(hfx_batch *b, void *stream)
 extern "C" int hfx_batch_set_precision(hfx_batch *b, uint32_t precision)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_precision: batch not planned");
-    HFX_REQUIRE(precision == HFX_BATCH_FP32_SPARSE || precision == HFX_BATCH_BF16X2_DENSE,
-                HFX_ERR_ARG, "hfx_batch_set_precision: unknown precision %u", precision);
-    HFX_REQUIRE(precision != HFX_BATCH_BF16X2_DENSE || b->sp.x_stride <= hfx_dense_max_len(),
+    HFX_REQUIRE(precision == HFX_BATCH_FP32_SPARSE || bt_is_dense(precision), HFX_ERR_ARG,
+                "hfx_batch_set_precision: unknown precision %u", precision);
+    HFX_REQUIRE(!bt_is_dense(precision) || b->sp.x_stride <= hfx_dense_max_len(),
                 HFX_ERR_ARG,
                 "hfx_batch_set_precision: the dense form takes spectra of up to %u bins "
                 "(x_stride %u)", hfx_dense_max_len(), b->sp.x_stride);
     int rc;
-    if (precision == HFX_BATCH_BF16X2_DENSE && (rc = bt_dense_alloc(b))) return rc;
+    if (bt_is_dense(precision) && (rc = bt_dense_alloc(b))) return rc;
+    if (bt_is_dense(precision) && precision != b->precision) {
+        /* the tuned chunk of this form for batches of this shape, if any */
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto &m = bt_tune_dense(precision);
+        auto jt = m.find(tune_key(b));
+        b->CH = jt != m.end() ? jt->second : 256;
+    }
     if (precision != b->precision && b->exec) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
@@ -1142,7 +1184,7 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     };
     float best = 1e30f;
     uint32_t bG = b->G, bC = b->C, bT = b->threads, bCH = b->CH;
-    if (b->precision == HFX_BATCH_BF16X2_DENSE) {
+    if (bt_is_dense(b->precision)) {
         const uint32_t CHs[5] = {256, 512, 1024, 2048, 4096};
         for (uint32_t CH : CHs) {
             b->CH = CH;
@@ -1171,7 +1213,7 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     b->CH = bCH;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        if (b->precision == HFX_BATCH_BF16X2_DENSE) g_tune_dense[tune_key(b)] = bCH;
+        if (bt_is_dense(b->precision)) bt_tune_dense(b->precision)[tune_key(b)] = bCH;
         else g_tune[tune_key(b)] = std::make_tuple(bG, bC, bT);
     }
     if (b->exec) {
@@ -1185,7 +1227,7 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     if ((rc = hfx_batch_run(b, stream))) return rc;
     if (out_cfg) {
         out_cfg[0] = bG; out_cfg[1] = bC; out_cfg[2] = bT;
-        if (b->precision == HFX_BATCH_BF16X2_DENSE) { out_cfg[0] = 0; out_cfg[1] = bCH; out_cfg[2] = 1024; }
+        if (bt_is_dense(b->precision)) { out_cfg[0] = 0; out_cfg[1] = bCH; out_cfg[2] = 1024; }
     }
     return HFX_OK;
 }
@@ -1210,7 +1252,7 @@ extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
     HFX_CHECK_HIP(hipEventRecord(ev[1], st));
     if (!rc) rc = bt_prep(b, st);
     HFX_CHECK_HIP(hipEventRecord(ev[2], st));
-    const bool scored = b->total != 0 && (b->precision == HFX_BATCH_BF16X2_DENSE || b->B != 0);
+    const bool scored = b->total != 0 && (bt_is_dense(b->precision) || b->B != 0);
     if (!rc) rc = bt_scores(b, st, kev);
     HFX_CHECK_HIP(hipEventRecord(ev[3], st));
     if (!rc) rc = bt_topn(b, st);
@@ -1225,10 +1267,12 @@ extern "C" int hfx_batch_time(hfx_batch *b, void *stream, float *ms)
 extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads)
 {
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_config: batch not planned");
-    if (b->precision == HFX_BATCH_BF16X2_DENSE) {
-        /* dense form: C = candidates per work item (G, threads fixed) */
-        HFX_REQUIRE(C >= 16 && C % 16 == 0 && C <= 8192, HFX_ERR_ARG,
-                    "hfx_batch_set_config: dense chunk must be a multiple of 16 in 16..8192");
+    if (bt_is_dense(b->precision)) {
+        /* dense forms: C = candidates per work item (G, threads fixed) */
+        const uint32_t step = b->precision == HFX_BATCH_I8X3_DENSE ? 64u : 16u;
+        HFX_REQUIRE(C >= step && C % step == 0 && C <= 8192, HFX_ERR_ARG,
+                    "hfx_batch_set_config: dense chunk must be a multiple of %u in %u..8192",
+                    step, step);
         b->CH = C;
         return HFX_OK;
     }

@@ -99,14 +99,16 @@ int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream);
 /* device scratch the top-N exact route needs for S spectra, total scores */
 size_t hfx_topn_scratch_bytes(uint32_t S, uint64_t total);
 /* hfx_topn over the unmodified candidates (batched path; no host sync), with
- * the error bound (rel0 + rel1 * fragment charges) * 50 * sum|x| + 1e-6.
+ * the error bound (rel0 + rel1 * fragment charges) * 50 * sum|x|
+ * + relq * max|x| * 50 * len + 1e-6.
  * scratch: hfx_topn_scratch_bytes(S, total) bytes of device memory, or null
  * for the context's own slot */
 int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                     uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
                     const float *d_scores, const uint64_t *d_off, uint64_t total,
                     const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
-                    const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream);
+                    const hfx_spectra *sp, float rel0, float rel1, float relq, void *scratch,
+                    void *stream);
 /* b ladders of the peptides in residual-sorted positions [q0, q0 + nq), for
  * the dense path.  Position k < tn - tc of sorted peptide q is
  * pos[off[q - q0] + k] = {b_k, r - b_k, q}, with b_k the sequential FP32 sum
@@ -140,6 +142,17 @@ int hfx_dense_ladder_build(hfx_ctx *ctx, const hfx_db *db, uint32_t q0, uint32_t
 size_t hfx_dense_scratch_bytes(uint32_t S);
 uint32_t hfx_dense_max_len(void);
 int hfx_dense_split(const hfx_spectra *sp, void *scratch, void *stream);
+/* the integer form (int8 MFMA over three base-256 digits of each spectrum,
+ * blocks of at most sixteen, chunks a multiple of 64): same arguments;
+ * scratch of hfx_dense8_scratch_bytes(S), digits written by
+ * hfx_dense8_split */
+size_t hfx_dense8_scratch_bytes(uint32_t S);
+int hfx_dense8_split(const hfx_spectra *sp, void *scratch, void *stream);
+int hfx_dense8_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                      const uint32_t *d_begin, const uint32_t *d_num, const uint32_t *d_order,
+                      const uint32_t *d_bstart, uint32_t nblocks, const hfx_dn_ladder *lad,
+                      const hfx_spectra *sp, uint32_t CH, void *scratch, int presplit,
+                      void *stream, void **ev);
 /* ev (optional): two hipEvent_t recorded right before and after the
  * contraction kernel itself */
 int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const uint32_t *d_begin,

@@ -193,18 +193,20 @@ __device__ __forceinline__ uint32_t dn_order(const uint32_t *order, uint32_t i)
 }
 
 /* spectra (positions in order) of block b: [bstart[b], bstart[b+1]), at
- * most DN_NS of them, or eight at a time without a table */
-__device__ __forceinline__ uint32_t dn_bfirst(const uint32_t *bstart, uint32_t b)
+ * most NS of them, or NS at a time without a table */
+__device__ __forceinline__ uint32_t dn_bfirst(const uint32_t *bstart, uint32_t b, uint32_t NS)
 {
-    return bstart ? bstart[b] : b * DN_NS;
+    return bstart ? bstart[b] : b * NS;
 }
-__device__ __forceinline__ uint32_t dn_bend(const uint32_t *bstart, uint32_t b, uint32_t S)
+__device__ __forceinline__ uint32_t dn_bend(const uint32_t *bstart, uint32_t b, uint32_t S,
+                                            uint32_t NS)
 {
-    return bstart ? bstart[b + 1] : min(S, b * DN_NS + DN_NS);
+    return bstart ? bstart[b + 1] : min(S, b * NS + NS);
 }
 
 /* block b: union window, fragment charge levels present (bit z), longest
  * row kb (multiple of 128), work items */
+template <uint32_t NS>
 __global__ void k_dn_blocks(uint32_t *ubeg, uint32_t *uend, uint32_t *binfo, uint32_t *ntiles,
                             const uint32_t *order, const uint32_t *bstart, const uint32_t *begin,
                             const uint32_t *num, hfx_spectra sp, uint32_t B, uint32_t CH)
@@ -212,8 +214,8 @@ __global__ void k_dn_blocks(uint32_t *ubeg, uint32_t *uend, uint32_t *binfo, uin
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
     uint32_t lo = 0xffffffffu, hi = 0, mask = 0, kmax = 0;
-    const uint32_t i0 = dn_bfirst(bstart, b), i1 = dn_bend(bstart, b, sp.num_spec);
-    for (uint32_t g = 0; g < DN_NS; ++g) {
+    const uint32_t i0 = dn_bfirst(bstart, b, NS), i1 = dn_bend(bstart, b, sp.num_spec, NS);
+    for (uint32_t g = 0; g < NS; ++g) {
         const uint32_t i = i0 + g;
         if (i >= i1) break;
         const uint32_t s = dn_order(order, i);
@@ -420,10 +422,10 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
         const uint32_t kb = info & 0xffffu, mask = info >> 16;
         if (b != cur) {
             if (tid < DN_NS) {
-                const uint32_t i = dn_bfirst(a.bstart, b) + tid;
+                const uint32_t i = dn_bfirst(a.bstart, b, DN_NS) + tid;
                 uint32_t s = DN_NONE, be = 0, en = 0, z = 0;
                 uint64_t of = 0;
-                if (i < dn_bend(a.bstart, b, a.sp.num_spec)) {
+                if (i < dn_bend(a.bstart, b, a.sp.num_spec, DN_NS)) {
                     s = dn_order(a.order, i);
                     be = a.begin[s];
                     en = be + a.num[s];
@@ -663,6 +665,367 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
 }
 
 /* ------------------------------------------------------------------ */
+/* Integer form: int8 MFMA over three limbs of each spectrum              */
+/* ------------------------------------------------------------------ */
+/*
+ * Same candidates, maps and peaks as above; the contraction is exact:
+ *
+ *   A [16 candidates x 64 bins]   weights 0/10/25/50 as int8
+ *   B [64 bins x 16 spectra]      one base-256 digit of each X_s
+ *   C [16 x 16] int32             sum of w * digit, exact
+ *
+ * with v_mfma_i32_16x16x64_i8, once per digit for each A fragment.  Spectrum
+ * s is scaled by the power of two 2^k_s (the largest with
+ * max|x_s| 2^k_s <= 8355711) and rounded to integers X = rint(x 2^k_s),
+ * written as balanced base-256 digits X = d2 2^16 + d1 2^8 + d0, each in
+ * [-128, 127] (8355711 = 0x7F7F7F is the largest |X| they reach).  Then
+ * score = (a2 2^16 + a1 2^8 + a0) 2^-k_s is formed exactly in FP64 and rounded
+ * once to FP32:
+ *   |score - exact| <= 2^-(k_s+1) sum_b w_b + 2^-24 |score|
+ *                   <= max|x_s| / 8355711 * 50 len_s + 2^-24 * 50 sum|x_s|
+ * (2^-k_s < 2 max|x_s| / 8355711).
+ *
+ * Sixteen spectra a block (the MFMA's N), 64 candidates a tile (four 16-row
+ * blocks of A) in one map; one persistent workgroup of twelve waves per CU
+ * (three a SIMD, so that each holds up to 168 registers):
+ *   - the tile's ladder positions go to the waves 64 at a time (wave w takes
+ *     positions 64 w + lane, 64 (w + 12) + lane, ...);
+ *   - wave w contracts k-steps w % 6, w % 6 + 6, ... (64 bins each) of row
+ *     blocks 2 (w / 6) and 2 (w / 6) + 1; the B fragments of its k-steps
+ *     (three digits each) stay in its registers for the whole block, loaded
+ *     once from HBM, so B never passes through LDS; lane (row, kg) reads map words 8 ks + kg and
+ *     8 ks + 4 + kg of its row (clearing them), conflict-free, and one v_perm
+ *     per four bins turns codes into weight bytes;
+ *   - each wave folds its three digit sums into one int64 per output
+ *     (a2 2^16 + a1 2^8 + a0, exact) and adds it into LDS (ds_add_u64, exact
+ *     in any order); after a barrier the threads form the tile's 64 x 16
+ *     scores.
+ * B's K order (the split's layout): byte 16 kg + j of a k-step's 64 is bin
+ * (j < 8 ? 0 : 32) + 8 kg + {0,2,4,6,1,3,5,7}[j % 8].
+ */
+#define D8_NS 16
+#define D8_RB 4                   /* 16-row blocks of A a tile */
+#define D8_ROWS (16 * D8_RB)
+#define D8_WAVES 12               /* three a SIMD: 168 registers each */
+#define D8_THREADS (D8_WAVES * HFX_WAVE)
+#define D8_MAXK DN_MAXK           /* bins of a row: 34 k-steps of 64 */
+#define D8_NKS (D8_MAXK / 64)
+#define D8_KG (D8_WAVES / 2)      /* waves splitting the k-steps of a row block */
+#define D8_KPW ((D8_NKS + D8_KG - 1) / D8_KG) /* k-steps a wave holds B for */
+#define D8_MAP_BYTES (D8_ROWS * DN_MSTR * 4)
+/* int64 sums [row block][16 rows][32 slots]: column c of row r in slot
+ * c + 16 ((r >> 2) & 1), so that the atomic adds (lanes of rows r and r + 4)
+ * and the epilogue's reads spread over all banks */
+#define D8_COMB_BYTES (D8_RB * 16 * 32 * 8)
+#define D8_LDS_BYTES (D8_MAP_BYTES + D8_COMB_BYTES)
+#define D8_FIXED_LDS (D8_NS * 4 * 5 + D8_NS * 8 + 2 * 4 + 16)
+#define D8_XMAX 8355711.0f
+static_assert(D8_LDS_BYTES + D8_FIXED_LDS <= 160 * 1024, "integer dense kernel LDS exceeds a CU");
+static_assert(D8_MAXK % 128 == 0, "rows of whole k-steps");
+static_assert(D8_WAVES % 2 == 0 && D8_RB == 4, "two row blocks for each half of the waves");
+
+typedef int32_t d8_i32x4 __attribute__((ext_vector_type(4)));
+
+/* bin (0..63 within a k-step) of MFMA K index kk (0..63) */
+__host__ __device__ inline uint32_t d8_bin_of(uint32_t kk)
+{
+    const uint32_t kg = kk >> 4, j = kk & 15u, t = j & 7u;
+    return (j < 8u ? 0u : 32u) + 8u * kg + (t < 4u ? 2u * t : 2u * (t - 4u) + 1u);
+}
+
+/* xq[s][d][K order] = digit d of X, qs[s] = 2^-k_s */
+__global__ void __launch_bounds__(256) k_d8_split(int8_t *xq, float *qs, hfx_spectra sp)
+{
+    __shared__ float red[256 / HFX_WAVE];
+    const uint32_t s = blockIdx.x;
+    const uint32_t len = min(sp.d_len[s], min(sp.x_stride, (uint32_t)D8_MAXK));
+    const float *x = sp.d_x + (size_t)s * sp.x_stride;
+    float m = 0.0f;
+    for (uint32_t i = threadIdx.x; i < len; i += 256) m = fmaxf(m, fabsf(x[i]));
+    for (uint32_t o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (hfx_lane() == 0) red[threadIdx.x / HFX_WAVE] = m;
+    __syncthreads();
+    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    /* k: m 2^k in [2^22, 2^23) from m's exponent, one less if that passes
+     * 8355711; at most 126 (a spectrum of denormals keeps fewer bits) */
+    int k = 0;
+    if (m > 0.0f) {
+        const int e = (int)((__float_as_uint(m) >> 23) & 255u) - 127;
+        k = 22 - e;
+        if (k > 126) k = 126;
+        else if (ldexpf(m, k) > D8_XMAX) --k;
+    }
+    const float sc = ldexpf(1.0f, k);
+    if (threadIdx.x == 0) qs[s] = ldexpf(1.0f, -k);
+    int8_t *o = xq + (size_t)s * 3 * D8_MAXK;
+    for (uint32_t t = threadIdx.x; t < D8_MAXK / 4; t += 256) {
+        uint32_t w[3] = {0u, 0u, 0u};
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            const uint32_t pos = 4 * t + i;
+            const uint32_t bin = (pos & ~63u) + d8_bin_of(pos & 63u);
+            const float v = bin < len ? x[bin] : 0.0f;
+            const int32_t X = (int32_t)rintf(v * sc);
+            const int32_t d0 = ((X + 128) & 255) - 128;
+            const int32_t X1 = (X - d0) >> 8;
+            const int32_t d1 = ((X1 + 128) & 255) - 128;
+            const int32_t d2 = (X1 - d1) >> 8;
+            w[0] |= (uint32_t)(d0 & 255) << (8 * i);
+            w[1] |= (uint32_t)(d1 & 255) << (8 * i);
+            w[2] |= (uint32_t)(d2 & 255) << (8 * i);
+        }
+#pragma unroll
+        for (uint32_t d = 0; d < 3; ++d) ((uint32_t *)(o + d * D8_MAXK))[t] = w[d];
+    }
+}
+
+/* four unary codes (one a byte, low nibble) to their weight bytes 0/10/25/50:
+ * code c selects byte c of the pool 00 0A -- 19 | -- -- -- 32 */
+__device__ __forceinline__ uint32_t d8_wbytes(uint32_t codes)
+{
+    return __builtin_amdgcn_perm(0x32000000u, 0x19000A00u, codes);
+}
+
+struct D8Pipe {
+    hfx_dn_pos pos; /* position 64 wave + lane of the tile */
+    uint64_t rng;   /* lanes 0, 1: the range of the tile after */
+    uint64_t L;     /* first position */
+    uint32_t np;    /* number of positions */
+    float top;      /* lane < 32: top of candidate lane */
+};
+
+struct D8Args {
+    float *scores;
+    const uint64_t *score_off;
+    const uint32_t *begin, *num, *order;
+    const uint32_t *bstart; /* [B + 1] block starts in order, or null: sixteen a block */
+    uint32_t B, CH;
+    const uint32_t *ubeg, *uend, *binfo;
+    const uint64_t *tile_off;
+    const int8_t *xq;
+    const float *qs;
+    uint32_t *counter;
+    float *dummy;
+    hfx_dn_ladder lad;
+    hfx_spectra sp;
+};
+
+__global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
+{
+    extern __shared__ __align__(16) unsigned char d8_lds[];
+    uint32_t *map = (uint32_t *)d8_lds;
+    uint64_t *comb = (uint64_t *)(d8_lds + D8_MAP_BYTES);
+    __shared__ uint32_t m_s[D8_NS], m_beg[D8_NS], m_end[D8_NS], m_z[D8_NS];
+    __shared__ float m_q[D8_NS];
+    __shared__ uint64_t m_off[D8_NS];
+    __shared__ uint32_t sh_t[2];
+
+    const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
+    for (uint32_t i = tid; i < D8_ROWS * DN_MSTR; i += D8_THREADS) map[i] = 0;
+    for (uint32_t i = tid; i < D8_COMB_BYTES / 8; i += D8_THREADS) comb[i] = 0;
+    const uint64_t ntiles = dn_sload(a.tile_off, a.B);
+    const uint32_t col = lane & 15u, kg = lane >> 4;
+    const uint32_t kw = wave % D8_KG, rg = wave / D8_KG;
+    /* B fragments of k-steps kw + 8 j, three digits each */
+    d8_i32x4 bq[D8_KPW][3];
+    uint32_t cur = DN_NONE;
+    if (tid == 0) sh_t[0] = atomicAdd(a.counter, 1u);
+    uint32_t next = 0;
+    for (uint32_t slot = 0;; slot ^= 1u) {
+        __syncthreads();
+        const uint32_t t0 = __builtin_amdgcn_readfirstlane(sh_t[slot]);
+        if (t0 >= ntiles) break;
+        const uint32_t t = (uint32_t)ntiles - 1u - t0;
+        if (tid == 0) next = atomicAdd(a.counter, 1u);
+        uint32_t lo = 0, hi = a.B;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (dn_sload(a.tile_off, mid) <= t) lo = mid; else hi = mid;
+        }
+        const uint32_t b = lo;
+        const uint32_t info = dn_sload(a.binfo, b);
+        const uint32_t kb = info & 0xffffu, mask = info >> 16;
+        if (b != cur) {
+            if (tid < D8_NS) {
+                const uint32_t i = dn_bfirst(a.bstart, b, D8_NS) + tid;
+                uint32_t s = DN_NONE, be = 0, en = 0, z = 0;
+                uint64_t of = 0;
+                float q = 0.0f;
+                if (i < dn_bend(a.bstart, b, a.sp.num_spec, D8_NS)) {
+                    s = dn_order(a.order, i);
+                    be = a.begin[s];
+                    en = be + a.num[s];
+                    z = hfx_zeff(a.sp.d_zmax[s]);
+                    of = a.score_off[s];
+                    q = a.qs[s];
+                }
+                m_s[tid] = s; m_beg[tid] = be; m_end[tid] = en; m_z[tid] = z; m_off[tid] = of;
+                m_q[tid] = q;
+            }
+            __syncthreads();
+            /* this wave's B fragments of the block, straight from HBM */
+            const uint32_t sc = m_s[col];
+#pragma unroll
+            for (uint32_t j = 0; j < D8_KPW; ++j) {
+                const uint32_t ks = kw + D8_KG * j;
+#pragma unroll
+                for (uint32_t d = 0; d < 3; ++d) {
+                    bq[j][d] = d8_i32x4{0, 0, 0, 0};
+                    if (sc != DN_NONE && ks * 64 < kb)
+                        bq[j][d] = *(const d8_i32x4 *)(a.xq + ((size_t)sc * 3 + d) * D8_MAXK +
+                                                       ks * 64 + kg * 16);
+                }
+            }
+            cur = b;
+        }
+        if (tid == 0) sh_t[slot ^ 1u] = next;
+        const uint32_t beg = dn_sload(a.ubeg, b), end = dn_sload(a.uend, b);
+        if (beg < a.lad.q0 || end > a.lad.q0 + a.lad.nq) continue;
+        const uint32_t nch = (end - beg + a.CH - 1) / a.CH;
+        const uint32_t local = (uint32_t)(t - dn_sload(a.tile_off, b));
+        const uint32_t li = local / nch, chunk = local - li * nch;
+        uint32_t m = mask;
+        for (uint32_t qq = 0; qq < li; ++qq) m &= m - 1u;
+        const uint32_t lvl = (uint32_t)__builtin_ctz(m);
+        const uint32_t cbeg = beg + chunk * a.CH, cend = min(cbeg + a.CH, end);
+        /* an even number of tiles for the two-tile loop; tiles past the
+         * chunk are empty */
+        const uint32_t ntile = ((cend - cbeg + D8_ROWS - 1) / D8_ROWS + 1u) & ~1u;
+        float *const dummy = a.dummy + lane;
+
+        auto rows_of = [&](uint32_t p) -> uint32_t {
+            return p < cend ? min((uint32_t)D8_ROWS, cend - p) : 0u;
+        };
+        /* lane 0: the tile's first position, lane 1: one past its last */
+        auto load_range = [&](uint32_t p) -> uint64_t {
+            const uint32_t nr = rows_of(p);
+            return a.lad.off[nr ? p + (lane ? nr : 0u) - a.lad.q0 : 0u];
+        };
+        /* lane < 64: the top of candidate lane */
+        auto load_top = [&](uint32_t p) -> float {
+            return a.lad.top[lane < rows_of(p) ? p + lane - a.lad.q0 : 0u];
+        };
+        auto lane64 = [&](uint64_t v, uint32_t l) -> uint64_t {
+            return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32;
+        };
+        auto load_pos = [&](uint64_t L0, uint32_t np) -> hfx_dn_pos {
+            const hfx_dn_pos *base = np ? a.lad.pos + L0 : a.lad.pos;
+            return base[min(HFX_WAVE * wave + lane, np ? np - 1u : 0u)];
+        };
+        D8Pipe X, Y;
+        {
+            const uint64_t r0 = load_range(cbeg);
+            X.rng = load_range(cbeg + D8_ROWS);
+            __builtin_amdgcn_sched_barrier(0);
+            X.L = lane64(r0, 0);
+            X.np = (uint32_t)(lane64(r0, 1) - X.L);
+            X.pos = load_pos(X.L, X.np);
+            X.top = load_top(cbeg);
+        }
+        auto tile = [&](D8Pipe &cp, D8Pipe &nx, uint32_t p0) {
+            const uint32_t nrows = rows_of(p0);
+            nx.rng = load_range(p0 + 2 * D8_ROWS);
+            nx.L = lane64(cp.rng, 0);
+            nx.np = (uint32_t)(lane64(cp.rng, 1) - nx.L);
+            nx.pos = load_pos(nx.L, nx.np);
+            nx.top = load_top(p0 + D8_ROWS);
+
+            /* bins of the tile: below top / 1.0005 + 2 for every candidate */
+            const float rm = dn_row_max(lane < nrows ? cp.top : 0.0f);
+            uint32_t mb = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 0);
+            mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 16));
+            mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 32));
+            mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 48));
+            const float mm = __uint_as_float(mb);
+            const uint32_t kneed = (uint32_t)(mm + 24.0f) + 4u;
+            const uint32_t kq = nrows ? min(kb, (kneed + 127u) & ~127u) : 0u;
+            auto peaks = [&](auto chk) {
+                constexpr bool CHECK = decltype(chk)::value;
+                if (HFX_WAVE * wave + lane < cp.np)
+                    dn_peaks_all<CHECK>(map + (cp.pos.q - p0) * DN_MSTR + 1, cp.pos.b, cp.pos.y, kq,
+                                        lane, lvl);
+                for (uint32_t i = HFX_WAVE * (wave + D8_WAVES) + lane; i < cp.np;
+                     i += HFX_WAVE * D8_WAVES) {
+                    const hfx_dn_pos v = a.lad.pos[cp.L + i];
+                    dn_peaks_all<CHECK>(map + (v.q - p0) * DN_MSTR + 1, v.b, v.y, kq, lane, lvl);
+                }
+            };
+            if (kneed <= kq) peaks(std::false_type{});
+            else peaks(std::true_type{});
+            __syncthreads(); /* (b) the tile's map rows are complete */
+            if (kneed > kq && wave == D8_WAVES - 1)
+                for (uint32_t k = lane; k < D8_ROWS * DN_TRASH; k += HFX_WAVE) {
+                    const uint32_t r = k / DN_TRASH;
+                    map[r * DN_MSTR + 1 + (kq >> 3) + (k - r * DN_TRASH)] = 0u;
+                }
+
+            /* the tile's 64 x 16 x kq contraction: this wave's k-steps of its
+             * two row blocks, where they hold candidates */
+            const uint32_t nks = kq >> 6;
+            const uint32_t nrb = (nrows + 15u) >> 4;
+            if (kw < nks) {
+#pragma unroll
+                for (uint32_t r2 = 0; r2 < D8_RB / 2; ++r2) {
+                    const uint32_t r = 2 * rg + r2;
+                    if (r < nrb) {
+                        d8_i32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+                        uint32_t *arow = map + (r * 16u + col) * DN_MSTR + 1 + kg;
+#pragma unroll
+                        for (uint32_t j = 0; j < D8_KPW; ++j) {
+                            const uint32_t ks = kw + D8_KG * j;
+                            if (ks < nks) {
+                                uint32_t *aw = arow + 8 * ks;
+                                const uint32_t w0 = aw[0], w1 = aw[4];
+                                aw[0] = 0u;
+                                aw[4] = 0u;
+                                d8_i32x4 av;
+                                av.x = (int32_t)d8_wbytes(w0 & 0x0f0f0f0fu);
+                                av.y = (int32_t)d8_wbytes((w0 >> 4) & 0x0f0f0f0fu);
+                                av.z = (int32_t)d8_wbytes(w1 & 0x0f0f0f0fu);
+                                av.w = (int32_t)d8_wbytes((w1 >> 4) & 0x0f0f0f0fu);
+#pragma unroll
+                                for (uint32_t d = 0; d < 3; ++d)
+                                    acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                        av, bq[j][d], acc[d], 0, 0, 0);
+                            }
+                        }
+                        /* C[16 r + 4 kg + i][col] of the three digits as one
+                         * int64 into the tile's sums */
+                        uint64_t *cb = comb + r * 512 + (4 * kg) * 32 + col + 16 * (kg & 1u);
+#pragma unroll
+                        for (uint32_t i = 0; i < 4; ++i) {
+                            const int64_t v = ((int64_t)acc[2][i] << 16) +
+                                              ((int64_t)acc[1][i] << 8) + (int64_t)acc[0][i];
+                            atomicAdd((unsigned long long *)(cb + 32 * i), (unsigned long long)v);
+                        }
+                    }
+                }
+            }
+            __syncthreads(); /* (c) the tile's sums are complete */
+            for (uint32_t o = tid; o < D8_ROWS * D8_NS; o += D8_THREADS) {
+                /* output (column g = o % 16, tile row 16 rbx + row): the two
+                 * half-waves take rows four apart */
+                const uint32_t g = o & 15u, qq = o >> 4, rbx = qq >> 4;
+                const uint32_t row = ((qq >> 3) & 1u) << 3 | (qq & 1u) << 2 | ((qq >> 1) & 3u);
+                const uint32_t r = rbx * 16u + row;
+                uint64_t *cb = comb + rbx * 512 + row * 32 + g + 16 * ((row >> 2) & 1u);
+                const int64_t v = (int64_t)*cb;
+                *cb = 0ull;
+                const float sc = (float)((double)v * (double)m_q[g]);
+                const uint32_t p = p0 + r;
+                const bool ok = r < nrows && m_z[g] == lvl && m_beg[g] <= p && p < m_end[g];
+                *(ok ? a.scores + m_off[g] + (p - m_beg[g]) : dummy) = sc;
+            }
+        };
+        for (uint32_t j = 0; j < ntile; j += 2) {
+            tile(X, Y, cbeg + j * D8_ROWS);
+            tile(Y, X, cbeg + (j + 1) * D8_ROWS);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Ladders                                                                */
 /* ------------------------------------------------------------------ */
 
@@ -799,7 +1162,7 @@ int hfx_dense_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off, const
     uint32_t *counter = (uint32_t *)(tile_off + B + 1);
     float *dummy = (float *)(((uintptr_t)(counter + 1) + 255) & ~(uintptr_t)255);
     if (!presplit) hipLaunchKernelGGL(k_dn_split, dim3(S), dim3(256), 0, st, xb, *sp);
-    hipLaunchKernelGGL(k_dn_blocks, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend, binfo,
+    hipLaunchKernelGGL(k_dn_blocks<DN_NS>, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend, binfo,
                        ntiles, d_order, d_bstart, d_begin, d_num, *sp, B, CH);
     if ((rc = hfx_scan_offsets(ctx, tile_off, ntiles, B, (void *)st))) return rc;
     hipLaunchKernelGGL(k_dn_reset, dim3(1), dim3(1), 0, st, counter);
@@ -833,4 +1196,104 @@ extern "C" int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint6
         return rc;
     return hfx_dense_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, nullptr, 0, &lad, sp,
                             256, nullptr, 0, stream, nullptr);
+}
+
+/* ------------------------------------------------------------------ */
+/* Integer form: host side                                               */
+/* ------------------------------------------------------------------ */
+
+size_t hfx_dense8_scratch_bytes(uint32_t S)
+{
+    const size_t B = S; /* blocks of at least one spectrum */
+    return (size_t)S * 3 * D8_MAXK + (size_t)S * 4 + 256 + B * 4 * 4 + (B + 1) * 8 + 1024;
+}
+
+/* scratch layout: xq [S][3][D8_MAXK] | qs [S] | block arrays */
+static void d8_carve(void *p, uint32_t S, int8_t **xq, float **qs, uint8_t **rest)
+{
+    *xq = (int8_t *)p;
+    *qs = (float *)(((uintptr_t)(*xq + (size_t)S * 3 * D8_MAXK) + 255) & ~(uintptr_t)255);
+    *rest = (uint8_t *)(*qs + S);
+}
+
+int hfx_dense8_split(const hfx_spectra *sp, void *scratch, void *stream)
+{
+    HFX_REQUIRE(sp && scratch, HFX_ERR_ARG, "dense split: null argument");
+    HFX_REQUIRE(sp->x_stride <= D8_MAXK, HFX_ERR_ARG,
+                "dense XCorr: spectra of up to %d bins (x_stride %u)", D8_MAXK, sp->x_stride);
+    int8_t *xq;
+    float *qs;
+    uint8_t *rest;
+    d8_carve(scratch, sp->num_spec, &xq, &qs, &rest);
+    if (sp->num_spec)
+        hipLaunchKernelGGL(k_d8_split, dim3(sp->num_spec), dim3(256), 0, hfx_stream(stream), xq,
+                           qs, *sp);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+int hfx_dense8_scores(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                      const uint32_t *d_begin, const uint32_t *d_num, const uint32_t *d_order,
+                      const uint32_t *d_bstart, uint32_t nblocks, const hfx_dn_ladder *lad,
+                      const hfx_spectra *sp, uint32_t CH, void *scratch, int presplit,
+                      void *stream, void **ev)
+{
+    HFX_REQUIRE(ctx && d_scores && d_off && d_begin && d_num && lad && sp, HFX_ERR_ARG,
+                "dense XCorr: null argument");
+    HFX_REQUIRE(sp->x_stride <= D8_MAXK, HFX_ERR_ARG,
+                "dense XCorr: spectra of up to %d bins (x_stride %u)", D8_MAXK, sp->x_stride);
+    HFX_REQUIRE(CH >= D8_ROWS && CH % D8_ROWS == 0 && CH <= 8192, HFX_ERR_ARG,
+                "dense XCorr: chunk %u must be a multiple of %d in %d..8192", CH, D8_ROWS,
+                D8_ROWS);
+    const uint32_t S = sp->num_spec;
+    if (S == 0) return HFX_OK;
+    hipStream_t st = hfx_stream(stream);
+    void *p = scratch;
+    int rc;
+    if (!p && (rc = hfx_ctx_scratch(ctx, HFX_SLOT_DENSE, hfx_dense8_scratch_bytes(S), &p)))
+        return rc;
+    HFX_REQUIRE(!d_bstart || (nblocks >= 1 && nblocks <= S), HFX_ERR_ARG,
+                "dense XCorr: %u blocks for %u spectra", nblocks, S);
+    const uint32_t B = d_bstart ? nblocks : (S + D8_NS - 1) / D8_NS;
+    int8_t *xq;
+    float *qs;
+    uint8_t *rest;
+    d8_carve(p, S, &xq, &qs, &rest);
+    uint32_t *ubeg = (uint32_t *)rest;
+    uint32_t *uend = ubeg + B, *binfo = uend + B, *ntiles = binfo + B;
+    uint64_t *tile_off = (uint64_t *)(((uintptr_t)(ntiles + B) + 7) & ~(uintptr_t)7);
+    uint32_t *counter = (uint32_t *)(tile_off + B + 1);
+    float *dummy = (float *)(((uintptr_t)(counter + 1) + 255) & ~(uintptr_t)255);
+    if (!presplit) hipLaunchKernelGGL(k_d8_split, dim3(S), dim3(256), 0, st, xq, qs, *sp);
+    hipLaunchKernelGGL(k_dn_blocks<D8_NS>, dim3((B + 255) / 256), dim3(256), 0, st, ubeg, uend,
+                       binfo, ntiles, d_order, d_bstart, d_begin, d_num, *sp, B, CH);
+    if ((rc = hfx_scan_offsets(ctx, tile_off, ntiles, B, (void *)st))) return rc;
+    hipLaunchKernelGGL(k_dn_reset, dim3(1), dim3(1), 0, st, counter);
+    D8Args args{d_scores, d_off, d_begin,  d_num, d_order, d_bstart, B,    CH,  ubeg,
+                uend,     binfo, tile_off, xq,    qs,      counter,  dummy, *lad, *sp};
+    HFX_CHECK_HIP(hipFuncSetAttribute((const void *)k_dense8,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, D8_LDS_BYTES));
+    if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[0], st));
+    hipLaunchKernelGGL(k_dense8, dim3((uint32_t)hfx_ctx_num_cus(ctx)), dim3(D8_THREADS),
+                       D8_LDS_BYTES, st, args);
+    if (ev) HFX_CHECK_HIP(hipEventRecord((hipEvent_t)ev[1], st));
+    hipLaunchKernelGGL(k_dn_zero_unscored, dim3(S), dim3(256), 0, st, d_scores, d_off, sp->d_zmax,
+                       S);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
+}
+
+extern "C" int hfx_xcorr_dense_i8x3(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                                    const uint32_t *d_begin, const uint32_t *d_num,
+                                    const uint32_t *d_order, const hfx_db *db,
+                                    const hfx_spectra *sp, void *stream)
+{
+    HFX_REQUIRE(db, HFX_ERR_ARG, "hfx_xcorr_dense_i8x3: null database");
+    if (sp && sp->num_spec == 0) return HFX_OK;
+    hfx_dn_ladder lad;
+    int rc;
+    if ((rc = hfx_dense_ladder_build(ctx, db, 0, db->num_pep, 0, &lad, nullptr, stream)))
+        return rc;
+    return hfx_dense8_scores(ctx, d_scores, d_off, d_begin, d_num, d_order, nullptr, 0, &lad, sp,
+                             256, nullptr, 0, stream, nullptr);
 }

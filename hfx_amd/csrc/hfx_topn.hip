@@ -20,7 +20,8 @@
  *      N.  If the band holds more than TN_EXT candidates, the spectrum is
  *      flagged and the exact route scores every candidate at or above
  *      exN - B exactly.
- * Bound: |score - exact| <= (rel0 + rel1 * z) * 50 * sum_b |x_b| + 1e-6, with
+ * Bound: |score - exact| <= (rel0 + rel1 * z) * 50 * sum_b |x_b|
+ *                           + relq * max_b |x_b| * 50 * len + 1e-6, with
  * z the spectrum's fragment charge count and sum|w*x| <= 50 * sum_b |x_b|
  * (each bin counts once, weights <= 50).  The sparse FP32 scores use
  * rel0 = 80 * 2^-24 (<= 71 roundings per score), rel1 = 0; the dense BF16
@@ -151,7 +152,8 @@ template <class Cands>
 __global__ void __launch_bounds__(TN_THREADS)
     k_topn(uint32_t *top_pos, double *top_exact, float *top_f32, uint32_t *top_cnt,
            uint32_t *flag, uint32_t N, uint32_t margin, const float *scores, Cands c,
-           hfx_spectra sp, float rel0, float rel1, double *tau_out, uint32_t *band_out,
+           hfx_spectra sp, float rel0, float rel1, float relq, double *tau_out,
+           uint32_t *band_out,
            const uint32_t *order)
 {
     extern __shared__ __align__(16) unsigned char tn_dyn[];
@@ -164,7 +166,7 @@ __global__ void __launch_bounds__(TN_THREADS)
     __shared__ int64_t cfx[TN_MAXC];
     __shared__ float cf32[TN_MAXC];
     __shared__ uint32_t wcnt[TN_WAVES];
-    __shared__ float sh_x1[TN_WAVES];
+    __shared__ float sh_x1[TN_WAVES], sh_xm[TN_WAVES];
     __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_last, sh_ext;
     __shared__ float sh_fN;
 
@@ -293,10 +295,17 @@ __global__ void __launch_bounds__(TN_THREADS)
     for (uint32_t w = lane; w < mwords; w += HFX_WAVE) map[w] = 0;
     for (uint32_t i = tid; i < T; i += TN_THREADS) cf32[i] = sc[cp[i]];
     __syncthreads();
-    float x1 = 0.0f;
-    for (uint32_t i = tid; i < len; i += TN_THREADS) x1 += fabsf(xs[i]);
+    float x1 = 0.0f, xm = 0.0f;
+    for (uint32_t i = tid; i < len; i += TN_THREADS) {
+        x1 += fabsf(xs[i]);
+        xm = fmaxf(xm, fabsf(xs[i]));
+    }
     x1 = hfx_wave_sum_f32(x1);
-    if (lane == 0) sh_x1[wave] = x1;
+    for (uint32_t o = 32; o > 0; o >>= 1) xm = fmaxf(xm, __shfl_xor(xm, o, 64));
+    if (lane == 0) {
+        sh_x1[wave] = x1;
+        sh_xm[wave] = xm;
+    }
     if (n > T)
         for (uint32_t i = tid; i < T; i += TN_THREADS) {
             const uint32_t ki = hfx_f2key(cf32[i]), pi = cp[i];
@@ -309,10 +318,14 @@ __global__ void __launch_bounds__(TN_THREADS)
         }
     __syncthreads();
     const uint32_t zeff = c.zeff(sp.d_zmax[s]);
-    float xa = 0.0f;
-    for (uint32_t w = 0; w < TN_WAVES; ++w) xa += sh_x1[w];
+    float xa = 0.0f, xmax = 0.0f;
+    for (uint32_t w = 0; w < TN_WAVES; ++w) {
+        xa += sh_x1[w];
+        xmax = fmaxf(xmax, sh_xm[w]);
+    }
     const double rel = (double)rel0 + (double)rel1 * (double)zeff;
-    const double bound = rel * 50.0 * (double)xa * 1.001 + 1e-6;
+    const double bound = rel * 50.0 * (double)xa * 1.001 +
+                         (double)relq * (double)xmax * 50.0 * (double)len * 1.001 + 1e-6;
 
     /* 3. the extension: every other candidate that could reach the N-th
      * exact score exN.  The N best FP32 scores are selected and each has an
@@ -848,7 +861,8 @@ template <class Cands>
 static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
                        uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
                        const float *d_scores, const Cands &c, uint64_t total,
-                       const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream)
+                       const hfx_spectra *sp, float rel0, float rel1, float relq, void *scratch,
+                       void *stream)
 {
     HFX_REQUIRE(ctx && d_top_pos && d_top_exact && d_top_f32 && d_top_cnt && d_flag && d_scores &&
                     sp,
@@ -868,8 +882,8 @@ static int launch_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                        sp->num_spec);
     hipLaunchKernelGGL(k_topn<Cands>, dim3(sp->num_spec), dim3(TN_THREADS),
                        tn_dyn_bytes(sp->x_stride), st, d_top_pos, d_top_exact, d_top_f32,
-                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, x.tau, x.band,
-                       x.list);
+                       d_top_cnt, d_flag, N, margin, d_scores, c, *sp, rel0, rel1, relq, x.tau,
+                       x.band, x.list);
     HFX_CHECK_LAUNCH();
     return exact_route(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, N, d_flag, d_scores, c,
                        total, sp, p, st);
@@ -904,13 +918,17 @@ int hfx_topn_device(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, floa
                     uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin,
                     const float *d_scores, const uint64_t *d_off, uint64_t total,
                     const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
-                    const hfx_spectra *sp, float rel0, float rel1, void *scratch, void *stream)
+                    const hfx_spectra *sp, float rel0, float rel1, float relq, void *scratch,
+                    void *stream)
 {
-    HFX_REQUIRE(rel0 >= 0.0f && rel1 >= 0.0f && rel0 < 1.0f && rel1 < 1.0f, HFX_ERR_ARG,
-                "top-N: error bound factors must lie in [0, 1) (rel0=%g rel1=%g)", rel0, rel1);
+    HFX_REQUIRE(rel0 >= 0.0f && rel1 >= 0.0f && relq >= 0.0f && rel0 < 1.0f && rel1 < 1.0f &&
+                    relq < 1.0f,
+                HFX_ERR_ARG,
+                "top-N: error bound factors must lie in [0, 1) (rel0=%g rel1=%g relq=%g)", rel0,
+                rel1, relq);
     HfxPlainCands c{*db, d_begin, d_num, d_off};
     return launch_topn(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-                       d_scores, c, total, sp, rel0, rel1, scratch, stream);
+                       d_scores, c, total, sp, rel0, rel1, relq, scratch, stream);
 }
 
 extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
@@ -922,7 +940,7 @@ extern "C" int hfx_topn(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, 
     HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn: null argument");
     return hfx_topn_device(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
                            d_scores, d_off, total, d_begin, d_num, db, sp, HFX_TOPN_REL_F32, 0.0f,
-                           nullptr, stream);
+                           0.0f, nullptr, stream);
 }
 
 extern "C" int hfx_topn_rel(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
@@ -934,8 +952,21 @@ extern "C" int hfx_topn_rel(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exa
 {
     HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn_rel: null argument");
     return hfx_topn_device(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-                           d_scores, d_off, total, d_begin, d_num, db, sp, rel0, rel1, nullptr,
-                           stream);
+                           d_scores, d_off, total, d_begin, d_num, db, sp, rel0, rel1, 0.0f,
+                           nullptr, stream);
+}
+
+extern "C" int hfx_topn_rel_q(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
+                              float *d_top_f32, uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N,
+                              uint32_t margin, float rel0, float rel1, float relq,
+                              const float *d_scores, const uint64_t *d_off, uint64_t total,
+                              const uint32_t *d_begin, const uint32_t *d_num, const hfx_db *db,
+                              const hfx_spectra *sp, void *stream)
+{
+    HFX_REQUIRE(d_off && d_begin && d_num && db, HFX_ERR_ARG, "hfx_topn_rel_q: null argument");
+    return hfx_topn_device(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
+                           d_scores, d_off, total, d_begin, d_num, db, sp, rel0, rel1, relq,
+                           nullptr, stream);
 }
 
 extern "C" int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,
@@ -962,7 +993,8 @@ extern "C" int hfx_topn_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exa
     HFX_REQUIRE(ms->num_ma <= HFX_MAX_MA, HFX_ERR_ARG, "hfx_topn_mod: num_ma %u", ms->num_ma);
     HfxModCands c{*db, *ms, d_mp_off};
     return launch_topn(ctx, d_top_pos, d_top_exact, d_top_f32, d_top_cnt, d_flag, N, margin,
-                       d_scores, c, ms->num_mpep, sp, HFX_TOPN_REL_F32, 0.0f, nullptr, stream);
+                       d_scores, c, ms->num_mpep, sp, HFX_TOPN_REL_F32, 0.0f, 0.0f, nullptr,
+                       stream);
 }
 
 extern "C" int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact,

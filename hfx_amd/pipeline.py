@@ -337,17 +337,23 @@ class PlainSearch:
     # |score - exact| <= REL * 50 * sum|x| for hfx_xcorr_dense_bf16x2 scores
     REL_DENSE = float(np.float32(2.0**-16 + 210 * 2.0**-24))
     REL_F32 = float(np.float32(80 * 2.0**-24))
+    # |score - exact| <= 2^-24 * 50 * sum|x| + max|x| / 8355711 * 50 * len
+    # for hfx_xcorr_dense_i8x3 scores
+    REL_I8 = float(2.0**-24)
+    RELQ_I8 = float(np.float32(1.0 / 8355711.0))
 
-    def score_dense(self, order: np.ndarray | None = None):
+    def score_dense(self, order: np.ndarray | None = None, form: str = "bf16x2"):
         """Scores by hfx_xcorr_dense_bf16x2 (blocks of eight spectra in
-        `order`, index order if None), same layout as score()."""
+        `order`, index order if None) or, form "i8x3", hfx_xcorr_dense_i8x3
+        (blocks of sixteen); same layout as score()."""
         t_order = dev(np.asarray(order, np.uint32)) if order is not None else None
+        fn = {"bf16x2": "hfx_xcorr_dense_bf16x2", "i8x3": "hfx_xcorr_dense_i8x3"}[form]
         if self.total:
-            check(lib().hfx_xcorr_dense_bf16x2(self.ctx.h, ptr(self.t_scores), ptr(self.t_off),
-                                               ptr(self.t_begin), ptr(self.t_num),
-                                               ptr(t_order) if t_order is not None else None,
-                                               C.byref(self.db.desc), C.byref(self.sp.desc),
-                                               stream_handle()), "hfx_xcorr_dense_bf16x2")
+            check(getattr(lib(), fn)(self.ctx.h, ptr(self.t_scores), ptr(self.t_off),
+                                     ptr(self.t_begin), ptr(self.t_num),
+                                     ptr(t_order) if t_order is not None else None,
+                                     C.byref(self.db.desc), C.byref(self.sp.desc),
+                                     stream_handle()), fn)
         sync()
 
     def select_rel(self, rel0: float, rel1: float = 0.0):
@@ -358,6 +364,15 @@ class PlainSearch:
                                  self.total, ptr(self.t_begin), ptr(self.t_num),
                                  C.byref(self.db.desc), C.byref(self.sp.desc), stream_handle()),
               "hfx_topn_rel")
+
+    def select_rel_q(self, rel0: float, rel1: float, relq: float):
+        """hfx_topn_rel_q: the bound with the max|x| * 50 * len term."""
+        check(lib().hfx_topn_rel_q(self.ctx.h, ptr(self.t_pos), ptr(self.t_exact),
+                                   ptr(self.t_f32), ptr(self.t_cnt), ptr(self.t_flag), self.topn,
+                                   self.margin, float(rel0), float(rel1), float(relq),
+                                   ptr(self.t_scores), ptr(self.t_off), self.total,
+                                   ptr(self.t_begin), ptr(self.t_num), C.byref(self.db.desc),
+                                   C.byref(self.sp.desc), stream_handle()), "hfx_topn_rel_q")
 
     def exact(self, spectra: np.ndarray):
         """hfx_topn_exact for the listed spectra (every candidate exact)."""
@@ -381,12 +396,16 @@ class PlainSearch:
         f = host(self.t_flag, np.uint32, self.sp.num_spec)
         return np.nonzero(f & FLAG_SHORT)[0].astype(np.uint32)
 
-    def run(self, keep_scores: bool = False, dense: bool = False) -> PlainResult:
-        """window -> XCorr (FP32 sparse, or dense BF16 hi+lo) -> top-N."""
+    def run(self, keep_scores: bool = False, dense: bool | str = False) -> PlainResult:
+        """window -> XCorr (FP32 sparse; dense=True: BF16 hi+lo; dense="i8x3":
+        int8 digits) -> top-N."""
         S, N = self.sp.num_spec, self.topn
         self.window()
         if self.total:
-            if dense:
+            if dense == "i8x3":
+                self.score_dense(form="i8x3")
+                self.select_rel_q(self.REL_I8, 0.0, self.RELQ_I8)
+            elif dense:
                 self.score_dense()
                 self.select_rel(self.REL_DENSE)
             else:
@@ -468,6 +487,7 @@ class BatchSearch:
             "zero", "precision", "chunk")
     FP32_SPARSE = 0
     BF16X2_DENSE = 1
+    I8X3_DENSE = 2
 
     def __init__(self, ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, eps: float, topn: int,
                  margin: int = 0, stream: Stream | None = None, index: PepIndex | None = None):
@@ -523,7 +543,8 @@ class BatchSearch:
         check(lib().hfx_batch_set_config(self.h, G, Cn, threads), "hfx_batch_set_config")
 
     def set_precision(self, precision: int):
-        """FP32_SPARSE (ion index + gathers) or BF16X2_DENSE (MFMA)."""
+        """FP32_SPARSE (ion index + gathers), BF16X2_DENSE (BF16 MFMA) or
+        I8X3_DENSE (int8 MFMA over three digits of each spectrum)."""
         check(lib().hfx_batch_set_precision(self.h, int(precision)), "hfx_batch_set_precision")
         self.captured = False
 

@@ -352,6 +352,28 @@ int hfx_xcorr_dense_bf16x2(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
                            const uint32_t *d_begin, const uint32_t *d_num, const uint32_t *d_order,
                            const hfx_db *db, const hfx_spectra *sp, void *stream);
 
+/* Integer precision form (new; same layout as hfx_xcorr_sparse): blocks of
+ * sixteen spectra in the order d_order, one theoretical row per candidate of
+ * the block's union window, contracted on the MFMA units in int8: weights
+ * 0/10/25/50 exactly, each spectrum as three balanced base-256 digits of
+ * X = rint(x * 2^k) (k the largest with max|x| * 2^k <= 8355711), int32
+ * sums exact, the score formed in FP64 and rounded once to FP32.
+ * |score - exact| <= max|x| / 8355711 * 50 * len + 2^-24 * 50 * sum|x|
+ * (top-N: hfx_topn_rel_q with rel0 = 2^-24, rel1 = 0, relq = 1 / 8355711).
+ * Needs sp->x_stride <= 2176. */
+int hfx_xcorr_dense_i8x3(hfx_ctx *ctx, float *d_scores, const uint64_t *d_off,
+                         const uint32_t *d_begin, const uint32_t *d_num, const uint32_t *d_order,
+                         const hfx_db *db, const hfx_spectra *sp, void *stream);
+
+/* hfx_topn_rel with one more term in the bound: relq * max_b |x_s[b]| * 50
+ * * len_s (scores whose error follows the spectrum's largest value, as the
+ * integer form's do). */
+int hfx_topn_rel_q(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
+                   uint32_t *d_top_cnt, uint32_t *d_flag, uint32_t N, uint32_t margin, float rel0,
+                   float rel1, float relq, const float *d_scores, const uint64_t *d_off,
+                   uint64_t total, const uint32_t *d_begin, const uint32_t *d_num,
+                   const hfx_db *db, const hfx_spectra *sp, void *stream);
+
 /* Exact top-N of the listed spectra (every candidate scored in fixed point;
  * d_top_f32 taken from d_scores). */
 int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float *d_top_f32,
@@ -453,6 +475,7 @@ int hfx_batch_use_index(hfx_batch *b, const hfx_pepix *ix);
 int hfx_batch_run(hfx_batch *b, void *stream);
 #define HFX_BATCH_FP32_SPARSE 0u  /* default after hfx_batch_plan */
 #define HFX_BATCH_BF16X2_DENSE 1u /* hfx_xcorr_dense_bf16x2 scores */
+#define HFX_BATCH_I8X3_DENSE 2u   /* hfx_xcorr_dense_i8x3 scores */
 int hfx_batch_set_precision(hfx_batch *b, uint32_t precision);
 /* capture hfx_batch_run on a non-null stream into a graph; replay it */
 int hfx_batch_capture(hfx_batch *b, void *stream);

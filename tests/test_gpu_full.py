@@ -60,10 +60,13 @@ def fenv(request, orc):
                             topn, want_all=True)
     x1 = np.array([np.abs(X[s, : lens[s]].astype(np.float64)).sum() for s in range(dsp.num_spec)])
     reps = np.diff(ref["off"].astype(np.int64))
+    xm = np.array([np.abs(X[s, : lens[s]]).max() if lens[s] else 0.0
+                   for s in range(dsp.num_spec)], np.float64)
+    b8 = (xm / 8355711.0 * 50 * lens.astype(np.float64) + 2.0**-24 * 50 * x1) * 1.001 + 1e-6
     print(f"{name}: {len(r)} peptides, {dsp.num_spec} spectra, {int(ref['off'][-1])} scores")
     return dict(name=name, ctx=ctx, db=db, dsp=dsp, ref=ref, topn=topn, pl=pipeline,
                 exact=orc.fx_to_double(ref["all_fx"]), scale=orc.fx_to_double(absx["all_fx"]),
-                x1=np.repeat(x1, reps))
+                x1=np.repeat(x1, reps), bound8=np.repeat(b8, reps))
 
 
 def check_topn(res, ref, orc, S):
@@ -103,6 +106,19 @@ def test_batched_fp32_full(fenv, orc):
     res = bs.results(keep_scores=True)
     check_topn(res, fenv["ref"], orc, fenv["dsp"].num_spec)
     check_scores(fenv, res, "batched fp32")
+
+
+def test_batched_i8_full(fenv, orc):
+    """the integer form: within max|x| / 8355711 * 50 len + 2^-24 * 50
+    sum|x| of the exact score, besides the common 1e-4 criterion"""
+    bs = fenv["pl"].BatchSearch(fenv["ctx"], fenv["db"], fenv["dsp"], eps=3.0, topn=fenv["topn"])
+    bs.set_precision(bs.I8X3_DENSE)
+    bs.run()
+    res = bs.results(keep_scores=True)
+    check_topn(res, fenv["ref"], orc, fenv["dsp"].num_spec)
+    err = check_scores(fenv, res, "batched i8x3")
+    assert np.all(err <= fenv["bound8"])
+    assert len(res.short) == 0
 
 
 def test_batched_dense_full(fenv, orc):
