@@ -339,7 +339,7 @@ __device__ __forceinline__ float dn_zdiv(float x)
  * sparse path (zh = H * Z, H2O / Z and NH3 / Z fold to the same FP32 values);
  * CHECK keeps only bins in [0, kq) (the others go to words this tile's
  * contraction does not read) */
-template <int Z, bool CHECK>
+template <int Z, bool CHECK, uint32_t SPREAD = 15u>
 __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32_t kq,
                                          uint32_t lane)
 {
@@ -350,7 +350,7 @@ __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32
     const float my = dn_zdiv<Z>((y + HFX_DEV_H2O) + zh);
     if constexpr (CHECK) {
         uint32_t *rowm = row - DN_MAGIC_WORDS;
-        const uint32_t uhi = 0x4B400000u + kq + 8u * (lane & 15u);
+        const uint32_t uhi = 0x4B400000u + kq + 8u * (lane & SPREAD);
         dn_put_u(rowm, dn_clamp(dn_binu(ma), uhi), HFX_CODE10);
         dn_put3_u(rowm, dn_clamp(dn_binu(mb) - 1u, uhi) + 1u);
         dn_put_u(rowm, dn_clamp(dn_binu(mb - h2o), uhi), HFX_CODE10);
@@ -368,14 +368,14 @@ __device__ __forceinline__ void dn_peaks(uint32_t *row, float b, float y, uint32
     }
 }
 
-template <bool CHECK>
+template <bool CHECK, uint32_t SPREAD = 15u>
 __device__ __forceinline__ void dn_peaks_all(uint32_t *row, float b, float y, uint32_t kq,
                                              uint32_t lane, uint32_t lvl)
 {
-    dn_peaks<1, CHECK>(row, b, y, kq, lane);
-    if (lvl >= 2) dn_peaks<2, CHECK>(row, b, y, kq, lane);
-    if (lvl >= 3) dn_peaks<3, CHECK>(row, b, y, kq, lane);
-    if (lvl >= 4) dn_peaks<4, CHECK>(row, b, y, kq, lane);
+    dn_peaks<1, CHECK, SPREAD>(row, b, y, kq, lane);
+    if (lvl >= 2) dn_peaks<2, CHECK, SPREAD>(row, b, y, kq, lane);
+    if (lvl >= 3) dn_peaks<3, CHECK, SPREAD>(row, b, y, kq, lane);
+    if (lvl >= 4) dn_peaks<4, CHECK, SPREAD>(row, b, y, kq, lane);
 }
 
 __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
@@ -686,43 +686,56 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
  * (2^-k_s < 2 max|x_s| / 8355711).
  *
  * Sixteen spectra a block (the MFMA's N), 64 candidates a tile (four 16-row
- * blocks of A) in one map; one persistent workgroup of twelve waves per CU
- * (three a SIMD, so that each holds up to 168 registers):
- *   - the tile's ladder positions go to the waves 64 at a time (wave w takes
- *     positions 64 w + lane, 64 (w + 12) + lane, ...);
- *   - wave w contracts k-steps w % 6, w % 6 + 6, ... (64 bins each) of row
- *     blocks 2 (w / 6) and 2 (w / 6) + 1; the B fragments of its k-steps
- *     (three digits each) stay in its registers for the whole block, loaded
- *     once from HBM, so B never passes through LDS; lane (row, kg) reads map words 8 ks + kg and
- *     8 ks + 4 + kg of its row (clearing them), conflict-free, and one v_perm
- *     per four bins turns codes into weight bytes;
- *   - each wave folds its three digit sums into one int64 per output
- *     (a2 2^16 + a1 2^8 + a0, exact) and adds it into LDS (ds_add_u64, exact
- *     in any order); after a barrier the threads form the tile's 64 x 16
- *     scores.
+ * blocks of A); one persistent workgroup of twelve waves per CU (three a
+ * SIMD, so that each holds up to 168 registers) in two groups of six that
+ * take turns, with two maps and two sum buffers:
+ *
+ *   phase k:  group k % 2      forms the scores of tile k - 2 and writes the
+ *                              peaks of tile k into map k % 2;
+ *             group (k+1) % 2  contracts tile k - 1 from map (k+1) % 2;
+ *             one workgroup barrier,
+ *
+ * so the peaks (VALU, LDS atomics) of one tile overlap the contraction (VALU,
+ * MFMA, LDS reads) of the previous one.
+ *   - peaks: the tile's ladder positions go to the group's waves 64 at a time
+ *     (group wave g takes positions 64 g + lane, 64 (g + 6) + lane, ...);
+ *   - contraction: group wave g takes k-steps g, g + 6, g + 12, ... (64 bins
+ *     each) of the four row blocks; the B fragments of its k-steps (three
+ *     digits each) stay in its registers for the whole block, loaded once
+ *     from HBM, so B never passes through LDS; lane (row, kg) reads map words
+ *     8 ks + kg and 8 ks + 4 + kg of its row (clearing them), conflict-free,
+ *     and one v_perm per four bins turns codes into weight bytes; each wave
+ *     folds its three digit sums into one int64 per output (a2 2^16 +
+ *     a1 2^8 + a0, exact) and adds it into LDS (ds_add_u64, exact in any
+ *     order).
  * B's K order (the split's layout): byte 16 kg + j of a k-step's 64 is bin
  * (j < 8 ? 0 : 32) + 8 kg + {0,2,4,6,1,3,5,7}[j % 8].
  */
 #define D8_NS 16
 #define D8_RB 4                   /* 16-row blocks of A a tile */
 #define D8_ROWS (16 * D8_RB)
-#define D8_WAVES 12               /* three a SIMD: 168 registers each */
+#define D8_WAVES 12
+#define D8_GW 6                   /* waves a group */
 #define D8_THREADS (D8_WAVES * HFX_WAVE)
+#define D8_GT (D8_GW * HFX_WAVE)  /* threads a group */
 #define D8_MAXK DN_MAXK           /* bins of a row: 34 k-steps of 64 */
 #define D8_NKS (D8_MAXK / 64)
-#define D8_KG (D8_WAVES / 2)      /* waves splitting the k-steps of a row block */
-#define D8_KPW ((D8_NKS + D8_KG - 1) / D8_KG) /* k-steps a wave holds B for */
-#define D8_MAP_BYTES (D8_ROWS * DN_MSTR * 4)
-/* int64 sums [row block][16 rows][32 slots]: column c of row r in slot
- * c + 16 ((r >> 2) & 1), so that the atomic adds (lanes of rows r and r + 4)
- * and the epilogue's reads spread over all banks */
-#define D8_COMB_BYTES (D8_RB * 16 * 32 * 8)
-#define D8_LDS_BYTES (D8_MAP_BYTES + D8_COMB_BYTES)
-#define D8_FIXED_LDS (D8_NS * 4 * 5 + D8_NS * 8 + 2 * 4 + 16)
+#define D8_KPW ((D8_NKS + D8_GW - 1) / D8_GW) /* k-steps a wave holds B for */
+/* map rows: bin x in word 1 + x / 8 (word 0 a margin); clamped peaks spread
+ * over four words past kq / 8 (D8_TRASH words with a triple's spill); a row
+ * stride of 2 mod 4 words keeps the contraction's reads conflict-free */
+#define D8_SPREAD 3u
+#define D8_TRASH 5u
+#define D8_MSTR 278
+static_assert(D8_MSTR >= 1 + D8_MAXK / 8 + D8_TRASH && D8_MSTR % 4 == 2, "map row stride");
+#define D8_MAP_BYTES (D8_ROWS * D8_MSTR * 4)
+/* int64 sums [row block][16 rows][16 columns] */
+#define D8_COMB_BYTES (D8_RB * 16 * 16 * 8)
+#define D8_LDS_BYTES (2 * D8_MAP_BYTES + 2 * D8_COMB_BYTES)
+#define D8_FIXED_LDS (D8_NS * 4 * 5 + D8_NS * 8 + 2 * 4 + 2 * 4 + 16)
 #define D8_XMAX 8355711.0f
 static_assert(D8_LDS_BYTES + D8_FIXED_LDS <= 160 * 1024, "integer dense kernel LDS exceeds a CU");
 static_assert(D8_MAXK % 128 == 0, "rows of whole k-steps");
-static_assert(D8_WAVES % 2 == 0 && D8_RB == 4, "two row blocks for each half of the waves");
 
 typedef int32_t d8_i32x4 __attribute__((ext_vector_type(4)));
 
@@ -813,20 +826,21 @@ struct D8Args {
 __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
 {
     extern __shared__ __align__(16) unsigned char d8_lds[];
-    uint32_t *map = (uint32_t *)d8_lds;
-    uint64_t *comb = (uint64_t *)(d8_lds + D8_MAP_BYTES);
+    uint32_t *maps = (uint32_t *)d8_lds;                          /* [2][64 rows][D8_MSTR] */
+    uint64_t *combs = (uint64_t *)(d8_lds + 2 * D8_MAP_BYTES);    /* [2][4][16][16] */
     __shared__ uint32_t m_s[D8_NS], m_beg[D8_NS], m_end[D8_NS], m_z[D8_NS];
     __shared__ float m_q[D8_NS];
     __shared__ uint64_t m_off[D8_NS];
     __shared__ uint32_t sh_t[2];
+    __shared__ uint32_t sh_kq[2], sh_kneed[2]; /* of the tile last written into map i */
 
     const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
-    for (uint32_t i = tid; i < D8_ROWS * DN_MSTR; i += D8_THREADS) map[i] = 0;
-    for (uint32_t i = tid; i < D8_COMB_BYTES / 8; i += D8_THREADS) comb[i] = 0;
+    const uint32_t grp = wave / D8_GW, gw = wave % D8_GW, gtid = tid % D8_GT;
+    for (uint32_t i = tid; i < 2 * D8_ROWS * D8_MSTR; i += D8_THREADS) maps[i] = 0;
+    for (uint32_t i = tid; i < 2 * D8_COMB_BYTES / 8; i += D8_THREADS) combs[i] = 0;
     const uint64_t ntiles = dn_sload(a.tile_off, a.B);
     const uint32_t col = lane & 15u, kg = lane >> 4;
-    const uint32_t kw = wave % D8_KG, rg = wave / D8_KG;
-    /* B fragments of k-steps kw + 8 j, three digits each */
+    /* B fragments of k-steps gw + 6 j, three digits each */
     d8_i32x4 bq[D8_KPW][3];
     uint32_t cur = DN_NONE;
     if (tid == 0) sh_t[0] = atomicAdd(a.counter, 1u);
@@ -867,7 +881,7 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
             const uint32_t sc = m_s[col];
 #pragma unroll
             for (uint32_t j = 0; j < D8_KPW; ++j) {
-                const uint32_t ks = kw + D8_KG * j;
+                const uint32_t ks = gw + D8_GW * j;
 #pragma unroll
                 for (uint32_t d = 0; d < 3; ++d) {
                     bq[j][d] = d8_i32x4{0, 0, 0, 0};
@@ -888,11 +902,10 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
         for (uint32_t qq = 0; qq < li; ++qq) m &= m - 1u;
         const uint32_t lvl = (uint32_t)__builtin_ctz(m);
         const uint32_t cbeg = beg + chunk * a.CH, cend = min(cbeg + a.CH, end);
-        /* an even number of tiles for the two-tile loop; tiles past the
-         * chunk are empty */
-        const uint32_t ntile = ((cend - cbeg + D8_ROWS - 1) / D8_ROWS + 1u) & ~1u;
+        const uint32_t ntile = (cend - cbeg + D8_ROWS - 1) / D8_ROWS;
         float *const dummy = a.dummy + lane;
 
+        auto p0_of = [&](uint32_t k) -> uint32_t { return cbeg + k * D8_ROWS; };
         auto rows_of = [&](uint32_t p) -> uint32_t {
             return p < cend ? min((uint32_t)D8_ROWS, cend - p) : 0u;
         };
@@ -901,7 +914,6 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
             const uint32_t nr = rows_of(p);
             return a.lad.off[nr ? p + (lane ? nr : 0u) - a.lad.q0 : 0u];
         };
-        /* lane < 64: the top of candidate lane */
         auto load_top = [&](uint32_t p) -> float {
             return a.lad.top[lane < rows_of(p) ? p + lane - a.lad.q0 : 0u];
         };
@@ -911,116 +923,136 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
         };
         auto load_pos = [&](uint64_t L0, uint32_t np) -> hfx_dn_pos {
             const hfx_dn_pos *base = np ? a.lad.pos + L0 : a.lad.pos;
-            return base[min(HFX_WAVE * wave + lane, np ? np - 1u : 0u)];
+            return base[min(HFX_WAVE * gw + lane, np ? np - 1u : 0u)];
         };
-        D8Pipe X, Y;
+        /* this group's pipeline: the tile it writes peaks for next (k0 = grp,
+         * then every other tile) is in pc, the range of the one after in rn */
+        D8Pipe pc;
+        uint64_t rn;
         {
-            const uint64_t r0 = load_range(cbeg);
-            X.rng = load_range(cbeg + D8_ROWS);
+            const uint64_t r0 = load_range(p0_of(grp));
+            rn = load_range(p0_of(grp + 2));
             __builtin_amdgcn_sched_barrier(0);
-            X.L = lane64(r0, 0);
-            X.np = (uint32_t)(lane64(r0, 1) - X.L);
-            X.pos = load_pos(X.L, X.np);
-            X.top = load_top(cbeg);
+            pc.L = lane64(r0, 0);
+            pc.np = (uint32_t)(lane64(r0, 1) - pc.L);
+            pc.pos = load_pos(pc.L, pc.np);
+            pc.top = load_top(p0_of(grp));
         }
-        auto tile = [&](D8Pipe &cp, D8Pipe &nx, uint32_t p0) {
-            const uint32_t nrows = rows_of(p0);
-            nx.rng = load_range(p0 + 2 * D8_ROWS);
-            nx.L = lane64(cp.rng, 0);
-            nx.np = (uint32_t)(lane64(cp.rng, 1) - nx.L);
-            nx.pos = load_pos(nx.L, nx.np);
-            nx.top = load_top(p0 + D8_ROWS);
-
-            /* bins of the tile: below top / 1.0005 + 2 for every candidate */
-            const float rm = dn_row_max(lane < nrows ? cp.top : 0.0f);
-            uint32_t mb = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 0);
-            mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 16));
-            mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 32));
-            mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 48));
-            const float mm = __uint_as_float(mb);
-            const uint32_t kneed = (uint32_t)(mm + 24.0f) + 4u;
-            const uint32_t kq = nrows ? min(kb, (kneed + 127u) & ~127u) : 0u;
-            auto peaks = [&](auto chk) {
-                constexpr bool CHECK = decltype(chk)::value;
-                if (HFX_WAVE * wave + lane < cp.np)
-                    dn_peaks_all<CHECK>(map + (cp.pos.q - p0) * DN_MSTR + 1, cp.pos.b, cp.pos.y, kq,
-                                        lane, lvl);
-                for (uint32_t i = HFX_WAVE * (wave + D8_WAVES) + lane; i < cp.np;
-                     i += HFX_WAVE * D8_WAVES) {
-                    const hfx_dn_pos v = a.lad.pos[cp.L + i];
-                    dn_peaks_all<CHECK>(map + (v.q - p0) * DN_MSTR + 1, v.b, v.y, kq, lane, lvl);
+        for (uint32_t k = 0; k < ntile + 2; ++k) {
+            const uint32_t mi = k & 1u;
+            if (grp == mi) {
+                /* 1. the scores of tile k - 2 (its sums, in buffer mi, are
+                 * complete since the last barrier) */
+                if (k >= 2) {
+                    const uint32_t pk = p0_of(k - 2), nr = rows_of(pk);
+                    uint64_t *cb = combs + mi * (D8_COMB_BYTES / 8);
+                    for (uint32_t o = gtid; o < D8_ROWS * D8_NS; o += D8_GT) {
+                        const uint32_t g = o & 15u, r = o >> 4;
+                        const int64_t v = (int64_t)cb[o];
+                        cb[o] = 0ull;
+                        const float sc = (float)((double)v * (double)m_q[g]);
+                        const uint32_t p = pk + r;
+                        const bool ok = r < nr && m_z[g] == lvl && m_beg[g] <= p && p < m_end[g];
+                        *(ok ? a.scores + m_off[g] + (p - m_beg[g]) : dummy) = sc;
+                    }
                 }
-            };
-            if (kneed <= kq) peaks(std::false_type{});
-            else peaks(std::true_type{});
-            __syncthreads(); /* (b) the tile's map rows are complete */
-            if (kneed > kq && wave == D8_WAVES - 1)
-                for (uint32_t k = lane; k < D8_ROWS * DN_TRASH; k += HFX_WAVE) {
-                    const uint32_t r = k / DN_TRASH;
-                    map[r * DN_MSTR + 1 + (kq >> 3) + (k - r * DN_TRASH)] = 0u;
-                }
-
-            /* the tile's 64 x 16 x kq contraction: this wave's k-steps of its
-             * two row blocks, where they hold candidates */
-            const uint32_t nks = kq >> 6;
-            const uint32_t nrb = (nrows + 15u) >> 4;
-            if (kw < nks) {
-#pragma unroll
-                for (uint32_t r2 = 0; r2 < D8_RB / 2; ++r2) {
-                    const uint32_t r = 2 * rg + r2;
-                    if (r < nrb) {
-                        d8_i32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-                        uint32_t *arow = map + (r * 16u + col) * DN_MSTR + 1 + kg;
-#pragma unroll
-                        for (uint32_t j = 0; j < D8_KPW; ++j) {
-                            const uint32_t ks = kw + D8_KG * j;
-                            if (ks < nks) {
-                                uint32_t *aw = arow + 8 * ks;
-                                const uint32_t w0 = aw[0], w1 = aw[4];
-                                aw[0] = 0u;
-                                aw[4] = 0u;
-                                d8_i32x4 av;
-                                av.x = (int32_t)d8_wbytes(w0 & 0x0f0f0f0fu);
-                                av.y = (int32_t)d8_wbytes((w0 >> 4) & 0x0f0f0f0fu);
-                                av.z = (int32_t)d8_wbytes(w1 & 0x0f0f0f0fu);
-                                av.w = (int32_t)d8_wbytes((w1 >> 4) & 0x0f0f0f0fu);
-#pragma unroll
-                                for (uint32_t d = 0; d < 3; ++d)
-                                    acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                                        av, bq[j][d], acc[d], 0, 0, 0);
-                            }
+                /* 2. the peaks of tile k into map mi */
+                if (k < ntile) {
+                    const uint32_t p0 = p0_of(k), nrows = rows_of(p0);
+                    /* issue: the range of tile k + 4, the positions and tops of
+                     * tile k + 2 (this group's next) */
+                    D8Pipe nx;
+                    const uint64_t rn2 = load_range(p0_of(k + 4));
+                    nx.L = lane64(rn, 0);
+                    nx.np = (uint32_t)(lane64(rn, 1) - nx.L);
+                    nx.pos = load_pos(nx.L, nx.np);
+                    nx.top = load_top(p0_of(k + 2));
+                    /* bins of the tile: below top / 1.0005 + 2 for every candidate */
+                    const float rm = dn_row_max(lane < nrows ? pc.top : 0.0f);
+                    uint32_t mb = (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 0);
+                    mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 16));
+                    mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 32));
+                    mb = max(mb, (uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(rm), 48));
+                    const float mm = __uint_as_float(mb);
+                    const uint32_t kneed = (uint32_t)(mm + 24.0f) + 4u;
+                    const uint32_t kq = nrows ? min(kb, (kneed + 127u) & ~127u) : 0u;
+                    if (gtid == 0) {
+                        sh_kq[mi] = kq;
+                        sh_kneed[mi] = kneed;
+                    }
+                    uint32_t *map = maps + mi * (D8_ROWS * D8_MSTR);
+                    auto peaks = [&](auto chk) {
+                        constexpr bool CHECK = decltype(chk)::value;
+                        if (HFX_WAVE * gw + lane < pc.np)
+                            dn_peaks_all<CHECK, D8_SPREAD>(map + (pc.pos.q - p0) * D8_MSTR + 1,
+                                                           pc.pos.b, pc.pos.y, kq, lane, lvl);
+                        for (uint32_t i = HFX_WAVE * (gw + D8_GW) + lane; i < pc.np;
+                             i += HFX_WAVE * D8_GW) {
+                            const hfx_dn_pos v = a.lad.pos[pc.L + i];
+                            dn_peaks_all<CHECK, D8_SPREAD>(map + (v.q - p0) * D8_MSTR + 1, v.b, v.y,
+                                                           kq, lane, lvl);
                         }
-                        /* C[16 r + 4 kg + i][col] of the three digits as one
-                         * int64 into the tile's sums */
-                        uint64_t *cb = comb + r * 512 + (4 * kg) * 32 + col + 16 * (kg & 1u);
+                    };
+                    if (kneed <= kq) peaks(std::false_type{});
+                    else peaks(std::true_type{});
+                    pc = nx;
+                    rn = rn2;
+                }
+            } else if (k >= 1 && k <= ntile) {
+                /* 3. the contraction of tile k - 1 from map mo into sums mo */
+                const uint32_t mo = mi ^ 1u;
+                const uint32_t nrows = rows_of(p0_of(k - 1));
+                const uint32_t kq = sh_kq[mo], kneed = sh_kneed[mo];
+                uint32_t *map = maps + mo * (D8_ROWS * D8_MSTR);
+                uint64_t *cb0 = combs + mo * (D8_COMB_BYTES / 8);
+                /* the clamped peaks' words past the tile's bins, cleared before
+                 * the map's next peaks (after the barrier) */
+                if (kneed > kq && gw == D8_GW - 1)
+                    for (uint32_t i = lane; i < D8_ROWS * D8_TRASH; i += HFX_WAVE) {
+                        const uint32_t r = i / D8_TRASH;
+                        map[r * D8_MSTR + 1 + (kq >> 3) + (i - r * D8_TRASH)] = 0u;
+                    }
+                const uint32_t nks = kq >> 6, nrb = (nrows + 15u) >> 4;
+                if (gw < nks) {
 #pragma unroll
-                        for (uint32_t i = 0; i < 4; ++i) {
-                            const int64_t v = ((int64_t)acc[2][i] << 16) +
-                                              ((int64_t)acc[1][i] << 8) + (int64_t)acc[0][i];
-                            atomicAdd((unsigned long long *)(cb + 32 * i), (unsigned long long)v);
+                    for (uint32_t r = 0; r < D8_RB; ++r) {
+                        if (r < nrb) {
+                            d8_i32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+                            uint32_t *arow = map + (r * 16u + col) * D8_MSTR + 1 + kg;
+#pragma unroll
+                            for (uint32_t j = 0; j < D8_KPW; ++j) {
+                                const uint32_t ks = gw + D8_GW * j;
+                                if (ks < nks) {
+                                    uint32_t *aw = arow + 8 * ks;
+                                    const uint32_t w0 = aw[0], w1 = aw[4];
+                                    aw[0] = 0u;
+                                    aw[4] = 0u;
+                                    d8_i32x4 av;
+                                    av.x = (int32_t)d8_wbytes(w0 & 0x0f0f0f0fu);
+                                    av.y = (int32_t)d8_wbytes((w0 >> 4) & 0x0f0f0f0fu);
+                                    av.z = (int32_t)d8_wbytes(w1 & 0x0f0f0f0fu);
+                                    av.w = (int32_t)d8_wbytes((w1 >> 4) & 0x0f0f0f0fu);
+#pragma unroll
+                                    for (uint32_t d = 0; d < 3; ++d)
+                                        acc[d] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                            av, bq[j][d], acc[d], 0, 0, 0);
+                                }
+                            }
+                            /* C[16 r + 4 kg + i][col] of the three digits as
+                             * one int64 into the tile's sums */
+                            uint64_t *cb = cb0 + r * 256 + (4 * kg) * 16 + col;
+#pragma unroll
+                            for (uint32_t i = 0; i < 4; ++i) {
+                                const int64_t v = ((int64_t)acc[2][i] << 16) +
+                                                  ((int64_t)acc[1][i] << 8) + (int64_t)acc[0][i];
+                                atomicAdd((unsigned long long *)(cb + 16 * i),
+                                          (unsigned long long)v);
+                            }
                         }
                     }
                 }
             }
-            __syncthreads(); /* (c) the tile's sums are complete */
-            for (uint32_t o = tid; o < D8_ROWS * D8_NS; o += D8_THREADS) {
-                /* output (column g = o % 16, tile row 16 rbx + row): the two
-                 * half-waves take rows four apart */
-                const uint32_t g = o & 15u, qq = o >> 4, rbx = qq >> 4;
-                const uint32_t row = ((qq >> 3) & 1u) << 3 | (qq & 1u) << 2 | ((qq >> 1) & 3u);
-                const uint32_t r = rbx * 16u + row;
-                uint64_t *cb = comb + rbx * 512 + row * 32 + g + 16 * ((row >> 2) & 1u);
-                const int64_t v = (int64_t)*cb;
-                *cb = 0ull;
-                const float sc = (float)((double)v * (double)m_q[g]);
-                const uint32_t p = p0 + r;
-                const bool ok = r < nrows && m_z[g] == lvl && m_beg[g] <= p && p < m_end[g];
-                *(ok ? a.scores + m_off[g] + (p - m_beg[g]) : dummy) = sc;
-            }
-        };
-        for (uint32_t j = 0; j < ntile; j += 2) {
-            tile(X, Y, cbeg + j * D8_ROWS);
-            tile(Y, X, cbeg + (j + 1) * D8_ROWS);
+            __syncthreads();
         }
     }
 }
