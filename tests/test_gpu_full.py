@@ -4,9 +4,9 @@
                    10,000 spectra at 2+/3+, top-10;
   tryptic-20k-20k  the same database, 20,000 spectra at 2+/3+, top-5;
 
-through the plain FP32 form, the batched FP32 form and the batched BF16-split
-dense form, each against the oracle (dense-row scoring, exact fixed-point
-scores, OpenMP):
+through the plain FP32 form, the batched FP32 form, the batched BF16-split
+dense form and the batched integer dense form, each against the oracle
+(dense-row scoring, exact fixed-point scores, OpenMP):
   - windows identical; top-N positions and exact scores identical for every
     spectrum;
   - every score of every form within 1e-4 * max(|exact|, FLOOR * scale),
@@ -18,8 +18,9 @@ scores, OpenMP):
     represents each x_b to 2^-17 relative, and each of the <= 2 * 2 * 4 * 49
     nonzero products adds at most 2^-24 of the running sum, a worst case of
     ~5e-5 * scale whose random-sign typical value is ~1e-6 * scale;
-  - every dense score within its derived bound (2^-16 + 210 * 2^-24) * 50 *
-    sum|x|;
+  - every BF16 dense score within its derived bound (2^-16 + 210 * 2^-24) *
+    50 * sum|x|, every integer-form score within max|x| / 8355711 * 50 len +
+    2^-24 * 50 sum|x|;
   - the dense scores of each spectrum's top N within 1e-4 relative of their
     exact values.
 """
