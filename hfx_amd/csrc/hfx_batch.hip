@@ -1227,7 +1227,11 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     if ((rc = hfx_batch_run(b, stream))) return rc;
     if (out_cfg) {
         out_cfg[0] = bG; out_cfg[1] = bC; out_cfg[2] = bT;
-        if (bt_is_dense(b->precision)) { out_cfg[0] = 0; out_cfg[1] = bCH; out_cfg[2] = 1024; }
+        if (bt_is_dense(b->precision)) {
+            out_cfg[0] = 0;
+            out_cfg[1] = bCH;
+            out_cfg[2] = b->precision == HFX_BATCH_I8X3_DENSE ? 768u : 1024u; /* workgroup threads */
+        }
     }
     return HFX_OK;
 }

@@ -482,7 +482,8 @@ int hfx_batch_capture(hfx_batch *b, void *stream);
 int hfx_batch_replay(hfx_batch *b, void *stream);
 /* time the scoring kernel over (spectra per block G, tile length C,
  * workgroup threads) -- dense form: candidates per work item C -- and keep
- * the fastest; out_cfg = {G, C, threads} ({0, C, 256} for the dense form) */
+ * the fastest; out_cfg = {G, C, threads} ({0, C, workgroup threads} for the
+ * dense forms: 1024 BF16, 768 integer) */
 int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg);
 int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads);
 /* one run, phase by phase, with event timings in ms (5 entries): {windows,
