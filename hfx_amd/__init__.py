@@ -194,6 +194,24 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P, C.POINTER(HfxModset), C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _P],
     ),
+    "hfx_evalue_fit": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _U32, _U32, _F, _P]),
+    "hfx_evalues": (_I, [_P, _P, _P, _P, _P, _U32, _U32, _P]),
+    "hfx_sp_peak_words": (_U32, [_U32]),
+    "hfx_sp_peaks": (_I, [_P, _P, _U32, C.POINTER(HfxSpectra), _P]),
+    "hfx_sp_ions": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, _P, _U64, _P, _U32, C.POINTER(HfxDb), C.POINTER(HfxSpectra),
+         _I, _P],
+    ),
+    "hfx_sp_ions_topn": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, _P, _U32, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _I, _P],
+    ),
+    "hfx_sp_ions_topn_mod": (
+        _I,
+        [_P, _P, _P, _P, _P, _P, _P, _U32, C.POINTER(HfxModset), C.POINTER(HfxDb),
+         C.POINTER(HfxSpectra), _I, _P],
+    ),
     "hfx_pepix_build": (_I, [_P, C.POINTER(HfxDb), _U32, _U32, _U32, _P, C.POINTER(_P)]),
     "hfx_pepix_destroy": (_I, [_P]),
     "hfx_pepix_info": (_I, [_P, _P]),

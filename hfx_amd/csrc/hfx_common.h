@@ -86,6 +86,7 @@ enum {
     HFX_SLOT_FLAGS = 11,  /* spectrum lists turned into flags    */
     HFX_SLOT_LADDER = 12, /* dense ladders: offsets, tops, counts */
     HFX_SLOT_LADDER2 = 13, /* dense ladders: positions           */
+    HFX_SLOT_STATS = 14,  /* SP ion peak flags                   */
 };
 /* Scratch owned by a context: grown on demand, never shrunk. */
 int hfx_ctx_scratch(hfx_ctx *ctx, int slot, size_t bytes, void **out);

@@ -374,6 +374,51 @@ class PlainSearch:
                                    ptr(self.t_begin), ptr(self.t_num), C.byref(self.db.desc),
                                    C.byref(self.sp.desc), stream_handle()), "hfx_topn_rel_q")
 
+    def evalue_fit(self, mods: "ModSearch | None" = None, bin_width: int = 350,
+                   perc: float = 3.0):
+        """hfx_evalue_fit over this search's scores (after run()), pooled with
+        the modified-peptide scores of `mods` (after its run()), as
+        Search.hs:118-121 pools allMatches.  Keeps the device fit for
+        evalues(); returns (fit [S, 2] alpha/beta, points [S], kept [S])."""
+        S = self.sp.num_spec
+        self.t_fit = empty(16 * S)
+        t_np, t_nu = empty(4 * S), empty(8 * S)
+        sb = ob = None
+        if mods is not None:
+            sb, ob = ptr(mods._scored[0]), ptr(mods._scored[1])
+        check(lib().hfx_evalue_fit(self.ctx.h, ptr(self.t_fit), ptr(t_np), ptr(t_nu),
+                                   ptr(self.t_scores), ptr(self.t_off), sb, ob, S,
+                                   int(bin_width), float(perc), stream_handle()),
+              "hfx_evalue_fit")
+        sync()
+        return (host(self.t_fit, np.float64, 2 * S).reshape(S, 2), host(t_np, np.uint32, S),
+                host(t_nu, np.uint64, S))
+
+    def evalues(self, scores: np.ndarray, cnt: np.ndarray | None = None) -> np.ndarray:
+        """hfx_evalues: E-values of [S, N] scores under the last fit."""
+        S = self.sp.num_spec
+        sc = np.ascontiguousarray(scores, np.float64).reshape(S, -1)
+        N = sc.shape[1]
+        t_sc, t_ev = dev(sc), empty(8 * S * N)
+        t_cnt = dev(np.asarray(cnt, np.uint32)) if cnt is not None else None
+        check(lib().hfx_evalues(self.ctx.h, ptr(t_ev), ptr(self.t_fit), ptr(t_sc),
+                                ptr(t_cnt) if t_cnt is not None else None, N, S,
+                                stream_handle()), "hfx_evalues")
+        sync()
+        return host(t_ev, np.float64, S * N).reshape(S, N)
+
+    def sp_ions(self, mono: bool = True):
+        """hfx_sp_ions_topn over the last top N: (b matched, y matched,
+        total), each [S, N]."""
+        S, N = self.sp.num_spec, self.topn
+        t = [empty(4 * S * N) for _ in range(3)]
+        check(lib().hfx_sp_ions_topn(self.ctx.h, *[ptr(x) for x in t], ptr(self.t_pos),
+                                     ptr(self.t_cnt), ptr(self.t_begin), N, C.byref(self.db.desc),
+                                     C.byref(self.sp.desc), int(mono), stream_handle()),
+              "hfx_sp_ions_topn")
+        sync()
+        return tuple(host(x, np.uint32, S * N).reshape(S, N) for x in t)
+
     def exact(self, spectra: np.ndarray):
         """hfx_topn_exact for the listed spectra (every candidate exact)."""
         lst = np.asarray(spectra, np.uint32)
@@ -731,6 +776,7 @@ class ModSearch:
         # keep the device arrays alive as long as the result references them
         self._keep = (t_mp, t_unrank, t_pep, t_pmod, t_pcnt, t_nmp, t_comb, t_mpoff)
         self._scored = (t_scores, t_mpoff, nmp)
+        self._top = (t_pos, t_cnt)
         return ModResult(
             spec_cand=spec_cand, num_valid=int(nv), num_mpep=int(nmp), len_unrank=int(lu),
             mp_off=mp_off, top_pos=host(t_pos, np.uint32, S * N).reshape(S, N),
@@ -760,7 +806,62 @@ class ModSearch:
                 host(t_ex, np.float64, S * N).reshape(S, N), host(t_cnt, np.uint32, S))
 
 
-def merge_rankings(plain: PlainResult, mods: ModResult, n: int):
+    def sp_ions(self, mono: bool = True):
+        """hfx_sp_ions_topn_mod over the last top N: (b matched, y matched,
+        total), each [S, N]; matched on the unmodified peptide."""
+        S, N = self.sp.num_spec, self.topn
+        t_pos, t_cnt = self._top
+        t = [empty(4 * S * N) for _ in range(3)]
+        check(lib().hfx_sp_ions_topn_mod(self.ctx.h, *[ptr(x) for x in t], ptr(t_pos), ptr(t_cnt),
+                                         ptr(self._scored[1]), N, C.byref(self.modset),
+                                         C.byref(self.db.desc), C.byref(self.sp.desc), int(mono),
+                                         stream_handle()), "hfx_sp_ions_topn_mod")
+        sync()
+        return tuple(host(x, np.uint32, S * N).reshape(S, N) for x in t)
+
+
+@dataclass
+class Match:
+    """One reported match (Match.hs:17-27): exact score (XCorr * 10000),
+    unmodified (0) or modified (1), window or modified-peptide position,
+    E-value, and Ions = (matched, total)."""
+    score: float
+    is_mod: int
+    index: int
+    evalue: float
+    ions: tuple
+
+
+def finish_matches(ps: PlainSearch, plain: PlainResult, n: int, ms: "ModSearch | None" = None,
+                   mods: ModResult | None = None, mono: bool = True, bin_width: int = 350,
+                   perc: float = 3.0) -> list[list[Match]]:
+    """Search.hs:115-141 (and :292-299 with modifications): per spectrum the
+    best n matches of both searches in reported order, each with its E-value
+    (fit over every score of the spectrum, unmodified and modified, on the
+    device) and its SP ion count (on the device)."""
+    ps.evalue_fit(ms if mods is not None else None, bin_width, perc)
+    ev_p = ps.evalues(plain.top_exact, plain.top_cnt)
+    nb, ny, tot = ps.sp_ions(mono)
+    if mods is not None:
+        ev_m = ps.evalues(mods.top_exact, mods.top_cnt)
+        mnb, mny, mtot = ms.sp_ions(mono)
+    out = []
+    for s, items in enumerate(merge_rankings(plain, mods, n)):
+        row = []
+        for sc, is_mod, idx in items:
+            if is_mod:
+                k = int(np.nonzero(mods.top_pos[s, : mods.top_cnt[s]] == idx)[0][0])
+                row.append(Match(sc, 1, idx, float(ev_m[s, k]),
+                                 (int(mnb[s, k] + mny[s, k]), int(mtot[s, k]))))
+            else:
+                k = int(np.nonzero(plain.top_pos[s, : plain.top_cnt[s]] == idx)[0][0])
+                row.append(Match(sc, 0, idx, float(ev_p[s, k]),
+                                 (int(nb[s, k] + ny[s, k]), int(tot[s, k]))))
+        out.append(row)
+    return out
+
+
+def merge_rankings(plain: PlainResult, mods: ModResult | None, n: int):
     """Search.hs:118-123: per spectrum the unmodified and modified matches
     merged by score descending; on equal scores unmodified matches first, then
     lower index.  Returns per spectrum a list of (exact score, is_mod, index)."""
@@ -768,8 +869,9 @@ def merge_rankings(plain: PlainResult, mods: ModResult, n: int):
     for s in range(len(plain.top_cnt)):
         items = [(float(plain.top_exact[s, k]), 0, int(plain.top_pos[s, k]))
                  for k in range(int(plain.top_cnt[s]))]
-        items += [(float(mods.top_exact[s, k]), 1, int(mods.top_pos[s, k]))
-                  for k in range(int(mods.top_cnt[s]))]
+        if mods is not None:
+            items += [(float(mods.top_exact[s, k]), 1, int(mods.top_pos[s, k]))
+                      for k in range(int(mods.top_cnt[s]))]
         items.sort(key=lambda t: (-t[0], t[1], t[2]))
         out.append(items[:n])
     return out

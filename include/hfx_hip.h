@@ -410,6 +410,60 @@ int hfx_topn_exact_mod(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, f
                        void *stream);
 
 /* ------------------------------------------------------------------ */
+/* E-values and SP ions of the matches (Search.hs:115-141, :292-299).   */
+/* The reference computes both on the host, over every score of a       */
+/* spectrum copied back from the device (Search.hs:250-276).            */
+/* ------------------------------------------------------------------ */
+/* Replaces Util/Statistics.hs:37-100 evalues/linearFitLine/histogram:
+ * per spectrum s the (alpha, beta) of the least-squares line through the
+ * histogram of its scores -- the scores of set a in [d_off_a[s],
+ * d_off_a[s+1]) pooled with those of the optional set b (the modified
+ * matches, Search.hs:118) -- after the reference's cut (sorted positions
+ * [round(0.3 n), round((1 - cutoff/100) n)), shift by 1 - min, keep > 4000),
+ * in bins of bin_width (evalue-bin-width, Config.hs:100-101).  d_fit[2s] =
+ * alpha, d_fit[2s+1] = beta; NaN when fewer than two histogram points (the
+ * reference's 0/0).  d_npts (optional): histogram points; d_nused (optional):
+ * scores kept.  No host synchronisation or allocation. */
+int hfx_evalue_fit(hfx_ctx *ctx, double *d_fit, uint32_t *d_npts, uint64_t *d_nused,
+                   const float *d_scores_a, const uint64_t *d_off_a, const float *d_scores_b,
+                   const uint64_t *d_off_b, uint32_t num_spec, uint32_t bin_width,
+                   float cutoff_perc, void *stream);
+/* Statistics.hs:41: d_ev[s][k] = 10 ** (alpha_s + beta_s * d_score[s][k] /
+ * 10000) for k < d_cnt[s] (all N when d_cnt is null), 0 beyond. */
+int hfx_evalues(hfx_ctx *ctx, double *d_ev, const double *d_fit, const double *d_score,
+                const uint32_t *d_cnt, uint32_t N, uint32_t num_spec, void *stream);
+/* Replaces IonSeries.hs:99-107 extractPeaks: per spectrum `words` uint32
+ * flag words, bit t set when bin t (= position div 3) holds one of the 200
+ * largest values and that value is above zero.  words >=
+ * hfx_sp_peak_words(x_stride); strides up to 24575. */
+uint32_t hfx_sp_peak_words(uint32_t x_stride);
+int hfx_sp_peaks(hfx_ctx *ctx, uint32_t *d_on, uint32_t words, const hfx_spectra *sp,
+                 void *stream);
+/* Replaces IonSeries.hs:136-147 matchIonSequence + Match.hs:33-40 scoreSP
+ * (with Location.hs:42-58 lookup) for num_match (peptide, spectrum) pairs:
+ * d_nb / d_ny = matched b / y ions, d_total = (residues - 1 + d_nmod) * 2
+ * (d_nmod optional: modified sites, Fragment.hs:84-106); a peptide index of
+ * 0xFFFFFFFF (or any index past the database, or a spectrum past
+ * num_spec) gives zeros.  mono selects the bin width (Correlation.hs:29-30). */
+int hfx_sp_ions(hfx_ctx *ctx, uint32_t *d_nb, uint32_t *d_ny, uint32_t *d_total,
+                const uint32_t *d_pep, const uint32_t *d_spec, const uint32_t *d_nmod,
+                uint64_t num_match, const uint32_t *d_on, uint32_t words, const hfx_db *db,
+                const hfx_spectra *sp, int mono, void *stream);
+/* hfx_sp_peaks + hfx_sp_ions over the top N of an unmodified search
+ * (hfx_topn layout: [S][N], entries beyond d_top_cnt[s] get zeros), flags in
+ * the context's scratch (allocated on first use). */
+int hfx_sp_ions_topn(hfx_ctx *ctx, uint32_t *d_nb, uint32_t *d_ny, uint32_t *d_total,
+                     const uint32_t *d_top_pos, const uint32_t *d_top_cnt,
+                     const uint32_t *d_begin, uint32_t N, const hfx_db *db,
+                     const hfx_spectra *sp, int mono, void *stream);
+/* the same over the top N of hfx_topn_mod: matched on the unmodified
+ * peptide (Search.hs:296-299), total grown by the modified sites */
+int hfx_sp_ions_topn_mod(hfx_ctx *ctx, uint32_t *d_nb, uint32_t *d_ny, uint32_t *d_total,
+                         const uint32_t *d_top_pos, const uint32_t *d_top_cnt,
+                         const uint64_t *d_mp_off, uint32_t N, const hfx_modset *ms,
+                         const hfx_db *db, const hfx_spectra *sp, int mono, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Batched search (new; Main.hs:129-142 scores 16 spectra per group).    */
 /* A batch of any number of spectra is planned once (windows, index      */
 /* sizes, every buffer allocated; synchronises), then run without host   */

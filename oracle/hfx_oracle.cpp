@@ -982,3 +982,154 @@ int orc_search_mods(const uint8_t *ions, const float *r, const uint32_t *tc, con
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------------
+ * E-values (Util/Statistics.hs) and SP ion matches (Sequence/IonSeries.hs,
+ * Sequence/Match.hs, Sequence/Location.hs)
+ * ------------------------------------------------------------------- */
+
+/* evalues / linearFitLine / histogram (Statistics.hs:37-100) over one
+ * spectrum's match scores in any order (Search.hs:118-121 sorts allMatches
+ * ascending first):
+ *   n = #scores; cut_end = round((1 - perc/100) * n) in Float (the
+ *   percentage is a Float, Config.hs:101); cut_beg = round(30/100 * n) in
+ *   Double; GHC's round takes halves to the even neighbour;
+ *   sc'' = [(x + 1) + (-1) * head sc | x <- drop cut_beg (take cut_end sc)];
+ *   sc' = filter (> 4000) sc'';
+ *   histogram: bin i = ceiling (x / w), centre i * w - w / 2, uniqCount of
+ *   the ascending centres, height count / w;
+ *   x = centre / 10000, logh = logBase 10 h = log h / log 10;
+ *   (alpha, beta) = linearRegression x logh -- statistics-linreg (a
+ *   dependency outside the reference tree): ordinary least squares,
+ *   beta = cov(x, logh) / var(x), alpha = mean logh - beta * mean x.
+ * Fewer than two histogram points leave 0/0 in beta, i.e. NaN, as in
+ * Haskell.  Returns the number of histogram points; *nused = #sc'. */
+uint32_t orc_evalue_fit(const float *scores, uint64_t n, uint32_t bin_width, float perc,
+                        double *alpha, double *beta, uint64_t *nused)
+{
+    *alpha = NAN;
+    *beta = NAN;
+    *nused = 0;
+    if (n == 0) return 0;
+    std::vector<double> sc(n);
+    for (uint64_t i = 0; i < n; ++i) sc[i] = (double)scores[i];
+    std::sort(sc.begin(), sc.end());
+    const float fe = (1.0f - perc / 100.0f) * (float)n;
+    const int64_t cut_end = (int64_t)std::nearbyint((double)fe);
+    const int64_t cut_beg = (int64_t)std::nearbyint((30.0 / 100.0) * (double)n);
+    const int64_t ce = std::min<int64_t>(std::max<int64_t>(cut_end, 0), (int64_t)n);
+    const int64_t cb = std::max<int64_t>(cut_beg, 0);
+    const double h0 = sc[0];
+    std::vector<double> kept;
+    for (int64_t i = cb; i < ce; ++i) {
+        const double y = (sc[i] + 1.0) + (-1.0) * h0;
+        if (y > 4000.0) kept.push_back(y);
+    }
+    *nused = kept.size();
+    const double w = (double)bin_width, half = w / 2.0;
+    std::vector<double> xs, ys;
+    for (size_t i = 0; i < kept.size();) {
+        const double c = std::ceil(kept[i] / w) * w - half;
+        size_t j = i + 1;
+        while (j < kept.size() && std::ceil(kept[j] / w) * w - half == c) ++j;
+        xs.push_back(c / 10000.0);
+        ys.push_back(std::log((double)(j - i) / w) / std::log(10.0));
+        i = j;
+    }
+    const size_t m = xs.size();
+    double mx = 0.0, my = 0.0;
+    for (size_t i = 0; i < m; ++i) {
+        mx += xs[i];
+        my += ys[i];
+    }
+    mx /= (double)m;
+    my /= (double)m;
+    double sxx = 0.0, sxy = 0.0;
+    for (size_t i = 0; i < m; ++i) {
+        const double dx = xs[i] - mx;
+        sxx += dx * dx;
+        sxy += dx * (ys[i] - my);
+    }
+    *beta = sxy / sxx;
+    *alpha = my - *beta * mx;
+    return (uint32_t)m;
+}
+
+/* Statistics.hs:41: E(x) = 10 ** (alpha + beta * (x / 10000)) */
+double orc_evalue(double alpha, double beta, double score)
+{
+    return std::pow(10.0, alpha + beta * (score / 10000.0));
+}
+
+/* extractPeaks (IonSeries.hs:99-107): the 200 largest values of the
+ * spectrum, each folded with max into entry i div 3 of len/3 + 1 zeros;
+ * out[t] = 1 where that entry ends above zero (the test matchIonSequence
+ * applies, IonSeries.hs:146-147).  Ties at the 200th value keep the lowest
+ * bins (the reference leaves them to vector-algorithms' introselect). */
+void orc_extract_peaks(const float *x, uint32_t len, uint8_t *out)
+{
+    std::vector<uint32_t> idx(len);
+    for (uint32_t i = 0; i < len; ++i) idx[i] = i;
+    const size_t k = std::min<size_t>(200, len);
+    std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](uint32_t a, uint32_t b) {
+        return x[a] > x[b] || (x[a] == x[b] && a < b);
+    });
+    std::vector<float> peaks(len / 3 + 1, 0.0f);
+    for (size_t i = 0; i < k; ++i) {
+        const uint32_t t = idx[i] / 3;
+        peaks[t] = peaks[t] <= x[idx[i]] ? x[idx[i]] : peaks[t];
+    }
+    for (uint32_t t = 0; t < len / 3 + 1; ++t) out[t] = peaks[t] > 0.0f;
+}
+
+/* lookup (Location.hs:42-58) + matchIonSequence (IonSeries.hs:136-147) +
+ * scoreSP (Match.hs:33-40) for one peptide against one spectrum's peak flags
+ * (orc_extract_peaks, npk = len/3 + 1 entries):
+ *   residues ions[tc .. tn], the last one dropped when tn is the database's
+ *   end; fragment mass r = (res + H) + H2O with the host constants;
+ *   ions = concat over c = 1 .. zmax of ionMZ(p_k, c) = (p_k + H * c) / c,
+ *   p = scanl1 (+) of the residue masses (FP32, left to right);
+ *   bin x = round (x / bin width / 3), halves to even; matched when
+ *   0 <= bin < npk and the flag is set;
+ *   b = init (matched ions), y = tail (matched (map (r -) ions));
+ *   total = (#residues - 1 + modified sites) * 2 (modifyFragment adds one
+ *   '*' per site, Fragment.hs:84-106).
+ * b_flags / y_flags (optional) receive the n_ions - 1 flags of each list. */
+void orc_match_ions(const uint8_t *ions, uint64_t num_ions, uint32_t tc, uint32_t tn, float res,
+                    const float *mass26, uint32_t zmax, int mono, const uint8_t *on, uint32_t npk,
+                    uint32_t nmod, uint32_t *out_nb, uint32_t *out_ny, uint32_t *out_total,
+                    uint8_t *b_flags, uint8_t *y_flags)
+{
+    const uint32_t ntake = (uint64_t)tn < num_ions ? tn - tc + 1 : tn - tc;
+    const float r = (res + HOST_H) + HOST_H2O;
+    const float bw = mono ? 1.0005079f : 1.0011413f;
+    auto matched = [&](float v) {
+        const float q = std::nearbyint((v / bw) / 3.0f);
+        return q >= 0.0f && q < (float)npk && on[(uint32_t)q] != 0;
+    };
+    const uint32_t nz = zmax < 1 ? 1 : zmax;
+    const uint64_t n_ions = (uint64_t)ntake * nz;
+    uint32_t nb = 0, ny = 0;
+    uint64_t j = 0;
+    for (uint32_t c = 1; c <= nz; ++c) {
+        const float cf = (float)c;
+        float p = 0.0f;
+        for (uint32_t k = 0; k < ntake; ++k, ++j) {
+            p = p + mass26[acid_idx(ions[tc + k])];
+            const float mz = (p + HOST_H * cf) / cf;
+            if (j + 1 < n_ions) {
+                const bool m = matched(mz);
+                nb += m;
+                if (b_flags) b_flags[j] = m;
+            }
+            if (j >= 1) {
+                const bool m = matched(r - mz);
+                ny += m;
+                if (y_flags) y_flags[j - 1] = m;
+            }
+        }
+    }
+    *out_nb = nb;
+    *out_ny = ny;
+    *out_total = (ntake - 1 + nmod) * 2;
+}

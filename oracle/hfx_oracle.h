@@ -173,6 +173,19 @@ int orc_search_mods(const uint8_t *ions, const float *r, const uint32_t *tc, con
                     uint32_t *out_unrank, uint64_t cap_mpep, uint64_t cap_unrank,
                     uint64_t *out_num_mpep, uint64_t *out_len_unrank, int nthreads);
 
+/* ---- E-values and SP ions ------------------------------------------ */
+/* (alpha, beta) of one spectrum's scores; returns the histogram points */
+uint32_t orc_evalue_fit(const float *scores, uint64_t n, uint32_t bin_width, float perc,
+                        double *alpha, double *beta, uint64_t *nused);
+double   orc_evalue(double alpha, double beta, double score);
+/* out[len/3 + 1]: 1 where extractPeaks leaves a value above zero */
+void     orc_extract_peaks(const float *x, uint32_t len, uint8_t *out);
+void     orc_match_ions(const uint8_t *ions, uint64_t num_ions, uint32_t tc, uint32_t tn,
+                        float res, const float *mass26, uint32_t zmax, int mono,
+                        const uint8_t *on, uint32_t npk, uint32_t nmod, uint32_t *out_nb,
+                        uint32_t *out_ny, uint32_t *out_total, uint8_t *b_flags,
+                        uint8_t *y_flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -382,3 +382,50 @@ def check_fast_div(d: float, lo: float, hi: float, corrected: bool = True) -> in
     (corrected=False: of the uncorrected x * RN(1/d))."""
     return int(lib().orc_check_fast_div(C.c_float(d), C.c_float(lo), C.c_float(hi),
                                         int(corrected)))
+
+
+def evalue_fit(scores, bin_width: int = 350, perc: float = 3.0):
+    """(alpha, beta, histogram points, scores kept) of one spectrum's scores
+    (Statistics.hs:37-100)."""
+    s = f32(scores)
+    a, b, nu = C.c_double(), C.c_double(), C.c_uint64()
+    L = lib()
+    L.orc_evalue_fit.restype = C.c_uint32
+    m = L.orc_evalue_fit(_p(s), C.c_uint64(len(s)), C.c_uint32(bin_width), C.c_float(perc),
+                         C.byref(a), C.byref(b), C.byref(nu))
+    return a.value, b.value, int(m), int(nu.value)
+
+
+def evalue(alpha: float, beta: float, score: float) -> float:
+    L = lib()
+    L.orc_evalue.restype = C.c_double
+    L.orc_evalue.argtypes = [C.c_double] * 3
+    return float(L.orc_evalue(alpha, beta, score))
+
+
+def extract_peaks(x) -> np.ndarray:
+    """uint8[len/3 + 1]: 1 where extractPeaks leaves a value above zero."""
+    x = f32(x)
+    out = np.zeros(len(x) // 3 + 1, np.uint8)
+    lib().orc_extract_peaks(_p(x), C.c_uint32(len(x)), _p(out))
+    return out
+
+
+def match_ions(ions_packed, tc, tn, res, mass26, zmax, on, mono=True, nmod=0, flags=False):
+    """(b matched, y matched, total) of one peptide against one spectrum's
+    peak flags (IonSeries.hs:136-147, Match.hs:33-40); flags=True also
+    returns the b and y flag lists."""
+    ions = np.ascontiguousarray(ions_packed, np.uint8)
+    on = np.ascontiguousarray(on, np.uint8)
+    nb, ny, tot = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    ntake = tn - tc + 1 if tn < len(ions) else tn - tc
+    nI = max(ntake * max(int(zmax), 1), 1)
+    bf = np.zeros(nI, np.uint8)
+    yf = np.zeros(nI, np.uint8)
+    lib().orc_match_ions(_p(ions), C.c_uint64(len(ions)), C.c_uint32(tc), C.c_uint32(tn),
+                         C.c_float(res), _p(f32(mass26)), C.c_uint32(zmax), C.c_int(int(mono)),
+                         _p(on), C.c_uint32(len(on)), C.c_uint32(nmod), C.byref(nb), C.byref(ny),
+                         C.byref(tot), _p(bf), _p(yf))
+    if flags:
+        return nb.value, ny.value, tot.value, bf[:nI - 1].copy(), yf[:nI - 1].copy()
+    return nb.value, ny.value, tot.value

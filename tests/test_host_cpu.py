@@ -148,3 +148,77 @@ def test_batch_sizing_and_split():
     nb = C.c_uint32()
     rc = L.hfx_batch_split(off.ctypes.data, S, 5, 2_700_000, 0, one // 2, None, 0, C.byref(nb))
     assert rc == 3 and "alone needs" in hfx_amd.last_error()
+
+
+def _reference_merge(plain, mods, n):
+    """Search.hs:118-123 with sequestXC's retrieval (Search.hs:257-276),
+    literally: each search returns `reverse` of its scores stably sorted
+    ascending (b40c radix sort over window order), the two lists are
+    concatenated modified first, stably sorted ascending by score
+    (matchScoreOrder) and reversed."""
+    def retrieved(items):
+        return list(reversed(sorted(items, key=lambda t: t[0])))
+
+    m = [(s, 0, p) for s, p in retrieved([(s, p) for p, s in enumerate(plain)])]
+    mm = [(s, 1, p) for s, p in retrieved([(s, p) for p, s in enumerate(mods)])]
+    return list(reversed(sorted(mm + m, key=lambda t: t[0])))[:n]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_merge_rankings_against_reference_order(seed):
+    """merge_rankings (score descending; unmodified first, then lower
+    position, on equal scores) against a literal restatement of the
+    reference's sorts and reverses, with many equal scores."""
+    rng = np.random.default_rng(seed)
+    S, N, n = 6, 5, 5
+    plains = [list(rng.integers(0, 6, int(rng.integers(0, 12))).astype(float)) for _ in range(S)]
+    modss = [list(rng.integers(0, 6, int(rng.integers(0, 12))).astype(float)) for _ in range(S)]
+
+    def top(lst):
+        order = sorted(range(len(lst)), key=lambda p: (-lst[p], p))[:N]
+        pos = np.zeros(N, np.uint32)
+        ex = np.zeros(N)
+        pos[:len(order)] = order
+        ex[:len(order)] = [lst[p] for p in order]
+        return pos, ex, len(order)
+
+    def result(lists, cls):
+        tp = np.zeros((S, N), np.uint32)
+        te = np.zeros((S, N))
+        tc = np.zeros(S, np.uint32)
+        for s, lst in enumerate(lists):
+            tp[s], te[s], tc[s] = top(lst)
+        z = np.zeros(S, np.uint32)
+        if cls is pipeline.PlainResult:
+            return cls(begin=z, num=z, off=z, top_pos=tp, top_exact=te, top_f32=te.astype(np.float32),
+                       top_cnt=tc, flagged=z[:0])
+        return cls(spec_cand=z, num_valid=0, num_mpep=0, len_unrank=0, mp_off=z, top_pos=tp,
+                   top_exact=te, top_f32=te.astype(np.float32), top_cnt=tc, flagged=z[:0],
+                   stages={})
+
+    got = pipeline.merge_rankings(result(plains, pipeline.PlainResult),
+                                  result(modss, pipeline.ModResult), n)
+    for s in range(S):
+        assert got[s] == _reference_merge(plains[s], modss[s], n), s
+
+
+def test_stats_entry_points_reject_bad_arguments():
+    """hfx_evalue_fit / hfx_sp_peaks / hfx_sp_ions validate before any device
+    work: a zero bin width, a cut-off outside [0, 100], a lone second score
+    set, too few flag words, a stride beyond the flag capacity."""
+    L = hfx_amd.lib()
+    buf = (C.c_uint64 * 64)()
+    p = C.cast(buf, C.c_void_p)
+    assert L.hfx_evalue_fit(None, p, None, None, p, p, None, None, 1, 0, 3.0, None) == 1
+    assert L.hfx_evalue_fit(None, p, None, None, p, p, None, None, 1, 350, 101.0, None) == 1
+    assert L.hfx_evalue_fit(None, p, None, None, p, p, p, None, 1, 350, 3.0, None) == 1
+    assert L.hfx_evalue_fit(None, None, None, None, p, p, None, None, 1, 350, 3.0, None) == 1
+    assert L.hfx_sp_peak_words(2112) == (2112 // 3 + 1 + 31) // 32
+    sp = hfx_amd.HfxSpectra(p.value, 2112, p.value, p.value, p.value, 1)
+    assert L.hfx_sp_peaks(None, p, L.hfx_sp_peak_words(2112) - 1, C.byref(sp), None) == 1
+    big = hfx_amd.HfxSpectra(p.value, 24576, p.value, p.value, p.value, 1)
+    assert L.hfx_sp_peaks(None, p, 4096, C.byref(big), None) == 1
+    db = hfx_amd.HfxDb()
+    assert L.hfx_sp_ions(None, p, p, p, p, p, None, 1, p, 1, C.byref(db), C.byref(sp), 1,
+                         None) == 1
+    assert "flag words" in hfx_amd.last_error()
