@@ -241,7 +241,58 @@ def bench_plain(env, topn, steps, warmup, margin=0):
     kern_ms = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
     flagged = len(srch.flagged())
     res = srch.run()
-    return res, elapsed, kern_ms, flagged
+    return res, elapsed, kern_ms, flagged, srch
+
+
+def bench_stats(srch, steps, res):
+    """E-value fit over every score of the batch (hfx_evalue_fit, one
+    workgroup per spectrum) and the SP ion counts of the top N
+    (hfx_sp_ions_topn: peak flags, then one match per thread), each timed
+    with events on the stream they run on, after the plain search."""
+    import ctypes as C
+
+    import hfx_amd
+    from hfx_amd import pipeline
+
+    L = hfx_amd.lib()
+    S, N = srch.sp.num_spec, srch.topn
+    t_fit, t_np, t_nu = pipeline.empty(16 * S), pipeline.empty(4 * S), pipeline.empty(8 * S)
+    outs = [pipeline.empty(4 * S * N) for _ in range(3)]
+    st = pipeline.stream_handle()
+
+    def fit():
+        hfx_amd.check(L.hfx_evalue_fit(srch.ctx.h, t_fit.p, t_np.p, t_nu.p, srch.t_scores.p,
+                                       srch.t_off.p, None, None, S, 350, 3.0, st),
+                      "hfx_evalue_fit")
+
+    def ions():
+        hfx_amd.check(L.hfx_sp_ions_topn(srch.ctx.h, *[o.p for o in outs], srch.t_pos.p,
+                                         srch.t_cnt.p, srch.t_begin.p, N, C.byref(srch.db.desc),
+                                         C.byref(srch.sp.desc), 1, st), "hfx_sp_ions_topn")
+
+    fit()
+    ions()
+    pipeline.sync()
+    evs = [(pipeline.Event(), pipeline.Event(), pipeline.Event()) for _ in range(max(steps, 3))]
+    for e in evs:
+        e[0].record(st)
+        fit()
+        e[1].record(st)
+        ions()
+        e[2].record(st)
+    pipeline.sync()
+    fit_ms = float(np.mean([a.elapsed_ms(b) for a, b, _ in evs]))
+    ions_ms = float(np.mean([b.elapsed_ms(c) for _, b, c in evs]))
+    total = int(res.off[-1])
+    npts = pipeline.host(t_np, np.uint32, S)
+    fit_bytes = 4.0 * total + 16.0 * S + 16.0 * S   # scores read once, offsets, fits
+    return {"evalue_fit": {"kernel": "k_evfit", "ms": fit_ms, "scores": total,
+                           "spectra_with_fit": int((npts >= 2).sum()),
+                           "algorithmic_bytes": fit_bytes,
+                           "achieved_gbs": fit_bytes / (fit_ms * 1e-3) / 1e9,
+                           "frac_hbm": fit_bytes / (fit_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "sp_ions": {"kernels": ["k_sp_peaks", "k_sp_topn"], "ms": ions_ms,
+                        "matches": int(res.top_cnt.sum())}}
 
 
 DENSE_K = 2176  # padded bins of a dense row (the kernel's row length)
@@ -327,8 +378,8 @@ def main():
     paths = {}
     plain_res = None
     if args.path in ("all", "plain"):
-        plain_res, el, kern_ms, flagged = bench_plain(env, topn, args.steps, args.warmup,
-                                                      args.margin)
+        plain_res, el, kern_ms, flagged, psrch = bench_plain(env, topn, args.steps,
+                                                             args.warmup, args.margin)
         sc = int(plain_res.off[-1])
         ab = algorithmic_bytes(env, plain_res)
         paths["plain_fp32_sparse"] = dict(
@@ -337,6 +388,9 @@ def main():
             kernel_gbs=ab / (kern_ms * 1e-3) / 1e9, algorithmic_bytes=ab,
             flagged_spectra_per_step=flagged)
         log(f"plain: {paths['plain_fp32_sparse']['value']:.3e} scores/s, kernel {kern_ms:.2f} ms")
+        stats = bench_stats(psrch, args.steps, plain_res)
+        log(f"stats: {stats}")
+        del psrch
     # the per-database peptide index, built once
     index = None
     forms = 0
@@ -455,6 +509,8 @@ def main():
                 "max_len": int(env["dsp"].lengths().max()), "topn": topn, "margin": args.margin,
                 "path": main_key, "scores_per_step": p["scores_per_step"]},
         roofline=roofline, db_setup_ms=db_setup.get("ms"), db_setup=db_setup, paths=paths)
+    if "plain_fp32_sparse" in paths:
+        out["stats"] = stats
     if "plain_fp32_sparse" in paths and main_key != "plain_fp32_sparse":
         out["batched_over_plain"] = p["value"] / paths["plain_fp32_sparse"]["value"]
         out["target_batched_over_plain"] = 10.0

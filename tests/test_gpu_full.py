@@ -66,6 +66,7 @@ def fenv(request, orc):
     b8 = (xm / 8355711.0 * 50 * lens.astype(np.float64) + 2.0**-24 * 50 * x1) * 1.001 + 1e-6
     print(f"{name}: {len(r)} peptides, {dsp.num_spec} spectra, {int(ref['off'][-1])} scores")
     return dict(name=name, ctx=ctx, db=db, dsp=dsp, ref=ref, topn=topn, pl=pipeline,
+                r=r, tc=tc, tn=tn, mass=mass, ions=ions, idx=idx,
                 exact=orc.fx_to_double(ref["all_fx"]), scale=orc.fx_to_double(absx["all_fx"]),
                 x1=np.repeat(x1, reps), bound8=np.repeat(b8, reps))
 
@@ -134,3 +135,34 @@ def test_batched_dense_full(fenv, orc):
         c = int(res.top_cnt[s])
         ex = res.top_exact[s, :c]
         assert np.all(np.abs(res.top_f32[s, :c] - ex) <= 1e-4 * np.maximum(np.abs(ex), 1.0)), s
+
+
+def test_stats_full(fenv, orc):
+    """E-value fit of every spectrum over its plain FP32 scores (histogram
+    points and kept counts identical, alpha and beta within 1e-9) and the SP
+    ion counts of every reported match, against the oracle."""
+    pl, dsp = fenv["pl"], fenv["dsp"]
+    ps = pl.PlainSearch(fenv["ctx"], fenv["db"], dsp, eps=3.0, topn=fenv["topn"])
+    res = ps.run(keep_scores=True)
+    fit, npt, nu = ps.evalue_fit()
+    bad = []
+    for s in range(dsp.num_spec):
+        a, b, m, nused = orc.evalue_fit(res.scores[int(res.off[s]):int(res.off[s + 1])])
+        ok = (int(npt[s]), int(nu[s])) == (m, nused)
+        for got, want in ((fit[s, 0], a), (fit[s, 1], b)):
+            ok &= (np.isnan(got) and np.isnan(want)) or abs(got - want) <= 1e-9 * max(abs(want), 1)
+        if not ok:
+            bad.append(s)
+    print(f"{fenv['name']}: {int((npt >= 2).sum())} of {dsp.num_spec} spectra with a fit")
+    assert not bad, bad[:10]
+    nb, ny, tot = ps.sp_ions()
+    X, lens = dsp.x(), dsp.lengths()
+    idx, ions = fenv["idx"], fenv["ions"]
+    r, tc, tn, mass = fenv["r"], fenv["tc"], fenv["tn"], fenv["mass"]
+    for s in range(dsp.num_spec):
+        flags = orc.extract_peaks(X[s, :int(lens[s])])
+        for k in range(int(res.top_cnt[s])):
+            p = int(idx[int(res.begin[s]) + int(res.top_pos[s, k])])
+            want = orc.match_ions(ions, int(tc[p]), int(tn[p]), r[p], mass, int(dsp.zmax[s]),
+                                  flags)
+            assert (int(nb[s, k]), int(ny[s, k]), int(tot[s, k])) == want, (s, k)
