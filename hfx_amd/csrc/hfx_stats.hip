@@ -25,9 +25,11 @@
  * SP ions (IonSeries.hs:99-147, Match.hs:33-40): extractPeaks keeps the 200
  * largest values of the spectrum and folds them by max into bins i div 3; a
  * b or y ion matches when its bin round(m/z / bin width / 3) holds a value
- * above zero.  k_sp_peaks turns each spectrum into those flags (one bit per
- * bin, a radix select of the 200th value over the float bits); k_sp_* match
- * one peptide per thread along its FP32 b ladder.
+ * above zero.  k_sp_peaks_w turns each spectrum into those flags (one bit
+ * per bin; one wave per spectrum, the row in registers, the 200th value
+ * selected bit by bit over the float bits by ballot counts; k_sp_peaks does
+ * the same with a workgroup for rows longer than 3072); k_sp_* match one
+ * peptide per thread along its FP32 b ladder.
  */
 #include "hfx_common.h"
 
@@ -74,48 +76,6 @@ __device__ __forceinline__ T st_block_sum(T v, T *red)
     return t;
 }
 
-__device__ __forceinline__ float st_block_min(float v, float *red)
-{
-#pragma unroll
-    for (int m = HFX_WAVE / 2; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, HFX_WAVE));
-    __syncthreads();
-    if (threadIdx.x % HFX_WAVE == 0) red[threadIdx.x / HFX_WAVE] = v;
-    __syncthreads();
-    float t = red[0];
-#pragma unroll
-    for (int i = 1; i < ST_WAVES; ++i) t = fminf(t, red[i]);
-    return t;
-}
-
-__device__ __forceinline__ float st_block_max(float v, float *red)
-{
-#pragma unroll
-    for (int m = HFX_WAVE / 2; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, HFX_WAVE));
-    __syncthreads();
-    if (threadIdx.x % HFX_WAVE == 0) red[threadIdx.x / HFX_WAVE] = v;
-    __syncthreads();
-    float t = red[0];
-#pragma unroll
-    for (int i = 1; i < ST_WAVES; ++i) t = fmaxf(t, red[i]);
-    return t;
-}
-
-__device__ __forceinline__ uint64_t st_block_min_u64(uint64_t v, uint64_t *red)
-{
-#pragma unroll
-    for (int m = HFX_WAVE / 2; m >= 1; m >>= 1) {
-        const uint64_t o = __shfl_xor(v, m, HFX_WAVE);
-        v = o < v ? o : v;
-    }
-    __syncthreads();
-    if (threadIdx.x % HFX_WAVE == 0) red[threadIdx.x / HFX_WAVE] = v;
-    __syncthreads();
-    uint64_t t = red[0];
-#pragma unroll
-    for (int i = 1; i < ST_WAVES; ++i) t = red[i] < t ? red[i] : t;
-    return t;
-}
-
 /* exclusive prefix sum over the block in thread order */
 template <typename T>
 __device__ __forceinline__ T st_block_excl_scan(T v, T *red)
@@ -139,6 +99,69 @@ __device__ __forceinline__ T st_block_excl_scan(T v, T *red)
 /* E-value fit                                                           */
 /* ------------------------------------------------------------------ */
 
+/* ------------------------------------------------------------------ */
+/* Wave reductions over DPP lane moves (no LDS round trips)              */
+/* ------------------------------------------------------------------ */
+
+/* quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror:
+ * every lane of a row of 16 ends with the row's total; row_bcast15 (rows 1
+ * and 3) and row_bcast31 (rows 2 and 3) carry the rows into lane 63. */
+#define ST_DPP32(old, v, ctrl, rm) \
+    ((uint32_t)__builtin_amdgcn_update_dpp((int)(old), (int)(v), (ctrl), (rm), 0xF, false))
+
+template <int CTRL, int RM>
+__device__ __forceinline__ uint64_t st_dpp64(uint64_t v, uint64_t old)
+{
+    const uint32_t lo = ST_DPP32((uint32_t)old, (uint32_t)v, CTRL, RM);
+    const uint32_t hi = ST_DPP32((uint32_t)(old >> 32), (uint32_t)(v >> 32), CTRL, RM);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t st_readlane63(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+/* op-reduction of a 64-bit value over the wave; ident is op's identity */
+template <class Op>
+__device__ __forceinline__ uint64_t st_wave_reduce64(uint64_t v, uint64_t ident, Op op)
+{
+    v = op(v, st_dpp64<0xB1, 0xF>(v, ident));
+    v = op(v, st_dpp64<0x4E, 0xF>(v, ident));
+    v = op(v, st_dpp64<0x141, 0xF>(v, ident));
+    v = op(v, st_dpp64<0x140, 0xF>(v, ident));
+    v = op(v, st_dpp64<0x142, 0xA>(v, ident));
+    v = op(v, st_dpp64<0x143, 0xC>(v, ident));
+    return st_readlane63(v);
+}
+
+__device__ __forceinline__ double st_wsum_f64(double v)
+{
+    const uint64_t r = st_wave_reduce64(__double_as_longlong(v), 0ull, [](uint64_t a, uint64_t b) {
+        return (uint64_t)__double_as_longlong(__longlong_as_double(a) + __longlong_as_double(b));
+    });
+    return __longlong_as_double(r);
+}
+
+/* inclusive prefix sum of uint32 over the wave: row_shr 1, 2, 4, 8 within
+ * rows of 16, then row_bcast15 / row_bcast31 across rows */
+__device__ __forceinline__ uint32_t st_wave_incl_scan32(uint32_t x)
+{
+    x += ST_DPP32(0u, x, 0x111, 0xF);
+    x += ST_DPP32(0u, x, 0x112, 0xF);
+    x += ST_DPP32(0u, x, 0x114, 0xF);
+    x += ST_DPP32(0u, x, 0x118, 0xF);
+    x += ST_DPP32(0u, x, 0x142, 0xA);
+    x += ST_DPP32(0u, x, 0x143, 0xC);
+    return x;
+}
+
+/* ------------------------------------------------------------------ */
+/* E-value fit                                                           */
+/* ------------------------------------------------------------------ */
+
 struct EvArgs {
     double *fit;        /* [S][2] alpha, beta          */
     uint32_t *npts;     /* [S] histogram points, opt.  */
@@ -151,12 +174,15 @@ struct EvArgs {
     float perc;
 };
 
+#define EV_NSUM 6   /* sum dx, sum y, sum dx^2, sum dx y, points, kept */
+
 struct EvShared {
     float cache[EV_CACHE];
-    uint32_t hist[EV_KEYS];
-    double redd[ST_WAVES];
-    uint64_t redu[ST_WAVES];
-    float redf[ST_WAVES];
+    uint32_t hist[EV_KEYS];       /* counts per key, then kept counts of the points */
+    uint16_t pkey[EV_KEYS];       /* key (less the range's base) of each point      */
+    double redd[EV_NSUM * ST_WAVES];
+    uint64_t redu[2 * ST_WAVES];
+    uint32_t red32[ST_WAVES];
 };
 
 /* one spectrum's view: n scores, the cut, the key map */
@@ -165,75 +191,188 @@ struct EvSpec {
     uint64_t na, n;
     bool cached;
     int64_t cb, ce;
-    double h0, w, kb;
+    double h0, w, kb, x0;
 
-    __device__ __forceinline__ float at(uint64_t i) const { return i < na ? sa[i] : sb[i - na]; }
+    double rw;   /* RN(1 / w) */
 
     /* 0 for (x + 1) - min <= 4000 (filtered out, Statistics.hs:76), else
-     * the bin ceiling(y / w) less kbase: monotone in x */
+     * the bin ceiling(y / w) less kbase: monotone in x.  q = y * RN(1/w) is
+     * within 2 ulp of y / w (below 2^20: 5e-10), so its ceiling is that of
+     * the correctly rounded quotient unless q lies within 1e-9 of an
+     * integer: only then (or for q >= 2^20) divide. */
     __device__ __forceinline__ uint64_t key(float v) const
     {
         const double y = ((double)v + 1.0) - h0;
         if (!(y > 4000.0)) return 0;
-        const double kd = ceil(y / w) - kb;
+        const double q = y * rw;
+        const double f = q - floor(q);
+        const double c = (f > 1e-9 && f < 1.0 - 1e-9 && q < 1048576.0) ? ceil(q) : ceil(y / w);
+        const double kd = c - kb;
         return kd < 4.6e18 ? (uint64_t)kd : (uint64_t)4.6e18;
+    }
+
+    /* valBin = i * w - w / 2 with i = ceiling(y / w); x = valBin / 10000
+     * (here times RN(1e-4): within an ulp) */
+    __device__ __forceinline__ double xof(uint64_t k) const
+    {
+        return (((double)k + kb) * w - w / 2.0) * 1e-4;
     }
 };
 
-/* Histogram of keys [base, base + EV_KEYS); each thread's EV_KPT keys become
- * points (x, logh) where the cut keeps some of their scores.  next = the
- * smallest key above the range (~0 if none), where the next pass starts. */
+/* Histogram of keys [base, base + EV_KEYS); the kept part of each key's
+ * scores becomes a point (x, logh), added to the thread's sums (x shifted by
+ * x0).  Key 0 (filtered) is counted with the keys below the range, in
+ * registers.  next = the smallest key above the range (~0 if none). */
 __device__ __forceinline__ void ev_window(const EvSpec &e, EvShared &sh, uint64_t base,
-                                          double *xv, double *yv, uint32_t &vm, uint64_t &used,
-                                          uint64_t &next)
+                                          double *acc, uint64_t &next)
 {
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid % HFX_WAVE, wv = tid / HFX_WAVE;
     for (uint32_t k = tid; k < EV_KEYS; k += ST_THREADS) sh.hist[k] = 0;
     __syncthreads();
-    uint64_t below = 0, nxt = ~(uint64_t)0;
-    for (uint64_t i = tid; i < e.n; i += ST_THREADS) {
-        const float v = e.cached ? sh.cache[i] : e.at(i);
+    uint32_t below = 0;
+    uint64_t nxt = ~(uint64_t)0;
+    auto count = [&](float v) {
         const uint64_t k = e.key(v);
-        if (k < base)
+        if (k < base || k == 0)
             ++below;
         else if (k - base < EV_KEYS)
             atomicAdd(&sh.hist[k - base], 1u);
         else
             nxt = k < nxt ? k : nxt;
+    };
+    if (e.cached) {
+        /* four scores per thread at a time: their keys' latencies overlap */
+        uint64_t i = tid;
+        for (; i + 3 * ST_THREADS < e.n; i += 4 * ST_THREADS) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = sh.cache[i + u * ST_THREADS];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) count(v[u]);
+        }
+        for (; i < e.n; i += ST_THREADS) count(sh.cache[i]);
+    } else {
+        for (uint64_t i = tid; i < e.na; i += ST_THREADS) count(e.sa[i]);
+        for (uint64_t i = tid; i < e.n - e.na; i += ST_THREADS) count(e.sb[i]);
+    }
+    /* block: sum of below, min of nxt (one barrier pair; the first barrier
+     * also ends the counting) */
+    const uint64_t bsum = st_wave_reduce64(below, 0ull, [](uint64_t a, uint64_t b) { return a + b; });
+    nxt = st_wave_reduce64(nxt, ~0ull, [](uint64_t a, uint64_t b) { return a < b ? a : b; });
+    __syncthreads();
+    if (lane == 0) {
+        sh.redu[2 * wv] = bsum;
+        sh.redu[2 * wv + 1] = nxt;
     }
     __syncthreads();
-    below = st_block_sum(below, sh.redu);
-    next = st_block_min_u64(nxt, sh.redu);
+    uint64_t tb = sh.redu[0];
+    next = sh.redu[1];
+#pragma unroll
+    for (int i = 1; i < ST_WAVES; ++i) {
+        tb += sh.redu[2 * i];
+        next = sh.redu[2 * i + 1] < next ? sh.redu[2 * i + 1] : next;
+    }
+    /* sorted position of each key: exclusive block scan of the counts */
     uint32_t c[EV_KPT];
-    uint64_t tot = 0;
+    uint32_t tot = 0;
 #pragma unroll
     for (int j = 0; j < EV_KPT; ++j) {
         c[j] = sh.hist[tid * EV_KPT + j];
         tot += c[j];
     }
-    int64_t pre = (int64_t)(below + st_block_excl_scan(tot, sh.redu));
-    vm = 0;
+    const uint32_t incl = st_wave_incl_scan32(tot);
+    if (lane == HFX_WAVE - 1) sh.red32[wv] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t i = 0; i < wv; ++i) woff += sh.red32[i];
+    int64_t pre = (int64_t)tb + (int64_t)(woff + incl - tot);
+    uint32_t kept[EV_KPT], nv = 0;
 #pragma unroll
     for (int j = 0; j < EV_KPT; ++j) {
         const uint64_t kk = base + tid * EV_KPT + j;
         const int64_t lo = pre, hi = pre + c[j];
         pre = hi;
-        const int64_t kept = min(hi, e.ce) - max(lo, e.cb);
-        if (kk >= 1 && kept > 0) {
-            /* valBin: i * w - w / 2 with i = ceiling(y / w); x = valBin / 10000;
-             * logh = logBase 10 (count / w) = log h / log 10 */
-            xv[j] = (((double)kk + e.kb) * e.w - e.w / 2.0) / 10000.0;
-            yv[j] = log((double)kept / e.w) / log(10.0);
-            vm |= 1u << j;
-            used += (uint64_t)kept;
+        const int64_t kp = min(hi, e.ce) - max(lo, e.cb);
+        kept[j] = (kk >= 1 && kp > 0) ? (uint32_t)kp : 0u;
+        nv += kept[j] != 0;
+    }
+    /* the points, compacted in key order, one per thread at a time */
+    const uint32_t incl2 = st_wave_incl_scan32(nv);
+    __syncthreads();   /* every thread has read its counts and the first scan */
+    if (lane == HFX_WAVE - 1) sh.red32[wv] = incl2;
+    __syncthreads();
+    uint32_t o = incl2 - nv, npt = 0;
+#pragma unroll
+    for (int i = 0; i < ST_WAVES; ++i) {
+        o += i < (int)wv ? sh.red32[i] : 0u;
+        npt += sh.red32[i];
+    }
+#pragma unroll
+    for (int j = 0; j < EV_KPT; ++j)
+        if (kept[j]) {
+            sh.hist[o] = kept[j];
+            sh.pkey[o] = (uint16_t)(tid * EV_KPT + j);
+            ++o;
+        }
+    __syncthreads();
+    for (uint32_t i = tid; i < npt; i += ST_THREADS) {
+        /* logh = logBase 10 (count / w) = log h / log 10 (here log10: within
+         * an ulp) */
+        const double dx = e.xof(base + sh.pkey[i]) - e.x0;
+        const double y = log10((double)sh.hist[i] / e.w);
+        acc[0] += dx;
+        acc[1] += y;
+        acc[2] += dx * dx;
+        acc[3] += dx * y;
+        acc[4] += 1.0;
+        acc[5] += (double)sh.hist[i];
+    }
+    __syncthreads();   /* the next range reuses hist */
+}
+
+/* min and max of p[0..cnt), copied to dst[0..cnt) when COPY; 16-byte loads
+ * over the aligned middle, up to eight per thread in flight */
+template <bool COPY>
+__device__ __forceinline__ void ev_scan(const float *p, uint64_t cnt, float *dst, float &lo,
+                                        float &hi)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint64_t h = min(cnt, (uint64_t)(((16u - ((uint32_t)(uintptr_t)p & 15u)) & 15u) / 4u));
+    auto one = [&](uint64_t i, float v) {
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+        if (COPY) dst[i] = v;
+    };
+    if (tid < h) one(tid, p[tid]);
+    const float4 *p4 = (const float4 *)(p + h);
+    const uint64_t n4 = (cnt - h) / 4;
+    for (uint64_t i = tid; i < n4; i += 8 * ST_THREADS) {
+        float4 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            q[u] = i + u * ST_THREADS < n4 ? p4[i + u * ST_THREADS]
+                                           : make_float4(p[0], p[0], p[0], p[0]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint64_t o = h + 4 * (i + u * ST_THREADS);
+            lo = fminf(fminf(lo, q[u].x), fminf(q[u].y, fminf(q[u].z, q[u].w)));
+            hi = fmaxf(fmaxf(hi, q[u].x), fmaxf(q[u].y, fmaxf(q[u].z, q[u].w)));
+            if (COPY && i + u * ST_THREADS < n4) {
+                dst[o] = q[u].x;
+                dst[o + 1] = q[u].y;
+                dst[o + 2] = q[u].z;
+                dst[o + 3] = q[u].w;
+            }
         }
     }
+    for (uint64_t t = h + 4 * n4 + tid; t < cnt; t += ST_THREADS) one(t, p[t]);
 }
 
 __global__ __launch_bounds__(ST_THREADS) void k_evfit(EvArgs a)
 {
     __shared__ EvShared sh;
-    const uint32_t s = blockIdx.x, tid = threadIdx.x;
+    const uint32_t s = blockIdx.x, tid = threadIdx.x, lane = tid % HFX_WAVE;
+    const uint32_t wv = tid / HFX_WAVE;
     EvSpec e;
     const uint64_t a0 = a.oa[s];
     e.na = a.oa[s + 1] - a0;
@@ -246,48 +385,54 @@ __global__ __launch_bounds__(ST_THREADS) void k_evfit(EvArgs a)
         e.sb = a.sb + b0;
     }
     e.n = e.na + nb;
-    if (e.n == 0) {
+    auto no_fit = [&]() {
         if (tid == 0) {
             a.fit[2 * s] = NAN;
             a.fit[2 * s + 1] = NAN;
             if (a.npts) a.npts[s] = 0;
             if (a.nused) a.nused[s] = 0;
         }
+    };
+    /* positions are counted in 32 bits */
+    if (e.n == 0 || e.n > 0xFFFFFFFFull) {
+        no_fit();
         return;
     }
     e.cached = e.n <= EV_CACHE;
 
     /* 1. min, max (and the LDS copy) */
     float lo = INFINITY, hi = -INFINITY;
-    uint64_t i = tid;
-    for (; i + 3 * ST_THREADS < e.n; i += 4 * ST_THREADS) {
-        const float v0 = e.at(i), v1 = e.at(i + ST_THREADS);
-        const float v2 = e.at(i + 2 * ST_THREADS), v3 = e.at(i + 3 * ST_THREADS);
-        lo = fminf(fminf(lo, v0), fminf(fminf(v1, v2), v3));
-        hi = fmaxf(fmaxf(hi, v0), fmaxf(fmaxf(v1, v2), v3));
-        if (e.cached) {
-            sh.cache[i] = v0;
-            sh.cache[i + ST_THREADS] = v1;
-            sh.cache[i + 2 * ST_THREADS] = v2;
-            sh.cache[i + 3 * ST_THREADS] = v3;
+    if (e.cached) {
+        ev_scan<true>(e.sa, e.na, sh.cache, lo, hi);
+        if (nb) ev_scan<true>(e.sb, nb, sh.cache + e.na, lo, hi);
+    } else {
+        ev_scan<false>(e.sa, e.na, nullptr, lo, hi);
+        if (nb) ev_scan<false>(e.sb, nb, nullptr, lo, hi);
+    }
+    {
+        const uint64_t mm = ((uint64_t)__float_as_uint(hi) << 32) | __float_as_uint(lo);
+        const uint64_t r = st_wave_reduce64(mm, ((uint64_t)__float_as_uint(-INFINITY) << 32) |
+                                                    __float_as_uint(INFINITY),
+                                            [](uint64_t x, uint64_t y) {
+            const float l = fminf(__uint_as_float((uint32_t)x), __uint_as_float((uint32_t)y));
+            const float h = fmaxf(__uint_as_float((uint32_t)(x >> 32)),
+                                  __uint_as_float((uint32_t)(y >> 32)));
+            return ((uint64_t)__float_as_uint(h) << 32) | __float_as_uint(l);
+        });
+        __syncthreads();
+        if (lane == 0) sh.redu[wv] = r;
+        __syncthreads();
+        lo = INFINITY;
+        hi = -INFINITY;
+#pragma unroll
+        for (int i = 0; i < ST_WAVES; ++i) {
+            lo = fminf(lo, __uint_as_float((uint32_t)sh.redu[i]));
+            hi = fmaxf(hi, __uint_as_float((uint32_t)(sh.redu[i] >> 32)));
         }
     }
-    for (; i < e.n; i += ST_THREADS) {
-        const float v = e.at(i);
-        lo = fminf(lo, v);
-        hi = fmaxf(hi, v);
-        if (e.cached) sh.cache[i] = v;
-    }
-    lo = st_block_min(lo, sh.redf);
-    hi = st_block_max(hi, sh.redf);
     if (!(isfinite(lo) && isfinite(hi))) {
         /* no histogram over non-finite scores */
-        if (tid == 0) {
-            a.fit[2 * s] = NAN;
-            a.fit[2 * s + 1] = NAN;
-            if (a.npts) a.npts[s] = 0;
-            if (a.nused) a.nused[s] = 0;
-        }
+        no_fit();
         return;
     }
 
@@ -298,54 +443,42 @@ __global__ __launch_bounds__(ST_THREADS) void k_evfit(EvArgs a)
     e.cb = max((int64_t)0, (int64_t)rint((30.0 / 100.0) * (double)e.n));
     e.h0 = (double)lo;
     e.w = a.w;
+    e.rw = 1.0 / a.w;
     e.kb = fmax(floor(4000.0 / a.w) - 1.0, 0.0);
+    /* sums are taken about x0, the middle of the key range of the scores,
+     * and centred at the end (no cancellation beyond the data's own spread) */
+    const uint64_t khi = e.key(hi);
+    e.x0 = khi ? 0.5 * (e.xof(1) + e.xof(khi)) : 0.0;
 
-    /* 2-3. histogram points, one pass per occupied range of EV_KEYS keys;
-     * 4a. their sums */
-    double xv[EV_KPT], yv[EV_KPT];
-    uint32_t vm = 0, nwin = 0;
-    uint64_t used = 0, next = 0;
-    double sx = 0.0, sy = 0.0;
-    uint64_t m = 0;
-    for (uint64_t base = 0; base != ~(uint64_t)0; base = next, ++nwin) {
-        ev_window(e, sh, base, xv, yv, vm, used, next);
-#pragma unroll
-        for (int j = 0; j < EV_KPT; ++j)
-            if (vm >> j & 1u) {
-                sx += xv[j];
-                sy += yv[j];
-                ++m;
-            }
-    }
-    sx = st_block_sum(sx, sh.redd);
-    sy = st_block_sum(sy, sh.redd);
-    m = st_block_sum(m, sh.redu);
-    const double mx = sx / (double)m, my = sy / (double)m;
+    /* 2-3. histogram points, one pass per occupied range of EV_KEYS keys */
+    double acc[EV_NSUM] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    uint64_t next = 0;
+    for (uint64_t base = 0; base != ~(uint64_t)0; base = next) ev_window(e, sh, base, acc, next);
 
-    /* 4b. centred sums (the last window's points are still in registers) */
-    double sxx = 0.0, sxy = 0.0;
-    for (uint64_t base = 0, wi = 0; wi < nwin; ++wi, base = next) {
-        if (nwin > 1) {
-            uint64_t dummy = 0;
-            ev_window(e, sh, base, xv, yv, vm, dummy, next);
-        }
+    /* 4. least squares: sums over the block (wave order, deterministic) */
 #pragma unroll
-        for (int j = 0; j < EV_KPT; ++j)
-            if (vm >> j & 1u) {
-                const double dx = xv[j] - mx;
-                sxx += dx * dx;
-                sxy += dx * (yv[j] - my);
-            }
-    }
-    sxx = st_block_sum(sxx, sh.redd);
-    sxy = st_block_sum(sxy, sh.redd);
-    used = st_block_sum(used, sh.redu);
+    for (int q = 0; q < EV_NSUM; ++q) acc[q] = st_wsum_f64(acc[q]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < EV_NSUM; ++q) sh.redd[EV_NSUM * wv + q] = acc[q];
+    __syncthreads();
     if (tid == 0) {
+        double t[EV_NSUM];
+#pragma unroll
+        for (int q = 0; q < EV_NSUM; ++q) {
+            t[q] = sh.redd[q];
+            for (int i = 1; i < ST_WAVES; ++i) t[q] += sh.redd[EV_NSUM * i + q];
+        }
+        const double m = t[4];
+        const double sxx = t[2] - t[0] * t[0] / m;
+        const double sxy = t[3] - t[0] * t[1] / m;
         const double beta = sxy / sxx;
+        const double mx = e.x0 + t[0] / m, my = t[1] / m;
         a.fit[2 * s] = my - beta * mx;
         a.fit[2 * s + 1] = beta;
         if (a.npts) a.npts[s] = (uint32_t)m;
-        if (a.nused) a.nused[s] = used;
+        if (a.nused) a.nused[s] = (uint64_t)t[5];
     }
 }
 
@@ -429,6 +562,81 @@ __global__ __launch_bounds__(ST_THREADS) void k_sp_peaks(uint32_t *on, uint32_t 
     }
     __syncthreads();
     for (uint32_t k = tid; k < W; k += ST_THREADS) on[(size_t)s * W + k] = sh.flags[k];
+}
+
+#define SPW_MAXE 48                                          /* values per lane */
+#define SPW_WORDS ((SPW_MAXE * HFX_WAVE / 3 + 1 + 31) / 32)     /* 33 flag words  */
+
+/* k_sp_peaks for rows of at most E * 64 values (E <= SPW_MAXE): one wave per
+ * spectrum, the row in registers (value j of a lane is position lane + 64 j,
+ * 0 past the row, which no count sees), every count a ballot; the flag words
+ * go through LDS. */
+template <int E>
+__global__ __launch_bounds__(ST_THREADS) void k_sp_peaks_w(uint32_t *on, uint32_t W,
+                                                            hfx_spectra sp)
+{
+    __shared__ uint32_t words[ST_WAVES * SPW_WORDS];
+    const uint32_t wv = threadIdx.x / HFX_WAVE, lane = threadIdx.x % HFX_WAVE;
+    const uint32_t s = blockIdx.x * ST_WAVES + wv;
+    uint32_t *fw = words + wv * SPW_WORDS;
+    for (uint32_t k = lane; k < SPW_WORDS; k += HFX_WAVE) fw[k] = 0;
+    __syncthreads();
+    const bool live = s < sp.num_spec;
+    const uint32_t L = live ? min(min(sp.d_len[s], sp.x_stride), (uint32_t)(E * HFX_WAVE)) : 0u;
+    const float *row = sp.d_x + (size_t)(live ? s : 0) * sp.x_stride;
+    float v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        /* unconditional loads (past the row: its first value, discarded) */
+        const uint32_t i = lane + HFX_WAVE * j;
+        const float t = row[i < L ? i : 0u];
+        v[j] = i < L ? t : 0.0f;
+    }
+    uint32_t kv[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) kv[j] = sp_key(v[j]);
+    uint32_t P = 0;
+#pragma unroll
+    for (int j = 0; j < E; ++j) P += __popcll(__builtin_amdgcn_ballot_w64(kv[j] != 0));
+    uint32_t T = 0, need = 0;
+    const bool all_pos = P <= SP_TOP;
+    if (!all_pos) {
+        /* T = key of the 200th largest value, bit by bit from the top */
+        for (int bit = 30; bit >= 0; --bit) {
+            const uint32_t cand = T | (1u << bit);
+            uint32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < E; ++j) c += __popcll(__builtin_amdgcn_ballot_w64(kv[j] >= cand));
+            if (c >= SP_TOP) T = cand;
+        }
+        uint32_t gt = 0;
+#pragma unroll
+        for (int j = 0; j < E; ++j) gt += __popcll(__builtin_amdgcn_ballot_w64(kv[j] > T));
+        need = SP_TOP - gt;
+    }
+    /* ties of T in position order: rows in order, lanes in order within */
+    const uint64_t below = __lanemask_lt();
+    uint32_t base = 0;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const uint32_t k = kv[j];
+        bool sel;
+        if (all_pos) {
+            sel = k != 0;
+        } else {
+            const uint64_t m = __builtin_amdgcn_ballot_w64(k == T);
+            sel = k > T || (k == T && base + (uint32_t)__popcll(m & below) < need);
+            base += (uint32_t)__popcll(m);
+        }
+        if (sel) {
+            const uint32_t t = (lane + HFX_WAVE * j) / 3;
+            atomicOr(&fw[t >> 5], 1u << (t & 31));
+        }
+    }
+    __syncthreads();
+    if (live)
+        for (uint32_t k = lane; k < W; k += HFX_WAVE)
+            on[(size_t)s * W + k] = k < SPW_WORDS ? fw[k] : 0u;
 }
 
 struct SpArgs {
@@ -589,8 +797,23 @@ extern "C" int hfx_sp_peaks(hfx_ctx *ctx, uint32_t *d_on, uint32_t words, const 
                 "hfx_sp_peaks: spectrum stride %u above %u", sp->x_stride,
                 3u * (32u * SP_WORDS - 1u) + 2u);
     if (sp->num_spec == 0) return HFX_OK;
-    hipLaunchKernelGGL(k_sp_peaks, dim3(sp->num_spec), dim3(ST_THREADS), 0, hfx_stream(stream),
-                       d_on, words, *sp);
+    const dim3 gw((sp->num_spec + ST_WAVES - 1) / ST_WAVES);
+    const uint32_t rows = (sp->x_stride + HFX_WAVE - 1) / HFX_WAVE;
+    if (rows <= 12)
+        hipLaunchKernelGGL(k_sp_peaks_w<12>, gw, dim3(ST_THREADS), 0, hfx_stream(stream), d_on,
+                           words, *sp);
+    else if (rows <= 24)
+        hipLaunchKernelGGL(k_sp_peaks_w<24>, gw, dim3(ST_THREADS), 0, hfx_stream(stream), d_on,
+                           words, *sp);
+    else if (rows <= 36)
+        hipLaunchKernelGGL(k_sp_peaks_w<36>, gw, dim3(ST_THREADS), 0, hfx_stream(stream), d_on,
+                           words, *sp);
+    else if (rows <= SPW_MAXE)
+        hipLaunchKernelGGL(k_sp_peaks_w<SPW_MAXE>, gw, dim3(ST_THREADS), 0, hfx_stream(stream),
+                           d_on, words, *sp);
+    else
+        hipLaunchKernelGGL(k_sp_peaks, dim3(sp->num_spec), dim3(ST_THREADS), 0,
+                           hfx_stream(stream), d_on, words, *sp);
     HFX_CHECK_LAUNCH();
     return HFX_OK;
 }

@@ -184,20 +184,25 @@ def test_sp_peaks_workload(senv, orc):
         assert _unpack(got[s], got.shape[1] * 32)[n // 3 + 1:].sum() == 0
 
 
-def test_sp_peaks_synthetic_rows(senv, orc):
+@pytest.mark.parametrize("stride", [3072, 12000])
+def test_sp_peaks_synthetic_rows(senv, orc, stride):
     """Ties at the 200th value, fewer than 200 positive values, empty and
-    one-value rows, rows longer than the LDS copy (stride 12000)."""
+    one-value rows; stride 3072: the wave-per-spectrum kernel up to its
+    longest row; stride 12000: the workgroup kernel, also with rows longer
+    than its LDS copy."""
     import hfx_amd
 
     pl, L = senv["pl"], senv["L"]
     rng = np.random.default_rng(7)
-    stride = 12000
     rows = []
-    for n in (0, 1, 2, 3, 250, 1000, 2999, 8192, 8193, 11999):
+    for n in (0, 1, 2, 3, 250, 1000, 2999, 3072, 8192, 8193, 11999):
+        if n > stride:
+            continue
         x = rng.normal(0, 1, n).astype(np.float32)
         rows.append(x)
         rows.append((np.round(x * 3) / 3).astype(np.float32))   # many ties
-    rows.append(np.where(rng.random(4000) < 0.02, 1.0, -1.0).astype(np.float32))  # ~80 positive
+    rows.append(np.where(rng.random(min(4000, stride)) < 0.02, 1.0, -1.0)
+                .astype(np.float32))  # about 60-80 positive
     S = len(rows)
     X = np.zeros((S, stride), np.float32)
     lens = np.zeros(S, np.uint32)
