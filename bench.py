@@ -97,6 +97,49 @@ def setup(workload: str):
     return dict(w=w, prot=prot, mass=mass, r=r, tc=tc, tn=tn, sp=sp, ctx=ctx, db=db, dsp=dsp)
 
 
+def bench_mods(steps, warmup, workload="mods-2k-50k"):
+    """The modified-candidate search (Search.hs:282-622) on the uneven
+    workload: hfx_mod_windows for every spectrum, the reference ABI's
+    enumeration stages (findModablePeptides, calcTotalModCands,
+    prepareGenMod, genModCands: each synchronises the host for its count, as
+    in the reference), hfx_mpep_offsets, hfx_xcorr_sparse_mod and
+    hfx_topn_mod.  Wall time per run (allocations included) and the scoring
+    and top-N kernels by events."""
+    from hfx_amd import pipeline, synth
+
+    w = synth.WORKLOADS[workload]
+    t0 = time.time()
+    prot = synth.make_proteins(w["proteins"], seed=w["seed"])
+    mass = pipeline.mass_table(w["static"])
+    r, tc, tn = pipeline.digest(prot.seq, prot.seg, mass, 1, w["missed"], 400.0, 7200.0,
+                                w["p_rule"], w["min_len"], w["max_len"])
+    sp = synth.make_spectra(prot, r, tc, tn, w["spectra"], seed=w["seed"] + 100,
+                            charges=w["charges"], charge_p=w["charge_p"],
+                            double_for_3plus=w["double3"], var_mods=w["var"])
+    ctx = pipeline.Ctx(0)
+    mi = pipeline.mod_info(w["var"], w["max_ma"])
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass), ma=mi.ma)
+    dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, 2112)
+    log(f"mods workload: {len(r)} peptides, {mi.num_mod} modifications, {dsp.num_spec} spectra "
+        f"({time.time() - t0:.1f}s)")
+    ms = pipeline.ModSearch(ctx, db, dsp, mi, eps=3.0, topn=w["topn"])
+    for _ in range(warmup):
+        res = ms.run(keep_stages=False)
+    pipeline.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = ms.run(keep_stages=False)
+    pipeline.sync()
+    el = (time.perf_counter() - t0) / max(steps, 1)
+    n = np.diff(res.mp_off).astype(np.int64)
+    return res, {"workload": workload, "spectra": int(dsp.num_spec), "peptides": int(len(r)),
+                 "modifications": int(mi.num_mod), "modified_candidates": int(res.num_mpep),
+                 "per_spectrum": {"min": int(n.min()), "median": int(np.median(n)),
+                                  "p99": int(np.percentile(n, 99)), "max": int(n.max())},
+                 "ms_per_run": el * 1e3, "value": res.num_mpep / el,
+                 "unit": "modified-candidate scores/s"}
+
+
 def algorithmic_bytes(env, res) -> float:
     """Compulsory HBM bytes of the scoring stage for the scored pairs: per
     pair the candidate's residues + 16 bytes of index/terminal/residual words
@@ -356,7 +399,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--margin", type=int, default=0,
                     help="extra candidates re-scored exactly beyond the top N")
-    ap.add_argument("--path", choices=("all", "plain", "batched", "dense", "i8"), default="all")
+    ap.add_argument("--path", choices=("all", "plain", "batched", "dense", "i8", "mods"),
+                    default="all")
     ap.add_argument("--calib-read", type=int, default=0, metavar="MIB",
                     help="only run a streaming read of MIB MiB (k_mvm_rows over a "
                          "row matrix) to calibrate HBM byte counters, then exit")
@@ -370,6 +414,15 @@ def main():
 
     from hfx_amd import pipeline
 
+    if args.path == "mods":
+        _, m = bench_mods(args.steps, args.warmup)
+        print(json.dumps({"metric": "modified-candidate XCorr scores/sec (uneven workload)",
+                          "value": m["value"], "unit": m["unit"], "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": m["ms_per_run"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                          "data": "synthetic", "config": m}), flush=True)
+        return
     env = setup(args.workload)
     topn = env["w"]["topn"]
     out = {"metric": METRIC, "unit": "scores/s", "n_gpus": 1, "steps": args.steps,

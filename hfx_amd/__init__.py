@@ -180,6 +180,8 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _P, C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],
     ),
     "hfx_mpep_offsets": (_I, [_P, _P, _P, _U32, _P, _U32, _P]),
+    "hfx_mod_windows": (
+        _I, [_P, _P, _P, _P, _P, C.POINTER(HfxDb), _P, _U32, _P, _U32, _F, _P]),
     "hfx_xcorr_sparse_mod": (
         _I,
         [_P, _P, _P, _U64, C.POINTER(HfxModset), C.POINTER(HfxDb), C.POINTER(HfxSpectra), _P],

@@ -100,6 +100,27 @@ struct HfxPlainCands {
     {
         peaks_prep(map, mtab, prep(s, k), len, zeff);
     }
+    /* one lane: the candidate's b ladder (sequential FP32 prefix of its
+     * first nl residue masses) into dst, four residues loaded ahead (the
+     * last one re-read past the end) */
+    __device__ __forceinline__ void ladder(const Prep &p, const float *mtab, float *dst,
+                                           uint32_t nl) const
+    {
+        const uint8_t *res = db.d_ions + p.tc;
+        float bl = 0.0f;
+        for (uint32_t k = 0; k < nl; k += 4) {
+            uint32_t a[4];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) a[j] = res[min(k + j, nl - 1u)];
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                if (k + j < nl) {
+                    bl = bl + mtab[HFX_ACID_IDX(a[j])];
+                    dst[k + j] = bl;
+                }
+            }
+        }
+    }
 };
 
 struct HfxModCands {
@@ -152,5 +173,51 @@ struct HfxModCands {
                                           uint32_t len, uint32_t zeff) const
     {
         peaks_prep(map, mtab, prep(s, k), len, zeff);
+    }
+    /* one lane: the modified candidate's b ladder over its first nl (<= 64)
+     * positions.  A residue of modable acid a (code field a + 1) that is the
+     * i-th of its acid in the peptide weighs table + ma_mass[a] when i is in
+     * the acid's part of the unrank slice (HfxModMass's rule, per lane):
+     * those parts become one 64-bit occurrence mask per acid. */
+    __device__ __forceinline__ void ladder(const Prep &p, const float *mtab, float *dst,
+                                           uint32_t nl) const
+    {
+        const uint32_t na = min(ms.num_ma, (uint32_t)HFX_MAX_MA);
+        const uint32_t *row = ms.d_mod_ma_count + (size_t)p.mod * ms.num_ma;
+        const uint32_t *u = ms.d_unrank + p.uoff;
+        uint64_t mask[HFX_MAX_MA];
+        uint32_t occ[HFX_MAX_MA];
+        float mam[HFX_MAX_MA];
+        uint32_t off = 0;
+#pragma unroll
+        for (uint32_t a = 0; a < HFX_MAX_MA; ++a) {
+            mask[a] = 0;
+            occ[a] = 0;
+            mam[a] = 0.0f;
+            if (a < na) {
+                const uint32_t k = row[a];
+                for (uint32_t q = 0; q < k; ++q) {
+                    const uint32_t i = u[off + q];
+                    if (i < 64) mask[a] |= 1ull << i;
+                }
+                off += k;
+                mam[a] = ms.d_ma_mass[a];
+            }
+        }
+        const uint8_t *res = db.d_ions + p.tc;
+        float bl = 0.0f;
+        for (uint32_t k = 0; k < nl; ++k) {
+            const uint32_t ion = res[k];
+            const uint32_t md = HFX_ACID_MOD(ion);
+            float m = mtab[HFX_ACID_IDX(ion)];
+#pragma unroll
+            for (uint32_t a = 0; a < HFX_MAX_MA; ++a)
+                if (md == a + 1) {
+                    if ((mask[a] >> occ[a]) & 1ull) m = m + mam[a];
+                    ++occ[a];
+                }
+            bl = bl + m;
+            dst[k] = bl;
+        }
     }
 };

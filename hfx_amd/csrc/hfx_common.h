@@ -87,6 +87,7 @@ enum {
     HFX_SLOT_LADDER = 12, /* dense ladders: offsets, tops, counts */
     HFX_SLOT_LADDER2 = 13, /* dense ladders: positions           */
     HFX_SLOT_STATS = 14,  /* SP ion peak flags                   */
+    HFX_SLOT_SCAN2 = 15,  /* tile sums of the in-place scans     */
 };
 /* Scratch owned by a context: grown on demand, never shrunk. */
 int hfx_ctx_scratch(hfx_ctx *ctx, int slot, size_t bytes, void **out);

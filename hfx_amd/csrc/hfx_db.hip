@@ -167,13 +167,6 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_downsweep(const uint32_t *kin
     }
 }
 
-int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream)
-{
-    if (n == 0) return HFX_OK;
-    hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(1024), 0, hfx_stream(stream), d_a, n);
-    HFX_CHECK_LAUNCH();
-    return HFX_OK;
-}
 
 /* keys in radix order: ascending float order, or (desc) its reverse --
  * the complement of an order-preserving key reverses the order, and a
@@ -356,12 +349,42 @@ __global__ void __launch_bounds__(1024) k_sum_u32(const uint32_t *a, uint32_t n,
     }
 }
 
+/* partial sums of n uint32: block b adds elements b, b + G*1024, ... (G
+ * blocks of 1024 threads) into part[b] */
+__global__ void __launch_bounds__(1024) k_sum_part(const uint32_t *a, uint32_t n, uint32_t *part)
+{
+    __shared__ uint32_t w[16];
+    uint32_t s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 1024 + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * 1024)
+        s += a[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < 16; ++i) t += w[i];
+        part[blockIdx.x] = t;
+    }
+}
+
+#define SUM_BLOCKS 512u
+
 uint32_t hfx_sum_u32_to_host(hfx_ctx *ctx, const uint32_t *d_a, uint32_t n)
 {
     void *p, *h;
-    if (hfx_ctx_scratch(ctx, HFX_SLOT_SUM, 16, &p) != HFX_OK) return 0;
+    if (hfx_ctx_scratch(ctx, HFX_SLOT_SUM, 16 + 4 * SUM_BLOCKS, &p) != HFX_OK) return 0;
     if (hfx_ctx_pinned(ctx, 16, &h) != HFX_OK) return 0;
-    hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(1024), 0, 0, d_a, n, (uint32_t *)p);
+    if (n > 65536u) {
+        /* many workgroups, then one over their sums (integer: any order) */
+        const uint32_t g = min(SUM_BLOCKS, (n + 8191u) / 8192u);
+        uint32_t *part = (uint32_t *)p + 4;
+        hipLaunchKernelGGL(k_sum_part, dim3(g), dim3(1024), 0, 0, d_a, n, part);
+        hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(1024), 0, 0, part, g, (uint32_t *)p);
+    } else {
+        hipLaunchKernelGGL(k_sum_u32, dim3(1), dim3(1024), 0, 0, d_a, n, (uint32_t *)p);
+    }
     if (hipMemcpyAsync(h, p, 4, hipMemcpyDeviceToHost, 0) != hipSuccess) return 0;
     if (hipStreamSynchronize(0) != hipSuccess) {
         hfx_set_error("sum_to_host: synchronize failed");
@@ -429,6 +452,49 @@ extern "C" uint32_t findBeginEnd_f(uint32_t *d_begin_raw, uint32_t *d_end_raw,
     hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(1024), 0, 0, d_num_pep_scan_raw, num_mod);
     if (hipStreamSynchronize(0) != hipSuccess) hfx_set_error("findBeginEnd_f: sync failed");
     return total;
+}
+
+/* findBeginEnd_f for many spectra in one launch: one thread per spectrum
+ * walks the modifications in order, so its exclusive scan of the window sizes
+ * is the reference's per-spectrum scan. */
+__global__ void k_mod_windows(uint32_t *begin, uint32_t *end, uint32_t *scan, uint32_t *spec_cand,
+                              const float *r, const uint32_t *idx, uint32_t num_pep,
+                              const float *delta, uint32_t num_mod, const float *mass,
+                              uint32_t num_spec, float eps)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= num_spec) return;
+    const float lo0 = mass[s] - eps, hi0 = mass[s] + eps;
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < num_mod; ++i) {
+        const float lo = lo0 - delta[i], hi = hi0 - delta[i];
+        const uint32_t b = lb_perm(r, idx, num_pep, lo), e = ub_perm(r, idx, num_pep, hi);
+        const size_t o = (size_t)s * num_mod + i;
+        begin[o] = b;
+        end[o] = e;
+        scan[o] = acc;
+        acc += e - b;
+    }
+    spec_cand[s] = acc;
+}
+
+extern "C" int hfx_mod_windows(hfx_ctx *ctx, uint32_t *d_begin, uint32_t *d_end,
+                               uint32_t *d_num_scan, uint32_t *d_spec_cand, const hfx_db *db,
+                               const float *d_spec_mass, uint32_t num_spec,
+                               const float *d_mod_delta, uint32_t num_mod, float eps,
+                               void *stream)
+{
+    HFX_REQUIRE(ctx && db && d_begin && d_end && d_num_scan && d_spec_cand && d_spec_mass,
+                HFX_ERR_ARG, "hfx_mod_windows: null argument");
+    HFX_REQUIRE(num_mod == 0 || d_mod_delta, HFX_ERR_ARG, "hfx_mod_windows: null deltas");
+    if (num_spec == 0) return HFX_OK;
+    const float *r = db->d_r_sorted ? db->d_r_sorted : db->d_r;
+    const uint32_t *idx = db->d_r_sorted ? nullptr : db->d_idx_sorted;
+    hipLaunchKernelGGL(k_mod_windows, dim3((num_spec + 255) / 256), dim3(256), 0,
+                       hfx_stream(stream), d_begin, d_end, d_num_scan, d_spec_cand, r, idx,
+                       db->num_pep, d_mod_delta, num_mod, d_spec_mass, num_spec, eps);
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
 }
 
 /* ------------------------------------------------------------------ */
@@ -573,6 +639,76 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(uint64_t *off, const 
         if (first + k < n) off[first + k] = pre;
         pre += v[k];
     }
+}
+
+/* In-place exclusive scan of n uint32 (wrapping) over many workgroups:
+ * tile sums (SC_TILE elements a tile) into scratch, one workgroup scans
+ * them, each tile then scans its own elements from its base. */
+__global__ void __launch_bounds__(SC_THREADS) k_scan32_tile_sums(const uint32_t *a, uint32_t n,
+                                                                 uint32_t *sums)
+{
+    __shared__ uint64_t part[SC_THREADS / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * SC_TILE;
+    uint64_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SC_ITEMS; ++k) {
+        const uint64_t i = base + k * SC_THREADS + threadIdx.x;
+        if (i < n) s += a[i];
+    }
+    const uint64_t t = sc_block_sum(s, part);
+    if (threadIdx.x == 0) sums[blockIdx.x] = (uint32_t)t;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan32_tiles(uint32_t *a, uint32_t n,
+                                                             const uint32_t *sums)
+{
+    __shared__ uint32_t warp_tot[SC_THREADS / 64];
+    const uint64_t first = (uint64_t)blockIdx.x * SC_TILE + (uint64_t)threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SC_ITEMS; ++k) {
+        v[k] = first + k < n ? a[first + k] : 0u;
+        s += v[k];
+    }
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) warp_tot[wid] = x;
+    __syncthreads();
+    uint32_t pre = sums[blockIdx.x] + x - s;
+    for (uint32_t w = 0; w < wid; ++w) pre += warp_tot[w];
+#pragma unroll
+    for (uint32_t k = 0; k < SC_ITEMS; ++k) {
+        if (first + k < n) a[first + k] = pre;
+        pre += v[k];
+    }
+}
+
+int hfx_scan_u32_inplace(uint32_t *d_a, uint32_t n, void *stream)
+{
+    if (n == 0) return HFX_OK;
+    hipStream_t st = hfx_stream(stream);
+    if (n < SC_SMALL) {
+        hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(1024), 0, st, d_a, n);
+    } else {
+        const uint32_t nt = (uint32_t)(((uint64_t)n + SC_TILE - 1) / SC_TILE);
+        hfx_ctx *ctx = hfx_default_ctx();
+        HFX_REQUIRE(ctx, HFX_ERR_HIP, "hfx_scan_u32_inplace: no context");
+        void *p = nullptr;
+        int rc = hfx_ctx_scratch(ctx, HFX_SLOT_SCAN2, (size_t)4 * nt, &p);
+        if (rc) return rc;
+        uint32_t *sums = (uint32_t *)p;
+        hipLaunchKernelGGL(k_scan32_tile_sums, dim3(nt), dim3(SC_THREADS), 0, st, d_a, n, sums);
+        hipLaunchKernelGGL(k_scan_u32_inplace, dim3(1), dim3(1024), 0, st, sums, nt);
+        hipLaunchKernelGGL(k_scan32_tiles, dim3(nt), dim3(SC_THREADS), 0, st, d_a, n, sums);
+    }
+    HFX_CHECK_LAUNCH();
+    return HFX_OK;
 }
 
 extern "C" int hfx_scan_offsets(hfx_ctx *ctx, uint64_t *d_off, const uint32_t *d_count, uint32_t n,

@@ -77,34 +77,16 @@ __device__ __forceinline__ void d_unrank_comb(uint32_t *ans, uint32_t n, uint32_
 /* findModablePeptides                                                   */
 /* ------------------------------------------------------------------ */
 
-/* rows (pep, mod): spectrum-major, mod-minor, residual-sorted within a range
- * (fillPepAndModIdx, filter_by_mod.cu:151-190); one wave per (spectrum, mod) */
-__global__ void __launch_bounds__(256)
-    k_mod_fill(uint32_t *rows_pep, uint32_t *rows_mod, const uint32_t *spec_num_cand_scan,
-               const uint32_t *idx_sorted, const uint32_t *begin, const uint32_t *end,
-               const uint32_t *num_pep_scan, uint32_t num_mod, uint64_t pairs)
+/* counts of each modable acid over [tc, tn) (fillPepMACount :193-231) of
+ * the peptide at every residual-sorted position below npos, once: a
+ * peptide's rows repeat once per modification */
+__global__ void k_pos_ma_count(uint32_t *pcnt, const uint32_t *idx_sorted, uint32_t npos,
+                               const uint8_t *ions, const uint32_t *tc, const uint32_t *tn,
+                               const uint8_t *ma, uint32_t num_ma)
 {
-    const uint64_t pr = (uint64_t)blockIdx.x * 4 + threadIdx.x / HFX_WAVE;
-    if (pr >= pairs) return;
-    const uint32_t m = (uint32_t)(pr % num_mod), s = (uint32_t)(pr / num_mod);
-    const uint32_t o = num_pep_scan[pr] + spec_num_cand_scan[s];
-    const uint32_t b = begin[pr], e = end[pr];
-    for (uint32_t i = b + hfx_lane(); i < e; i += HFX_WAVE) {
-        rows_pep[o + (i - b)] = idx_sorted[i];
-        rows_mod[o + (i - b)] = m;
-    }
-}
-
-/* counts of each modable acid over [tc, tn) and the modability test
- * (fillPepMACount :193-231, checkModable :234-262) */
-__global__ void k_mod_count(uint32_t *cnt, uint32_t *flag, const uint32_t *rows_pep,
-                            const uint32_t *rows_mod, uint32_t n, const uint8_t *ions,
-                            const uint32_t *tc, const uint32_t *tn, const uint32_t *mod_ma_count,
-                            const uint8_t *ma, uint32_t num_ma)
-{
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n) return;
-    const uint32_t pep = rows_pep[row];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npos) return;
+    const uint32_t pep = idx_sorted[k];
     uint32_t c[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint8_t mac[10];
     for (uint32_t a = 0; a < num_ma; ++a) mac[a] = ma[a];
@@ -112,13 +94,55 @@ __global__ void k_mod_count(uint32_t *cnt, uint32_t *flag, const uint32_t *rows_
         const uint8_t ch = (uint8_t)(HFX_ACID_IDX(ions[j]) + 'A');
         for (uint32_t a = 0; a < num_ma; ++a) c[a] += (ch == mac[a]) ? 1u : 0u;
     }
-    const uint32_t *need = mod_ma_count + (size_t)rows_mod[row] * num_ma;
-    bool ok = true;
-    for (uint32_t a = 0; a < num_ma; ++a) {
-        cnt[(size_t)row * num_ma + a] = c[a];
-        ok = ok && c[a] >= need[a];
+    for (uint32_t a = 0; a < num_ma; ++a) pcnt[(size_t)k * num_ma + a] = c[a];
+}
+
+/* rows (pep, mod): spectrum-major, mod-minor, residual-sorted within a range
+ * (fillPepAndModIdx, filter_by_mod.cu:151-190), with each row's acid counts
+ * and the modability test (checkModable :234-262); one wave per
+ * (spectrum, mod) */
+__global__ void __launch_bounds__(256)
+    k_mod_fill(uint32_t *rows_pep, uint32_t *rows_mod, uint32_t *cnt, uint32_t *flag,
+               const uint32_t *spec_num_cand_scan, const uint32_t *idx_sorted,
+               const uint32_t *begin, const uint32_t *end, const uint32_t *num_pep_scan,
+               uint32_t num_mod, uint64_t pairs, const uint32_t *pcnt,
+               const uint32_t *mod_ma_count, uint32_t num_ma)
+{
+    const uint64_t pr = (uint64_t)blockIdx.x * 4 + threadIdx.x / HFX_WAVE;
+    if (pr >= pairs) return;
+    const uint32_t m = (uint32_t)(pr % num_mod), s = (uint32_t)(pr / num_mod);
+    const uint32_t o = num_pep_scan[pr] + spec_num_cand_scan[s];
+    const uint32_t b = begin[pr], e = end[pr];
+    const uint32_t *need = mod_ma_count + (size_t)m * num_ma;
+    for (uint32_t i = b + hfx_lane(); i < e; i += HFX_WAVE) {
+        const uint32_t row = o + (i - b);
+        rows_pep[row] = idx_sorted[i];
+        rows_mod[row] = m;
+        bool ok = true;
+        for (uint32_t a = 0; a < num_ma; ++a) {
+            const uint32_t v = pcnt[(size_t)i * num_ma + a];
+            cnt[(size_t)row * num_ma + a] = v;
+            ok = ok && v >= need[a];
+        }
+        flag[row] = ok ? 1u : 0u;
     }
-    flag[row] = ok ? 1u : 0u;
+}
+
+/* largest window end (every window position lies below it) */
+__global__ void __launch_bounds__(1024) k_max_end(const uint32_t *end, uint64_t n, uint32_t *out)
+{
+    __shared__ uint32_t w[16];
+    uint32_t v = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += 1024) v = max(v, end[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < 16; ++i) t = max(t, w[i]);
+        *out = t;
+    }
 }
 
 /* stable compaction (remove_if, :345-358) */
@@ -166,21 +190,37 @@ extern "C" uint32_t findModablePeptides(
     }
     const uint32_t n = num_cand_total;
     if (num_spec == 0 || num_mod == 0) return 0;
+    const uint64_t pairs = (uint64_t)num_spec * num_mod;
+    /* the positions the windows reach (one small read back) */
+    uint32_t npos = 0;
+    if (n) {
+        void *q, *h;
+        if (hfx_ctx_scratch(ctx, HFX_SLOT_MODS2, 16, &q) != HFX_OK) return 0;
+        if (hfx_ctx_pinned(ctx, 16, &h) != HFX_OK) return 0;
+        hipLaunchKernelGGL(k_max_end, dim3(1), dim3(1024), 0, 0, d_end_raw, pairs, (uint32_t *)q);
+        if (hipMemcpy(h, q, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            hfx_set_error("findModablePeptides: read back failed");
+            return 0;
+        }
+        npos = *(uint32_t *)h;
+    }
     void *p;
-    size_t bytes = (size_t)n * (12 + 4 * (size_t)num_ma) + ((size_t)n + 1) * 8 + 64;
+    size_t bytes = (size_t)n * (12 + 4 * (size_t)num_ma) + ((size_t)n + 1) * 8 +
+                   (size_t)npos * 4 * num_ma + 64;
     if (hfx_ctx_scratch(ctx, HFX_SLOT_MODS, bytes, &p) != HFX_OK) return 0;
     uint64_t *pos = (uint64_t *)p;
     uint32_t *rows_pep = (uint32_t *)(pos + n + 1);
     uint32_t *rows_mod = rows_pep + n;
     uint32_t *flag = rows_mod + n;
     uint32_t *cnt = flag + n;
-    const uint64_t pairs = (uint64_t)num_spec * num_mod;
+    uint32_t *pcnt = cnt + (size_t)n * num_ma;
     if (n) {
+        hipLaunchKernelGGL(k_pos_ma_count, dim3((npos + 255) / 256), dim3(256), 0, 0, pcnt,
+                           d_pep_idx_r_sorted_raw, npos, d_ions, d_tc, d_tn, d_ma, num_ma);
         hipLaunchKernelGGL(k_mod_fill, dim3((uint32_t)((pairs + 3) / 4)), dim3(256), 0, 0, rows_pep,
-                           rows_mod, d_spec_num_cand_scan, d_pep_idx_r_sorted_raw, d_begin_raw,
-                           d_end_raw, d_num_pep_scan_raw, num_mod, pairs);
-        hipLaunchKernelGGL(k_mod_count, dim3((n + 255) / 256), dim3(256), 0, 0, cnt, flag, rows_pep,
-                           rows_mod, n, d_ions, d_tc, d_tn, d_mod_ma_count, d_ma, num_ma);
+                           rows_mod, cnt, flag, d_spec_num_cand_scan, d_pep_idx_r_sorted_raw,
+                           d_begin_raw, d_end_raw, d_num_pep_scan_raw, num_mod, pairs, pcnt,
+                           d_mod_ma_count, num_ma);
     }
     if (hfx_scan_offsets(ctx, pos, flag, n, nullptr) != HFX_OK) return 0;
     if (n)

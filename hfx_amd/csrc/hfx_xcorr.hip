@@ -338,9 +338,9 @@ __device__ __forceinline__ void xp_pack(uint32_t *maps, uint32_t mstr, const flo
     }
 }
 
+template <class Cands>
 __global__ void __launch_bounds__(XP_THREADS)
-    k_xcorr_pack(float *scores, const uint64_t *tile_off, HfxPlainCands c, hfx_spectra sp,
-                 uint32_t mstr)
+    k_xcorr_pack(float *scores, const uint64_t *tile_off, Cands c, hfx_spectra sp, uint32_t mstr)
 {
     extern __shared__ __align__(16) float xp_lds[];
     __shared__ float mtab[32];
@@ -381,12 +381,15 @@ __global__ void __launch_bounds__(XP_THREADS)
         return;
     }
     /* lane i < rn: candidate c0 + i's first residue, positions, residual */
-    HfxPlainCands::Prep pr{0u, 0u, 0.0f};
+    typename Cands::Prep pr{};
     if (lane < rn) pr = c.prep(s, first + c0 + lane);
-    const uint8_t *ions = c.db.d_ions;
+    /* the packed peaks serve fragment charges 1..4; above (modified
+     * candidates only: addModIons goes to 10) every candidate takes the
+     * general path */
+    const bool general = zeff > 4;
     /* ladder positions of the packable candidates (long ones take none):
      * exclusive prefix over the wave's candidates */
-    const uint32_t nl = lane < rn && pr.n <= HFX_WAVE ? pr.n : 0u;
+    const uint32_t nl = lane < rn && pr.n <= HFX_WAVE && !general ? pr.n : 0u;
     uint32_t incl = nl;
 #pragma unroll
     for (int o = 1; o < HFX_WAVE; o <<= 1) {
@@ -401,32 +404,14 @@ __global__ void __launch_bounds__(XP_THREADS)
         const uint32_t base = hfx_take(pofs, p);
         const uint64_t fit = __ballot(lane >= p && lane < rn && incl - base <= XP_LB);
         const uint32_t q = p + (uint32_t)__builtin_popcountll(fit);
-        if (lane >= p && lane < q && nl) {
-            float bl = 0.0f;
-            const uint8_t *res = ions + pr.tc;
-            float *dst = lb + (pofs - base);
-            /* four residues loaded ahead (the last one re-read past the end) */
-            for (uint32_t k = 0; k < nl; k += 4) {
-                uint32_t a[4];
-#pragma unroll
-                for (uint32_t j = 0; j < 4; ++j) a[j] = res[min(k + j, nl - 1u)];
-#pragma unroll
-                for (uint32_t j = 0; j < 4; ++j) {
-                    if (k + j < nl) {
-                        bl = bl + mtab[HFX_ACID_IDX(a[j])];
-                        dst[k + j] = bl;
-                    }
-                }
-            }
-        }
+        if (lane >= p && lane < q && nl) c.ladder(pr, mtab, lb + (pofs - base), nl);
         hfx_wave_sync();
         while (p < q) {
             const uint32_t n0 = hfx_take(pr.n, p);
-            if (n0 > HFX_WAVE) {
+            if (n0 > HFX_WAVE || general) {
                 /* a long candidate: the general path over chunks of 64 */
-                const HfxPlainCands::Prep qq = HfxPlainCands::take(pr, p);
-                hfx_wave_peaks(maps + XP_PAD / 8, qq.n, qq.r, zeff, len,
-                               HfxPlainMass{ions + qq.tc, mtab});
+                const typename Cands::Prep qq = Cands::take(pr, p);
+                c.peaks_prep(maps + XP_PAD / 8, mtab, qq, len, zeff);
                 hfx_wave_sync();
                 const float sc = hfx_wave_dot_f32(maps + XP_PAD / 8, xs, len);
                 hfx_wave_sync();
@@ -516,8 +501,9 @@ static int launch_xcorr(void *scratch, float *d_scores, const Cands &c, uint64_t
     if (S == 0 || total == 0) return HFX_OK;
     HFX_REQUIRE(sp->x_stride <= HFX_MAX_LEN, HFX_ERR_ARG,
                 "sparse XCorr: spectra of up to %d bins (x_stride %u)", HFX_MAX_LEN, sp->x_stride);
-    /* unmodified candidates take the packed kernel */
-    constexpr bool PACK = std::is_same<Cands, HfxPlainCands>::value;
+    /* unmodified and modified candidates take the packed kernel */
+    constexpr bool PACK = std::is_same<Cands, HfxPlainCands>::value ||
+                          std::is_same<Cands, HfxModCands>::value;
     const uint32_t chunk = PACK ? XP_CHUNK : XC_CHUNK;
     hipStream_t st = hfx_stream(stream);
     int rc;
@@ -530,7 +516,7 @@ static int launch_xcorr(void *scratch, float *d_scores, const Cands &c, uint64_t
     const uint64_t grid = (total + chunk - 1) / chunk + S;
     HFX_REQUIRE(grid < (1ull << 31), HFX_ERR_ARG, "xcorr: %llu tiles", (unsigned long long)grid);
     if constexpr (PACK) {
-        hipLaunchKernelGGL(k_xcorr_pack, dim3((uint32_t)grid), dim3(XP_THREADS),
+        hipLaunchKernelGGL(k_xcorr_pack<Cands>, dim3((uint32_t)grid), dim3(XP_THREADS),
                            xp_lds_bytes(sp->x_stride), st, d_scores, tile_off, c, *sp,
                            xp_map_words(sp->x_stride));
     } else {

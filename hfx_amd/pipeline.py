@@ -708,46 +708,58 @@ class ModSearch:
         self.t_mod_sum = dev(mi.mod_sum)
         self.t_mod_delta = dev(mi.mod_delta)
 
-    def run(self, keep_scores: bool = False) -> ModResult:
+    def run(self, keep_scores: bool = False, per_spectrum: bool = False,
+            keep_stages: bool = True) -> ModResult:
+        """per_spectrum: the windows by one findBeginEnd_f call per spectrum
+        (Search.hs:317-322, the reference ABI); otherwise hfx_mod_windows
+        for all spectra in one launch.  keep_stages: copy every intermediate
+        array to the host (for parity checks)."""
         L = lib()
         S, M, na = self.sp.num_spec, self.mi.num_mod, self.mi.num_ma
         db = self.db
         t_b, t_e, t_sc = empty(4 * S * M), empty(4 * S * M), empty(4 * S * M)
-        spec_cand = np.zeros(S, np.uint32)
-        # Search.hs:317-322: one findBeginEnd per spectrum
-        for s in range(S):
-            o = 4 * s * M
-            spec_cand[s] = L.findBeginEnd_f(t_b.p + o, t_e.p + o, t_sc.p + o, db.t_r.p, db.t_idx.p,
-                                            db.num_pep, self.t_mod_delta.p, M,
-                                            float(self.sp.mass[s]), self.eps)
+        if per_spectrum:
+            spec_cand = np.zeros(S, np.uint32)
+            for s in range(S):
+                o = 4 * s * M
+                spec_cand[s] = L.findBeginEnd_f(t_b.p + o, t_e.p + o, t_sc.p + o, db.t_r.p,
+                                                db.t_idx.p, db.num_pep, self.t_mod_delta.p, M,
+                                                float(self.sp.mass[s]), self.eps)
+        else:
+            t_cand = empty(4 * S)
+            check(L.hfx_mod_windows(self.ctx.h, t_b.p, t_e.p, t_sc.p, t_cand.p,
+                                    C.byref(db.desc), self.sp.t_mass.p, S, self.t_mod_delta.p,
+                                    M, self.eps, None), "hfx_mod_windows")
+            spec_cand = host(t_cand, np.uint32, S)
         scan = np.zeros(S + 1, np.uint32)
         np.cumsum(spec_cand, out=scan[1:])
         total = int(scan[-1])
         t_scan = dev(scan)
-        st = {"spec_cand": spec_cand, "begin": host(t_b, np.uint32, S * M),
-              "end": host(t_e, np.uint32, S * M), "num_scan": host(t_sc, np.uint32, S * M)}
+        keep = host if keep_stages else (lambda *a: None)
+        st = {"spec_cand": spec_cand, "begin": keep(t_b, np.uint32, S * M),
+              "end": keep(t_e, np.uint32, S * M), "num_scan": keep(t_sc, np.uint32, S * M)}
         t_pep, t_pmod = empty(4 * total), empty(4 * total)
         t_pcnt, t_vscan = empty(4 * total * na), empty(4 * S)
         nv = L.findModablePeptides(t_pep.p, t_pmod.p, t_pcnt.p, t_vscan.p, t_scan.p, S, total,
                                    db.t_ions.p, db.t_tc.p, db.t_tn.p, db.t_idx.p, t_b.p, t_e.p,
                                    t_sc.p, self.t_mod_cnt.p, M, self.t_ma.p, na)
-        st.update(valid_pep=host(t_pep, np.uint32, nv), valid_mod=host(t_pmod, np.uint32, nv),
-                  valid_cnt=host(t_pcnt, np.uint32, nv * na), valid_scan=host(t_vscan, np.uint32, S))
+        st.update(valid_pep=keep(t_pep, np.uint32, nv), valid_mod=keep(t_pmod, np.uint32, nv),
+                  valid_cnt=keep(t_pcnt, np.uint32, nv * na), valid_scan=keep(t_vscan, np.uint32, S))
         t_nmp, t_comb = empty(4 * nv), empty(4 * nv * na)
         nmp = L.calcTotalModCands(t_nmp.p, t_comb.p, self.t_mod_cnt.p, t_pep.p, t_pmod.p, t_pcnt.p,
                                   nv, M, na) if nv else 0
-        st.update(pep_num_mpep=host(t_nmp, np.uint32, nv), comb=host(t_comb, np.uint32, nv * na))
+        st.update(pep_num_mpep=keep(t_nmp, np.uint32, nv), comb=keep(t_comb, np.uint32, nv * na))
         t_mp = [empty(4 * nmp) for _ in range(5)]
         lu = L.prepareGenMod(*[t.p for t in t_mp], self.t_mod_sum.p, t_pep.p, t_pmod.p, t_nmp.p,
                              nv, nmp) if nmp else 0
         for name, t in zip(("mp_pep", "mp_mod", "mp_rank", "mp_ith", "mp_scan"), t_mp):
-            st[name] = host(t, np.uint32, nmp)
+            st[name] = keep(t, np.uint32, nmp)
         t_unrank = empty(4 * max(lu, 1))
         if nmp:
             L.genModCands(t_unrank.p, self.t_mod_cnt.p, t_mp[3].p, t_mp[2].p, t_mp[4].p, t_pmod.p,
                           t_pcnt.p, t_comb.p, nmp, na)
         sync()
-        st["unrank"] = host(t_unrank, np.uint32, lu)
+        st["unrank"] = keep(t_unrank, np.uint32, lu)
         t_mpoff = empty(8 * (S + 1))
         check(L.hfx_mpep_offsets(self.ctx.h, t_mpoff.p, t_nmp.p, nv, t_vscan.p, S, None),
               "hfx_mpep_offsets")

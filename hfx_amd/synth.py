@@ -80,8 +80,15 @@ def peptide_residues(prot: Proteins, tc: int, tn: int) -> bytes:
 
 def make_spectra(prot: Proteins, r: np.ndarray, tc: np.ndarray, tn: np.ndarray, n: int,
                  seed: int, charges=(2,), charge_p=None, static_mods: dict | None = None,
-                 double_for_3plus: bool = False, mz_max: float = 2000.0) -> Spectra:
+                 double_for_3plus: bool = False, mz_max: float = 2000.0,
+                 var_mods: dict | None = None, max_sites: int = 3,
+                 mod_p: float = 0.5) -> Spectra:
+    """var_mods: with probability mod_p a spectrum's peptide carries 1..max_sites
+    variable modifications (residue -> mass delta) at random modifiable
+    residues (drawn from a second generator, so the rest of the workload is
+    the same with or without them)."""
     rng = np.random.default_rng(seed)
+    mrng = np.random.default_rng(seed + 7919)
     masses = dict(MONO)
     for k, v in (static_mods or {}).items():
         masses[k] = masses[k] + v
@@ -90,9 +97,17 @@ def make_spectra(prot: Proteins, r: np.ndarray, tc: np.ndarray, tn: np.ndarray, 
     pcr = np.empty(n, dtype=np.float32)
     mz_all, in_all = [], []
     off = np.zeros(n + 1, dtype=np.uint32)
+    nmod = np.zeros(n, np.int32)
     for i in range(n):
         res = peptide_residues(prot, int(tc[src[i]]), int(tn[src[i]]))
         m = np.array([masses[chr(c)] for c in res])
+        if var_mods and mrng.random() < mod_p:
+            sites = [k for k, c in enumerate(res) if chr(c) in var_mods]
+            if sites:
+                k = int(mrng.integers(1, min(max_sites, len(sites)) + 1))
+                for q in mrng.choice(sites, size=k, replace=False):
+                    m[q] += var_mods[chr(res[q])]
+                nmod[i] = k
         b = np.cumsum(m)[:-1]
         tot = float(m.sum())
         y = tot - b
@@ -119,6 +134,7 @@ def make_spectra(prot: Proteins, r: np.ndarray, tc: np.ndarray, tn: np.ndarray, 
         inten=np.concatenate(in_all) if in_all else np.zeros(0, np.float32),
         off=off,
         source=src.astype(np.int64),
+        info={"mod_sites": nmod} if var_mods else {},
     )
 
 
@@ -137,5 +153,14 @@ WORKLOADS = {
     "tryptic-20k-20k": dict(
         proteins=20000, spectra=20000, seed=4, missed=2, p_rule=0, min_len=0, max_len=0,
         charges=(2, 3), charge_p=(0.5, 0.5), static={"C": 57.02146}, double3=True, topn=5,
+    ),
+    # the uneven workload: the 2,000-protein database, variable Met oxidation
+    # and S/T/Y phosphorylation with at most 3 sites (modification vectors
+    # summing below 4), 50,000 spectra at 1+..4+, half of them from modified
+    # peptides; per-spectrum modified-candidate counts are heavy-tailed
+    "mods-2k-50k": dict(
+        proteins=2000, spectra=50000, seed=5, missed=1, p_rule=1, min_len=7, max_len=30,
+        charges=(1, 2, 3, 4), charge_p=(0.1, 0.4, 0.35, 0.15), static={}, double3=True,
+        topn=5, var={"M": 15.9949, "S": 79.9663, "T": 79.9663, "Y": 79.9663}, max_ma=4,
     ),
 }

@@ -382,6 +382,18 @@ int hfx_topn_exact(hfx_ctx *ctx, uint32_t *d_top_pos, double *d_top_exact, float
                    uint64_t total, const uint32_t *d_begin, const uint32_t *d_num,
                    const hfx_db *db, const hfx_spectra *sp, void *stream);
 
+/* findBeginEnd_f (filter_by_mod.cu:35-143) for num_spec spectra in one
+ * launch, where Search.hs:317-322 calls it once per spectrum: for spectrum s
+ * and modification i, [d_begin[s*M+i], d_end[s*M+i]) is the residual-sorted
+ * window of peptides with (m_s - eps) - delta_i <= r <= (m_s + eps) - delta_i
+ * (FP32), d_num_scan[s*M+i] the exclusive scan of the window sizes over i and
+ * d_spec_cand[s] their sum (findBeginEnd_f's return).  M = num_mod.  No host
+ * synchronisation. */
+int hfx_mod_windows(hfx_ctx *ctx, uint32_t *d_begin, uint32_t *d_end, uint32_t *d_num_scan,
+                    uint32_t *d_spec_cand, const hfx_db *db, const float *d_spec_mass,
+                    uint32_t num_spec, const float *d_mod_delta, uint32_t num_mod, float eps,
+                    void *stream);
+
 /* Search.hs:381-386 on the device: spectrum s owns the modified peptides of
  * rows [d_spec_num_valid_scan[s], d_spec_num_valid_scan[s+1]) (the last bound
  * is num_rows), i.e. d_mp_off[s] = sum of d_pep_num_mpep over the rows before

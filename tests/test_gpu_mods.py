@@ -167,3 +167,66 @@ def test_exact_route_matches(menv, orc):
         assert (ex[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
         # the gated route agrees with the all-exact route
         assert pos[s, :c].tolist() == res.top_pos[s, :c].tolist(), s
+
+
+@pytest.fixture(scope="module")
+def cfg3(orc):
+    """The uneven modified workload's database (2,000 proteins, M ox + S/T/Y
+    phospho, at most 3 sites) with 1,000 of its spectra (1+..4+, half from
+    modified peptides)."""
+    import hfx_amd
+    from hfx_amd import pipeline, synth
+
+    w = synth.WORKLOADS["mods-2k-50k"]
+    prot = synth.make_proteins(w["proteins"], seed=w["seed"])
+    mass = orc.mass_table(w["static"])
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=w["missed"],
+                           p_rule=w["p_rule"], min_len=w["min_len"], max_len=w["max_len"])
+    sp = synth.make_spectra(prot, r, tc, tn, 1000, seed=w["seed"] + 100, charges=w["charges"],
+                            charge_p=w["charge_p"], double_for_3plus=w["double3"],
+                            var_mods=w["var"])
+    ctx = pipeline.Ctx(0)
+    mi = pipeline.mod_info(w["var"], w["max_ma"])
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass), ma=mi.ma)
+    dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE)
+    ms = pipeline.ModSearch(ctx, db, dsp, mi, eps=3.0, topn=w["topn"])
+    return dict(prot=prot, mass=mass, r=r, tc=tc, tn=tn, sp=sp, ctx=ctx, mi=mi, db=db,
+                dsp=dsp, ms=ms, w=w, L=hfx_amd.lib(), pl=pipeline)
+
+
+def test_mod_windows_all_spectra_match_per_spectrum_calls(cfg3):
+    """hfx_mod_windows (one launch) against findBeginEnd_f called once per
+    spectrum (Search.hs:317-322): identical windows, scans and counts."""
+    a = cfg3["ms"].run(per_spectrum=True)
+    b = cfg3["ms"].run()
+    for k in ("begin", "end", "num_scan"):
+        assert (a.stages[k] == b.stages[k]).all(), k
+    assert (a.spec_cand == b.spec_cand).all()
+    assert (a.top_cnt == b.top_cnt).all()
+    for s in range(len(a.top_cnt)):
+        c = int(a.top_cnt[s])
+        assert (a.top_pos[s, :c] == b.top_pos[s, :c]).all(), s
+        assert (a.top_exact[s, :c] == b.top_exact[s, :c]).all(), s
+
+
+def test_mods_uneven_workload_against_oracle(cfg3, orc):
+    """1,000 spectra of the uneven workload: per-spectrum modified-candidate
+    counts (heavy-tailed), top-5 positions and exact scores identical to the
+    oracle's search."""
+    mi, dsp = cfg3["mi"], cfg3["dsp"]
+    res = cfg3["ms"].run()
+    idx, _ = orc.sort_idx(cfg3["r"])
+    ions = orc.prepare_ions(cfg3["prot"].seq, mi.ma)
+    ref = orc.search_mods(ions, cfg3["r"], cfg3["tc"], cfg3["tn"], cfg3["mass"], idx, mi.ma,
+                          mi.ma_mass, mi.mod_ma_count, mi.mod_sum, mi.mod_delta, dsp.x(),
+                          dsp.lengths(), dsp.zmax, dsp.mass, 3.0, cfg3["w"]["topn"])
+    n = np.diff(res.mp_off).astype(np.uint32)
+    assert (n == ref["spec_num_mpep"]).all()
+    print(f"modified candidates per spectrum: {int(n.sum())} total, max {int(n.max())}, "
+          f"median {int(np.median(n))}, zero in {int((n == 0).sum())}")
+    assert (res.top_cnt == ref["top_cnt"]).all()
+    bad = [s for s in range(dsp.num_spec)
+           if res.top_pos[s, :res.top_cnt[s]].tolist() != ref["top_pos"][s, :ref["top_cnt"][s]].tolist()
+           or not (res.top_exact[s, :res.top_cnt[s]] ==
+                   orc.fx_to_double(ref["top_fx"][s, :ref["top_cnt"][s]])).all()]
+    assert not bad, bad[:10]
