@@ -564,6 +564,11 @@ def main():
         roofline=roofline, db_setup_ms=db_setup.get("ms"), db_setup=db_setup, paths=paths)
     if "plain_fp32_sparse" in paths:
         out["stats"] = stats
+    if args.path == "all":
+        # the modified-candidate search on the uneven workload (not the headline)
+        _, m = bench_mods(min(args.steps, 3), min(args.warmup, 1))
+        paths["mods_fp32_sparse"] = m
+        log(f"mods: {m['value']:.3e} modified-candidate scores/s, {m['ms_per_run']:.1f} ms a run")
     if "plain_fp32_sparse" in paths and main_key != "plain_fp32_sparse":
         out["batched_over_plain"] = p["value"] / paths["plain_fp32_sparse"]["value"]
         out["target_batched_over_plain"] = 10.0
