@@ -214,6 +214,26 @@ _SIGS = {
         [_P, _P, _P, _P, _P, _P, _P, _U32, C.POINTER(HfxModset), C.POINTER(HfxDb),
          C.POINTER(HfxSpectra), _I, _P],
     ),
+    "hfx_seqdb_from_fasta": (_I, [C.c_char_p, C.c_char_p, _U64, _P, _I, _U32, _F, _F,
+                                  C.POINTER(_P)]),
+    "hfx_seqdb_free": (_I, [_P]),
+    "hfx_seqdb_info": (_I, [_P, _P]),
+    "hfx_seqdb_copy": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "hfx_seqdb_write_index": (_I, [_P, C.c_char_p, C.c_char_p]),
+    "hfx_seqdb_read_index": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "hfx_seqdb_lookup": (_I, [_P, _U32, _P, _U32, C.POINTER(_U32), C.POINTER(_U32),
+                              C.POINTER(_F)]),
+    "hfx_digestion_rule_name": (C.c_char_p, [_I]),
+    "hfx_dta_parse": (_I, [C.c_char_p, _U64, C.POINTER(_F), C.POINTER(_F), _P, _P, _U32,
+                           C.POINTER(_U32)]),
+    "hfx_mgf_parse": (_I, [C.c_char_p, _U64, C.POINTER(_P)]),
+    "hfx_mgf_info": (_I, [_P, _P]),
+    "hfx_mgf_copy": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "hfx_mgf_free": (_I, [_P]),
+    "hfx_format_number": (_I, [C.c_double, _I, _I, _I, _P, _U32]),
+    "hfx_format_table": (_I, [_P, _U32, _U32, _I, _P, _U64, C.POINTER(_U64)]),
+    "hfx_format_detail": (_I, [_P, _U32, _P, _U64, C.POINTER(_U64)]),
+    "hfx_format_scan": (_I, [_P, _P, _U32, _U32, _U32, _P, _U64, C.POINTER(_U64)]),
     "hfx_pepix_build": (_I, [_P, C.POINTER(HfxDb), _U32, _U32, _U32, _P, C.POINTER(_P)]),
     "hfx_pepix_destroy": (_I, [_P]),
     "hfx_pepix_info": (_I, [_P, _P]),

@@ -476,6 +476,86 @@ int hfx_sp_ions_topn_mod(hfx_ctx *ctx, uint32_t *d_nb, uint32_t *d_ny, uint32_t 
                          const hfx_db *db, const hfx_spectra *sp, int mono, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Host formats on either side of the path (hfx_io.cpp): FASTA -> the    */
+/* sequence database and its .index file, DTA/MGF spectra, the display   */
+/* lookup and the result tables.  Host memory only.                      */
+/* ------------------------------------------------------------------ */
+/* Fragment.hs:145-166 SequenceDB: headers, residues, per-protein residue
+ * segments, fragments (residual, c, n) and per-protein fragment segments. */
+typedef struct hfx_seqdb hfx_seqdb;
+/* Fasta.hs:30-35 readFasta (".gz" inflated) + Fragment.hs:197-233
+ * makeSeqDB: the proteins of a FASTA file (path) or text (path null), each
+ * digested by `rule` (Config.hs:416-430) with `missed` missed cleavages and
+ * min_mass <= (M+H)+ <= max_mass. */
+int hfx_seqdb_from_fasta(const char *path, const char *text, uint64_t len, const float *h_mass26,
+                         int rule, uint32_t missed, float min_mass, float max_mass,
+                         hfx_seqdb **out);
+int hfx_seqdb_free(hfx_seqdb *db);
+/* info = {proteins, residues, header bytes, fragments, parameter bytes} */
+int hfx_seqdb_info(const hfx_seqdb *db, uint64_t *info);
+/* any output may be null; h_header_off and the segments take count + 1 */
+int hfx_seqdb_copy(const hfx_seqdb *db, uint8_t *h_ions, uint32_t *h_ion_seg, char *h_headers,
+                   uint64_t *h_header_off, float *h_r, uint32_t *h_tc, uint32_t *h_tn,
+                   uint32_t *h_frag_seg, char *h_params);
+/* Index.hs:43-63 writeIndex: the parameter block (config-file text) as a
+ * Data.Binary ByteString, then the database gzip-compressed in
+ * Data.Binary's encoding; Index.hs:70-81 readIndex reads both back. */
+int hfx_seqdb_write_index(const hfx_seqdb *db, const char *path, const char *params);
+int hfx_seqdb_read_index(const char *path, hfx_seqdb **out);
+/* Location.hs:42-58 lookup: fragment k as "X.PEP.Y" (flanks or '-'; the
+ * residue at n kept unless n is past the database, Q28), its protein and
+ * (M+H)+ = (residual + H) + H2O. */
+int hfx_seqdb_lookup(const hfx_seqdb *db, uint32_t k, char *out, uint32_t cap, uint32_t *out_len,
+                     uint32_t *out_protein, float *out_mh);
+/* Config.hs:416-430: the description line of a digestion rule (0..13) */
+const char *hfx_digestion_rule_name(int rule);
+/* DTA.hs:36-67: first line MH+ and charge (an integer), then m/z intensity
+ * pairs; *out_pcr = (MH + (z - 1) H) / z.  Up to cap peaks copied. */
+int hfx_dta_parse(const char *text, uint64_t len, float *out_pcr, float *out_charge, float *h_mz,
+                  float *h_inten, uint32_t cap, uint32_t *out_npk);
+/* MGF.hs:38-100: BEGIN IONS / TITLE, CHARGE, PEPMASS / peaks / END IONS */
+typedef struct hfx_mgf hfx_mgf;
+int hfx_mgf_parse(const char *text, uint64_t len, hfx_mgf **out);
+/* info = {spectra, peaks, title bytes} */
+int hfx_mgf_info(const hfx_mgf *m, uint64_t *info);
+int hfx_mgf_copy(const hfx_mgf *m, float *h_pcr, float *h_charge, uint32_t *h_off, float *h_mz,
+                 float *h_inten, char *h_titles, uint64_t *h_title_off);
+int hfx_mgf_free(hfx_mgf *m);
+/* PrettyPrint.hs:83-180.  Numbers as Haskell prints them: kind 0 show,
+ * 1 showFFloat (Just digits), 2 showEFloat (Just digits); is_float: the
+ * value is a Float (its own shortest digits). */
+int hfx_format_number(double v, int is_float, int kind, int digits, char *out, uint32_t cap);
+/* ppAsRows q over nrows x ncols NUL-terminated cells (row-major) */
+int hfx_format_table(const char *const *cells, uint32_t nrows, uint32_t ncols, int q, char *out,
+                     uint64_t cap, uint64_t *out_len);
+/* printResultsDetail: the full headers of n matches */
+int hfx_format_detail(const char *const *headers, uint32_t n, char *out, uint64_t cap,
+                      uint64_t *out_len);
+typedef struct hfx_scan_header {
+    const char *title;         /* ms2info (empty for DTA files) */
+    const char *spectrum_file;
+    const char *database;
+    const char *enzyme;        /* hfx_digestion_rule_name(rule) */
+    float precursor;           /* ms2precursor, m/z */
+    float charge;              /* ms2charge */
+    float tolerance;           /* mass-tolerance */
+} hfx_scan_header;
+typedef struct hfx_match_row {
+    float mh;                  /* fragmass, (M+H)+ */
+    float xcorr;               /* scoreXC / 10000 */
+    uint32_t ions_matched, ions_total;
+    double evalue;
+    const char *reference;     /* fraglabel: first word of the header */
+    const char *peptide;       /* "X.PEP.Y" ('*' after modified residues) */
+    const char *header;        /* the protein's full header */
+} hfx_match_row;
+/* printScanResults for one scan: printConfig, printResults of the first
+ * nshow rows, printResultsDetail of the first ndetail; out gets at most
+ * cap - 1 bytes and a NUL, *out_len the full length. */
+int hfx_format_scan(const hfx_scan_header *hdr, const hfx_match_row *rows, uint32_t n,
+                    uint32_t nshow, uint32_t ndetail, char *out, uint64_t cap, uint64_t *out_len);
+
+/* ------------------------------------------------------------------ */
 /* Batched search (new; Main.hs:129-142 scores 16 spectra per group).    */
 /* A batch of any number of spectra is planned once (windows, index      */
 /* sizes, every buffer allocated; synchronises), then run without host   */

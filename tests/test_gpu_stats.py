@@ -331,3 +331,56 @@ def test_finish_matches(menv, orc):
             b, y, t = orc.match_ions(ions, int(menv["tc"][p]), int(menv["tn"][p]), menv["r"][p],
                                      menv["mass"], int(dsp.zmax[s]), flags[s], nmod=nm)
             assert m.ions == (b + y, t)
+
+
+def test_files_to_ranked_output(tmp_path, orc):
+    """The whole path from files: a FASTA database (readFasta + makeSeqDB) and
+    DTA spectra, the search, E-values and SP ions on the device, and the
+    ranked tables (printScanResults) -- every printed match the lookup of its
+    window position, with the device's ion counts and E-value."""
+    from hfx_amd import formats, pipeline, synth
+
+    prot = synth.make_proteins(200, seed=77)
+    seq = prot.seq.tobytes().decode()
+    fasta = "".join(f">sp|P{i:05d}|PROT{i}_TEST Protein {i} of the test database\n"
+                    f"{seq[prot.seg[i]:prot.seg[i + 1]]}\n" for i in range(prot.count))
+    mass = orc.mass_table({"C": 57.021464})
+    db = formats.read_fasta(mass, text=fasta, rule=1, missed=1, min_mass=400.0, max_mass=7200.0)
+    assert db.ions.tobytes() == prot.seq.tobytes()
+    sp = synth.make_spectra(prot, db.r, db.tc, db.tn, 6, seed=78, charges=(2, 3),
+                            static_mods={"C": 57.021464})
+    pcr, z, mzs, its = [], [], [], []
+    for s in range(sp.count):
+        mh = float(np.float32(sp.pcr[s]) * sp.charge[s] - (sp.charge[s] - 1) * 1.007276)
+        lines = [f"{mh:.4f} {int(sp.charge[s])}"]
+        a, b = int(sp.off[s]), int(sp.off[s + 1])
+        lines += [f"{float(m):.5f} {float(i):.5f}" for m, i in zip(sp.mz[a:b], sp.inten[a:b])]
+        p, c, m, i = formats.parse_dta("\n".join(lines) + "\n")
+        pcr.append(p), z.append(c), mzs.append(m), its.append(i)
+    off = np.concatenate([[0], np.cumsum([len(m) for m in mzs])]).astype(np.uint32)
+    ctx = pipeline.Ctx(0)
+    ddb = pipeline.DeviceDB(pipeline.HostDB(db.ions, db.r, db.tc, db.tn, mass))
+    dsp = pipeline.DeviceSpectra(ctx, np.float32(pcr), np.float32(z), np.concatenate(mzs),
+                                 np.concatenate(its), off, STRIDE)
+    ps = pipeline.PlainSearch(ctx, ddb, dsp, eps=3.0, topn=5)
+    res = ps.run()
+    out = pipeline.finish_matches(ps, res, 5)
+    idx = ddb.idx_sorted()
+    for s, matches in enumerate(out):
+        rows = []
+        for m in matches:
+            k = int(idx[int(res.begin[s]) + m.index])
+            label, p, mh = db.lookup(k)
+            rows.append(dict(mh=float(mh), xcorr=float(np.float32(m.score / 10000.0)),
+                             ions=m.ions, evalue=m.evalue, reference=db.headers[p].split()[0],
+                             peptide=label, header=db.headers[p]))
+        text = formats.format_scan("", f"spec{s}.dta", "test.fasta", 1, float(pcr[s]), float(z[s]),
+                                   3.0, rows, 5, 3)
+        lines = text.split("\n")
+        assert lines[2].startswith("Database    :: test.fasta")
+        body = [l for l in lines[8:8 + len(rows)]]
+        for r, l in zip(rows, body):
+            cells = l.split()
+            assert cells[4] == f"{r['ions'][0]}/{r['ions'][1]}" and cells[6] == r["peptide"]
+            assert cells[5] == r["reference"]
+            assert cells[7] == formats.format_number(r["evalue"], 2, 4)
