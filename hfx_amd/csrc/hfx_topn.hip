@@ -36,6 +36,9 @@
 #define TN_EXT 256
 #define TN_MAXC (TN_MAXT + TN_EXT)
 
+/* T at most this: the selection keeps per-thread lists in registers */
+#define TN_LT 8
+
 /* the first select pass histograms the key's 11 high bits */
 #define TN_HBITS 11
 #define TN_HBINS (1u << TN_HBITS)
@@ -100,6 +103,19 @@ __device__ __forceinline__ void tn_select_digit(const uint32_t *hist, uint32_t n
         }
     }
     __syncthreads();
+}
+
+/* the largest of the wave's 64-bit values, in every lane */
+__device__ __forceinline__ uint64_t tn_wave_max_u64(uint64_t v)
+{
+#pragma unroll
+    for (uint32_t o = 1; o < HFX_WAVE; o <<= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, o, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), o, 64);
+        const uint64_t y = (uint64_t)hi << 32 | lo;
+        v = y > v ? y : v;
+    }
+    return v;
 }
 
 /* Histogram increment of digit d for the active lanes of a wave: lanes that
@@ -167,7 +183,8 @@ __global__ void __launch_bounds__(TN_THREADS)
     __shared__ float cf32[TN_MAXC];
     __shared__ uint32_t wcnt[TN_WAVES];
     __shared__ float sh_x1[TN_WAVES], sh_xm[TN_WAVES];
-    __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_last, sh_ext;
+    __shared__ uint32_t sh_digit, sh_rem, sh_gt, sh_last, sh_ext, sh_wide;
+    __shared__ uint64_t sh_wtop[TN_WAVES * TN_LT];
     __shared__ float sh_fN;
 
     const uint32_t s = order[blockIdx.x], tid = threadIdx.x, wave = tid / HFX_WAVE;
@@ -213,9 +230,60 @@ __global__ void __launch_bounds__(TN_THREADS)
         sh_gt = 0;
         sh_ext = 0;
         sh_last = 0;
+        sh_wide = 0;
     }
+    /* T <= TN_LT: one sweep instead.  Each thread keeps its TN_LT best
+     * (key, ~position) values in registers, sorted; the waves merge them.
+     * Anything a thread left out ranks below its TN_LT-th value, which the
+     * band step uses to take the band from the lists. */
+    const bool local = T <= TN_LT;
+    uint64_t lst[TN_LT];
+#pragma unroll
+    for (uint32_t j = 0; j < TN_LT; ++j) lst[j] = 0; /* 0: empty (no score has key 0 with position ~0) */
     bool ranked = false;
-    for (int pass = 0; pass < 4; ++pass) {
+    if (local) {
+        for_keys([&](uint32_t i, uint32_t k) {
+            uint64_t x = i < n ? (uint64_t)k << 32 | (0xffffffffu - i) : 0ull;
+            if (x > lst[TN_LT - 1]) {
+#pragma unroll
+                for (uint32_t j = 0; j < TN_LT; ++j) {
+                    const uint64_t a = lst[j];
+                    lst[j] = x > a ? x : a;
+                    x = x > a ? a : x;
+                }
+            }
+        });
+        /* the wave's T best: T rounds of a wave max over the lists' heads */
+        uint32_t h = 0;
+        for (uint32_t r = 0; r < T; ++r) {
+            uint64_t hv = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < TN_LT; ++j) hv = h == j ? lst[j] : hv;
+            const uint64_t m = tn_wave_max_u64(hv);
+            if (m != 0 && hv == m) ++h;
+            if (lane == 0) sh_wtop[wave * TN_LT + r] = m;
+        }
+        __syncthreads();
+        /* the T best of the waves' 4T, ranked exactly */
+        if (tid < TN_WAVES * T) {
+            const uint64_t v = sh_wtop[(tid / T) * TN_LT + tid % T];
+            uint32_t rk = 0;
+            for (uint32_t b = 0; b < TN_WAVES * T; ++b)
+                rk += sh_wtop[(b / T) * TN_LT + b % T] > v ? 1u : 0u;
+            if (v != 0 && rk < T) {
+                const uint32_t pa = 0xffffffffu - (uint32_t)v;
+                cp[rk] = pa;
+                if (rk == N - 1u) sh_fN = hfx_key2f((uint32_t)(v >> 32));
+                if (rk == T - 1u) {
+                    sh_digit = (uint32_t)(v >> 32);
+                    sh_last = pa;
+                }
+            }
+        }
+        __syncthreads();
+        ranked = true;
+    }
+    for (int pass = 0; pass < 4 && !local; ++pass) {
         const uint32_t shift = pass == 0 ? 32u - TN_HBITS : pass == 1 ? 13u : pass == 2 ? 5u : 0u;
         const uint32_t nb = pass == 0 ? TN_HBINS : pass == 3 ? 32u : 256u;
         for (uint32_t b = tid; b < nb; b += TN_THREADS) hist[b] = 0;
@@ -306,7 +374,7 @@ __global__ void __launch_bounds__(TN_THREADS)
         sh_x1[wave] = x1;
         sh_xm[wave] = xm;
     }
-    if (n > T)
+    if (n > T && !local)
         for (uint32_t i = tid; i < T; i += TN_THREADS) {
             const uint32_t ki = hfx_f2key(cf32[i]), pi = cp[i];
             uint32_t rk = 0;
@@ -341,13 +409,35 @@ __global__ void __launch_bounds__(TN_THREADS)
         const double fN = (double)sh_fN;
         const double tau2 = fN - 2.0 * bound - 1e-12 * (fabs(fN) + 2.0 * bound);
         const uint32_t last = sh_last;
-        for_keys([&](uint32_t i, uint32_t k) {
-            const bool selected = k > K || (k == K && i <= last);
-            if (i < n && !selected && (double)hfx_key2f(k) >= tau2) {
-                const uint32_t slot = atomicAdd(&sh_ext, 1u);
-                if (slot < TN_EXT) cp[T + slot] = i;
-            }
-        });
+        /* from the lists: a thread whose TN_LT-th value is below tau2 left
+         * out only values below tau2; if every thread's is, the band is the
+         * lists' unselected values at or above tau2 */
+        bool wide = false;
+        if (local) {
+            const uint64_t vT = (uint64_t)K << 32 | (0xffffffffu - last);
+            wide = lst[TN_LT - 1] != 0 &&
+                   (double)hfx_key2f((uint32_t)(lst[TN_LT - 1] >> 32)) >= tau2;
+            if (wide) sh_wide = 1u;
+            __syncthreads();
+            wide = sh_wide != 0;
+            if (!wide)
+#pragma unroll
+                for (uint32_t j = 0; j < TN_LT; ++j) {
+                    const uint64_t v = lst[j];
+                    if (v != 0 && v < vT && (double)hfx_key2f((uint32_t)(v >> 32)) >= tau2) {
+                        const uint32_t slot = atomicAdd(&sh_ext, 1u);
+                        if (slot < TN_EXT) cp[T + slot] = 0xffffffffu - (uint32_t)v;
+                    }
+                }
+        }
+        if (!local || wide)
+            for_keys([&](uint32_t i, uint32_t k) {
+                const bool selected = k > K || (k == K && i <= last);
+                if (i < n && !selected && (double)hfx_key2f(k) >= tau2) {
+                    const uint32_t slot = atomicAdd(&sh_ext, 1u);
+                    if (slot < TN_EXT) cp[T + slot] = i;
+                }
+            });
         __syncthreads();
         ne = sh_ext;
         if (ne > TN_EXT) {
