@@ -834,7 +834,8 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
     __shared__ uint32_t sh_t[2];
     __shared__ uint32_t sh_kq[2], sh_kneed[2]; /* of the tile last written into map i */
 
-    const uint32_t tid = threadIdx.x, lane = hfx_lane(), wave = tid / HFX_WAVE;
+    const uint32_t tid = threadIdx.x, lane = hfx_lane();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / HFX_WAVE);
     const uint32_t grp = wave / D8_GW, gw = wave % D8_GW, gtid = tid % D8_GT;
     for (uint32_t i = tid; i < 2 * D8_ROWS * D8_MSTR; i += D8_THREADS) maps[i] = 0;
     for (uint32_t i = tid; i < 2 * D8_COMB_BYTES / 8; i += D8_THREADS) combs[i] = 0;
@@ -981,16 +982,22 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
                         sh_kneed[mi] = kneed;
                     }
                     uint32_t *map = maps + mi * (D8_ROWS * D8_MSTR);
+                    /* a lane's positions i, i + 384, ...: the first came a
+                     * tile ahead, each later one is loaded while the peaks
+                     * of the one before are written */
                     auto peaks = [&](auto chk) {
                         constexpr bool CHECK = decltype(chk)::value;
-                        if (HFX_WAVE * gw + lane < pc.np)
-                            dn_peaks_all<CHECK, D8_SPREAD>(map + (pc.pos.q - p0) * D8_MSTR + 1,
-                                                           pc.pos.b, pc.pos.y, kq, lane, lvl);
-                        for (uint32_t i = HFX_WAVE * (gw + D8_GW) + lane; i < pc.np;
-                             i += HFX_WAVE * D8_GW) {
-                            const hfx_dn_pos v = a.lad.pos[pc.L + i];
-                            dn_peaks_all<CHECK, D8_SPREAD>(map + (v.q - p0) * D8_MSTR + 1, v.b, v.y,
-                                                           kq, lane, lvl);
+                        constexpr uint32_t STEP = HFX_WAVE * D8_GW;
+                        uint32_t i = HFX_WAVE * gw + lane;
+                        hfx_dn_pos v = pc.pos;
+                        while (i < pc.np) {
+                            const uint32_t in = i + STEP;
+                            hfx_dn_pos vn = v;
+                            if (in < pc.np) vn = a.lad.pos[pc.L + in];
+                            dn_peaks_all<CHECK, D8_SPREAD>(map + (v.q - p0) * D8_MSTR + 1, v.b,
+                                                           v.y, kq, lane, lvl);
+                            v = vn;
+                            i = in;
                         }
                     };
                     if (kneed <= kq) peaks(std::false_type{});
@@ -1002,7 +1009,8 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
                 /* 3. the contraction of tile k - 1 from map mo into sums mo */
                 const uint32_t mo = mi ^ 1u;
                 const uint32_t nrows = rows_of(p0_of(k - 1));
-                const uint32_t kq = sh_kq[mo], kneed = sh_kneed[mo];
+                const uint32_t kq = __builtin_amdgcn_readfirstlane(sh_kq[mo]);
+                const uint32_t kneed = __builtin_amdgcn_readfirstlane(sh_kneed[mo]);
                 uint32_t *map = maps + mo * (D8_ROWS * D8_MSTR);
                 uint64_t *cb0 = combs + mo * (D8_COMB_BYTES / 8);
                 /* the clamped peaks' words past the tile's bins, cleared before
@@ -1019,12 +1027,19 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
                         if (r < nrb) {
                             d8_i32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
                             uint32_t *arow = map + (r * 16u + col) * D8_MSTR + 1 + kg;
+                            /* the words of k-step ks + 6 are loaded before
+                             * those of ks are turned into weights */
+                            uint32_t n0 = arow[8 * gw], n1 = arow[8 * gw + 4];
 #pragma unroll
                             for (uint32_t j = 0; j < D8_KPW; ++j) {
                                 const uint32_t ks = gw + D8_GW * j;
                                 if (ks < nks) {
                                     uint32_t *aw = arow + 8 * ks;
-                                    const uint32_t w0 = aw[0], w1 = aw[4];
+                                    const uint32_t w0 = n0, w1 = n1;
+                                    if (j + 1 < D8_KPW && ks + D8_GW < nks) {
+                                        n0 = aw[8 * D8_GW];
+                                        n1 = aw[8 * D8_GW + 4];
+                                    }
                                     aw[0] = 0u;
                                     aw[4] = 0u;
                                     d8_i32x4 av;
