@@ -1230,7 +1230,7 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
         if (bt_is_dense(b->precision)) {
             out_cfg[0] = 0;
             out_cfg[1] = bCH;
-            out_cfg[2] = b->precision == HFX_BATCH_I8X3_DENSE ? 768u : 1024u; /* workgroup threads */
+            out_cfg[2] = 1024u; /* workgroup threads of both dense kernels */
         }
     }
     return HFX_OK;

@@ -686,8 +686,8 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
  * (2^-k_s < 2 max|x_s| / 8355711).
  *
  * Sixteen spectra a block (the MFMA's N), 64 candidates a tile (four 16-row
- * blocks of A); one persistent workgroup of twelve waves per CU (three a
- * SIMD, so that each holds up to 168 registers) in two groups of six that
+ * blocks of A); one persistent workgroup of sixteen waves per CU (four a
+ * SIMD, so that each holds up to 128 registers) in two groups of eight that
  * take turns, with two maps and two sum buffers:
  *
  *   phase k:  group k % 2      forms the scores of tile k - 2 and writes the
@@ -698,8 +698,8 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
  * so the peaks (VALU, LDS atomics) of one tile overlap the contraction (VALU,
  * MFMA, LDS reads) of the previous one.
  *   - peaks: the tile's ladder positions go to the group's waves 64 at a time
- *     (group wave g takes positions 64 g + lane, 64 (g + 6) + lane, ...);
- *   - contraction: group wave g takes k-steps g, g + 6, g + 12, ... (64 bins
+ *     (group wave g takes positions 64 g + lane, 64 (g + 8) + lane, ...);
+ *   - contraction: group wave g takes k-steps g, g + 8, g + 16, ... (64 bins
  *     each) of the four row blocks; the B fragments of its k-steps (three
  *     digits each) stay in its registers for the whole block, loaded once
  *     from HBM, so B never passes through LDS; lane (row, kg) reads map words
@@ -714,8 +714,8 @@ __global__ void __launch_bounds__(DN_THREADS, 1) k_dense(DnArgs a)
 #define D8_NS 16
 #define D8_RB 4                   /* 16-row blocks of A a tile */
 #define D8_ROWS (16 * D8_RB)
-#define D8_WAVES 12
-#define D8_GW 6                   /* waves a group */
+#define D8_WAVES 16
+#define D8_GW 8                   /* waves a group */
 #define D8_THREADS (D8_WAVES * HFX_WAVE)
 #define D8_GT (D8_GW * HFX_WAVE)  /* threads a group */
 #define D8_MAXK DN_MAXK           /* bins of a row: 34 k-steps of 64 */
@@ -1022,14 +1022,20 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
                     }
                 const uint32_t nks = kq >> 6, nrb = (nrows + 15u) >> 4;
                 if (gw < nks) {
+                    /* the words of the wave's next k-step (the next row
+                     * block's first after a block's last) are loaded before
+                     * the current ones are turned into weights */
+                    uint32_t n0, n1;
+                    {
+                        const uint32_t *a0 = map + col * D8_MSTR + 1 + kg + 8 * gw;
+                        n0 = a0[0];
+                        n1 = a0[4];
+                    }
 #pragma unroll
                     for (uint32_t r = 0; r < D8_RB; ++r) {
                         if (r < nrb) {
                             d8_i32x4 acc[3] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
                             uint32_t *arow = map + (r * 16u + col) * D8_MSTR + 1 + kg;
-                            /* the words of k-step ks + 6 are loaded before
-                             * those of ks are turned into weights */
-                            uint32_t n0 = arow[8 * gw], n1 = arow[8 * gw + 4];
 #pragma unroll
                             for (uint32_t j = 0; j < D8_KPW; ++j) {
                                 const uint32_t ks = gw + D8_GW * j;
@@ -1039,6 +1045,9 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
                                     if (j + 1 < D8_KPW && ks + D8_GW < nks) {
                                         n0 = aw[8 * D8_GW];
                                         n1 = aw[8 * D8_GW + 4];
+                                    } else if (r + 1 < nrb) {
+                                        n0 = arow[16 * D8_MSTR + 8 * gw];
+                                        n1 = arow[16 * D8_MSTR + 8 * gw + 4];
                                     }
                                     aw[0] = 0u;
                                     aw[4] = 0u;

@@ -140,7 +140,7 @@ def test_batch_i8_graph_and_autotune(denv, orc):  # noqa: F811
     bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
     bs.set_precision(bs.I8X3_DENSE)
     G, Cn, T = bs.autotune()
-    assert G == 0 and Cn in (256, 512, 1024, 2048, 4096) and T == 768
+    assert G == 0 and Cn in (256, 512, 1024, 2048, 4096) and T == 1024
     bs.run()
     first = bs.results(keep_scores=True)
     check_topn(first, denv["ref"], orc)
