@@ -736,6 +736,9 @@ static_assert(D8_MSTR >= 1 + D8_MAXK / 8 + D8_TRASH && D8_MSTR % 4 == 2, "map ro
 #define D8_XMAX 8355711.0f
 static_assert(D8_LDS_BYTES + D8_FIXED_LDS <= 160 * 1024, "integer dense kernel LDS exceeds a CU");
 static_assert(D8_MAXK % 128 == 0, "rows of whole k-steps");
+/* a wave's digit sums: at most D8_KPW k-steps of 64 bins, weights <= 50,
+ * digits in [-128, 127]; 257 times that stays below 2^31 */
+static_assert((int64_t)D8_KPW * 64 * 50 * 128 * 257 < ((int64_t)1 << 31), "int32 digit fold");
 
 typedef int32_t d8_i32x4 __attribute__((ext_vector_type(4)));
 
@@ -1063,12 +1066,17 @@ __global__ void __launch_bounds__(D8_THREADS, 1) k_dense8(D8Args a)
                                 }
                             }
                             /* C[16 r + 4 kg + i][col] of the three digits as
-                             * one int64 into the tile's sums */
+                             * one int64 into the tile's sums: a wave's digit
+                             * sums are below 2^31 / 257 in magnitude, so
+                             * a1 2^8 + a0 is exact in int32 and one 64-bit
+                             * shift-add takes a2 2^16 on */
                             uint64_t *cb = cb0 + r * 256 + (4 * kg) * 16 + col;
 #pragma unroll
                             for (uint32_t i = 0; i < 4; ++i) {
-                                const int64_t v = ((int64_t)acc[2][i] << 16) +
-                                                  ((int64_t)acc[1][i] << 8) + (int64_t)acc[0][i];
+                                const int32_t v10 = (int32_t)(((uint32_t)acc[1][i] << 8) +
+                                                              (uint32_t)acc[0][i]);
+                                const int64_t v = (int64_t)(((uint64_t)(int64_t)acc[2][i] << 16) +
+                                                            (uint64_t)(int64_t)v10);
                                 atomicAdd((unsigned long long *)(cb + 16 * i),
                                           (unsigned long long)v);
                             }
