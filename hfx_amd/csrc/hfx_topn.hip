@@ -37,7 +37,7 @@
 #define TN_MAXC (TN_MAXT + TN_EXT)
 
 /* T at most this: the selection keeps per-thread lists in registers */
-#define TN_LT 8
+#define TN_LT 6
 
 /* the first select pass histograms the key's 11 high bits */
 #define TN_HBITS 11
