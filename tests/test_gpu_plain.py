@@ -269,20 +269,25 @@ def test_empty_windows_and_high_charge(env, orc):
     assert b > a and np.all(res.scores[a:b] == 0)
 
 
-def test_tie_heavy_window_takes_exact_route(orc):
-    """400 proteins carry the same tryptic peptide: a spectrum made from it has
-    400 candidates with one identical theoretical spectrum, so its N-th place
-    is an exact tie among far more candidates than the re-score set holds.
-    The spectrum goes down the on-device exact route (flagged), and the top N
-    (the tied copies in ascending position order, then the rest) equals the
-    oracle's.  Both scoring forms feed the same top-N."""
+@pytest.mark.parametrize("copies,topn", [(400, 10), (400, 5), (400, 1), (120, 5)])
+def test_tie_heavy_window_takes_exact_route(orc, copies, topn):
+    """`copies` proteins carry the same tryptic peptide: a spectrum made from
+    it has that many candidates with one identical theoretical spectrum, so
+    its N-th place is an exact tie.  With 400 copies the tie holds far more
+    candidates than the re-score set (256 past the selection): the spectrum
+    goes down the on-device exact route (flagged); with 120 the tied band is
+    re-scored inside the top-N kernel.  N of 1 and 5 select through the
+    per-thread lists (the band taken from the lists), N of 10 through the
+    radix select.  The top N (the tied copies in ascending position order,
+    then the rest) equals the oracle's.  Both scoring forms feed the same
+    top-N."""
     from hfx_amd import pipeline, synth
 
     base = synth.make_proteins(600, seed=61)
     motif = np.frombuffer(b"GASGTPAGEGSK", np.uint8)
     rng = np.random.default_rng(62)
     parts, seg = [base.seq], list(base.seg)
-    for _ in range(400):
+    for _ in range(copies):
         head = np.frombuffer(b"MKR", np.uint8)
         tail = synth.make_proteins(1, seed=int(rng.integers(1 << 30)), min_len=20, max_len=60).seq
         p = np.concatenate([head, motif, tail])
@@ -296,7 +301,7 @@ def test_tie_heavy_window_takes_exact_route(orc):
     # spectrum 0 from the shared peptide
     hit = [i for i in range(len(r))
            if synth.peptide_residues(prot, int(tc[i]), int(tn[i])) == motif.tobytes()]
-    assert len(hit) == 400
+    assert len(hit) == copies
     one = synth.make_spectra(prot, r[hit[0]:hit[0] + 1], tc[hit[0]:hit[0] + 1],
                              tn[hit[0]:hit[0] + 1], 1, seed=64, charges=(2,))
     n0 = int(one.off[1])
@@ -311,10 +316,10 @@ def test_tie_heavy_window_takes_exact_route(orc):
     dsp = pipeline.DeviceSpectra(ctx, pcr, ch, mz, inten, off, 2112)
     idx, _ = orc.sort_idx(r)
     ref = orc.search_plain(orc.prepare_ions(prot.seq), r, tc, tn, mass, idx, dsp.x(),
-                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, 10)
+                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, topn)
     for dense in (False, True):
-        res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=10).run(dense=dense)
-        assert 0 in res.flagged.tolist(), dense
+        res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=topn).run(dense=dense)
+        assert (0 in res.flagged.tolist()) == (copies - topn > 256), dense
         assert (res.top_cnt == ref["top_cnt"]).all()
         for s in range(dsp.num_spec):
             c = int(res.top_cnt[s])
@@ -322,11 +327,14 @@ def test_tie_heavy_window_takes_exact_route(orc):
             assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
 
 
-def test_exact_route_past_the_lds_selection(orc):
+@pytest.mark.parametrize("topn", [10, 5])
+def test_exact_route_past_the_lds_selection(orc, topn):
     """5000 proteins carry the same tryptic peptide: spectrum 0's window holds
     5000 candidates tied exactly, more than the exact route's LDS selection
     takes (4096 live keys), so its top N come from the radix select over
-    global memory.  Top N equal the oracle's for both scoring forms."""
+    global memory.  With N = 5 the per-thread lists fill with tied values
+    (each thread sees about twenty), so the band is swept instead of taken
+    from the lists.  Top N equal the oracle's for both scoring forms."""
     from hfx_amd import pipeline, synth
 
     copies = 5000
@@ -363,10 +371,10 @@ def test_exact_route_past_the_lds_selection(orc):
     dsp = pipeline.DeviceSpectra(ctx, pcr, ch, mz, inten, off, 2112)
     idx, _ = orc.sort_idx(r)
     ref = orc.search_plain(orc.prepare_ions(prot.seq), r, tc, tn, mass, idx, dsp.x(),
-                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, 10)
+                           dsp.lengths(), dsp.zmax, dsp.mass, 3.0, topn)
     assert int(ref["num"][0]) > copies
     for dense in (False, True):
-        res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=10).run(dense=dense)
+        res = pipeline.PlainSearch(ctx, db, dsp, eps=3.0, topn=topn).run(dense=dense)
         assert 0 in res.flagged.tolist(), dense
         assert (res.top_cnt == ref["top_cnt"]).all()
         for s in range(dsp.num_spec):
