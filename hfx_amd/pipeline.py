@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import HfxDb, HfxModset, HfxSpectra, check, lib
+from . import HfxDb, HfxError, HfxModset, HfxSpectra, check, lib
 
 # per-spectrum flags of the top-N (include/hfx_hip.h)
 FLAG_EXACT = 1
@@ -75,6 +75,23 @@ class DevBuf:
 
     def zero(self, stream=None):
         check(lib().hfx_memset(self.p, 0, self.nbytes, stream), "hfx_memset")
+
+
+class DevView(DevBuf):
+    """A window [offset, offset + nbytes) of another device buffer, kept
+    alive with it; never freed itself."""
+
+    __slots__ = ("owner",)
+
+    def __init__(self, owner: DevBuf, offset: int, nbytes: int):
+        if offset < 0 or offset + nbytes > owner.nbytes:
+            raise ValueError("view outside its buffer")
+        self.owner = owner
+        self.p = owner.p + int(offset)
+        self.nbytes = max(int(nbytes), 1)
+
+    def __del__(self):
+        pass
 
 
 def dev(arr: np.ndarray) -> DevBuf:
@@ -266,6 +283,35 @@ class DeviceSpectra:
 
     def x(self) -> np.ndarray:
         return host(self.t_x, np.float32).reshape(self.num_spec, self.x_stride)
+
+    def view(self, s0: int, s1: int) -> "SpectraView":
+        """Spectra [s0, s1) as a search input, sharing this object's device
+        arrays (no copy)."""
+        return SpectraView(self, s0, s1)
+
+
+class SpectraView:
+    """Spectra [s0, s1) of a DeviceSpectra: the same preprocessed arrays
+    from spectrum s0 on, with the fields a search reads."""
+
+    def __init__(self, sp: DeviceSpectra, s0: int, s1: int):
+        if not 0 <= s0 <= s1 <= sp.num_spec:
+            raise ValueError(f"spectra {s0}..{s1} of {sp.num_spec}")
+        n, st = s1 - s0, sp.x_stride
+        self.base, self.s0 = sp, s0
+        self.num_spec, self.x_stride = n, st
+        self.zmax, self.mass = sp.zmax[s0:s1], sp.mass[s0:s1]
+        self.t_x = DevView(sp.t_x, 4 * s0 * st, 4 * n * st)
+        self.t_len = DevView(sp.t_len, 4 * s0, 4 * n)
+        self.t_status = DevView(sp.t_status, 4 * s0, 4 * n)
+        self.t_zmax = DevView(sp.t_zmax, 4 * s0, 4 * n)
+        self.t_mass = DevView(sp.t_mass, 4 * s0, 4 * n)
+        self.desc = HfxSpectra(ptr(self.t_x), st, ptr(self.t_len), ptr(self.t_zmax),
+                               ptr(self.t_mass), n)
+
+    lengths = DeviceSpectra.lengths
+    status = DeviceSpectra.status
+    x = DeviceSpectra.x
 
 
 @dataclass
@@ -708,6 +754,41 @@ class ModSearch:
         self.t_mod_sum = dev(mi.mod_sum)
         self.t_mod_delta = dev(mi.mod_delta)
 
+    def mpep_per_spectrum(self) -> np.ndarray:
+        """The modified peptides each spectrum would score, from the stages
+        whose arrays grow with the candidates by mass (the windows,
+        findModablePeptides, calcTotalModCands), without expanding them."""
+        L = lib()
+        S, M, na = self.sp.num_spec, self.mi.num_mod, self.mi.num_ma
+        db = self.db
+        if S == 0:
+            return np.zeros(0, np.uint64)
+        t_b, t_e, t_sc, t_cand = empty(4 * S * M), empty(4 * S * M), empty(4 * S * M), empty(4 * S)
+        check(L.hfx_mod_windows(self.ctx.h, t_b.p, t_e.p, t_sc.p, t_cand.p, C.byref(db.desc),
+                                self.sp.t_mass.p, S, self.t_mod_delta.p, M, self.eps, None),
+              "hfx_mod_windows")
+        spec_cand = host(t_cand, np.uint32, S)
+        scan = np.zeros(S + 1, np.uint32)
+        np.cumsum(spec_cand, out=scan[1:])
+        total = int(scan[-1])
+        t_scan = dev(scan)
+        t_pep, t_pmod = empty(4 * total), empty(4 * total)
+        t_pcnt, t_vscan = empty(4 * total * na), empty(4 * S)
+        nv = L.findModablePeptides(t_pep.p, t_pmod.p, t_pcnt.p, t_vscan.p, t_scan.p, S, total,
+                                   db.t_ions.p, db.t_tc.p, db.t_tn.p, db.t_idx.p, t_b.p, t_e.p,
+                                   t_sc.p, self.t_mod_cnt.p, M, self.t_ma.p, na)
+        if nv == 0:
+            return np.zeros(S, np.uint64)
+        t_nmp, t_comb = empty(4 * nv), empty(4 * nv * na)
+        L.calcTotalModCands(t_nmp.p, t_comb.p, self.t_mod_cnt.p, t_pep.p, t_pmod.p, t_pcnt.p,
+                            nv, M, na)
+        sync()
+        # valid rows are spectrum-major; vscan[s] is spectrum s's first
+        rows = np.zeros(nv + 1, np.uint64)
+        np.cumsum(host(t_nmp, np.uint32, nv), out=rows[1:])
+        vb = np.append(host(t_vscan, np.uint32, S).astype(np.int64), nv)
+        return rows[vb[1:]] - rows[vb[:-1]]
+
     def run(self, keep_scores: bool = False, per_spectrum: bool = False,
             keep_stages: bool = True) -> ModResult:
         """per_spectrum: the windows by one findBeginEnd_f call per spectrum
@@ -830,6 +911,74 @@ class ModSearch:
                                          stream_handle()), "hfx_sp_ions_topn_mod")
         sync()
         return tuple(host(x, np.uint32, S * N).reshape(S, N) for x in t)
+
+
+# Device bytes one modified peptide holds while its batch is searched: the
+# five arrays of prepareGenMod (20), its score (4), its unrank slice (at most
+# four sites, 16), the top-N exact route's per-score scratch (about 8.2) and
+# its share of the per-spectrum arrays; an upper bound.
+MPEP_BYTES = 64
+
+
+def mod_groups(spec_mpep, budget: int, per: int = MPEP_BYTES) -> list[tuple[int, int]]:
+    """Consecutive spectrum ranges [s0, s1) whose modified peptides take at
+    most `budget` bytes at `per` bytes each, as few as possible (a greedy
+    cut in spectrum order).  A spectrum that alone exceeds the budget is an
+    error."""
+    cap = int(budget) // int(per)
+    groups, s0, acc = [], 0, 0
+    for s, c in enumerate(np.asarray(spec_mpep, np.int64).tolist()):
+        if c > cap:
+            raise HfxError(f"spectrum {s}: {c} modified peptides need {c * per} bytes, "
+                           f"more than the {budget}-byte budget")
+        if acc + c > cap:
+            groups.append((s0, s))
+            s0, acc = s, 0
+        acc += c
+    if s0 < len(spec_mpep):
+        groups.append((s0, len(spec_mpep)))
+    return groups
+
+
+def search_mods_split(ctx: Ctx, db: DeviceDB, sp: DeviceSpectra, mi: ModInfo, eps: float,
+                      topn: int, margin: int = 0, budget: int | None = None):
+    """searchWithMods over batches of spectra sized to device memory
+    (Execution.hs:77-93 cuts a search's spectra by the memory its buffers
+    need): each spectrum's modified-peptide count comes from the stages
+    that do not expand them, consecutive spectra are grouped so that a
+    group's modified peptides fit `budget` bytes (default: half the free
+    device memory), and each group is searched on a view of the spectra.
+    Returns the merged result (positions index each spectrum's own modified
+    peptides, as in one batch) and the groups."""
+    if budget is None:
+        free, total = C.c_size_t(), C.c_size_t()
+        check(lib().hfx_mem_info(C.byref(free), C.byref(total)), "hfx_mem_info")
+        budget = free.value // 2
+    per = ModSearch(ctx, db, sp, mi, eps, topn, margin).mpep_per_spectrum()
+    groups = mod_groups(per, budget)
+    S, N = sp.num_spec, int(topn)
+    spec_cand = np.zeros(S, np.uint32)
+    mp_off = np.zeros(S + 1, np.uint64)
+    top_pos = np.zeros((S, N), np.uint32)
+    top_exact = np.zeros((S, N), np.float64)
+    top_f32 = np.zeros((S, N), np.float32)
+    top_cnt = np.zeros(S, np.uint32)
+    flagged, short = [], []
+    nv = nmp = lu = 0
+    for s0, s1 in groups:
+        r = ModSearch(ctx, db, sp.view(s0, s1), mi, eps, topn, margin).run(keep_stages=False)
+        spec_cand[s0:s1] = r.spec_cand
+        mp_off[s0:s1 + 1] = r.mp_off + np.uint64(nmp)
+        top_pos[s0:s1], top_exact[s0:s1], top_f32[s0:s1] = r.top_pos, r.top_exact, r.top_f32
+        top_cnt[s0:s1] = r.top_cnt
+        flagged.append(r.flagged + np.uint32(s0))
+        short.append(r.short + np.uint32(s0))
+        nv, nmp, lu = nv + r.num_valid, nmp + r.num_mpep, lu + r.len_unrank
+    cat = (lambda xs: np.concatenate(xs).astype(np.uint32) if xs else np.zeros(0, np.uint32))
+    res = ModResult(spec_cand=spec_cand, num_valid=nv, num_mpep=nmp, len_unrank=lu,
+                    mp_off=mp_off, top_pos=top_pos, top_exact=top_exact, top_f32=top_f32,
+                    top_cnt=top_cnt, flagged=cat(flagged), stages={}, short=cat(short))
+    return res, groups
 
 
 @dataclass

@@ -230,3 +230,34 @@ def test_mods_uneven_workload_against_oracle(cfg3, orc):
            or not (res.top_exact[s, :res.top_cnt[s]] ==
                    orc.fx_to_double(ref["top_fx"][s, :ref["top_cnt"][s]])).all()]
     assert not bad, bad[:10]
+
+
+def test_mods_split_by_memory_matches_one_batch(cfg3):
+    """search_mods_split with a budget that cuts the 1,000 spectra into
+    several batches: the per-spectrum modified-peptide counts it sizes with
+    equal the one-batch run's, every batch fits the budget, and the merged
+    result (windows, counts, offsets, top N, exact scores, flags) equals the
+    one-batch run's."""
+    pl, ms = cfg3["pl"], cfg3["ms"]
+    one = ms.run(keep_stages=False)
+    per = ms.mpep_per_spectrum()
+    assert (per == np.diff(one.mp_off)).all()
+    budget = int(per.sum()) * pl.MPEP_BYTES // 5
+    res, groups = pl.search_mods_split(cfg3["ctx"], cfg3["db"], cfg3["dsp"], cfg3["mi"], 3.0,
+                                       cfg3["w"]["topn"], budget=budget)
+    print(f"{len(groups)} batches for {int(per.sum())} modified peptides")
+    assert len(groups) >= 5
+    for a, b in groups:
+        assert int(per[a:b].sum()) * pl.MPEP_BYTES <= budget
+    assert (res.spec_cand == one.spec_cand).all()
+    assert res.num_valid == one.num_valid and res.num_mpep == one.num_mpep
+    assert res.len_unrank == one.len_unrank
+    assert (res.mp_off == one.mp_off).all()
+    assert (res.top_cnt == one.top_cnt).all()
+    for s in range(len(one.top_cnt)):
+        c = int(one.top_cnt[s])
+        assert (res.top_pos[s, :c] == one.top_pos[s, :c]).all(), s
+        assert (res.top_exact[s, :c] == one.top_exact[s, :c]).all(), s
+        assert (res.top_f32[s, :c] == one.top_f32[s, :c]).all(), s
+    assert res.flagged.tolist() == one.flagged.tolist()
+    assert res.short.tolist() == one.short.tolist()

@@ -222,3 +222,31 @@ def test_stats_entry_points_reject_bad_arguments():
     assert L.hfx_sp_ions(None, p, p, p, p, p, None, 1, p, 1, C.byref(db), C.byref(sp), 1,
                          None) == 1
     assert "flag words" in hfx_amd.last_error()
+
+
+def test_mod_groups_cut_spectra_by_memory():
+    """The modified search's batches (Execution.hs:77-93 cuts a search's
+    spectra by the memory its buffers need): consecutive ranges covering
+    every spectrum once, each within the budget, a new range only where the
+    next spectrum would overflow it, and an error for a spectrum that alone
+    exceeds the budget."""
+    rng = np.random.default_rng(11)
+    per = pipeline.MPEP_BYTES
+    for trial in range(200):
+        n = int(rng.integers(0, 60))
+        counts = rng.integers(0, 50, n) * (rng.random(n) < 0.8)
+        budget = per * int(rng.integers(max(int(counts.max(initial=0)), 1), 400))
+        g = pipeline.mod_groups(counts, budget)
+        if n == 0:
+            assert g == []
+            continue
+        assert [a for a, _ in g] == [0] + [b for _, b in g[:-1]]
+        assert (g[-1][1] if g else 0) == n
+        for k, (a, b) in enumerate(g):
+            assert b > a
+            assert int(counts[a:b].sum()) * per <= budget
+            if k + 1 < len(g):
+                assert (int(counts[a:b].sum()) + int(counts[b])) * per > budget
+    assert pipeline.mod_groups([], per) == []
+    with pytest.raises(hfx_amd.HfxError):
+        pipeline.mod_groups([1, 5, 2], 4 * per)
