@@ -505,20 +505,43 @@ __global__ void k_ord_rank(uint32_t *order, const uint32_t *tmp, const uint64_t 
  * minimise the sum over blocks of (charge levels x rows of the hull of the
  * windows) -- a block spanning a gap in precursor mass would score every
  * candidate in the gap.  A dynamic programme per segment of CUT_SEG order
- * positions (one thread each; segments are cut independently, which costs
- * about 1% more rows than one programme over the whole order). */
+ * positions (segments are cut independently, which costs about 1% more
+ * rows than one programme over the whole order), one row of 16 lanes a
+ * segment: lane i holds the window of the (i+1)-th newest position and the
+ * best cost of the positions before it.  Each step shifts both by one lane
+ * (DPP row_shr:1, the new position entering lane 0), takes the hull of the
+ * newest i+1 windows by inclusive row scans (min of starts, max of ends, OR
+ * of charge levels) and the best block length by a row minimum of
+ * (cost << 4 | i): the shortest block among equal costs, the one a serial
+ * scan of lengths 1, 2, ... with strict improvement keeps. */
 #define CUT_SEG 32
-#define CUT_WG 64
+#define CUT_ROW 16
+#define CUT_WG 256
+#define CUT_SPW (CUT_WG / CUT_ROW) /* segments a workgroup */
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t cut_dpp(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t cut_dpp64(uint64_t old, uint64_t v)
+{
+    return (uint64_t)cut_dpp<CTRL>((uint32_t)(old >> 32), (uint32_t)(v >> 32)) << 32 |
+           cut_dpp<CTRL>((uint32_t)old, (uint32_t)v);
+}
+
 template <int CUT_NS>
 __global__ void __launch_bounds__(CUT_WG) k_cut_seg(uint32_t *seg_cuts, uint32_t *seg_n,
                                                    const uint32_t *order, const uint32_t *begin,
                                                    const uint32_t *num, const uint32_t *zmax,
                                                    uint32_t S)
 {
-    __shared__ uint32_t lo_s[CUT_WG * CUT_SEG], hi_s[CUT_WG * CUT_SEG], lv_s[CUT_WG * CUT_SEG];
-    __shared__ uint8_t from_s[CUT_WG][CUT_SEG + 1];
-    const uint32_t base = blockIdx.x * CUT_WG * CUT_SEG;
-    for (uint32_t i = threadIdx.x; i < CUT_WG * CUT_SEG; i += CUT_WG) {
+    static_assert(CUT_NS <= CUT_ROW, "every block length has its lane");
+    __shared__ uint32_t lo_s[CUT_SPW * CUT_SEG], hi_s[CUT_SPW * CUT_SEG], lv_s[CUT_SPW * CUT_SEG];
+    __shared__ uint8_t from_s[CUT_SPW][CUT_SEG + 1];
+    const uint32_t base = blockIdx.x * CUT_SPW * CUT_SEG;
+    for (uint32_t i = threadIdx.x; i < CUT_SPW * CUT_SEG; i += CUT_WG) {
         const uint32_t p = base + i;
         uint32_t lo = 0xffffffffu, hi = 0, lv = 0;
         if (p < S) {
@@ -534,55 +557,51 @@ __global__ void __launch_bounds__(CUT_WG) k_cut_seg(uint32_t *seg_cuts, uint32_t
         lv_s[i] = lv;
     }
     __syncthreads();
-    const uint32_t seg = blockIdx.x * CUT_WG + threadIdx.x;
+    const uint32_t r = threadIdx.x / CUT_ROW, i = threadIdx.x % CUT_ROW;
+    const uint32_t seg = blockIdx.x * CUT_SPW + r;
     const uint32_t p0 = seg * CUT_SEG;
-    if (p0 >= S) return;
+    if (p0 >= S) return; /* the whole row leaves together */
     const uint32_t len = min((uint32_t)CUT_SEG, S - p0);
-    const uint32_t *lo = lo_s + threadIdx.x * CUT_SEG, *hi = hi_s + threadIdx.x * CUT_SEG,
-                   *lv = lv_s + threadIdx.x * CUT_SEG;
-    uint8_t *from = from_s[threadIdx.x];
-    /* registers hold the last CUT_NS positions (index 0 the newest) and the
-     * best costs before them: bst[k - 1] = best cost of the first j - k
-     * positions */
-    uint64_t bst[CUT_NS];
-    uint32_t rlo[CUT_NS], rhi[CUT_NS], rlv[CUT_NS];
-#pragma unroll
-    for (int k = 0; k < CUT_NS; ++k) {
-        bst[k] = k == 0 ? 0ull : ~0ull;
-        rlo[k] = 0xffffffffu;
-        rhi[k] = 0;
-        rlv[k] = 0;
-    }
+    const uint32_t *lo = lo_s + r * CUT_SEG, *hi = hi_s + r * CUT_SEG, *lv = lv_s + r * CUT_SEG;
+    uint8_t *from = from_s[r];
+    uint32_t wlo = 0xffffffffu, whi = 0, wlv = 0;
+    uint64_t bst = i == 0 ? 0ull : ~0ull;
     for (uint32_t j = 1; j <= len; ++j) {
-#pragma unroll
-        for (int k = CUT_NS - 1; k > 0; --k) {
-            rlo[k] = rlo[k - 1];
-            rhi[k] = rhi[k - 1];
-            rlv[k] = rlv[k - 1];
-        }
-        rlo[0] = lo[j - 1];
-        rhi[0] = hi[j - 1];
-        rlv[0] = lv[j - 1];
-        uint32_t l = 0xffffffffu, h = 0, v = 0, fj = j - 1;
-        uint64_t bj = ~0ull;
-#pragma unroll
-        for (uint32_t k = 1; k <= CUT_NS; ++k) {
-            l = min(l, rlo[k - 1]);
-            h = max(h, rhi[k - 1]);
-            v |= rlv[k - 1];
-            const uint64_t rows = h > l ? (uint64_t)(h - l) : 0ull;
-            const uint64_t c = bst[k - 1] + rows * (uint64_t)__builtin_popcount(v);
-            /* bst[k - 1] is ~0 for k > j: never chosen */
-            if (k <= j && c < bj) {
-                bj = c;
-                fj = j - k;
-            }
-        }
-#pragma unroll
-        for (int k = CUT_NS - 1; k > 0; --k) bst[k] = bst[k - 1];
-        bst[0] = bj;
-        from[j] = (uint8_t)fj;
+        wlo = cut_dpp<0x111>(lo[j - 1], wlo);
+        whi = cut_dpp<0x111>(hi[j - 1], whi);
+        wlv = cut_dpp<0x111>(lv[j - 1], wlv);
+        uint32_t l = wlo, h = whi, v = wlv;
+        l = min(l, cut_dpp<0x111>(0xffffffffu, l));
+        h = max(h, cut_dpp<0x111>(0u, h));
+        v |= cut_dpp<0x111>(0u, v);
+        l = min(l, cut_dpp<0x112>(0xffffffffu, l));
+        h = max(h, cut_dpp<0x112>(0u, h));
+        v |= cut_dpp<0x112>(0u, v);
+        l = min(l, cut_dpp<0x114>(0xffffffffu, l));
+        h = max(h, cut_dpp<0x114>(0u, h));
+        v |= cut_dpp<0x114>(0u, v);
+        l = min(l, cut_dpp<0x118>(0xffffffffu, l));
+        h = max(h, cut_dpp<0x118>(0u, h));
+        v |= cut_dpp<0x118>(0u, v);
+        /* lane i: the block of the newest i + 1 positions after the best cut
+         * of the ones before (bst, finite for i < j) */
+        const uint64_t rows = h > l ? (uint64_t)(h - l) : 0ull;
+        uint64_t key = ~0ull;
+        if (i < (uint32_t)CUT_NS && i < j)
+            key = (bst + rows * (uint64_t)__builtin_popcount(v)) << 4 | i;
+        uint64_t t;
+        t = cut_dpp64<0xB1>(0ull, key);  /* quad_perm [1,0,3,2] */
+        key = t < key ? t : key;
+        t = cut_dpp64<0x4E>(0ull, key);  /* quad_perm [2,3,0,1] */
+        key = t < key ? t : key;
+        t = cut_dpp64<0x141>(0ull, key); /* row_half_mirror */
+        key = t < key ? t : key;
+        t = cut_dpp64<0x140>(0ull, key); /* row_mirror */
+        key = t < key ? t : key;
+        bst = cut_dpp64<0x111>(key >> 4, bst);
+        if (i == 0) from[j] = (uint8_t)(j - 1u - (uint32_t)(key & 15u));
     }
+    if (i != 0) return;
     /* block starts, back from the segment's end, then written in order */
     uint32_t cnt = 0;
     for (uint32_t j = len; j > 0; j = from[j]) ++cnt;
@@ -979,10 +998,10 @@ static int bt_dense_prep(hfx_batch *b, hipStream_t st)
     const uint32_t S = b->S;
     const bool i8 = b->precision == HFX_BATCH_I8X3_DENSE;
     if (i8)
-        hipLaunchKernelGGL(k_cut_seg<16>, dim3((b->nseg + CUT_WG - 1) / CUT_WG), dim3(CUT_WG), 0,
+        hipLaunchKernelGGL(k_cut_seg<16>, dim3((b->nseg + CUT_SPW - 1) / CUT_SPW), dim3(CUT_WG), 0,
                            st, b->cut_seg, b->cut_n, b->order, b->begin, b->num, b->sp.d_zmax, S);
     else
-        hipLaunchKernelGGL(k_cut_seg<8>, dim3((b->nseg + CUT_WG - 1) / CUT_WG), dim3(CUT_WG), 0,
+        hipLaunchKernelGGL(k_cut_seg<8>, dim3((b->nseg + CUT_SPW - 1) / CUT_SPW), dim3(CUT_WG), 0,
                            st, b->cut_seg, b->cut_n, b->order, b->begin, b->num, b->sp.d_zmax, S);
     int rc;
     if ((rc = hfx_scan_offsets(b->ctx, b->cut_off, b->cut_n, b->nseg, (void *)st))) return rc;
