@@ -756,8 +756,23 @@ __global__ void __launch_bounds__(256) k_d8_split(int8_t *xq, float *qs, hfx_spe
     const uint32_t s = blockIdx.x;
     const uint32_t len = min(sp.d_len[s], min(sp.x_stride, (uint32_t)D8_MAXK));
     const float *x = sp.d_x + (size_t)s * sp.x_stride;
+    /* the row once, in registers: thread t takes the bins of output words
+     * t + 256 u (the K order permutes bins within each 64), which cover
+     * every bin once, so max |x| comes from them too */
+    constexpr uint32_t NW = D8_MAXK / 4, NU = (NW + 255) / 256;
+    float v[NU][4];
     float m = 0.0f;
-    for (uint32_t i = threadIdx.x; i < len; i += 256) m = fmaxf(m, fabsf(x[i]));
+#pragma unroll
+    for (uint32_t u = 0; u < NU; ++u) {
+        const uint32_t t = threadIdx.x + 256 * u;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            const uint32_t pos = 4 * t + i;
+            const uint32_t bin = (pos & ~63u) + d8_bin_of(pos & 63u);
+            v[u][i] = t < NW && bin < len ? x[bin] : 0.0f;
+            m = fmaxf(m, fabsf(v[u][i]));
+        }
+    }
     for (uint32_t o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if (hfx_lane() == 0) red[threadIdx.x / HFX_WAVE] = m;
     __syncthreads();
@@ -774,14 +789,14 @@ __global__ void __launch_bounds__(256) k_d8_split(int8_t *xq, float *qs, hfx_spe
     const float sc = ldexpf(1.0f, k);
     if (threadIdx.x == 0) qs[s] = ldexpf(1.0f, -k);
     int8_t *o = xq + (size_t)s * 3 * D8_MAXK;
-    for (uint32_t t = threadIdx.x; t < D8_MAXK / 4; t += 256) {
+#pragma unroll
+    for (uint32_t u = 0; u < NU; ++u) {
+        const uint32_t t = threadIdx.x + 256 * u;
+        if (t >= NW) break;
         uint32_t w[3] = {0u, 0u, 0u};
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
-            const uint32_t pos = 4 * t + i;
-            const uint32_t bin = (pos & ~63u) + d8_bin_of(pos & 63u);
-            const float v = bin < len ? x[bin] : 0.0f;
-            const int32_t X = (int32_t)rintf(v * sc);
+            const int32_t X = (int32_t)rintf(v[u][i] * sc);
             const int32_t d0 = ((X + 128) & 255) - 128;
             const int32_t X1 = (X - d0) >> 8;
             const int32_t d1 = ((X1 + 128) & 255) - 128;
