@@ -215,16 +215,21 @@ __global__ void k_dn_blocks(uint32_t *ubeg, uint32_t *uend, uint32_t *binfo, uin
     if (b >= B) return;
     uint32_t lo = 0xffffffffu, hi = 0, mask = 0, kmax = 0;
     const uint32_t i0 = dn_bfirst(bstart, b, NS), i1 = dn_bend(bstart, b, sp.num_spec, NS);
+    /* the block's spectra first, then their windows, charges and lengths:
+     * two rounds of independent loads */
+    uint32_t sv[NS];
+#pragma unroll
+    for (uint32_t g = 0; g < NS; ++g) sv[g] = i0 + g < i1 ? dn_order(order, i0 + g) : DN_NONE;
+#pragma unroll
     for (uint32_t g = 0; g < NS; ++g) {
-        const uint32_t i = i0 + g;
-        if (i >= i1) break;
-        const uint32_t s = dn_order(order, i);
-        const uint32_t z = hfx_zeff(sp.d_zmax[s]);
-        if (z == 0 || num[s] == 0) continue;
-        lo = min(lo, begin[s]);
-        hi = max(hi, begin[s] + num[s]);
+        const uint32_t s = sv[g];
+        if (s == DN_NONE) continue;
+        const uint32_t z = hfx_zeff(sp.d_zmax[s]), n = num[s], bg = begin[s], ln = sp.d_len[s];
+        if (z == 0 || n == 0) continue;
+        lo = min(lo, bg);
+        hi = max(hi, bg + n);
         mask |= 1u << z;
-        kmax = max(kmax, min(sp.d_len[s], min(sp.x_stride, (uint32_t)DN_MAXK)));
+        kmax = max(kmax, min(ln, min(sp.x_stride, (uint32_t)DN_MAXK)));
     }
     if (hi <= lo) { lo = 0; hi = 0; mask = 0; }
     const uint32_t kb = min((kmax + 127u) & ~127u, (uint32_t)DN_MAXK);
