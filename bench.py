@@ -28,7 +28,7 @@ time is reported beside the step (`db_setup_ms`), as is the one-off plan
 value = scored pairs per second of the batched BF16-split form (whole job,
 one GPU); every form's top-N must equal the plain form's.
 
-Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -406,7 +406,7 @@ def main():
                          "row matrix) to calibrate HBM byte counters, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1"}))
         raise SystemExit(2)
     if args.calib_read:
         calib_read(args.calib_read)
