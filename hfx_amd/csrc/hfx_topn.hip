@@ -7,8 +7,10 @@
  * (stable ascending sort, then reverse twice: Search.hs:119-123, :276).
  *
  * Here one workgroup per spectrum:
- *   1. radix-selects the T = min(N + margin, n) best FP32 scores (key desc,
- *      position asc);
+ *   1. selects the T = min(N + margin, n) best FP32 scores (key desc,
+ *      position asc): for T <= TN_LT in one sweep, each thread keeping its
+ *      TN_LT best in registers and the waves merging them; otherwise by a
+ *      radix select;
  *   2. takes the N-th best FP32 score fN and collects every other candidate
  *      whose FP32 score is >= fN - 2B (B bounds |FP32 - exact|): the N-th
  *      best exact score exN is >= fN - B, so no candidate left out can reach
