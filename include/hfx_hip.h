@@ -629,7 +629,7 @@ int hfx_batch_replay(hfx_batch *b, void *stream);
 /* time the scoring kernel over (spectra per block G, tile length C,
  * workgroup threads) -- dense form: candidates per work item C -- and keep
  * the fastest; out_cfg = {G, C, threads} ({0, C, workgroup threads} for the
- * dense forms: 1024 BF16, 768 integer) */
+ * dense forms: 1024 in both) */
 int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg);
 int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32_t threads);
 /* one run, phase by phase, with event timings in ms (5 entries): {windows,
@@ -640,8 +640,8 @@ int hfx_batch_time(hfx_batch *b, void *stream, float *ms);
  *         G, C, threads, LDS stride, 0, precision, dense chunk} (12 entries) */
 int hfx_batch_info(const hfx_batch *b, uint64_t *info);
 /* blocks of the dense form's last run (new): the spectra, in fragment charge
- * then precursor mass order, cut into runs of at most eight whose precursor
- * windows overlap most.  *num_blocks = their number; h_bstart (host,
+ * then precursor mass order, cut into runs of at most eight (BF16 form) or
+ * sixteen (integer form) whose precursor windows overlap most.  *num_blocks = their number; h_bstart (host,
  * optional) receives min(cap, num_blocks + 1) block starts as positions in
  * that order, the last one the spectrum count.  Synchronises. */
 int hfx_batch_dense_blocks(const hfx_batch *b, uint32_t *h_bstart, uint32_t cap,
