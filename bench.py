@@ -103,8 +103,9 @@ def bench_mods(steps, warmup, workload="mods-2k-50k"):
     enumeration stages (findModablePeptides, calcTotalModCands,
     prepareGenMod, genModCands: each synchronises the host for its count, as
     in the reference), hfx_mpep_offsets, hfx_xcorr_sparse_mod and
-    hfx_topn_mod.  Wall time per run (allocations included) and the scoring
-    and top-N kernels by events."""
+    hfx_topn_mod.  Wall time per run (host synchronisations and copies
+    included; a run reuses the previous run's device buffers when they are
+    large enough) and the scoring and top-N kernels by events."""
     from hfx_amd import pipeline, synth
 
     w = synth.WORKLOADS[workload]

@@ -753,6 +753,17 @@ class ModSearch:
         self.t_mod_cnt = dev(mi.mod_ma_count.reshape(-1))
         self.t_mod_sum = dev(mi.mod_sum)
         self.t_mod_delta = dev(mi.mod_delta)
+        self._pool: dict[str, DevBuf] = {}
+
+    def _buf(self, name: str, nbytes: int) -> DevBuf:
+        """A device buffer of at least nbytes for one of run()'s arrays: the
+        last run's, when it is large enough (a run's arrays stay valid until
+        the next run), else a new one."""
+        b = self._pool.get(name)
+        if b is None or b.nbytes < max(int(nbytes), 1):
+            self._pool.pop(name, None)
+            b = self._pool[name] = empty(nbytes)
+        return b
 
     def mpep_per_spectrum(self) -> np.ndarray:
         """The modified peptides each spectrum would score, from the stages
@@ -798,7 +809,7 @@ class ModSearch:
         L = lib()
         S, M, na = self.sp.num_spec, self.mi.num_mod, self.mi.num_ma
         db = self.db
-        t_b, t_e, t_sc = empty(4 * S * M), empty(4 * S * M), empty(4 * S * M)
+        t_b, t_e, t_sc = (self._buf(k, 4 * S * M) for k in ("b", "e", "sc"))
         if per_spectrum:
             spec_cand = np.zeros(S, np.uint32)
             for s in range(S):
@@ -807,7 +818,7 @@ class ModSearch:
                                                 db.t_idx.p, db.num_pep, self.t_mod_delta.p, M,
                                                 float(self.sp.mass[s]), self.eps)
         else:
-            t_cand = empty(4 * S)
+            t_cand = self._buf("cand", 4 * S)
             check(L.hfx_mod_windows(self.ctx.h, t_b.p, t_e.p, t_sc.p, t_cand.p,
                                     C.byref(db.desc), self.sp.t_mass.p, S, self.t_mod_delta.p,
                                     M, self.eps, None), "hfx_mod_windows")
@@ -815,42 +826,44 @@ class ModSearch:
         scan = np.zeros(S + 1, np.uint32)
         np.cumsum(spec_cand, out=scan[1:])
         total = int(scan[-1])
-        t_scan = dev(scan)
+        t_scan = self._buf("scan", scan.nbytes)
+        t_scan.upload(scan)
         keep = host if keep_stages else (lambda *a: None)
         st = {"spec_cand": spec_cand, "begin": keep(t_b, np.uint32, S * M),
               "end": keep(t_e, np.uint32, S * M), "num_scan": keep(t_sc, np.uint32, S * M)}
-        t_pep, t_pmod = empty(4 * total), empty(4 * total)
-        t_pcnt, t_vscan = empty(4 * total * na), empty(4 * S)
+        t_pep, t_pmod = self._buf("pep", 4 * total), self._buf("pmod", 4 * total)
+        t_pcnt, t_vscan = self._buf("pcnt", 4 * total * na), self._buf("vscan", 4 * S)
         nv = L.findModablePeptides(t_pep.p, t_pmod.p, t_pcnt.p, t_vscan.p, t_scan.p, S, total,
                                    db.t_ions.p, db.t_tc.p, db.t_tn.p, db.t_idx.p, t_b.p, t_e.p,
                                    t_sc.p, self.t_mod_cnt.p, M, self.t_ma.p, na)
         st.update(valid_pep=keep(t_pep, np.uint32, nv), valid_mod=keep(t_pmod, np.uint32, nv),
                   valid_cnt=keep(t_pcnt, np.uint32, nv * na), valid_scan=keep(t_vscan, np.uint32, S))
-        t_nmp, t_comb = empty(4 * nv), empty(4 * nv * na)
+        t_nmp, t_comb = self._buf("nmp", 4 * nv), self._buf("comb", 4 * nv * na)
         nmp = L.calcTotalModCands(t_nmp.p, t_comb.p, self.t_mod_cnt.p, t_pep.p, t_pmod.p, t_pcnt.p,
                                   nv, M, na) if nv else 0
         st.update(pep_num_mpep=keep(t_nmp, np.uint32, nv), comb=keep(t_comb, np.uint32, nv * na))
-        t_mp = [empty(4 * nmp) for _ in range(5)]
+        t_mp = [self._buf(f"mp{k}", 4 * nmp) for k in range(5)]
         lu = L.prepareGenMod(*[t.p for t in t_mp], self.t_mod_sum.p, t_pep.p, t_pmod.p, t_nmp.p,
                              nv, nmp) if nmp else 0
         for name, t in zip(("mp_pep", "mp_mod", "mp_rank", "mp_ith", "mp_scan"), t_mp):
             st[name] = keep(t, np.uint32, nmp)
-        t_unrank = empty(4 * max(lu, 1))
+        t_unrank = self._buf("unrank", 4 * max(lu, 1))
         if nmp:
             L.genModCands(t_unrank.p, self.t_mod_cnt.p, t_mp[3].p, t_mp[2].p, t_mp[4].p, t_pmod.p,
                           t_pcnt.p, t_comb.p, nmp, na)
         sync()
         st["unrank"] = keep(t_unrank, np.uint32, lu)
-        t_mpoff = empty(8 * (S + 1))
+        t_mpoff = self._buf("mpoff", 8 * (S + 1))
         check(L.hfx_mpep_offsets(self.ctx.h, t_mpoff.p, t_nmp.p, nv, t_vscan.p, S, None),
               "hfx_mpep_offsets")
         mp_off = host(t_mpoff, np.uint64, S + 1)
         self.modset = HfxModset(t_mp[0].p, t_mp[1].p, t_mp[4].p, t_unrank.p, self.t_mod_cnt.p,
                                 self.t_mod_delta.p, self.t_ma_mass.p, na, nmp)
         N = self.topn
-        t_scores = empty(4 * max(nmp, 1))
-        t_pos, t_ex = empty(4 * S * N), empty(8 * S * N)
-        t_f32, t_cnt, t_flag = empty(4 * S * N), empty(4 * S), empty(4 * S)
+        t_scores = self._buf("scores", 4 * max(nmp, 1))
+        t_pos, t_ex = self._buf("pos", 4 * S * N), self._buf("ex", 8 * S * N)
+        t_f32, t_cnt, t_flag = (self._buf(k, n) for k, n in (("f32", 4 * S * N), ("cnt", 4 * S),
+                                                              ("flag", 4 * S)))
         flagged = short = np.zeros(0, np.uint32)
         if nmp:
             check(L.hfx_xcorr_sparse_mod(self.ctx.h, t_scores.p, t_mpoff.p, nmp,
