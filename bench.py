@@ -226,6 +226,7 @@ def cpu_baseline(env, res, topn, target_s=1.0, reps=5):
         for i, s in enumerate(sel))
     dt_one, sel1, ref1 = timed(1)
     scores1 = int(ref1["num"].sum())
+    acc = score_error(orc, args, X, lens, dsp, sel, res, topn, threads)
     return dict(value=scores / dt_all, unit="scores/s", cores=threads, kind="port",
                 sample=f"{len(sel)} of {S} spectra drawn across the mass range ({scores} scored "
                        f"pairs): oracle dense-row scoring with exact top-{topn}, OpenMP over "
@@ -235,7 +236,35 @@ def cpu_baseline(env, res, topn, target_s=1.0, reps=5):
                                 sample=f"{len(sel1)} spectra ({scores1} scored pairs), best of "
                                        f"{reps}, {dt_one:.2f}s"),
                 build="g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
-                topn_identical_to_gpu=bool(same))
+                topn_identical_to_gpu=bool(same), accuracy=acc)
+
+
+def score_error(orc, args, X, lens, dsp, sel, res, topn, threads):
+    """Every score of the sampled spectra from the measured form against the
+    oracle's exact score of the same pair (outside the timed runs): the
+    relative error with the floor the full-size tests use (|exact| or a
+    quarter of the pair's sum w|x|, whichever is larger) and the plain
+    relative error."""
+    if res.scores is None:
+        return None
+    ex = orc.search_plain(*args, X[sel], lens[sel], dsp.zmax[sel], dsp.mass[sel], 3.0, topn,
+                          want_all=True, nthreads=threads)
+    ab = orc.search_plain(*args, np.abs(X[sel]), lens[sel], dsp.zmax[sel], dsp.mass[sel], 3.0,
+                          topn, want_all=True, nthreads=threads)
+    if not (np.diff(ex["off"].astype(np.int64)) == res.num[sel]).all():
+        raise SystemExit("sampled windows differ from the measured form's")
+    e = orc.fx_to_double(ex["all_fx"])[: int(ex["off"][-1])]
+    scale = orc.fx_to_double(ab["all_fx"])[: int(ex["off"][-1])]
+    g = np.concatenate([res.scores[int(res.off[s]):int(res.off[s + 1])] for s in sel])
+    d = np.abs(g.astype(np.float64) - e)
+    rel = d / np.maximum(np.abs(e), 0.25 * scale)
+    nz = np.abs(e) > 0
+    return dict(pairs=int(len(e)), max_rel=float(rel.max(initial=0.0)),
+                within_1e4=float((rel <= 1e-4).mean()) if len(e) else 1.0,
+                max_rel_of_exact=float((d[nz] / np.abs(e[nz])).max(initial=0.0)),
+                within_1e4_of_exact=float((d[nz] <= 1e-4 * np.abs(e[nz])).mean()) if nz.any() else 1.0,
+                rule="rel = |score - exact| / max(|exact|, 0.25 * sum w|x| of the pair); "
+                     "of_exact: |score - exact| / |exact| over pairs with exact != 0")
 
 
 def calib_read(mib: int):
@@ -380,7 +409,7 @@ def bench_batched(env, topn, steps, warmup, index, precision=0, margin=0):
         bs.replay()
     bs.stream.sync()
     elapsed = time.perf_counter() - t0
-    res = bs.results()
+    res = bs.results(keep_scores=precision != 0)
     # the same per-batch pipeline run phase by phase, with events on the
     # batch's stream around each phase and around the scoring kernel itself
     ph = [bs.time_phases() for _ in range(max(steps, 3))]
