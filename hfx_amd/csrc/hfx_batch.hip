@@ -28,12 +28,14 @@
  */
 #include "hfx_cand.h"
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 #include <numeric>
 #include <tuple>
 #include <vector>
@@ -743,6 +745,57 @@ static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
     return std::make_tuple(lg(b->S), lg(b->total), b->L);
 }
 
+/* The tuned shapes persist across processes when HFX_TUNE_CACHE names a
+ * file: one line a shape, "sparse <S bucket> <pairs bucket> <levels> <G> <C>
+ * <threads>", "bf16 ... <chunk>" or "int8 ... <chunk>" (other lines are
+ * ignored).  It is read when a lookup first sees the path (its entries
+ * replace this process's for the same shapes) and rewritten whole, through a
+ * temporary file and a rename, after each tuning.  Callers hold g_tune_mu. */
+static std::string g_tune_path;
+static void tune_load_locked()
+{
+    const char *p = getenv("HFX_TUNE_CACHE");
+    const std::string path = p ? p : "";
+    if (path == g_tune_path) return;
+    g_tune_path = path;
+    if (path.empty()) return;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char kind[16];
+    int sb, pb;
+    unsigned L, a, c, t;
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        const int n = sscanf(line, "%15s %d %d %u %u %u %u", kind, &sb, &pb, &L, &a, &c, &t);
+        const auto key = std::make_tuple(sb, pb, (uint32_t)L);
+        if (n == 7 && !strcmp(kind, "sparse") && a && c && t)
+            g_tune[key] = std::make_tuple((uint32_t)a, (uint32_t)c, (uint32_t)t);
+        else if (n == 5 && !strcmp(kind, "bf16") && a)
+            g_tune_dense[key] = a;
+        else if (n == 5 && !strcmp(kind, "int8") && a)
+            g_tune_dense8[key] = a;
+    }
+    fclose(f);
+}
+static void tune_save_locked()
+{
+    if (g_tune_path.empty()) return;
+    const std::string tmp = g_tune_path + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    for (const auto &e : g_tune)
+        fprintf(f, "sparse %d %d %u %u %u %u\n", std::get<0>(e.first), std::get<1>(e.first),
+                std::get<2>(e.first), std::get<0>(e.second), std::get<1>(e.second),
+                std::get<2>(e.second));
+    for (const auto &e : g_tune_dense)
+        fprintf(f, "bf16 %d %d %u %u\n", std::get<0>(e.first), std::get<1>(e.first),
+                std::get<2>(e.first), e.second);
+    for (const auto &e : g_tune_dense8)
+        fprintf(f, "int8 %d %d %u %u\n", std::get<0>(e.first), std::get<1>(e.first),
+                std::get<2>(e.first), e.second);
+    if (fclose(f) == 0) rename(tmp.c_str(), g_tune_path.c_str());
+}
+
 /* the form of the peptide index the current precision reads */
 static uint32_t bt_form(const hfx_batch *b)
 {
@@ -916,6 +969,7 @@ extern "C" int hfx_batch_plan(hfx_batch *b, const hfx_db *db, const hfx_spectra 
     b->precision = HFX_BATCH_FP32_SPARSE;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
         auto it = g_tune.find(tune_key(b));
         if (it != g_tune.end()) std::tie(b->G, b->C, b->threads) = it->second;
         auto jt = g_tune_dense.find(tune_key(b));
@@ -1127,6 +1181,7 @@ extern "C" int hfx_batch_set_precision(hfx_batch *b, uint32_t precision)
     if (bt_is_dense(precision) && precision != b->precision) {
         /* the tuned chunk of this form for batches of this shape, if any */
         std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
         auto &m = bt_tune_dense(precision);
         auto jt = m.find(tune_key(b));
         b->CH = jt != m.end() ? jt->second : 256;
@@ -1232,8 +1287,10 @@ extern "C" int hfx_batch_autotune(hfx_batch *b, void *stream, uint32_t *out_cfg)
     b->CH = bCH;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
         if (bt_is_dense(b->precision)) bt_tune_dense(b->precision)[tune_key(b)] = bCH;
         else g_tune[tune_key(b)] = std::make_tuple(bG, bC, bT);
+        tune_save_locked();
     }
     if (b->exec) {
         /* the captured graph holds the old launch shapes */
