@@ -383,7 +383,8 @@ def test_exact_route_past_the_lds_selection(orc, topn):
             assert (res.top_exact[s, :c] == orc.fx_to_double(ref["top_fx"][s, :c])).all()
 
 
-@pytest.mark.parametrize("n", [0, 1, 1000, 8191, 8192, 8193, 8192 * 7, 8192 * 7 + 1, 5_450_001])
+@pytest.mark.parametrize("n", [0, 1, 1000, 8191, 8192, 8193, 20_000, 32_767, 32_768, 32_769,
+                               8192 * 7, 8192 * 7 + 1, 5_450_001])
 def test_scan_offsets(env, n):
     """hfx_scan_offsets (one workgroup below 8192 counts, three passes of
     8192-count tiles above) against numpy's cumulative sum, with counts up to
