@@ -440,7 +440,10 @@ struct HfxModMass {
         float m = act ? mtab[HFX_ACID_IDX(ion)] : 0.0f;
         const uint64_t lt = (1ull << lane) - 1ull;
         uint32_t off = 0;
-        for (uint32_t a = 0; a < num_ma; ++a) {
+        /* unrolled to the largest acid count, so that occ stays in registers */
+#pragma unroll
+        for (uint32_t a = 0; a < HFX_MAX_MA; ++a) {
+            if (a >= num_ma) break;
             const bool is_a = act && mod == a + 1;
             const uint64_t bal = __ballot(is_a);
             const uint32_t k = mod_row[a];
