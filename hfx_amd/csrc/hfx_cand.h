@@ -13,6 +13,8 @@
 
 #include "hfx_common.h"
 
+#include <type_traits>
+
 /* Stage spectrum s of sp in LDS (xs) and the mass table (mtab); returns the
  * effective length, never above HFX_MAX_LEN or the stride. */
 __device__ __forceinline__ uint32_t hfx_stage_spectrum(float *xs, float *mtab, const hfx_db &db,
@@ -204,20 +206,27 @@ struct HfxModCands {
                 mam[a] = ms.d_ma_mass[a];
             }
         }
+        /* the residue loop tests only as many acids as the batch has: one
+         * instance for up to four (the usual M, S, T, Y), one for more */
         const uint8_t *res = db.d_ions + p.tc;
-        float bl = 0.0f;
-        for (uint32_t k = 0; k < nl; ++k) {
-            const uint32_t ion = res[k];
-            const uint32_t md = HFX_ACID_MOD(ion);
-            float m = mtab[HFX_ACID_IDX(ion)];
+        auto walk = [&](auto nac) {
+            constexpr uint32_t NA = decltype(nac)::value;
+            float bl = 0.0f;
+            for (uint32_t k = 0; k < nl; ++k) {
+                const uint32_t ion = res[k];
+                const uint32_t md = HFX_ACID_MOD(ion);
+                float m = mtab[HFX_ACID_IDX(ion)];
 #pragma unroll
-            for (uint32_t a = 0; a < HFX_MAX_MA; ++a)
-                if (md == a + 1) {
-                    if ((mask[a] >> occ[a]) & 1ull) m = m + mam[a];
-                    ++occ[a];
-                }
-            bl = bl + m;
-            dst[k] = bl;
-        }
+                for (uint32_t a = 0; a < NA; ++a)
+                    if (md == a + 1) {
+                        if ((mask[a] >> occ[a]) & 1ull) m = m + mam[a];
+                        ++occ[a];
+                    }
+                bl = bl + m;
+                dst[k] = bl;
+            }
+        };
+        if (na <= 4) walk(std::integral_constant<uint32_t, 4>{});
+        else walk(std::integral_constant<uint32_t, HFX_MAX_MA>{});
     }
 };
