@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <map>
@@ -780,7 +781,9 @@ static void tune_load_locked()
 static void tune_save_locked()
 {
     if (g_tune_path.empty()) return;
-    const std::string tmp = g_tune_path + ".tmp";
+    /* a temporary name of this process's own, so that processes tuning at
+     * once never write one file together (the last rename wins) */
+    const std::string tmp = g_tune_path + ".tmp." + std::to_string((long)getpid());
     FILE *f = fopen(tmp.c_str(), "w");
     if (!f) return;
     for (const auto &e : g_tune)
