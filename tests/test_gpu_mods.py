@@ -261,3 +261,45 @@ def test_mods_split_by_memory_matches_one_batch(cfg3):
         assert (res.top_f32[s, :c] == one.top_f32[s, :c]).all(), s
     assert res.flagged.tolist() == one.flagged.tolist()
     assert res.short.tolist() == one.short.tolist()
+
+
+def test_mods_full_uneven_workload_against_oracle(orc):
+    """The whole uneven workload searched on the GPU (50,000 spectra, 107 M
+    modified candidates); every fifth spectrum (10,000) checked against the
+    oracle's search on all host threads: per-spectrum counts, top-5
+    positions and exact scores identical."""
+    import time
+
+    from hfx_amd import pipeline, synth
+
+    w = synth.WORKLOADS["mods-2k-50k"]
+    prot = synth.make_proteins(w["proteins"], seed=w["seed"])
+    mass = orc.mass_table(w["static"])
+    r, tc, tn = orc.digest(prot.seq, prot.seg, mass, rule=1, missed=w["missed"],
+                           p_rule=w["p_rule"], min_len=w["min_len"], max_len=w["max_len"])
+    sp = synth.make_spectra(prot, r, tc, tn, w["spectra"], seed=w["seed"] + 100,
+                            charges=w["charges"], charge_p=w["charge_p"],
+                            double_for_3plus=w["double3"], var_mods=w["var"])
+    ctx = pipeline.Ctx(0)
+    mi = pipeline.mod_info(w["var"], w["max_ma"])
+    db = pipeline.DeviceDB(pipeline.HostDB(prot.seq, r, tc, tn, mass), ma=mi.ma)
+    dsp = pipeline.DeviceSpectra(ctx, sp.pcr, sp.charge, sp.mz, sp.inten, sp.off, STRIDE)
+    res = pipeline.ModSearch(ctx, db, dsp, mi, eps=3.0, topn=w["topn"]).run(keep_stages=False)
+    n = np.diff(res.mp_off).astype(np.uint32)
+    sel = np.arange(0, dsp.num_spec, 5)
+    t0 = time.time()
+    idx, _ = orc.sort_idx(r)
+    ref = orc.search_mods(orc.prepare_ions(prot.seq, mi.ma), r, tc, tn, mass, idx, mi.ma,
+                          mi.ma_mass, mi.mod_ma_count, mi.mod_sum, mi.mod_delta, dsp.x()[sel],
+                          dsp.lengths()[sel], dsp.zmax[sel], dsp.mass[sel], 3.0, w["topn"],
+                          cap_mpep=int(n[sel].sum()) + 1024,
+                          cap_unrank=int(res.len_unrank) + 1024)
+    print(f"{dsp.num_spec} spectra, {int(n.sum())} modified candidates; {len(sel)} checked, "
+          f"oracle {time.time() - t0:.1f}s", flush=True)
+    assert (n[sel] == ref["spec_num_mpep"]).all()
+    assert (res.top_cnt[sel] == ref["top_cnt"]).all()
+    bad = [int(s) for i, s in enumerate(sel)
+           if res.top_pos[s, :res.top_cnt[s]].tolist() != ref["top_pos"][i, :ref["top_cnt"][i]].tolist()
+           or not (res.top_exact[s, :res.top_cnt[s]] ==
+                   orc.fx_to_double(ref["top_fx"][i, :ref["top_cnt"][i]])).all()]
+    assert not bad, bad[:10]
