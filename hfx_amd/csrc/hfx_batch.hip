@@ -740,6 +740,23 @@ static size_t bt_dense_scratch(uint32_t S)
     return std::max(hfx_dense_scratch_bytes(S), hfx_dense8_scratch_bytes(S));
 }
 
+/* the launch shapes the kernels support: sparse G in {4, 8}, C in 1..4096,
+ * threads in {256, 512}; a dense chunk a multiple of its tile (16 candidates
+ * for BF16, 64 for int8) up to 8192 */
+static bool tune_sparse_ok(uint32_t G, uint32_t C, uint32_t threads)
+{
+    return (G == 4 || G == 8) && C >= 1 && C <= 4096 && (threads == 256 || threads == 512);
+}
+static uint32_t tune_dense_step(uint32_t precision)
+{
+    return precision == HFX_BATCH_I8X3_DENSE ? 64u : 16u;
+}
+static bool tune_dense_ok(uint32_t precision, uint32_t CH)
+{
+    const uint32_t step = tune_dense_step(precision);
+    return CH >= step && CH % step == 0 && CH <= 8192;
+}
+
 static std::tuple<int, int, uint32_t> tune_key(const hfx_batch *b)
 {
     auto lg = [](uint64_t v) { int k = 0; while (v > 1) { v >>= 1; ++k; } return k; };
@@ -769,11 +786,13 @@ static void tune_load_locked()
     while (fgets(line, sizeof line, f)) {
         const int n = sscanf(line, "%15s %d %d %u %u %u %u", kind, &sb, &pb, &L, &a, &c, &t);
         const auto key = std::make_tuple(sb, pb, (uint32_t)L);
-        if (n == 7 && !strcmp(kind, "sparse") && a && c && t)
+        /* entries outside the supported shapes (a damaged or hand-edited
+         * file) are skipped, never launched */
+        if (n == 7 && !strcmp(kind, "sparse") && tune_sparse_ok(a, c, t))
             g_tune[key] = std::make_tuple((uint32_t)a, (uint32_t)c, (uint32_t)t);
-        else if (n == 5 && !strcmp(kind, "bf16") && a)
+        else if (n == 5 && !strcmp(kind, "bf16") && tune_dense_ok(HFX_BATCH_BF16X2_DENSE, a))
             g_tune_dense[key] = a;
-        else if (n == 5 && !strcmp(kind, "int8") && a)
+        else if (n == 5 && !strcmp(kind, "int8") && tune_dense_ok(HFX_BATCH_I8X3_DENSE, a))
             g_tune_dense8[key] = a;
     }
     fclose(f);
@@ -1352,15 +1371,15 @@ extern "C" int hfx_batch_set_config(hfx_batch *b, uint32_t G, uint32_t C, uint32
     HFX_REQUIRE(b && b->planned, HFX_ERR_ARG, "hfx_batch_set_config: batch not planned");
     if (bt_is_dense(b->precision)) {
         /* dense forms: C = candidates per work item (G, threads fixed) */
-        const uint32_t step = b->precision == HFX_BATCH_I8X3_DENSE ? 64u : 16u;
-        HFX_REQUIRE(C >= step && C % step == 0 && C <= 8192, HFX_ERR_ARG,
+        const uint32_t step = tune_dense_step(b->precision);
+        HFX_REQUIRE(tune_dense_ok(b->precision, C), HFX_ERR_ARG,
                     "hfx_batch_set_config: dense chunk must be a multiple of %u in %u..8192",
                     step, step);
         b->CH = C;
         return HFX_OK;
     }
-    HFX_REQUIRE((G == 4 || G == 8) && C >= 1 && C <= 4096 && (threads == 256 || threads == 512),
-                HFX_ERR_ARG, "hfx_batch_set_config: G in {4,8}, C in 1..4096, threads in {256,512}");
+    HFX_REQUIRE(tune_sparse_ok(G, C, threads), HFX_ERR_ARG,
+                "hfx_batch_set_config: G in {4,8}, C in 1..4096, threads in {256,512}");
     b->G = G;
     b->C = C;
     b->threads = threads;

@@ -203,19 +203,24 @@ def test_i8_light_first_residue(orc):
 def test_tuned_shapes_persist_in_a_cache_file(denv, orc, tmp_path, monkeypatch):  # noqa: F811
     """HFX_TUNE_CACHE: a batch takes each dense form's chunk from the file's
     line for its shape (S, pairs and charge levels bucketed as the library
-    does), malformed lines are ignored, and a tuning rewrites the file with
-    the chunk it chose; rankings are unchanged."""
+    does), malformed lines and shapes no kernel supports are ignored (the
+    later lines would otherwise replace the valid ones), and a tuning
+    rewrites the file with the chunk it chose; rankings are unchanged."""
     pl = denv["pl"]
     path = tmp_path / "tune.txt"
     bs = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
     inf = bs.info()
     lg = lambda v: max(int(v), 1).bit_length() - 1  # noqa: E731
     key = f"{lg(inf['spectra'])} {lg(inf['scores'])} {inf['levels']}"
-    path.write_text(f"int8 {key} 512\nbf16 {key} 1024\nint8 not a line\n")
+    path.write_text(f"int8 {key} 512\nbf16 {key} 1024\nint8 not a line\n"
+                    f"int8 {key} 100\nint8 {key} 16384\nbf16 {key} 24\n"
+                    f"sparse {key} 3 64 300\nsparse {key} 8 0 256\n")
     monkeypatch.setenv("HFX_TUNE_CACHE", str(path))
     bs.set_precision(bs.I8X3_DENSE)
     assert bs.info()["chunk"] == 512
     other = pl.BatchSearch(denv["ctx"], denv["db"], denv["dsp"], eps=3.0, topn=10)
+    oinf = other.info()
+    assert oinf["G"] in (4, 8) and 1 <= oinf["C"] <= 4096 and oinf["threads"] in (256, 512)
     other.set_precision(other.BF16X2_DENSE)
     assert other.info()["chunk"] == 1024
     G, Cn, T = bs.autotune()
