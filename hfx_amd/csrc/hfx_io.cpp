@@ -31,6 +31,14 @@ void hfx_set_error(const char *fmt, ...);
 
 namespace {
 
+/* a container's bytes to dst (when dst is given): nothing for an empty one,
+ * whose data() may be null, which memcpy does not accept even for 0 bytes */
+template <class V>
+void copy_out(void *dst, const V &v)
+{
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+}
+
 /* ------------------------------------------------------------------ */
 /* Files                                                                 */
 /* ------------------------------------------------------------------ */
@@ -282,6 +290,8 @@ std::string digits_str(const std::vector<int> &v, size_t a, size_t b)
  * 2: showEFloat (Just dec) */
 std::string hs_format(double x, bool is_float, int kind, int dec, bool half_even = true)
 {
+    /* a Float is the value rounded to FP32 (beyond its range: Infinity) */
+    if (is_float) x = (double)(float)x;
     if (std::isnan(x)) return "NaN";
     if (std::isinf(x)) return x < 0 ? "-Infinity" : "Infinity";
     if (x < 0 || (x == 0 && std::signbit(x))) return "-" + hs_format(-x, is_float, kind, dec, half_even);
@@ -540,22 +550,22 @@ extern "C" int hfx_seqdb_copy(const hfx_seqdb *db, uint8_t *h_ions, uint32_t *h_
         hfx_set_error("hfx_seqdb_copy: null database");
         return HFX_ERR_ARG;
     }
-    if (h_ions) std::memcpy(h_ions, db->ions.data(), db->ions.size());
-    if (h_ion_seg) std::memcpy(h_ion_seg, db->ion_seg.data(), db->ion_seg.size() * 4);
+    copy_out(h_ions, db->ions);
+    copy_out(h_ion_seg, db->ion_seg);
     if (h_headers || h_header_off) {
         uint64_t o = 0;
         for (size_t i = 0; i < db->headers.size(); ++i) {
             if (h_header_off) h_header_off[i] = o;
-            if (h_headers) std::memcpy(h_headers + o, db->headers[i].data(), db->headers[i].size());
+            if (h_headers) copy_out(h_headers + o, db->headers[i]);
             o += db->headers[i].size();
         }
         if (h_header_off) h_header_off[db->headers.size()] = o;
     }
-    if (h_r) std::memcpy(h_r, db->r.data(), db->r.size() * 4);
-    if (h_tc) std::memcpy(h_tc, db->c.data(), db->c.size() * 4);
-    if (h_tn) std::memcpy(h_tn, db->n.data(), db->n.size() * 4);
-    if (h_frag_seg) std::memcpy(h_frag_seg, db->frag_seg.data(), db->frag_seg.size() * 4);
-    if (h_params) std::memcpy(h_params, db->params.data(), db->params.size());
+    copy_out(h_r, db->r);
+    copy_out(h_tc, db->c);
+    copy_out(h_tn, db->n);
+    copy_out(h_frag_seg, db->frag_seg);
+    copy_out(h_params, db->params);
     return HFX_OK;
 }
 
@@ -871,15 +881,15 @@ extern "C" int hfx_mgf_copy(const hfx_mgf *m, float *h_pcr, float *h_charge, uin
         hfx_set_error("hfx_mgf_copy: null argument");
         return HFX_ERR_ARG;
     }
-    if (h_pcr) std::memcpy(h_pcr, m->pcr.data(), m->pcr.size() * 4);
-    if (h_charge) std::memcpy(h_charge, m->charge.data(), m->charge.size() * 4);
-    if (h_off) std::memcpy(h_off, m->off.data(), m->off.size() * 4);
-    if (h_mz) std::memcpy(h_mz, m->mz.data(), m->mz.size() * 4);
-    if (h_inten) std::memcpy(h_inten, m->inten.data(), m->inten.size() * 4);
+    copy_out(h_pcr, m->pcr);
+    copy_out(h_charge, m->charge);
+    copy_out(h_off, m->off);
+    copy_out(h_mz, m->mz);
+    copy_out(h_inten, m->inten);
     uint64_t o = 0;
     for (size_t i = 0; i < m->titles.size(); ++i) {
         if (h_title_off) h_title_off[i] = o;
-        if (h_titles) std::memcpy(h_titles + o, m->titles[i].data(), m->titles[i].size());
+        if (h_titles) copy_out(h_titles + o, m->titles[i]);
         o += m->titles[i].size();
     }
     if (h_title_off) h_title_off[m->titles.size()] = o;

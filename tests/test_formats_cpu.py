@@ -304,6 +304,13 @@ def test_number_formats_against_decimal():
     for v, want in [(3.0, "3.0"), (0.1, "0.1"), (0.01, "1.0e-2"), (1595.7012, "1595.7012"),
                     (1.0e7, "1.0e7"), (9999999.0, "9999999.0"), (123.456, "123.456")]:
         assert formats.format_number(v, 0, 0, is_float=True) == want, v
+    # a Float is the value rounded to FP32: Doubles beyond its range are
+    # infinite, below its smallest subnormal zero
+    for kind in (0, 1, 2):
+        assert formats.format_number(1e300, kind, 4, is_float=True) == "Infinity"
+        assert formats.format_number(-1e300, kind, 4, is_float=True) == "-Infinity"
+    assert formats.format_number(1e-300, 0, 0, is_float=True) == "0.0"
+    assert formats.format_number(1e300, 0, 0) == "1.0e300"
 
 
 def test_format_scan_layout():
