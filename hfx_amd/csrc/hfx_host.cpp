@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/hfx_hip.h"
@@ -185,18 +186,38 @@ extern "C" int hfx_mod_info(const float *var26, uint32_t max_ma, uint8_t *out_ma
     *out_num_ma = na;
     *out_num_mod = 0;
     if (na == 0) return HFX_OK;
-    /* a vector with sum < max_ma has every entry < max_ma; enumerate the
-     * lexicographic order directly as a mixed-radix counter */
+    /* the count first: vectors of na counts with sum <= max_ma - 1 number
+     * C(na + max_ma - 1, k) with k = min(na, max_ma - 1), the zero vector
+     * among them; C(n, i) rises with i up to k <= n / 2, so it saturates at
+     * 2^40, far beyond any table a caller can hold, once a partial product
+     * passes that */
+    uint64_t total = 0;
+    if (max_ma > 0) {
+        const uint64_t n = (uint64_t)max_ma - 1 + na, k = std::min<uint64_t>(na, max_ma - 1);
+        unsigned __int128 c = 1;
+        for (uint64_t i = 0; i < k && c <= ((uint64_t)1 << 40); ++i) c = c * (n - i) / (i + 1);
+        total = c > ((uint64_t)1 << 40) ? (uint64_t)1 << 40 : (uint64_t)c - 1;
+    }
+    if (total > cap_mod) {
+        *out_num_mod = total > UINT32_MAX ? UINT32_MAX : (uint32_t)total;
+        hfx_set_error("hfx_mod_info: %llu modifications exceed capacity %u",
+                      (unsigned long long)total, cap_mod);
+        return HFX_ERR_ARG;
+    }
+    /* lexicographic order, the last acid fastest, over the vectors with sum
+     * < max_ma only: the next one raises the rightmost count whose raise
+     * (with every count right of it cleared) keeps the sum below max_ma */
     std::vector<uint32_t> v(na, 0);
-    uint32_t nm = 0;
+    uint32_t nm = 0, sum = 0;
     for (;;) {
         int d = (int)na - 1;
-        while (d >= 0 && v[d] == max_ma) v[d--] = 0;
+        while (d >= 0 && sum + 1 >= max_ma) {
+            sum -= v[d];
+            v[d--] = 0;
+        }
         if (d < 0) break;
         ++v[d];
-        uint32_t sum = 0;
-        for (uint32_t a = 0; a < na; ++a) sum += v[a];
-        if (sum >= max_ma) continue;
+        ++sum;
         if (nm < cap_mod && out_mod_ma_count && out_mod_sum && out_mod_delta) {
             float delta = 0.0f;
             for (uint32_t a = 0; a < na; ++a) {

@@ -950,6 +950,10 @@ extern "C" int hfx_format_table(const char *const *cells, uint32_t nrows, uint32
         hfx_set_error("hfx_format_table: null cells");
         return HFX_ERR_ARG;
     }
+    if (q < 0) {
+        hfx_set_error("hfx_format_table: negative column spacing %d", q);
+        return HFX_ERR_ARG;
+    }
     std::vector<std::vector<std::string>> rows(nrows, std::vector<std::string>(ncols));
     for (uint32_t i = 0; i < nrows; ++i)
         for (uint32_t j = 0; j < ncols; ++j) rows[i][j] = cells[(size_t)i * ncols + j];

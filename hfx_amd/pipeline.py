@@ -701,6 +701,11 @@ class ModInfo:
         return len(self.mod_delta)
 
 
+# the largest modification table mod_info builds (count vectors of up to 26
+# acids, 4 bytes a count)
+MAX_MODS = 1 << 20
+
+
 def mod_info(var: dict, max_ma: int) -> ModInfo:
     L = lib()
     v = np.zeros(26, np.float32)
@@ -712,6 +717,9 @@ def mod_info(var: dict, max_ma: int) -> ModInfo:
     L.hfx_mod_info(v.ctypes.data, max_ma, ma.ctypes.data, mam.ctypes.data, C.byref(na), None,
                    None, None, 0, C.byref(nm))
     a, m = na.value, nm.value
+    if m > MAX_MODS:
+        raise HfxError(f"hfx_mod_info: {m} modifications of at most {max_ma - 1} sites over "
+                       f"{a} acids exceed {MAX_MODS}")
     cnt = np.zeros(max(m * a, 1), np.uint32)
     sm = np.zeros(max(m, 1), np.uint32)
     dl = np.zeros(max(m, 1), np.float32)

@@ -78,6 +78,38 @@ def test_mod_info_matches_oracle(orc):
         assert (sm[:m] == o[3]).all() and (dl[:m] == o[4]).all()
 
 
+def test_mod_info_against_brute_force():
+    """hfx_mod_info walks only the count vectors with sum < max_ma; the table
+    equals a filter of every vector in 0..max_ma per acid (lexicographic, the
+    last acid fastest, zero dropped), and a table too large to hold is
+    refused at once with its count (saturated) instead of enumerated."""
+    import itertools
+    rng = np.random.default_rng(4)
+    for _ in range(40):
+        acids = sorted(rng.choice(list("ACDEFGHIKLMNPQRSTVWY"), rng.integers(1, 6), replace=False))
+        var = {a: float(np.float32(rng.uniform(-20, 80))) or 1.0 for a in acids}
+        mx = int(rng.integers(0, 6))
+        want = [v for v in itertools.product(range(mx + 1), repeat=len(acids))
+                if 0 < sum(v) < mx]
+        mi = pipeline.mod_info(var, mx)
+        assert mi.mod_ma_count.tolist() == [list(v) for v in want], (var, mx)
+        assert mi.mod_sum.tolist() == [sum(v) for v in want]
+    L = hfx_amd.lib()
+    v = np.full(26, 1.5, np.float32)
+    ma = np.zeros(26, np.uint8)
+    mam = np.zeros(26, np.float32)
+    na, nm = C.c_uint32(), C.c_uint32()
+    rc = L.hfx_mod_info(v.ctypes.data, 60, ma.ctypes.data, mam.ctypes.data, C.byref(na), None,
+                        None, None, 1024, C.byref(nm))
+    assert rc != 0 and na.value == 26 and nm.value == 2**32 - 1
+    # C(28, 2) - 1 vectors of one or two sites over 26 acids
+    rc = L.hfx_mod_info(v.ctypes.data, 3, ma.ctypes.data, mam.ctypes.data, C.byref(na), None,
+                        None, None, 0, C.byref(nm))
+    assert rc != 0 and nm.value == 377
+    with pytest.raises(hfx_amd.HfxError):
+        pipeline.mod_info({c: 1.0 for c in "ACDEFGHIKLMNPQRSTVWY"}, 40)
+
+
 def test_precursor_helpers_match_oracle(orc):
     L = hfx_amd.lib()
     rng = np.random.default_rng(0)
