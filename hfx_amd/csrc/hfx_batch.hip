@@ -897,6 +897,9 @@ extern "C" int hfx_batch_split(const uint64_t *h_off, uint32_t S, uint32_t N, ui
     HFX_REQUIRE(h_off && num_batches, HFX_ERR_ARG, "hfx_batch_split: null argument");
     HFX_REQUIRE(precision <= HFX_BATCH_I8X3_DENSE, HFX_ERR_ARG,
                 "hfx_batch_split: unknown precision %u", precision);
+    for (uint32_t s = 0; s < S; ++s)
+        HFX_REQUIRE(h_off[s + 1] >= h_off[s], HFX_ERR_ARG,
+                    "hfx_batch_split: score offsets decrease at spectrum %u", s);
     uint32_t nb = 0, a = 0;
     while (a < S) {
         /* the longest run [a, e) that fits: the requirement grows with the

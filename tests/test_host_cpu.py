@@ -180,6 +180,10 @@ def test_batch_sizing_and_split():
     nb = C.c_uint32()
     rc = L.hfx_batch_split(off.ctypes.data, S, 5, 2_700_000, 0, one // 2, None, 0, C.byref(nb))
     assert rc == 3 and "alone needs" in hfx_amd.last_error()
+    bad = off.copy()
+    bad[7] = bad[8] + 1
+    rc = L.hfx_batch_split(bad.ctypes.data, S, 5, 2_700_000, 0, one * 4, None, 0, C.byref(nb))
+    assert rc != 0 and "decrease at spectrum 7" in hfx_amd.last_error()
 
 
 def _reference_merge(plain, mods, n):
