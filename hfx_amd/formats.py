@@ -48,11 +48,15 @@ class SeqDB:
 
     def lookup(self, k: int):
         """(label "X.PEP.Y", protein, (M+H)+) of fragment k."""
-        buf = C.create_string_buffer(4096)
+        cap = 4096
         n, p, mh = C.c_uint32(), C.c_uint32(), C.c_float()
-        check(lib().hfx_seqdb_lookup(self.h, k, buf, 4096, C.byref(n), C.byref(p), C.byref(mh)),
-              "hfx_seqdb_lookup")
-        return buf.value.decode(), int(p.value), np.float32(mh.value)
+        while True:
+            buf = C.create_string_buffer(cap)
+            check(lib().hfx_seqdb_lookup(self.h, k, buf, cap, C.byref(n), C.byref(p),
+                                         C.byref(mh)), "hfx_seqdb_lookup")
+            if n.value < cap:  # the whole label fitted (n is its full length)
+                return buf.value.decode(), int(p.value), np.float32(mh.value)
+            cap = n.value + 1
 
     def write_index(self, path: str, params: str):
         check(lib().hfx_seqdb_write_index(self.h, path.encode(), params.encode()),

@@ -159,6 +159,11 @@ def test_lookup_labels(orc):
     # protein 1's last piece "GSAFIKTRE" ends at its end: shows P3's G
     last = [db.lookup(k)[0] for k in range(len(db.r)) if db.lookup(k)[1] == 1][-1]
     assert last == "R.EG.-"
+    # a label longer than the first buffer comes back whole
+    long = "ACDEFGHIKLMNPQRSTVWY" * 250
+    db = formats.read_fasta(mass, text=f">big\n{long}\n", rule=0, missed=0, min_mass=0.0,
+                            max_mass=1e9)
+    assert db.lookup(0)[0] == f"-.{long}.-"
 
 
 def test_dta_parse(orc):
