@@ -938,8 +938,15 @@ extern "C" int hfx_format_number(double v, int is_float, int kind, int digits, c
         hfx_set_error("hfx_format_number: kind %d", kind);
         return HFX_ERR_ARG;
     }
+    const std::string s = hs_format(v, is_float != 0, kind, digits);
     uint64_t n = 0;
-    return put_out(hs_format(v, is_float != 0, kind, digits), out, cap, &n);
+    put_out(s, out, cap, &n);
+    if (!out || s.size() >= cap) {
+        hfx_set_error("hfx_format_number: the text needs %llu bytes, the buffer holds %u",
+                      (unsigned long long)s.size() + 1, cap);
+        return HFX_ERR_ARG;
+    }
+    return HFX_OK;
 }
 
 /* ppAsRows q over nrows x ncols cells (row-major, NUL-terminated strings) */

@@ -140,8 +140,9 @@ def parse_mgf(text: str | bytes):
 
 
 def format_number(v: float, kind: int, digits: int = 4, is_float: bool = False) -> str:
-    buf = C.create_string_buffer(128)
-    check(lib().hfx_format_number(float(v), int(is_float), kind, digits, buf, 128),
+    cap = 320 + max(int(digits), 0)  # a sign, 309 integer digits, a point, the decimals
+    buf = C.create_string_buffer(cap)
+    check(lib().hfx_format_number(float(v), int(is_float), kind, digits, buf, cap),
           "hfx_format_number")
     return buf.value.decode()
 

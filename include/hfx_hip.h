@@ -523,7 +523,9 @@ int hfx_mgf_copy(const hfx_mgf *m, float *h_pcr, float *h_charge, uint32_t *h_of
 int hfx_mgf_free(hfx_mgf *m);
 /* PrettyPrint.hs:83-180.  Numbers as Haskell prints them: kind 0 show,
  * 1 showFFloat (Just digits), 2 showEFloat (Just digits); is_float: the
- * value is a Float (its own shortest digits). */
+ * value is a Float (its own shortest digits).  A text that does not fit
+ * cap - 1 bytes is cut there and HFX_ERR_ARG returned (it needs at most
+ * 320 + digits bytes). */
 int hfx_format_number(double v, int is_float, int kind, int digits, char *out, uint32_t cap);
 /* ppAsRows q over nrows x ncols NUL-terminated cells (row-major) */
 int hfx_format_table(const char *const *cells, uint32_t nrows, uint32_t ncols, int q, char *out,

@@ -10,6 +10,7 @@ showFFloat (Just 4) byte for byte.  The files were written by an older
 version of the tables (no Title row, no Ions and E-value columns, the
 hydrogen atom's mass); the layout functions are the same.
 """
+import ctypes as C
 import gzip
 import json
 import math
@@ -20,6 +21,7 @@ from decimal import ROUND_HALF_EVEN, Decimal
 import numpy as np
 import pytest
 
+import hfx_amd
 from hfx_amd import formats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -316,6 +318,12 @@ def test_number_formats_against_decimal():
         assert formats.format_number(-1e300, kind, 4, is_float=True) == "-Infinity"
     assert formats.format_number(1e-300, 0, 0, is_float=True) == "0.0"
     assert formats.format_number(1e300, 0, 0) == "1.0e300"
+    # the widest fixed form comes back whole; a buffer too small is an error
+    big = formats.format_number(-1.7976931348623157e308, 1, 4)
+    assert len(big) == 1 + 309 + 5 and big.startswith("-17976931348623157") and big.endswith(".0000")
+    buf = C.create_string_buffer(8)
+    assert hfx_amd.lib().hfx_format_number(12345.678, 0, 1, 4, buf, 8) != 0
+    assert buf.value == b"12345.6" and "needs 11 bytes" in hfx_amd.last_error()
 
 
 def test_format_scan_layout():
